@@ -1,0 +1,111 @@
+#!/usr/bin/env python3
+"""Benchmark of the radiative-transfer operator and its PCG on one MI355X.
+
+Workload (the ball64 configuration): cubes of a 64^3 grid on [-1,1]^3 with
+centroid r <= 1 (824,256 tets, 147,253 vertices), kappa = 50 exp(-r/0.2),
+I_p = (1 - 0.8 r)^4, no scattering, harmonic order K = 3 (28 harmonics,
+the 110-point Lebedev order), FP64.
+
+A step is one application of the operator A (computeLhs: volume + vacuum
+boundary kernels) to a device-resident vector.  value = DOF*ordinate/s =
+nP * slm * steps / wall seconds of the timed loop (device-synchronised on
+both sides).  Also reported: time to a 1e-8 relative FP64 residual of the
+plain FP64 PCG (the reference algorithm, main.cpp:125-162), the roofline of
+the volume kernel (HIP events on its stream) and the oracle timed on the
+host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-1e-8 residual & DOF·ordinate/s, learned+mixed vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def volume_kernel_bytes(nT, nP, slm, p):
+    """Algorithmic bytes of one operator application's volume part: per tet
+    4 u32 indices + 1/(kappa V) + 3 stored s rows (16 + 10p), and f read /
+    r written once (2 nP slm p)."""
+    return nT * (16 + 10 * p) + 2 * nP * slm * p
+
+
+def cpu_baseline(P, seconds=12.0):
+    import numpy as np
+    import oracle
+    a = P.ang
+    op = oracle.OracleProblem(P.mesh.arrays(), dict(omega=a.omega, omega_pos=a.omega_pos, Ox=a.Ox, Oy=a.Oy,
+                                                    Oz=a.Oz, massfac=a.massfac, slm=a.slm))
+    f = op.rhs()
+    op.lhs(f)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        op.lhs(f)
+        n += 1
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+    return dict(value=op.nP * op.slm * n / dt, unit="DOF*ordinate/s", cores=cores, kind="port",
+                sample="%d applications of the FP64 oracle operator (kernels.cu restated, OpenMP over "
+                       "vertices) on the full ball64 mesh, %.1f s" % (n, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--n", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(the multi-GPU sharding is a design note, not an implementation)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import transfer_amd as ta
+    P = ta.problems.ball(args.n)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    nP, nT, slm = P.mesh.nP, P.mesh.nT, P.ang.slm
+    print("[bench] %s nP=%d nT=%d nF=%d slm=%d" % (P.name, nP, nT, P.mesh.nF, slm), file=sys.stderr, flush=True)
+    wall, vol_ms, bnd_ms, step_ms = ctx.bench_operator(args.warmup, args.steps)
+    value = nP * slm * args.steps / wall
+    ms_per_step = wall / args.steps * 1e3
+    vbytes = volume_kernel_bytes(nT, nP, slm, 8)
+    achieved = vbytes / (vol_ms * 1e-3) / 1e9
+    print("[bench] operator %.1f us/step (volume %.1f us, boundary %.1f us)" % (ms_per_step * 1e3, vol_ms * 1e3,
+                                                                              bnd_ms * 1e3), file=sys.stderr, flush=True)
+    out = dict(metric=METRIC, value=value, unit="DOF*ordinate/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="%s: ball of 64^3-grid cubes, K=3 (slm 28), kappa=50exp(-r/0.2), "
+                                    "I_p=(1-0.8r)^4, FP64 operator apply" % P.name,
+                           nP=nP, nT=nT, nF=P.mesh.nF, slm=slm))
+    out["roofline"] = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                           frac=achieved / HBM_PEAK_GBS, traffic=None,
+                           kernel="k_lhs_volume<double,4,7>", kernel_ms=vol_ms,
+                           algorithmic_bytes=vbytes)
+    if not args.no_solve:
+        solves = {}
+        for name, prec in (("pcg_fp64_plain", False),):
+            d_f = ctx.allocVector()
+            st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_REFERENCE, use_prec=prec, maxit=20000,
+                              tol_abs=0.0, tol_rel=1e-8)
+            solves[name] = dict(seconds=st.seconds, iters=st.iters, preck=st.preck, true_rel=st.true_rel,
+                                converged=st.status == 0)
+            ctx.deviceFree(d_f)
+            print("[bench] %s: %.3f s, %d iters, true rel %.2e" % (name, st.seconds, st.iters, st.true_rel),
+                  file=sys.stderr, flush=True)
+        out["time_to_1e-8"] = solves
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(P)
+        print("[bench] cpu baseline %.3g DOF*ordinate/s" % out["cpu_baseline"]["value"], file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

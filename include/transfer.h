@@ -1,0 +1,265 @@
+/*
+ * transfer.h -- C ABI of transfer_amd, an MI355X-native drop-in for the
+ * CudaContext + kernels.cu path of uranix/Transfer.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Every call returns 0 on
+ * success and a negative code on failure (tr_last_error() has the text),
+ * except the few that return a value; those document their failure value.
+ *
+ * Vector convention (the reference's): a "vector" is nP*aslm REAL values laid
+ * out [vertex][aslm] in the caller's vertex order, aslm = slm rounded up to
+ * 64 bytes (DeviceAngularData.cpp:10, common.cuh:5-6), padding lanes zero.
+ * REAL is double (the reference's USE_DOUBLE build, util.h:24-28).
+ * Device vectors returned by tr_alloc_vector() are stored internally in a
+ * renumbered, unpadded layout; tr_copy_to_dev/tr_copy_to_host convert when
+ * the device side is such a vector, so callers only ever see the reference
+ * layout.  Raw buffers from tr_device_alloc() are copied byte for byte.
+ */
+#ifndef TRANSFER_AMD_H
+#define TRANSFER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TR_OK 0
+#define TR_ERR_ARG (-1)
+#define TR_ERR_HIP (-2)
+#define TR_ERR_STATE (-3)
+#define TR_ERR_NOCONV (-4)
+
+/* ------------------------------------------------------------------------ */
+/* Host data structures                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Same bytes as the reference's tetrahedron (tetrahedron.h:7-20) and face
+ * (face.h:8-18) in the USE_DOUBLE build: 128 and 64 bytes. */
+#pragma pack(push, 1)
+typedef struct tr_tetrahedron {
+    uint32_t p[4];
+    double kappa_volume; /* kappa * V */
+    double I_p;
+    double s[4][3];      /* s_j = 3 V grad(e_j) = -S_j n_j (MeshData.cpp:65-73) */
+} tr_tetrahedron;
+
+typedef struct tr_face {
+    uint32_t p[4];       /* p[3] unused */
+    double s[3];         /* S * n (MeshData.cpp:79-91) */
+    double _padd[3];
+} tr_face;
+#pragma pack(pop)
+
+/* Kernel-argument view of MeshData (MeshData.h:13-32, DeviceMeshDataRaw
+ * MeshData.h:34-47).  Arrays are borrowed for the duration of the call that
+ * takes the view. */
+typedef struct tr_mesh_view {
+    uint32_t nP, nF, nT;
+    const uint32_t *tetstart, *tetidx, *tetpos;
+    const tr_tetrahedron *mesh;
+    const uint32_t *facestart, *faceidx, *facepos;
+    const tr_face *bnd;
+    /* Optional, may be NULL.  coords[nP][3] drives the Morton renumbering
+     * (without it the device numbering comes from a reverse Cuthill-McKee
+     * pass over the vertex graph).  face_kind[nF]: 0 = vacuum (the
+     * reference's only kind), 1 = specular reflection (no boundary term). */
+    const double *coords;
+    const uint8_t *face_kind;
+} tr_mesh_view;
+
+/* View of AngularData (AngularData.h:8-18): unpadded host arrays
+ * omega/omega_pos [3][slm][slm], Ox/Oy/Oz [slm][slm]; massfac[slm] is an
+ * extension (collision factor 1 - albedo*g_l per harmonic; NULL = 1, the
+ * reference's scattering-free operator). */
+typedef struct tr_angular_view {
+    uint32_t slm;
+    const double *omega;
+    const uint32_t *omega_pos;
+    const double *Ox, *Oy, *Oz;
+    const double *massfac;
+} tr_angular_view;
+
+/* ------------------------------------------------------------------------ */
+/* Front end: AngularData, MeshData, Config                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct tr_angular tr_angular;
+typedef struct tr_mesh tr_mesh;
+typedef struct tr_config tr_config;
+
+/* Replaces AngularData(int maxk) (AngularData.cpp:9-68).  albedo/g/hg_lmax
+ * add scattering: albedo = 0 gives the reference's matrices; hg_lmax = 0
+ * with albedo > 0 is isotropic scattering; hg_lmax = L is Henyey-Greenstein
+ * with asymmetry g truncated after P_L. */
+int tr_angular_create(int maxk, double albedo, double g, int hg_lmax,
+                      tr_angular **out);
+int tr_angular_destroy(tr_angular *a);
+int tr_angular_get_view(const tr_angular *a, tr_angular_view *v);
+
+/* Replaces MeshData(const Config&) (MeshData.cpp:9-94), whose mesh came from
+ * meshProcessor's Netgen reader: here meshes come from generators.
+ * A box [lo,hi] split into nx*ny*nz cubes of 6 Kuhn tetrahedra each.
+ * jitter > 0 moves every vertex not on the boundary surface by
+ * U[-jitter, jitter] cell widths per axis (seeded).  ball_radius > 0 keeps
+ * only cubes whose centroid lies within ball_radius of ball_center. */
+typedef struct tr_mesh_spec {
+    int nx, ny, nz;
+    double lo[3], hi[3];
+    double jitter;
+    double ball_radius;
+    double ball_center[3];
+    uint64_t seed;
+} tr_mesh_spec;
+int tr_mesh_create_box(const tr_mesh_spec *spec, tr_mesh **out);
+/* General meshes: vertices coords[nP][3], tets tets[nT][4]; per-tet
+ * kappa/Ip may be NULL (defaults 1 and 0). */
+int tr_mesh_create(uint32_t nP, const double *coords, uint32_t nT,
+                   const uint32_t *tets, const double *kappa, const double *Ip,
+                   tr_mesh **out);
+int tr_mesh_destroy(tr_mesh *m);
+int tr_mesh_get_view(const tr_mesh *m, tr_mesh_view *v);
+/* coords [nP][3], tets [nT][4], centroids [nT][3], volumes [nT] */
+int tr_mesh_get_geometry(const tr_mesh *m, double *coords, uint32_t *tets,
+                         double *centroids, double *volumes);
+/* Per-tet fields (the reference's [region] kappa / Ip, Config.cpp:88-101). */
+int tr_mesh_set_fields(tr_mesh *m, const double *kappa, const double *Ip);
+/* Boundary-face kinds (0 vacuum, 1 reflective) by an axis-aligned plane
+ * test: faces lying on x_axis = value (|x - value| < tol) get `kind`. */
+int tr_mesh_set_face_kind_plane(tr_mesh *m, int axis, double value, double tol,
+                                int kind);
+
+/* Per-tet field generators (seeded; SplitMix64 counter-based RNG). */
+enum {
+    TR_FIELD_CONST = 0,      /* p[0]                                          */
+    TR_FIELD_RADIAL_EXP = 1, /* p[0] * exp(-r / p[1]), r = |centroid - p[2..4]| */
+    TR_FIELD_UNIFORM = 2,    /* p[0] + p[1] * U[0,1)                          */
+    TR_FIELD_LOGNORMAL = 3,  /* exp(p[0] + p[1] * N(0,1))                     */
+    TR_FIELD_BLOBS = 4,      /* p[0] + sum of n=p[1] Gaussians: amplitude
+                                log-uniform [p[2],p[3]], width U[p[4],p[5]],
+                                centre U[lo,hi]^3 of the mesh bounding box     */
+    TR_FIELD_PLANCK = 5,     /* (1 - p[0] * r)^4, r as RADIAL_EXP (p[2..4])   */
+    TR_FIELD_STEP = 6        /* centroid[axis=p[0]] < p[1] ? p[2] : p[3]       */
+};
+int tr_field_eval(const tr_mesh *m, int kind, const double *params,
+                  uint64_t seed, uint64_t stream, double *out /* [nT] */);
+/* SplitMix64 uniforms in [0,1): out[i] = U(seed, stream, i) */
+int tr_rng_uniform(uint64_t seed, uint64_t stream, uint64_t n, double *out);
+
+/* Config (Config.h:4-48, Config.cpp:8-131): same cfg grammar and keys. */
+int tr_config_load(const char *path, tr_config **out);
+int tr_config_destroy(tr_config *c);
+/* ints: maxk, device, dump, precondition, soltype, numregs */
+int tr_config_ints(const tr_config *c, int *ints6);
+int tr_config_region(const tr_config *c, int region, double *kappa, double *Ip);
+int tr_config_strings(const tr_config *c, char *meshfile, char *outfile,
+                      size_t cap);
+
+/* ------------------------------------------------------------------------ */
+/* Device context: the CudaContext of CudaContext.h:9-50                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct tr_ctx tr_ctx;
+
+/* CudaContext(dev, MeshData, AngularData) (CudaContext.cpp:76-102):
+ * uploads mesh and angular data (DeviceMeshData.cpp:8-52,
+ * DeviceAngularData.cpp:7-67).  flags: reserved, pass 0. */
+int tr_ctx_create(int device, const tr_mesh_view *mesh,
+                  const tr_angular_view *ang, int flags, tr_ctx **out);
+int tr_ctx_destroy(tr_ctx *ctx);               /* ~CudaContext :104-110 */
+uint32_t tr_ctx_N(const tr_ctx *ctx);          /* N() = nP*aslm :119-121 */
+uint32_t tr_ctx_slm(const tr_ctx *ctx);
+uint32_t tr_ctx_aslm(const tr_ctx *ctx);
+uint32_t tr_ctx_nP(const tr_ctx *ctx);
+/* Returns the internal numbering: perm[new] = caller vertex id. */
+int tr_ctx_permutation(const tr_ctx *ctx, uint32_t *perm);
+
+void *tr_device_alloc(tr_ctx *ctx, size_t bytes);   /* zero-filled, :112-117; NULL on failure */
+double *tr_alloc_vector(tr_ctx *ctx);               /* :123-125; NULL on failure */
+int tr_device_free(tr_ctx *ctx, void *mem);         /* :127-129 */
+int tr_copy_to_dev(tr_ctx *ctx, void *dst, const void *src, size_t bytes);   /* :131-133 */
+int tr_copy_to_host(tr_ctx *ctx, void *dst, const void *src, size_t bytes);  /* :135-137 */
+/* getHostSmall (:139-150): dst[nP][slm] (caller-allocated, not malloc'd) */
+int tr_get_host_small(tr_ctx *ctx, const double *src, double *dst);
+
+int tr_compute_lhs(tr_ctx *ctx, const double *f, double *Af);      /* :160-172 */
+int tr_compute_lhs_diag(tr_ctx *ctx, const double *f, double *Af); /* :174-186 */
+int tr_compute_rhs(tr_ctx *ctx, double *b);                        /* :151-158 */
+int tr_scale(tr_ctx *ctx, double *x, double wx);                   /* :188-196 */
+int tr_add_prod(tr_ctx *ctx, double *x, const double *y, double wy);          /* :198-206 */
+int tr_mul_add(tr_ctx *ctx, double *x, double wx, const double *y);           /* :208-216 */
+int tr_mul_add_prod(tr_ctx *ctx, double *x, double wx, const double *y, double wy); /* :218-225 */
+/* norm/dot (:227-250) over the slm real lanes; NaN on failure */
+double tr_norm(tr_ctx *ctx, const double *x);
+double tr_dot(tr_ctx *ctx, const double *x, const double *y);
+int tr_synchronize(tr_ctx *ctx);
+
+const char *tr_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Solvers (extensions of main.cpp:125-162 / 223-248)                        */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    TR_SOLVER_REFERENCE = 0, /* main.cpp's PCG + invprec, host-driven, FP64 */
+    TR_SOLVER_PCG_FP64 = 1   /* same algorithm, device-resident scalars     */
+};
+
+typedef struct tr_solve_params {
+    int solver;          /* TR_SOLVER_* */
+    int use_prec;        /* 1: invprec inner CG on the Diag operator */
+    int maxit;           /* outer cap (main.cpp:142: 1000) */
+    double tol_abs;      /* stop when ||r|| < tol_abs (main.cpp:149: 1e-4) */
+    double tol_rel;      /* and/or ||r|| <= tol_rel*||b|| (0 = off) */
+    int inner_maxit;     /* main.cpp:232: 1000 */
+    double inner_tol;    /* main.cpp:239: 1e-8 absolute */
+    int record;          /* leading iterations whose (alpha, beta, ||r||) to record */
+} tr_solve_params;
+
+typedef struct tr_solve_stats {
+    int iters;           /* outer iterations (main.cpp's k at exit) */
+    int preck;           /* total inner iterations */
+    double rnorm;        /* final recursive ||r|| */
+    double bnorm;        /* ||b|| */
+    double true_rel;     /* ||b - A f|| / ||b|| with the FP64 operator */
+    double seconds;      /* wall time of the solve, device-synchronised */
+    int status;          /* 0 converged, TR_ERR_NOCONV otherwise */
+} tr_solve_stats;
+
+/* Solves A f = b, b = computeRhs, from f = 0.  f is a device vector.
+ * hist: 3*record doubles (alpha, beta, ||r||) or NULL. */
+int tr_solve(tr_ctx *ctx, const tr_solve_params *prm, double *f,
+             tr_solve_stats *st, double *hist);
+
+/* Seconds per operator application (median over reps), for benchmarking. */
+double tr_bench_lhs(tr_ctx *ctx, int reps);
+/* Times `steps` operator applications after `warmup` untimed ones, with HIP
+ * events on the context's stream.  out[0] wall seconds of the timed loop
+ * (device-synchronised on both sides), out[1] mean volume-kernel ms,
+ * out[2] mean boundary-kernel ms, out[3] mean ms per application. */
+int tr_bench_operator(tr_ctx *ctx, int warmup, int steps, double *out4);
+
+/* ------------------------------------------------------------------------ */
+/* Introspection of the assembled operator data (host only, no device work). */
+/* No reference counterpart: the reference rebuilds the couplings inside     */
+/* kernels.cu on every application.  Tests use it to check the assembly.    */
+/* ------------------------------------------------------------------------ */
+typedef struct tr_stencil tr_stencil;
+int tr_stencil_create(const tr_mesh_view *mesh, const tr_angular_view *ang,
+                      tr_stencil **out);
+int tr_stencil_destroy(tr_stencil *s);
+/* info: nP, slm, NC, W, nB, WB, arow, brow */
+int tr_stencil_info(const tr_stencil *s, uint32_t *info8);
+/* Any pointer may be NULL.  perm[nP]; nbr[W][nP]; coef[W][NC][nP];
+ * bvert[nB]; bnbr[WB][nB]; bcoef[WB][3][nB]; omc[NC-1][slm][slm];
+ * onc[3][slm][slm]; massfac[slm]; rhs[nP] (all in the internal numbering) */
+int tr_stencil_export(const tr_stencil *s, uint32_t *perm, uint32_t *nbr,
+                      double *coef, uint32_t *bvert, uint32_t *bnbr,
+                      double *bcoef, double *omc, double *onc, double *massfac,
+                      double *rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,135 @@
+"""Pins the oracle against the reference's own host code (oracle/_ref, built
+from the reference sources where present), closed forms and independent
+numpy/scipy evaluations."""
+import numpy as np
+import pytest
+from scipy import integrate
+from scipy.special import expn
+
+import oracle
+from angular_np import couplings, harmonics, reference_storage
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 5])
+def test_angular_matches_reference_angulardata(K):
+    ref = oracle.ref_angular(K)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    o = oracle.angular(K)
+    assert np.array_equal(o["omega_pos"], ref["omega_pos"])
+    for k in ("omega", "Ox", "Oy", "Oz"):
+        np.testing.assert_allclose(o[k], ref[k], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("K", [1, 3, 5])
+def test_angular_matches_independent_quadrature(K):
+    E, O, _ = couplings(K)
+    om, pos = reference_storage(E)
+    o = oracle.angular(K)
+    assert np.array_equal(o["omega_pos"], pos.ravel())
+    np.testing.assert_allclose(o["omega"], om.ravel(), atol=1e-13)
+    for n, k in enumerate(("Ox", "Oy", "Oz")):
+        # reference storage O[in][out]; O is symmetric
+        np.testing.assert_allclose(o[k].reshape(O[n].shape), O[n].T, atol=1e-13)
+
+
+def test_harmonics_orthonormal_and_convention():
+    G = oracle.sh_gram(11)
+    np.testing.assert_allclose(G, np.eye(len(G)), atol=1e-13)
+    # point values against scipy's Legendre functions without the CS phase
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        mine = oracle.sh_eval(6, *v)
+        mu, ph = np.array([v[2]]), np.array([np.arctan2(v[1], v[0])])
+        want = harmonics(range(7), mu, ph)[:, 0]
+        np.testing.assert_allclose(mine, want, atol=1e-12)
+
+
+@pytest.mark.parametrize("K", [1, 2, 5])
+def test_angular_identities(K):
+    o = oracle.angular(K)
+    s = o["slm"]
+    om, pos = o["omega"].reshape(3, s, s), o["omega_pos"].reshape(3, s, s)
+    full = np.zeros((3, 3, s, s))
+    for a in range(3):
+        for b in range(3):
+            full[a, b] = np.where(pos[a] == b, om[a], 0.0)
+    # sum_a <Y Y' Omega_a^2> = <Y Y'> = delta
+    np.testing.assert_allclose(full[0, 0] + full[1, 1] + full[2, 2], np.eye(s), atol=1e-13)
+    # <Omega_a Omega_b> = delta_ab / 3
+    np.testing.assert_allclose(full[:, :, 0, 0], np.eye(3) / 3, atol=1e-15)
+    for k in ("Ox", "Oy", "Oz"):
+        O = o[k].reshape(s, s)
+        assert abs(O[0, 0] - 0.5) < 1e-15  # <|Omega_n|> = 1/2
+        np.testing.assert_allclose(O, O.T, atol=1e-15)
+        assert np.linalg.eigvalsh(O).min() > 0
+    # the 3s x 3s matrix [<Y Omega_a Omega_b Y'>] is a Gram matrix: PSD
+    big = full.transpose(0, 2, 1, 3).reshape(3 * s, 3 * s)
+    assert np.linalg.eigvalsh(0.5 * (big + big.T)).min() > -1e-13
+    # exactly one b per (a, lm, lm') carries the coupling (the omega_pos rule)
+    nz = (np.abs(full) > 1e-12).sum(axis=1)
+    assert nz.max() <= 1
+
+
+def test_isotropic_scattering_reduces_to_completeness():
+    # with isotropic scattering the odd-harmonic weights are all 1, so the
+    # effective couplings equal <Y Omega_a Omega_b Y'> by completeness
+    plain = oracle.angular(3)
+    iso = oracle.angular(3, albedo=0.5, g=0.0, hg_lmax=0)
+    np.testing.assert_allclose(iso["omega"], plain["omega"], atol=1e-13)
+    assert np.array_equal(iso["omega_pos"], plain["omega_pos"])
+    assert iso["massfac"][0] == 0.5 and np.all(iso["massfac"][1:] == 1.0)
+
+
+def test_henyey_greenstein_couplings_match_independent_quadrature():
+    K, albedo, g, L = 2, 0.9, 0.6, 7
+    E, O, mf = couplings(K, albedo, g, L)
+    om, pos = reference_storage(E)
+    o = oracle.angular(K, albedo, g, L)
+    np.testing.assert_allclose(o["massfac"], mf, atol=1e-15)
+    assert np.array_equal(o["omega_pos"], pos.ravel())
+    np.testing.assert_allclose(o["omega"], om.ravel(), atol=1e-12)
+    # the anisotropic couplings are not symmetric in a <-> b
+    assert np.abs(E[0, 1] - E[1, 0]).max() > 1e-2
+    np.testing.assert_allclose(E[0, 1], E[1, 0].T, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
+def test_expn_matches_scipy(n):
+    for x in np.concatenate([np.logspace(-4, 0, 13), np.linspace(1.01, 40, 23)]):
+        np.testing.assert_allclose(oracle.expn(n, x), expn(n, x), rtol=2e-13)
+
+
+def test_reference_xE1_agrees_below_one_and_drops_exp_above():
+    r = oracle.ref()
+    if r is None:
+        pytest.skip("oracle/_ref not built")
+    for x in (0.01, 0.3, 0.99):
+        assert abs(r.ref_xE1(x) - x * expn(1, x)) < 1e-12
+    # E1.cpp:148-157 returns x e^x E1(x) for x > 1 (missing factor e^-x)
+    for x in (1.5, 3.0, 8.0):
+        assert abs(r.ref_xE1(x) - x * np.exp(x) * expn(1, x)) < 1e-10
+
+
+def test_step_and_slab_analytic():
+    # main.cpp:498-515 forms restated with E_n
+    for x in (0.1, 0.7, 2.5):
+        assert abs(oracle.step_U(x) - 2 * np.pi * expn(2, x)) < 1e-13
+        assert abs(oracle.step_U(-x) - (4 * np.pi - 2 * np.pi * expn(2, x))) < 1e-13
+        assert abs(oracle.step_W(x) - 2 * np.pi * expn(3, x)) < 1e-13
+        assert abs(oracle.step_T(x) - 2 * np.pi * expn(4, x)) < 1e-13
+    # finite slab against direct integration of the 1-D transport solution
+    L = 2.0
+
+    def U(x):
+        def I(mu, sgn):  # intensity along +-mu from the source on [-L, 0]
+            if sgn > 0:
+                a, b = -L, min(x, 0.0)
+                return 0.0 if b <= a else np.exp(-(x - b) / mu) - np.exp(-(x - a) / mu)
+            a, b = max(x, -L), 0.0
+            return 0.0 if b <= a else np.exp(-(a - x) / mu) - np.exp(-(b - x) / mu)
+        return 2 * np.pi * sum(integrate.quad(lambda m: I(m, s), 0, 1, epsabs=1e-13)[0] for s in (1, -1))
+    for x in (-1.5, -0.3, 0.4, 1.7):
+        assert abs(oracle.slab_U(x, L) - U(x)) < 1e-9

@@ -1,0 +1,435 @@
+"""ctypes bindings of libtransfer_amd.so (include/transfer.h) and a Python
+mirror of the reference's CudaContext interface (CudaContext.h:29-47).
+
+The library is required: importing this module raises if it is missing, and
+there is no Python or CPU fallback for anything that runs on the device.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtransfer_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("transfer_amd: %s is missing; run `python transfer_amd/build.py`" % LIB_PATH)
+lib = C.CDLL(LIB_PATH)
+
+u32p = C.POINTER(C.c_uint32)
+f64p = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+vp = C.c_void_p
+
+TR_SOLVER_REFERENCE = 0
+TR_SOLVER_PCG_FP64 = 1
+
+FIELD = dict(const=0, radial_exp=1, uniform=2, lognormal=3, blobs=4, planck=5, step=6)
+
+TET_DTYPE = np.dtype([("p", "<u4", 4), ("kappa_volume", "<f8"), ("I_p", "<f8"), ("s", "<f8", (4, 3))])
+FACE_DTYPE = np.dtype([("p", "<u4", 4), ("s", "<f8", 3), ("_padd", "<f8", 3)])
+assert TET_DTYPE.itemsize == 128 and FACE_DTYPE.itemsize == 64
+
+
+class MeshSpec(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("jitter", C.c_double), ("ball_radius", C.c_double),
+                ("ball_center", C.c_double * 3), ("seed", C.c_uint64)]
+
+
+class MeshView(C.Structure):
+    _fields_ = [("nP", C.c_uint32), ("nF", C.c_uint32), ("nT", C.c_uint32),
+                ("tetstart", u32p), ("tetidx", u32p), ("tetpos", u32p), ("mesh", vp),
+                ("facestart", u32p), ("faceidx", u32p), ("facepos", u32p), ("bnd", vp),
+                ("coords", f64p), ("face_kind", u8p)]
+
+
+class AngularView(C.Structure):
+    _fields_ = [("slm", C.c_uint32), ("omega", f64p), ("omega_pos", u32p),
+                ("Ox", f64p), ("Oy", f64p), ("Oz", f64p), ("massfac", f64p)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("solver", C.c_int), ("use_prec", C.c_int), ("maxit", C.c_int),
+                ("tol_abs", C.c_double), ("tol_rel", C.c_double),
+                ("inner_maxit", C.c_int), ("inner_tol", C.c_double), ("record", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("preck", C.c_int), ("rnorm", C.c_double),
+                ("bnorm", C.c_double), ("true_rel", C.c_double), ("seconds", C.c_double),
+                ("status", C.c_int)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("tr_last_error", C.c_char_p)
+_sig("tr_angular_create", C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(vp))
+_sig("tr_angular_destroy", C.c_int, vp)
+_sig("tr_angular_get_view", C.c_int, vp, C.POINTER(AngularView))
+_sig("tr_mesh_create_box", C.c_int, C.POINTER(MeshSpec), C.POINTER(vp))
+_sig("tr_mesh_create", C.c_int, C.c_uint32, f64p, C.c_uint32, u32p, f64p, f64p, C.POINTER(vp))
+_sig("tr_mesh_destroy", C.c_int, vp)
+_sig("tr_mesh_get_view", C.c_int, vp, C.POINTER(MeshView))
+_sig("tr_mesh_get_geometry", C.c_int, vp, f64p, u32p, f64p, f64p)
+_sig("tr_mesh_set_fields", C.c_int, vp, f64p, f64p)
+_sig("tr_mesh_set_face_kind_plane", C.c_int, vp, C.c_int, C.c_double, C.c_double, C.c_int)
+_sig("tr_field_eval", C.c_int, vp, C.c_int, f64p, C.c_uint64, C.c_uint64, f64p)
+_sig("tr_rng_uniform", C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, f64p)
+_sig("tr_config_load", C.c_int, C.c_char_p, C.POINTER(vp))
+_sig("tr_config_destroy", C.c_int, vp)
+_sig("tr_config_ints", C.c_int, vp, C.POINTER(C.c_int))
+_sig("tr_config_region", C.c_int, vp, C.c_int, f64p, f64p)
+_sig("tr_config_strings", C.c_int, vp, C.c_char_p, C.c_char_p, C.c_size_t)
+_sig("tr_ctx_create", C.c_int, C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), C.c_int, C.POINTER(vp))
+_sig("tr_ctx_destroy", C.c_int, vp)
+_sig("tr_ctx_N", C.c_uint32, vp)
+_sig("tr_ctx_slm", C.c_uint32, vp)
+_sig("tr_ctx_aslm", C.c_uint32, vp)
+_sig("tr_ctx_nP", C.c_uint32, vp)
+_sig("tr_ctx_permutation", C.c_int, vp, u32p)
+_sig("tr_device_alloc", vp, vp, C.c_size_t)
+_sig("tr_alloc_vector", vp, vp)
+_sig("tr_device_free", C.c_int, vp, vp)
+_sig("tr_copy_to_dev", C.c_int, vp, vp, vp, C.c_size_t)
+_sig("tr_copy_to_host", C.c_int, vp, vp, vp, C.c_size_t)
+_sig("tr_get_host_small", C.c_int, vp, vp, f64p)
+for _n in ("tr_compute_lhs", "tr_compute_lhs_diag"):
+    _sig(_n, C.c_int, vp, vp, vp)
+_sig("tr_compute_rhs", C.c_int, vp, vp)
+_sig("tr_scale", C.c_int, vp, vp, C.c_double)
+_sig("tr_add_prod", C.c_int, vp, vp, vp, C.c_double)
+_sig("tr_mul_add", C.c_int, vp, vp, C.c_double, vp)
+_sig("tr_mul_add_prod", C.c_int, vp, vp, C.c_double, vp, C.c_double)
+_sig("tr_norm", C.c_double, vp, vp)
+_sig("tr_dot", C.c_double, vp, vp, vp)
+_sig("tr_synchronize", C.c_int, vp)
+_sig("tr_solve", C.c_int, vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats), f64p)
+_sig("tr_bench_lhs", C.c_double, vp, C.c_int)
+_sig("tr_bench_operator", C.c_int, vp, C.c_int, C.c_int, f64p)
+_sig("tr_stencil_create", C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), C.POINTER(vp))
+_sig("tr_stencil_destroy", C.c_int, vp)
+_sig("tr_stencil_info", C.c_int, vp, u32p)
+_sig("tr_stencil_export", C.c_int, vp, u32p, u32p, f64p, u32p, u32p, f64p, f64p, f64p, f64p, f64p)
+
+
+class TransferError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise TransferError("transfer_amd error %d: %s" % (rc, lib.tr_last_error().decode()))
+    return rc
+
+
+def ptr(a, t=f64p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def last_error():
+    return lib.tr_last_error().decode()
+
+
+# ---------------------------------------------------------------------------
+class Angular:
+    """AngularData(maxk) (AngularData.cpp:9-68) plus the scattering extension."""
+
+    def __init__(self, maxk, albedo=0.0, g=0.0, hg_lmax=0):
+        self._h = vp()
+        check(lib.tr_angular_create(int(maxk), float(albedo), float(g), int(hg_lmax), C.byref(self._h)))
+        self.maxk = maxk
+        self.view = AngularView()
+        check(lib.tr_angular_get_view(self._h, C.byref(self.view)))
+        s = self.slm = int(self.view.slm)
+        self.omega = np.ctypeslib.as_array(self.view.omega, (3 * s * s,)).copy()
+        self.omega_pos = np.ctypeslib.as_array(self.view.omega_pos, (3 * s * s,)).copy()
+        self.Ox = np.ctypeslib.as_array(self.view.Ox, (s * s,)).copy()
+        self.Oy = np.ctypeslib.as_array(self.view.Oy, (s * s,)).copy()
+        self.Oz = np.ctypeslib.as_array(self.view.Oz, (s * s,)).copy()
+        self.massfac = np.ctypeslib.as_array(self.view.massfac, (s,)).copy()
+
+    @staticmethod
+    def view_of(omega, omega_pos, Ox, Oy, Oz, massfac=None):
+        """An AngularView over caller-held arrays (e.g. the reference's own)."""
+        v = AngularView()
+        v.slm = int(round(np.sqrt(len(Ox))))
+        v.omega, v.omega_pos = ptr(omega), ptr(omega_pos, u32p)
+        v.Ox, v.Oy, v.Oz = ptr(Ox), ptr(Oy), ptr(Oz)
+        v.massfac = ptr(massfac)
+        v._keep = (omega, omega_pos, Ox, Oy, Oz, massfac)
+        return v
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.tr_angular_destroy(self._h)
+            self._h = None
+
+
+class Mesh:
+    """MeshData (MeshData.h:13-32) built from a generator or raw geometry."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.view = MeshView()
+        check(lib.tr_mesh_get_view(self._h, C.byref(self.view)))
+        self.nP, self.nT, self.nF = int(self.view.nP), int(self.view.nT), int(self.view.nF)
+
+    @classmethod
+    def box(cls, n, lo, hi, jitter=0.0, ball_radius=0.0, ball_center=(0, 0, 0), seed=0):
+        sp = MeshSpec()
+        sp.nx, sp.ny, sp.nz = [int(x) for x in n]
+        sp.lo[:] = list(map(float, lo))
+        sp.hi[:] = list(map(float, hi))
+        sp.jitter, sp.ball_radius = float(jitter), float(ball_radius)
+        sp.ball_center[:] = list(map(float, ball_center))
+        sp.seed = int(seed)
+        h = vp()
+        check(lib.tr_mesh_create_box(C.byref(sp), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, coords, tets, kappa=None, Ip=None):
+        coords = np.ascontiguousarray(coords, np.float64)
+        tets = np.ascontiguousarray(tets, np.uint32)
+        kappa = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        Ip = None if Ip is None else np.ascontiguousarray(Ip, np.float64)
+        h = vp()
+        check(lib.tr_mesh_create(len(coords), ptr(coords), len(tets), ptr(tets, u32p),
+                                 ptr(kappa), ptr(Ip), C.byref(h)))
+        return cls(h)
+
+    def refresh_view(self):
+        check(lib.tr_mesh_get_view(self._h, C.byref(self.view)))
+
+    def set_fields(self, kappa=None, Ip=None):
+        k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        i = None if Ip is None else np.ascontiguousarray(Ip, np.float64)
+        check(lib.tr_mesh_set_fields(self._h, ptr(k), ptr(i)))
+        self.refresh_view()
+
+    def set_face_kind_plane(self, axis, value, kind, tol=1e-9):
+        check(lib.tr_mesh_set_face_kind_plane(self._h, axis, value, tol, kind))
+
+    def field(self, kind, params, seed=0, stream=0):
+        p = np.zeros(8)
+        p[:len(params)] = params
+        out = np.empty(self.nT)
+        check(lib.tr_field_eval(self._h, FIELD[kind], ptr(p), int(seed), int(stream), ptr(out)))
+        return out
+
+    def geometry(self):
+        coords = np.empty((self.nP, 3))
+        tets = np.empty((self.nT, 4), np.uint32)
+        cen = np.empty((self.nT, 3))
+        vol = np.empty(self.nT)
+        check(lib.tr_mesh_get_geometry(self._h, ptr(coords), ptr(tets, u32p), ptr(cen), ptr(vol)))
+        return coords, tets, cen, vol
+
+    def arrays(self):
+        """Copies of the MeshData arrays (the reference's layouts)."""
+        v = self.view
+        nP, nT, nF = self.nP, self.nT, self.nF
+        tc = int(np.ctypeslib.as_array(v.tetstart, (nP + 1,))[-1])
+        fc = int(np.ctypeslib.as_array(v.facestart, (nP + 1,))[-1])
+        a = lambda p_, n: np.ctypeslib.as_array(p_, (n,)).copy()
+        mesh = np.frombuffer((C.c_char * (128 * nT)).from_address(v.mesh), TET_DTYPE).copy()
+        bnd = np.frombuffer((C.c_char * (64 * max(nF, 1))).from_address(v.bnd), FACE_DTYPE)[:nF].copy()
+        out = dict(tetstart=a(v.tetstart, nP + 1), tetidx=a(v.tetidx, tc), tetpos=a(v.tetpos, tc),
+                   mesh=mesh, facestart=a(v.facestart, nP + 1), faceidx=a(v.faceidx, fc),
+                   facepos=a(v.facepos, fc), bnd=bnd, coords=a(v.coords, 3 * nP).reshape(nP, 3),
+                   face_kind=a(v.face_kind, nF) if nF else np.zeros(0, np.uint8))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.tr_mesh_destroy(self._h)
+            self._h = None
+
+
+def rng_uniform(seed, stream, n):
+    out = np.empty(n)
+    check(lib.tr_rng_uniform(int(seed), int(stream), int(n), ptr(out)))
+    return out
+
+
+class Config:
+    """Config(cfgfile) (Config.cpp:8-131)."""
+
+    def __init__(self, path):
+        self._h = vp()
+        check(lib.tr_config_load(path.encode(), C.byref(self._h)))
+        v = (C.c_int * 6)()
+        check(lib.tr_config_ints(self._h, v))
+        self.maxk, self.device, self.dump, self.prec, self.soltype, self.nreg = list(v)
+        self.kappa, self.Ip = {}, {}
+        for r in range(0, self.nreg + 1):
+            k, i = C.c_double(), C.c_double()
+            check(lib.tr_config_region(self._h, r, C.byref(k), C.byref(i)))
+            self.kappa[r], self.Ip[r] = k.value, i.value
+        m, o = C.create_string_buffer(1024), C.create_string_buffer(1024)
+        check(lib.tr_config_strings(self._h, m, o, 1024))
+        self.meshfile, self.outfile = m.value.decode(), o.value.decode()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.tr_config_destroy(self._h)
+            self._h = None
+
+
+class Stencil:
+    """The assembled operator data (introspection for tests)."""
+
+    def __init__(self, mesh_view, ang_view):
+        self._h = vp()
+        check(lib.tr_stencil_create(C.byref(mesh_view), C.byref(ang_view), C.byref(self._h)))
+        info = np.zeros(8, np.uint32)
+        check(lib.tr_stencil_info(self._h, ptr(info, u32p)))
+        nP, slm, NC, W, nB, WB, arow, brow = [int(x) for x in info]
+        self.nP, self.slm, self.NC, self.W, self.nB, self.WB = nP, slm, NC, W, nB, WB
+        self.perm = np.empty(nP, np.uint32)
+        self.nbr = np.empty((W, nP), np.uint32)
+        self.coef = np.empty((W, NC, nP))
+        self.bvert = np.empty(nB, np.uint32)
+        self.bnbr = np.empty((WB, nB), np.uint32)
+        self.bcoef = np.empty((WB, 3, nB))
+        self.omc = np.empty((NC - 1, slm, slm))
+        self.onc = np.empty((3, slm, slm))
+        self.massfac = np.empty(slm)
+        self.rhs = np.empty(nP)
+        check(lib.tr_stencil_export(self._h, ptr(self.perm, u32p), ptr(self.nbr, u32p), ptr(self.coef),
+                                    ptr(self.bvert, u32p), ptr(self.bnbr, u32p), ptr(self.bcoef),
+                                    ptr(self.omc), ptr(self.onc), ptr(self.massfac), ptr(self.rhs)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.tr_stencil_destroy(self._h)
+            self._h = None
+
+
+# ---------------------------------------------------------------------------
+class HipContext:
+    """Mirror of CudaContext (CudaContext.h:29-47) over the C ABI.
+
+    Device vectors are opaque handles (int addresses).  Host arrays use the
+    reference layout [nP][aslm] in the caller's vertex order."""
+
+    def __init__(self, mesh, ang, device=0, mesh_view=None, ang_view=None):
+        self._h = vp()
+        mv = mesh_view if mesh_view is not None else mesh.view
+        av = ang_view if ang_view is not None else ang.view
+        check(lib.tr_ctx_create(int(device), C.byref(mv), C.byref(av), 0, C.byref(self._h)))
+        self._keep = (mesh, ang, mv, av)
+        self.nP = int(lib.tr_ctx_nP(self._h))
+        self.slm = int(lib.tr_ctx_slm(self._h))
+        self.aslm = int(lib.tr_ctx_aslm(self._h))
+
+    def N(self):
+        return int(lib.tr_ctx_N(self._h))
+
+    def permutation(self):
+        p = np.empty(self.nP, np.uint32)
+        check(lib.tr_ctx_permutation(self._h, ptr(p, u32p)))
+        return p
+
+    def deviceAlloc(self, nbytes):
+        p = lib.tr_device_alloc(self._h, nbytes)
+        if not p:
+            raise TransferError(last_error())
+        return p
+
+    def allocVector(self):
+        p = lib.tr_alloc_vector(self._h)
+        if not p:
+            raise TransferError(last_error())
+        return p
+
+    def deviceFree(self, p):
+        check(lib.tr_device_free(self._h, p))
+
+    def copyToDev(self, dst, host):
+        host = np.ascontiguousarray(host)
+        check(lib.tr_copy_to_dev(self._h, dst, host.ctypes.data, host.nbytes))
+
+    def copyToHost(self, src, nbytes=None, dtype=np.float64):
+        nbytes = nbytes if nbytes is not None else 8 * self.N()
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+        check(lib.tr_copy_to_host(self._h, out.ctypes.data, src, nbytes))
+        return out
+
+    def vector_from(self, host):
+        v = self.allocVector()
+        self.copyToDev(v, np.ascontiguousarray(host, np.float64).reshape(-1))
+        return v
+
+    def getHostSmall(self, src):
+        out = np.empty(self.nP * self.slm)
+        check(lib.tr_get_host_small(self._h, src, ptr(out)))
+        return out
+
+    def computeLhs(self, f, Af):
+        check(lib.tr_compute_lhs(self._h, f, Af))
+
+    def computeLhsDiag(self, f, Af):
+        check(lib.tr_compute_lhs_diag(self._h, f, Af))
+
+    def computeRhs(self, b):
+        check(lib.tr_compute_rhs(self._h, b))
+
+    def scale(self, x, wx):
+        check(lib.tr_scale(self._h, x, wx))
+
+    def addProd(self, x, y, wy):
+        check(lib.tr_add_prod(self._h, x, y, wy))
+
+    def mulAdd(self, x, wx, y):
+        check(lib.tr_mul_add(self._h, x, wx, y))
+
+    def mulAddProd(self, x, wx, y, wy):
+        check(lib.tr_mul_add_prod(self._h, x, wx, y, wy))
+
+    def norm(self, x):
+        v = lib.tr_norm(self._h, x)
+        if v != v:
+            raise TransferError(last_error())
+        return v
+
+    def dot(self, x, y):
+        v = lib.tr_dot(self._h, x, y)
+        if v != v:
+            raise TransferError(last_error())
+        return v
+
+    def synchronize(self):
+        check(lib.tr_synchronize(self._h))
+
+    def solve(self, f, solver=TR_SOLVER_REFERENCE, use_prec=False, maxit=1000, tol_abs=1e-4,
+              tol_rel=0.0, inner_maxit=1000, inner_tol=1e-8, record=0):
+        prm = SolveParams(solver, int(use_prec), maxit, tol_abs, tol_rel, inner_maxit, inner_tol, record)
+        st = SolveStats()
+        hist = np.zeros(3 * max(record, 1))
+        check(lib.tr_solve(self._h, C.byref(prm), f, C.byref(st), ptr(hist)))
+        return st, hist[:3 * record].reshape(-1, 3)
+
+    def bench_lhs(self, reps=20):
+        return lib.tr_bench_lhs(self._h, reps)
+
+    def bench_operator(self, warmup, steps):
+        """(wall s, volume-kernel ms, boundary-kernel ms, ms per application)"""
+        out = np.zeros(4)
+        check(lib.tr_bench_operator(self._h, warmup, steps, ptr(out)))
+        return tuple(out)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.tr_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

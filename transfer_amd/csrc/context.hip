@@ -1,0 +1,563 @@
+// context.hip -- tr_ctx: the device context behind the C ABI, replacing
+// CudaContext (CudaContext.h:9-50, CudaContext.cpp:76-250).
+//
+// Differences from the reference that a caller can observe are none by
+// design: vectors keep the reference's [vertex][aslm] layout at the copy
+// boundary, norm/dot exclude the padding lanes, deviceAlloc zero-fills, and
+// every call is ordered on one stream.  Internally vectors live renumbered
+// and unpadded ([vertex][slm]) and the copy calls convert.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <set>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+#include "stencil.h"
+
+#define HIPCHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            return tr::fail(TR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct tr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    tr::HostOperator op;
+    tr::LaneMap map, bmap;
+    // device operator data (FP64)
+    uint32_t *d_nbr = nullptr, *d_bvert = nullptr, *d_bnbr = nullptr;
+    uint32_t *d_perm = nullptr, *d_iperm = nullptr;
+    uint16_t *d_aidx = nullptr, *d_bidx = nullptr;
+    double *d_coef = nullptr, *d_bcoef = nullptr, *d_aval = nullptr, *d_bval = nullptr;
+    double *d_massfac = nullptr, *d_ddiag = nullptr, *d_bdiag = nullptr, *d_rhs = nullptr;
+    double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
+    std::set<void *> vectors, raw;
+    std::vector<void *> owned;
+
+    tr::VolArgs<double> vol() const {
+        tr::VolArgs<double> a;
+        a.nP = op.nP;
+        a.slm = op.slm;
+        a.W = op.W;
+        a.arow = op.arow;
+        a.L = map.L;
+        a.VPW = map.VPW;
+        a.nbr = d_nbr;
+        a.coef = d_coef;
+        a.aidx = d_aidx;
+        a.aval = d_aval;
+        a.massfac = d_massfac;
+        a.ddiag = d_ddiag;
+        return a;
+    }
+    tr::BndArgs<double> bnd() const {
+        tr::BndArgs<double> a;
+        a.nB = op.nB;
+        a.slm = op.slm;
+        a.WB = op.WB;
+        a.brow = op.brow;
+        a.L = bmap.L;
+        a.VPW = bmap.VPW;
+        a.bvert = d_bvert;
+        a.bnbr = d_bnbr;
+        a.bcoef = d_bcoef;
+        a.bidx = d_bidx;
+        a.bval = d_bval;
+        a.bdiag = d_bdiag;
+        return a;
+    }
+    size_t nint() const { return (size_t)op.nP * op.slm; }
+    size_t next() const { return (size_t)op.nP * op.aslm; }
+};
+
+namespace tr {
+
+// Picks the lane mapping: J must divide slm with slm/J <= 64 lanes; prefer
+// the best lane utilisation, then the smaller J (fewer registers).
+LaneMap choose_map(uint32_t slm, int maxJ) {
+    static const int Js[] = {1, 2, 3, 4, 5, 6, 7, 11};
+    LaneMap best;
+    double bu = -1;
+    for (int J : Js) {
+        if (J > maxJ || slm % J) continue;
+        int L = (int)(slm / J);
+        if (L > 64) continue;
+        int VPW = 64 / L;
+        double u = (double)VPW * L / 64.0;
+        if (u > bu + 1e-9) {
+            bu = u;
+            best.J = J;
+            best.L = L;
+            best.VPW = VPW;
+        }
+    }
+    return best;
+}
+
+template <typename T>
+static int upload(tr_ctx *c, T **dst, const std::vector<T> &src) {
+    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    HIPCHECK(hipMalloc((void **)dst, bytes));
+    c->owned.push_back(*dst);
+    if (!src.empty())
+        HIPCHECK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return TR_OK;
+}
+
+static int lhs(tr_ctx *c, const double *f, double *r) {
+    HIPCHECK(launch_lhs_volume<double>(c->vol(), (int)c->op.NC, c->map, f, r, c->stream));
+    HIPCHECK(launch_lhs_boundary<double>(c->bnd(), c->bmap, f, r, c->stream));
+    return TR_OK;
+}
+
+static int lhs_diag(tr_ctx *c, const double *f, double *r) {
+    HIPCHECK(launch_diag_volume<double>(c->vol(), (int)c->op.NC, c->map, f, r, c->stream));
+    HIPCHECK(launch_diag_boundary<double>(c->bnd(), c->bmap, f, r, c->stream));
+    return TR_OK;
+}
+
+static double dot(tr_ctx *c, const double *x, const double *y) {
+    hipError_t e = launch_dot(x, y, c->nint(), c->d_part, c->d_red, c->stream);
+    double h = NAN;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        fail(TR_ERR_HIP, std::string("dot: ") + hipGetErrorString(e));
+        return NAN;
+    }
+    return h;
+}
+
+static int axpby(tr_ctx *c, double *x, double wx, const double *y, double wy) {
+    HIPCHECK(launch_axpby(x, wx, y, wy, c->nint(), c->stream));
+    return TR_OK;
+}
+
+}  // namespace tr
+
+using namespace tr;
+
+extern "C" {
+
+int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *ang, int flags,
+                  tr_ctx **out) {
+    (void)flags;
+    if (!mesh || !ang || !out) return fail(TR_ERR_ARG, "tr_ctx_create: null argument");
+    *out = nullptr;
+    tr_ctx *c = new tr_ctx();
+    c->device = device;
+    int rc = assemble_operator(*mesh, *ang, c->op);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
+    }
+    c->map = choose_map(c->op.slm, 4);
+    c->bmap = c->map;
+    int maxlds = 65536;
+    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    while (c->map.WPB > 1 && lhs_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds)
+        c->map.WPB /= 2;
+    if (lhs_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds) {
+        tr_ctx_destroy(c);
+        return fail(TR_ERR_ARG, "tr_ctx_create: angular order too large for LDS staging");
+    }
+    const HostOperator &op = c->op;
+    std::vector<double> stage;
+    if ((rc = upload(c, &c->d_nbr, op.nbr)) || (rc = upload(c, &c->d_coef, op.coef)) ||
+        (rc = upload(c, &c->d_bvert, op.bvert)) || (rc = upload(c, &c->d_bnbr, op.bnbr)) ||
+        (rc = upload(c, &c->d_bcoef, op.bcoef)) || (rc = upload(c, &c->d_aidx, op.aidx)) ||
+        (rc = upload(c, &c->d_aval, op.aval)) || (rc = upload(c, &c->d_bidx, op.bidx)) ||
+        (rc = upload(c, &c->d_bval, op.bval)) || (rc = upload(c, &c->d_massfac, op.massfac)) ||
+        (rc = upload(c, &c->d_ddiag, op.ddiag)) || (rc = upload(c, &c->d_bdiag, op.bdiag)) ||
+        (rc = upload(c, &c->d_rhs, op.rhs)) || (rc = upload(c, &c->d_perm, op.perm)) ||
+        (rc = upload(c, &c->d_iperm, op.iperm))) {
+        tr_ctx_destroy(c);
+        return rc;
+    }
+    e = hipMalloc((void **)&c->d_part, sizeof(double) * (red_partials() + 8));
+    if (e == hipSuccess) {
+        c->owned.push_back(c->d_part);
+        c->d_red = c->d_part + red_partials();
+        e = hipMalloc((void **)&c->d_stage, sizeof(double) * c->next());
+    }
+    if (e == hipSuccess) {
+        c->owned.push_back(c->d_stage);
+        e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) {
+        tr_ctx_destroy(c);
+        return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return TR_OK;
+}
+
+int tr_ctx_destroy(tr_ctx *c) {
+    if (!c) return TR_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : c->owned) hipFree(p);
+    for (void *p : c->vectors) hipFree(p);
+    for (void *p : c->raw) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return TR_OK;
+}
+
+uint32_t tr_ctx_N(const tr_ctx *c) { return c ? c->op.nP * c->op.aslm : 0; }
+uint32_t tr_ctx_slm(const tr_ctx *c) { return c ? c->op.slm : 0; }
+uint32_t tr_ctx_aslm(const tr_ctx *c) { return c ? c->op.aslm : 0; }
+uint32_t tr_ctx_nP(const tr_ctx *c) { return c ? c->op.nP : 0; }
+
+int tr_ctx_permutation(const tr_ctx *c, uint32_t *perm) {
+    if (!c || !perm) return fail(TR_ERR_ARG, "tr_ctx_permutation: null argument");
+    std::memcpy(perm, c->op.perm.data(), sizeof(uint32_t) * c->op.nP);
+    return TR_OK;
+}
+
+void *tr_device_alloc(tr_ctx *c, size_t bytes) {
+    if (!c) return nullptr;
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) {
+        fail(TR_ERR_HIP, "tr_device_alloc: hipMalloc failed");
+        return nullptr;
+    }
+    if (hipMemsetAsync(p, 0, bytes, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        hipFree(p);
+        fail(TR_ERR_HIP, "tr_device_alloc: memset failed");
+        return nullptr;
+    }
+    c->raw.insert(p);
+    return p;
+}
+
+double *tr_alloc_vector(tr_ctx *c) {
+    if (!c) return nullptr;
+    void *p = nullptr;
+    size_t bytes = sizeof(double) * c->nint();
+    if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) {
+        fail(TR_ERR_HIP, "tr_alloc_vector: hipMalloc failed");
+        return nullptr;
+    }
+    if (hipMemsetAsync(p, 0, bytes, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        hipFree(p);
+        fail(TR_ERR_HIP, "tr_alloc_vector: memset failed");
+        return nullptr;
+    }
+    c->vectors.insert(p);
+    return (double *)p;
+}
+
+int tr_device_free(tr_ctx *c, void *mem) {
+    if (!c) return fail(TR_ERR_ARG, "tr_device_free: null context");
+    if (!mem) return TR_OK;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->vectors.erase(mem) || c->raw.erase(mem)) {
+        HIPCHECK(hipFree(mem));
+        return TR_OK;
+    }
+    return fail(TR_ERR_ARG, "tr_device_free: pointer not allocated by this context");
+}
+
+int tr_copy_to_dev(tr_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c || !dst || !src) return fail(TR_ERR_ARG, "tr_copy_to_dev: null argument");
+    if (c->vectors.count(dst)) {
+        if (bytes != sizeof(double) * c->next())
+            return fail(TR_ERR_ARG, "tr_copy_to_dev: vector copies must be N()*sizeof(REAL) bytes");
+        HIPCHECK(hipMemcpyAsync(c->d_stage, src, bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(launch_to_internal(c->d_stage, (double *)dst, c->d_perm, c->op.nP, c->op.slm,
+                                    c->op.aslm, c->stream));
+    } else {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return TR_OK;
+}
+
+int tr_copy_to_host(tr_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!c || !dst || !src) return fail(TR_ERR_ARG, "tr_copy_to_host: null argument");
+    if (c->vectors.count(const_cast<void *>(src))) {
+        if (bytes != sizeof(double) * c->next())
+            return fail(TR_ERR_ARG, "tr_copy_to_host: vector copies must be N()*sizeof(REAL) bytes");
+        HIPCHECK(launch_to_external((const double *)src, c->d_stage, c->d_iperm, c->op.nP,
+                                    c->op.slm, c->op.aslm, c->stream));
+        HIPCHECK(hipMemcpyAsync(dst, c->d_stage, bytes, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return TR_OK;
+}
+
+int tr_get_host_small(tr_ctx *c, const double *src, double *dst) {
+    if (!c || !src || !dst) return fail(TR_ERR_ARG, "tr_get_host_small: null argument");
+    std::vector<double> q(c->next());
+    int rc = tr_copy_to_host(c, q.data(), src, sizeof(double) * q.size());
+    if (rc) return rc;
+    for (size_t i = 0; i < c->op.nP; i++)
+        for (size_t j = 0; j < c->op.slm; j++) dst[i * c->op.slm + j] = q[i * c->op.aslm + j];
+    return TR_OK;
+}
+
+int tr_compute_lhs(tr_ctx *c, const double *f, double *Af) {
+    if (!c || !f || !Af) return fail(TR_ERR_ARG, "tr_compute_lhs: null argument");
+    return lhs(c, f, Af);
+}
+
+int tr_compute_lhs_diag(tr_ctx *c, const double *f, double *Af) {
+    if (!c || !f || !Af) return fail(TR_ERR_ARG, "tr_compute_lhs_diag: null argument");
+    return lhs_diag(c, f, Af);
+}
+
+int tr_compute_rhs(tr_ctx *c, double *b) {
+    if (!c || !b) return fail(TR_ERR_ARG, "tr_compute_rhs: null argument");
+    HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, b, c->stream));
+    return TR_OK;
+}
+
+int tr_scale(tr_ctx *c, double *x, double wx) {
+    if (!c || !x) return fail(TR_ERR_ARG, "tr_scale: null argument");
+    return axpby(c, x, wx, nullptr, 0.0);
+}
+int tr_add_prod(tr_ctx *c, double *x, const double *y, double wy) {
+    if (!c || !x || !y) return fail(TR_ERR_ARG, "tr_add_prod: null argument");
+    return axpby(c, x, 1.0, y, wy);
+}
+int tr_mul_add(tr_ctx *c, double *x, double wx, const double *y) {
+    if (!c || !x || !y) return fail(TR_ERR_ARG, "tr_mul_add: null argument");
+    return axpby(c, x, wx, y, 1.0);
+}
+int tr_mul_add_prod(tr_ctx *c, double *x, double wx, const double *y, double wy) {
+    if (!c || !x || !y) return fail(TR_ERR_ARG, "tr_mul_add_prod: null argument");
+    return axpby(c, x, wx, y, wy);
+}
+
+double tr_norm(tr_ctx *c, const double *x) {
+    if (!c || !x) {
+        fail(TR_ERR_ARG, "tr_norm: null argument");
+        return NAN;
+    }
+    return std::sqrt(dot(c, x, x));
+}
+
+double tr_dot(tr_ctx *c, const double *x, const double *y) {
+    if (!c || !x || !y) {
+        fail(TR_ERR_ARG, "tr_dot: null argument");
+        return NAN;
+    }
+    return dot(c, x, y);
+}
+
+int tr_synchronize(tr_ctx *c) {
+    if (!c) return fail(TR_ERR_ARG, "tr_synchronize: null context");
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return TR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Solver: main.cpp:125-162 (outer PCG, Fletcher-Reeves beta) with
+// invprec (main.cpp:223-248, inner CG on the angular-diagonal operator).
+namespace tr {
+
+struct Work {
+    tr_ctx *c;
+    std::vector<double *> v;
+    explicit Work(tr_ctx *c_) : c(c_) {}
+    double *get() {
+        double *p = nullptr;
+        size_t bytes = sizeof(double) * c->nint();
+        if (hipMalloc((void **)&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+        hipMemsetAsync(p, 0, bytes, c->stream);
+        v.push_back(p);
+        return p;
+    }
+    ~Work() {
+        hipStreamSynchronize(c->stream);
+        for (double *p : v) hipFree(p);
+    }
+};
+
+static int invprec(tr_ctx *c, const tr_solve_params *prm, double *f, const double *b, double *Ap,
+                   double *p, double *r, int *kout) {
+    int rc;
+    if ((rc = axpby(c, f, 0, b, 0)) || (rc = lhs_diag(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) ||
+        (rc = axpby(c, r, 1, Ap, -1)) || (rc = axpby(c, p, 0, r, 1)))
+        return rc;
+    int k = 0;
+    double nrz = std::sqrt(dot(c, r, r));  // main.cpp:230-231: norm, then squared
+    nrz *= nrz;
+    while (k < prm->inner_maxit) {
+        if ((rc = lhs_diag(c, p, Ap))) return rc;
+        double alpha = nrz / dot(c, p, Ap);
+        if ((rc = axpby(c, f, 1, p, alpha)) || (rc = axpby(c, r, 1, Ap, -alpha))) return rc;
+        double nr = std::sqrt(dot(c, r, r));
+        if (!(nr >= prm->inner_tol)) break;
+        double nrz2 = nr * nr, beta = nrz2 / nrz;
+        nrz = nrz2;
+        if ((rc = axpby(c, p, beta, r, 1))) return rc;
+        k++;
+    }
+    *kout = k;
+    return TR_OK;
+}
+
+static int solve_reference(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st,
+                           double *hist) {
+    Work w(c);
+    double *b = w.get(), *p = w.get(), *Ap = w.get(), *z = w.get(), *r = w.get(), *pp = w.get(),
+           *pAp = w.get(), *pr = w.get();
+    if (!pr) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+    int rc, preck = 0, ki;
+    auto t0 = std::chrono::steady_clock::now();
+    if ((rc = axpby(c, f, 0, nullptr, 0)) || (rc = tr_compute_rhs(c, b))) return rc;
+    double bnorm = std::sqrt(dot(c, b, b));
+    if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+    if (prm->use_prec) {
+        if ((rc = invprec(c, prm, z, r, pAp, pp, pr, &ki))) return rc;
+        preck += ki;
+    } else if ((rc = axpby(c, z, 0, r, 1)))
+        return rc;
+    if ((rc = axpby(c, p, 0, z, 1))) return rc;
+    int k = 0;
+    double nrz = dot(c, r, z), nr = std::sqrt(dot(c, r, r));
+    bool conv = false;
+    while (k < prm->maxit) {
+        if ((rc = lhs(c, p, Ap))) return rc;
+        double alpha = nrz / dot(c, p, Ap);
+        if ((rc = axpby(c, f, 1, p, alpha)) || (rc = axpby(c, r, 1, Ap, -alpha))) return rc;
+        nr = std::sqrt(dot(c, r, r));
+        conv = nr < prm->tol_abs || (prm->tol_rel > 0 && nr <= prm->tol_rel * bnorm);
+        double beta = 0;
+        if (!conv) {
+            if (prm->use_prec) {
+                if ((rc = invprec(c, prm, z, r, pAp, pp, pr, &ki))) return rc;
+                preck += ki;
+            } else if ((rc = axpby(c, z, 0, r, 1)))
+                return rc;
+            double nrz2 = dot(c, z, r);
+            beta = nrz2 / nrz;
+            nrz = nrz2;
+            if ((rc = axpby(c, p, beta, z, 1))) return rc;
+        }
+        if (hist && k < prm->record) {
+            hist[3 * k] = alpha;
+            hist[3 * k + 1] = beta;
+            hist[3 * k + 2] = nr;
+        }
+        if (conv || !std::isfinite(nr)) break;
+        k++;
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    // true residual with the FP64 operator
+    if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+    double tr_ = std::sqrt(dot(c, r, r));
+    if (st) {
+        st->iters = k;
+        st->preck = preck;
+        st->rnorm = nr;
+        st->bnorm = bnorm;
+        st->true_rel = bnorm > 0 ? tr_ / bnorm : tr_;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->status = conv ? TR_OK : TR_ERR_NOCONV;
+    }
+    return TR_OK;
+}
+
+}  // namespace tr
+
+extern "C" {
+
+int tr_solve(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st, double *hist) {
+    if (!c || !prm || !f) return fail(TR_ERR_ARG, "tr_solve: null argument");
+    if (!c->vectors.count(f)) return fail(TR_ERR_ARG, "tr_solve: f must come from tr_alloc_vector");
+    switch (prm->solver) {
+        case TR_SOLVER_REFERENCE:
+        case TR_SOLVER_PCG_FP64:
+            return solve_reference(c, prm, f, st, hist);
+        default:
+            return fail(TR_ERR_ARG, "tr_solve: unknown solver");
+    }
+}
+
+double tr_bench_lhs(tr_ctx *c, int reps) {
+    if (!c || reps < 1) return NAN;
+    Work w(c);
+    double *f = w.get(), *r = w.get();
+    if (!r) return NAN;
+    std::vector<float> ms(reps);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    lhs(c, f, r);
+    for (int i = 0; i < reps; i++) {
+        hipEventRecord(e0, c->stream);
+        lhs(c, f, r);
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        hipEventElapsedTime(&ms[i], e0, e1);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    std::sort(ms.begin(), ms.end());
+    return ms[reps / 2] * 1e-3;
+}
+
+}  // extern "C"
+
+extern "C" int tr_bench_operator(tr_ctx *c, int warmup, int steps, double *out) {
+    if (!c || !out || steps < 1 || warmup < 0) return fail(TR_ERR_ARG, "tr_bench_operator: bad argument");
+    Work w(c);
+    double *f = w.get(), *r = w.get();
+    if (!r) return fail(TR_ERR_HIP, "tr_bench_operator: out of device memory");
+    // a non-trivial input: f = rhs, then one application
+    HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, f, c->stream));
+    for (int i = 0; i < warmup; i++) {
+        int rc = lhs(c, f, r);
+        if (rc) return rc;
+    }
+    std::vector<hipEvent_t> ev(3 * (size_t)steps);
+    for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; i++) {
+        HIPCHECK(hipEventRecord(ev[3 * i], c->stream));
+        HIPCHECK(launch_lhs_volume<double>(c->vol(), (int)c->op.NC, c->map, f, r, c->stream));
+        HIPCHECK(hipEventRecord(ev[3 * i + 1], c->stream));
+        HIPCHECK(launch_lhs_boundary<double>(c->bnd(), c->bmap, f, r, c->stream));
+        HIPCHECK(hipEventRecord(ev[3 * i + 2], c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    double vol = 0, bnd = 0, all = 0;
+    for (int i = 0; i < steps; i++) {
+        float a = 0, b = 0;
+        HIPCHECK(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+        HIPCHECK(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
+        vol += a;
+        bnd += b;
+        all += a + b;
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    out[0] = std::chrono::duration<double>(t1 - t0).count();
+    out[1] = vol / steps;
+    out[2] = bnd / steps;
+    out[3] = all / steps;
+    return TR_OK;
+}

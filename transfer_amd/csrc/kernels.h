@@ -1,0 +1,79 @@
+// kernels.h -- launch interface of the gfx950 kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tr {
+
+// Lane mapping of the vertex kernels: a wavefront of 64 lanes holds VPW
+// vertices, L = slm / J lanes each, every lane owning J consecutive
+// harmonics.  WPB wavefronts per workgroup.
+struct LaneMap {
+    int J = 1, L = 1, VPW = 1, WPB = 4;
+};
+
+template <typename T>
+struct VolArgs {
+    uint32_t nP, slm, W, arow;
+    int L, VPW;
+    const uint32_t *nbr;   // [W][nP]
+    const T *coef;         // [W][NC][nP]
+    const uint16_t *aidx;  // [arow][slm]
+    const T *aval;         // [arow][slm]
+    const T *massfac;      // [slm]
+    const T *ddiag;        // [3][slm]
+};
+
+template <typename T>
+struct BndArgs {
+    uint32_t nB, slm, WB, brow;
+    int L, VPW;
+    const uint32_t *bvert; // [nB]
+    const uint32_t *bnbr;  // [WB][nB]
+    const T *bcoef;        // [WB][3][nB]
+    const uint16_t *bidx;  // [brow][slm]
+    const T *bval;         // [brow][slm]
+    const T *bdiag;        // [3][slm]
+};
+
+// r = volume part (overwrite); NC = 7 or 10 channels
+template <typename T>
+hipError_t launch_lhs_volume(const VolArgs<T> &a, int NC, const LaneMap &m,
+                             const T *f, T *r, hipStream_t s);
+// r += vacuum-boundary part
+template <typename T>
+hipError_t launch_lhs_boundary(const BndArgs<T> &a, const LaneMap &m, const T *f,
+                               T *r, hipStream_t s);
+template <typename T>
+hipError_t launch_diag_volume(const VolArgs<T> &a, int NC, const LaneMap &m,
+                              const T *f, T *r, hipStream_t s);
+template <typename T>
+hipError_t launch_diag_boundary(const BndArgs<T> &a, const LaneMap &m,
+                                const T *f, T *r, hipStream_t s);
+size_t lhs_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem);
+size_t bnd_lds_bytes(const LaneMap &m, uint32_t slm, size_t elem);
+
+// b[v][lm] = lm == 0 ? rhs[v] : 0
+hipError_t launch_rhs(const double *rhs, uint32_t nP, uint32_t slm, double *b,
+                      hipStream_t s);
+
+// elementwise x = wx*x + wy*y over n entries (y may be null when wy == 0)
+hipError_t launch_axpby(double *x, double wx, const double *y, double wy, size_t n,
+                        hipStream_t s);
+
+// out[0] = sum_i x[i]*y[i] in FP64, deterministic two-pass reduction;
+// part needs red_partials() doubles.
+int red_partials();
+hipError_t launch_dot(const double *x, const double *y, size_t n, double *part,
+                      double *out, hipStream_t s);
+
+// layout conversions between the caller's [vertex][aslm] and the internal
+// renumbered [vertex][slm]
+hipError_t launch_to_internal(const double *src, double *dst, const uint32_t *perm,
+                              uint32_t nP, uint32_t slm, uint32_t aslm, hipStream_t s);
+hipError_t launch_to_external(const double *src, double *dst, const uint32_t *iperm,
+                              uint32_t nP, uint32_t slm, uint32_t aslm, hipStream_t s);
+
+}  // namespace tr

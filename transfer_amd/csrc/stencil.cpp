@@ -1,0 +1,299 @@
+// stencil.cpp -- renumbering and FP64 assembly of the device operator data
+// (see stencil.h for the formulation).
+#include "stencil.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "internal.h"
+
+namespace tr {
+
+static uint64_t spread3(uint64_t x) {
+    x &= 0x1fffffULL;
+    x = (x | x << 32) & 0x1f00000000ffffULL;
+    x = (x | x << 16) & 0x1f0000ff0000ffULL;
+    x = (x | x << 8) & 0x100f00f00f00f00fULL;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+    x = (x | x << 2) & 0x1249249249249249ULL;
+    return x;
+}
+
+// Neighbour sets (vertices sharing a tet) in caller numbering.
+static void vertex_graph(const tr_mesh_view &mv, std::vector<uint32_t> &start,
+                         std::vector<uint32_t> &adj) {
+    start.assign(mv.nP + 1, 0);
+    adj.clear();
+    std::vector<uint32_t> tmp;
+    for (uint32_t v = 0; v < mv.nP; v++) {
+        tmp.clear();
+        for (uint32_t j = mv.tetstart[v]; j < mv.tetstart[v + 1]; j++) {
+            const tr_tetrahedron &t = mv.mesh[mv.tetidx[j]];
+            for (int k = 0; k < 4; k++)
+                if (t.p[k] != v) tmp.push_back(t.p[k]);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        adj.insert(adj.end(), tmp.begin(), tmp.end());
+        start[v + 1] = (uint32_t)adj.size();
+    }
+}
+
+// Reverse Cuthill-McKee over every connected component.
+static void rcm_order(uint32_t nP, const std::vector<uint32_t> &start,
+                      const std::vector<uint32_t> &adj, std::vector<uint32_t> &perm) {
+    std::vector<uint32_t> deg(nP), order;
+    for (uint32_t v = 0; v < nP; v++) deg[v] = start[v + 1] - start[v];
+    std::vector<uint32_t> byDeg(nP);
+    std::iota(byDeg.begin(), byDeg.end(), 0);
+    std::stable_sort(byDeg.begin(), byDeg.end(),
+                     [&](uint32_t a, uint32_t b) { return deg[a] < deg[b]; });
+    std::vector<char> seen(nP, 0);
+    order.reserve(nP);
+    std::vector<uint32_t> nb;
+    for (uint32_t root : byDeg) {
+        if (seen[root]) continue;
+        size_t head = order.size();
+        order.push_back(root);
+        seen[root] = 1;
+        while (head < order.size()) {
+            uint32_t v = order[head++];
+            nb.clear();
+            for (uint32_t j = start[v]; j < start[v + 1]; j++)
+                if (!seen[adj[j]]) nb.push_back(adj[j]);
+            std::stable_sort(nb.begin(), nb.end(),
+                             [&](uint32_t a, uint32_t b) { return deg[a] < deg[b]; });
+            for (uint32_t u : nb) {
+                seen[u] = 1;
+                order.push_back(u);
+            }
+        }
+    }
+    perm.assign(order.rbegin(), order.rend());
+}
+
+int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
+                      HostOperator &op) {
+    const uint32_t nP = mv.nP, slm = av.slm;
+    if (nP == 0 || slm == 0 || slm > 128)
+        return fail(TR_ERR_ARG, "assemble_operator: empty mesh or slm out of 1..128");
+    op.nP = nP;
+    op.slm = slm;
+    op.aslm = (slm + 7) & ~7u;
+
+    // ---- renumbering
+    std::vector<uint32_t> gstart, gadj;
+    vertex_graph(mv, gstart, gadj);
+    op.perm.resize(nP);
+    if (mv.coords) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (uint32_t v = 0; v < nP; v++)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], mv.coords[3 * v + a]);
+                hi[a] = std::max(hi[a], mv.coords[3 * v + a]);
+            }
+        double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+        std::vector<uint64_t> code(nP);
+        for (uint32_t v = 0; v < nP; v++) {
+            uint64_t q[3];
+            for (int a = 0; a < 3; a++)
+                q[a] = (uint64_t)std::min(2097151.0, std::floor((mv.coords[3 * v + a] - lo[a]) / span * 2097151.0));
+            code[v] = spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
+        }
+        std::iota(op.perm.begin(), op.perm.end(), 0);
+        std::stable_sort(op.perm.begin(), op.perm.end(),
+                         [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+    } else {
+        rcm_order(nP, gstart, gadj, op.perm);
+    }
+    op.iperm.resize(nP);
+    for (uint32_t i = 0; i < nP; i++) op.iperm[op.perm[i]] = i;
+
+    // ---- angular channel matrices, [out][in] (kernels.cu:276-282 reads
+    // omega[a][in][out] with b = omega_pos[a][in][out])
+    std::vector<double> Om((size_t)9 * slm * slm, 0.0);  // [a][b][out][in]
+    for (uint32_t a = 0; a < 3; a++)
+        for (uint32_t in = 0; in < slm; in++)
+            for (uint32_t out = 0; out < slm; out++) {
+                size_t s = (size_t)a * slm * slm + (size_t)in * slm + out;
+                uint32_t b = av.omega_pos[s];
+                if (b > 2) return fail(TR_ERR_ARG, "omega_pos entry out of 0..2");
+                Om[((size_t)(a * 3 + b) * slm + out) * slm + in] = av.omega[s];
+            }
+    auto O = [&](int a, int b, uint32_t out, uint32_t in) {
+        return Om[((size_t)(a * 3 + b) * slm + out) * slm + in];
+    };
+    double asym = 0, omax = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (uint32_t o = 0; o < slm; o++)
+                for (uint32_t i = 0; i < slm; i++) {
+                    omax = std::max(omax, std::fabs(O(a, b, o, i)));
+                    if (a < b) asym = std::max(asym, std::fabs(O(a, b, o, i) - O(b, a, o, i)));
+                }
+    op.NC = asym > 1e-12 * std::max(omax, 1e-300) ? 10 : 7;
+    const uint32_t NC = op.NC;
+    static const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+    op.omc.assign((size_t)(NC - 1) * slm * slm, 0.0);
+    for (uint32_t o = 0; o < slm; o++)
+        for (uint32_t i = 0; i < slm; i++) {
+            for (int a = 0; a < 3; a++)
+                op.omc[((size_t)a * slm + o) * slm + i] = O(a, a, o, i);
+            for (int q = 0; q < 3; q++) {
+                double ab = O(pa[q], pb[q], o, i), ba = O(pb[q], pa[q], o, i);
+                op.omc[((size_t)(3 + q) * slm + o) * slm + i] = 0.5 * (ab + ba);
+                if (NC == 10) op.omc[((size_t)(6 + q) * slm + o) * slm + i] = 0.5 * (ab - ba);
+            }
+        }
+    op.onc.assign((size_t)3 * slm * slm, 0.0);
+    const double *On[3] = {av.Ox, av.Oy, av.Oz};
+    for (int n = 0; n < 3; n++)
+        for (uint32_t o = 0; o < slm; o++)
+            for (uint32_t i = 0; i < slm; i++)
+                op.onc[((size_t)n * slm + o) * slm + i] = On[n][(size_t)i * slm + o];
+
+    auto build_table = [&](const std::vector<double> &mats, uint32_t nmat, uint32_t &rows,
+                           std::vector<uint16_t> &idx, std::vector<double> &val) {
+        double mx = 0;
+        for (double v : mats) mx = std::max(mx, std::fabs(v));
+        const double thr = 1e-14 * std::max(mx, 1e-300);
+        std::vector<std::vector<std::pair<uint16_t, double>>> R(slm);
+        for (uint32_t o = 0; o < slm; o++)
+            for (uint32_t c = 0; c < nmat; c++)
+                for (uint32_t i = 0; i < slm; i++) {
+                    double v = mats[((size_t)c * slm + o) * slm + i];
+                    if (std::fabs(v) > thr) R[o].push_back({(uint16_t)(c * slm + i), v});
+                }
+        rows = 0;
+        for (auto &r : R) rows = std::max<uint32_t>(rows, (uint32_t)r.size());
+        rows = std::max<uint32_t>(rows, 1);
+        idx.assign((size_t)rows * slm, 0);
+        val.assign((size_t)rows * slm, 0.0);
+        for (uint32_t o = 0; o < slm; o++)
+            for (size_t e = 0; e < R[o].size(); e++) {
+                idx[e * slm + o] = R[o][e].first;
+                val[e * slm + o] = R[o][e].second;
+            }
+    };
+    build_table(op.omc, NC - 1, op.arow, op.aidx, op.aval);
+    build_table(op.onc, 3, op.brow, op.bidx, op.bval);
+
+    op.massfac.assign(slm, 1.0);
+    if (av.massfac)
+        for (uint32_t lm = 0; lm < slm; lm++) op.massfac[lm] = av.massfac[lm];
+    op.ddiag.resize((size_t)3 * slm);
+    op.bdiag.resize((size_t)3 * slm);
+    for (int a = 0; a < 3; a++)
+        for (uint32_t lm = 0; lm < slm; lm++) {
+            op.ddiag[(size_t)a * slm + lm] = av.omega[(size_t)a * slm * slm + (size_t)lm * slm + lm];
+            op.bdiag[(size_t)a * slm + lm] = On[a][(size_t)lm * slm + lm];
+        }
+
+    // ---- volume stencil in the new numbering
+    std::vector<uint32_t> cnt(nP);
+    uint32_t W = 1;
+    for (uint32_t i = 0; i < nP; i++) {
+        uint32_t vo = op.perm[i];
+        cnt[i] = 1 + gstart[vo + 1] - gstart[vo];
+        W = std::max(W, cnt[i]);
+    }
+    op.W = W;
+    op.nbr.assign((size_t)W * nP, 0);
+    op.coef.assign((size_t)W * NC * nP, 0.0);
+    std::vector<uint32_t> slots;
+    for (uint32_t i = 0; i < nP; i++) {
+        uint32_t vo = op.perm[i];
+        slots.clear();
+        slots.push_back(i);
+        for (uint32_t j = gstart[vo]; j < gstart[vo + 1]; j++) slots.push_back(op.iperm[gadj[j]]);
+        std::sort(slots.begin() + 1, slots.end());
+        for (uint32_t s = 0; s < W; s++)
+            op.nbr[(size_t)s * nP + i] = s < slots.size() ? slots[s] : i;
+        auto slot_of = [&](uint32_t v) {
+            for (uint32_t s = 0; s < slots.size(); s++)
+                if (slots[s] == v) return s;
+            return (uint32_t)0;
+        };
+        for (uint32_t j = mv.tetstart[vo]; j < mv.tetstart[vo + 1]; j++) {
+            const tr_tetrahedron &t = mv.mesh[mv.tetidx[j]];
+            const uint32_t li = mv.tetpos[j];
+            const double kv = t.kappa_volume, inv = 1.0 / (9.0 * kv);
+            const double *si = t.s[li];
+            for (int k = 0; k < 4; k++) {
+                uint32_t s = slot_of(op.iperm[t.p[k]]);
+                const double *sk = t.s[k];
+                double ch[10];
+                ch[0] = kv * (1.0 / 20.0) * (k == (int)li ? 2.0 : 1.0);
+                ch[1] = si[0] * sk[0] * inv;
+                ch[2] = si[1] * sk[1] * inv;
+                ch[3] = si[2] * sk[2] * inv;
+                ch[4] = (si[0] * sk[1] + si[1] * sk[0]) * inv;
+                ch[5] = (si[0] * sk[2] + si[2] * sk[0]) * inv;
+                ch[6] = (si[1] * sk[2] + si[2] * sk[1]) * inv;
+                ch[7] = (si[0] * sk[1] - si[1] * sk[0]) * inv;
+                ch[8] = (si[0] * sk[2] - si[2] * sk[0]) * inv;
+                ch[9] = (si[1] * sk[2] - si[2] * sk[1]) * inv;
+                for (uint32_t c = 0; c < NC; c++) op.coef[((size_t)s * NC + c) * nP + i] += ch[c];
+            }
+        }
+    }
+
+    // ---- vacuum-boundary stencil
+    op.bvert.clear();
+    std::vector<std::vector<uint32_t>> bsl;
+    std::vector<std::vector<double>> bco;
+    for (uint32_t i = 0; i < nP; i++) {
+        uint32_t vo = op.perm[i];
+        std::vector<uint32_t> sl;
+        std::vector<double> co;
+        for (uint32_t j = mv.facestart[vo]; j < mv.facestart[vo + 1]; j++) {
+            uint32_t fi = mv.faceidx[j];
+            if (mv.face_kind && mv.face_kind[fi] != 0) continue;
+            const tr_face &F = mv.bnd[fi];
+            const uint32_t lf = mv.facepos[j];
+            double surf = std::fabs(F.s[0] + F.s[1] + F.s[2]);
+            int axis = (2 * std::fabs(F.s[0]) > surf) ? 0 : ((2 * std::fabs(F.s[1]) > surf) ? 1 : 2);
+            for (int k = 0; k < 3; k++) {
+                uint32_t v = op.iperm[F.p[k]];
+                size_t s = std::find(sl.begin(), sl.end(), v) - sl.begin();
+                if (s == sl.size()) {
+                    sl.push_back(v);
+                    co.insert(co.end(), {0.0, 0.0, 0.0});
+                }
+                co[3 * s + axis] += surf * (1.0 / 12.0) * (k == (int)lf ? 2.0 : 1.0);
+            }
+        }
+        if (!sl.empty()) {
+            op.bvert.push_back(i);
+            bsl.push_back(sl);
+            bco.push_back(co);
+        }
+    }
+    op.nB = (uint32_t)op.bvert.size();
+    op.WB = 1;
+    for (auto &s : bsl) op.WB = std::max<uint32_t>(op.WB, (uint32_t)s.size());
+    op.bnbr.assign((size_t)op.WB * std::max<uint32_t>(op.nB, 1), 0);
+    op.bcoef.assign((size_t)op.WB * 3 * std::max<uint32_t>(op.nB, 1), 0.0);
+    for (uint32_t b = 0; b < op.nB; b++)
+        for (uint32_t s = 0; s < op.WB; s++) {
+            op.bnbr[(size_t)s * op.nB + b] = s < bsl[b].size() ? bsl[b][s] : op.bvert[b];
+            for (int n = 0; n < 3; n++)
+                op.bcoef[((size_t)s * 3 + n) * op.nB + b] = s < bsl[b].size() ? bco[b][3 * s + n] : 0.0;
+        }
+
+    // ---- right-hand side (kernels.cu:157-163, same summation order)
+    op.rhs.assign(nP, 0.0);
+    for (uint32_t i = 0; i < nP; i++) {
+        uint32_t vo = op.perm[i];
+        double sum = 0;
+        for (uint32_t j = mv.tetstart[vo]; j < mv.tetstart[vo + 1]; j++) {
+            const tr_tetrahedron &t = mv.mesh[mv.tetidx[j]];
+            sum += t.kappa_volume * t.I_p * (1. / 4.);
+        }
+        op.rhs[i] = sum;
+    }
+    return TR_OK;
+}
+
+}  // namespace tr

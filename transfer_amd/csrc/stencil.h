@@ -1,0 +1,63 @@
+// stencil.h -- the device operator's data, assembled on the host in FP64.
+//
+// The reference applies A vertex by vertex, walking each vertex's incident
+// tets and rebuilding every tet's gradient once per vertex
+// (kernels.cu:200-287).  Here the spatial part is assembled once into
+// per-edge couplings ("channels") and the angular part stays matrix-free:
+//
+//   (Af)_i = sum_k massfac * M_ik f_k
+//          + sum_c Omega_c ( sum_k K^c_ik f_k )            c = 1..NC-1
+//          + sum_n O_n ( sum_k B^n_ik f_k )                n = x, y, z
+//
+//   M_ik   = sum_{t ∋ i,k} kappa V / 20 (1 + delta_ik)                  (kernels.cu:249)
+//   K^ab_ik= sum_{t ∋ i,k} s^a_{t,i} s^b_{t,k} / (9 kappa V)            (kernels.cu:251-260)
+//   channels 1..3 = K^xx, K^yy, K^zz;  4..6 = K^ab + K^ba (xy, xz, yz);
+//   7..9 = K^ab - K^ba (only when the angular matrices are not symmetric
+//   in a <-> b, i.e. with anisotropic scattering)
+//   B^n_ik = sum over vacuum faces F ∋ i,k with axis n of S_F/12 (1 + delta_ik)
+//            (kernels.cu:414-435, axis rule :419-425)
+//
+// Vertices are renumbered (Morton order of coordinates, or reverse
+// Cuthill-McKee without coordinates) so that the neighbour gathers of a
+// wavefront hit nearby memory.  Neighbour lists are ELL (fixed width W,
+// slot 0 = the vertex itself, unused slots point at the vertex with zero
+// coefficients) stored slot-major so that a wave's vertices read adjacent
+// words.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "transfer.h"
+
+namespace tr {
+
+struct HostOperator {
+    uint32_t nP = 0, slm = 0, aslm = 0, NC = 7, W = 0;
+    std::vector<uint32_t> perm;   // perm[new] = caller vertex id
+    std::vector<uint32_t> iperm;  // iperm[caller id] = new
+    std::vector<uint32_t> nbr;    // [W][nP]
+    std::vector<double> coef;     // [W][NC][nP]
+    // vacuum-boundary stencil over the boundary vertices only
+    uint32_t nB = 0, WB = 0;
+    std::vector<uint32_t> bvert;  // [nB]
+    std::vector<uint32_t> bnbr;   // [WB][nB]
+    std::vector<double> bcoef;    // [WB][3][nB]
+    // angular tables: for output lm, the padded list of (channel, lm')
+    // couplings; idx = (c-1)*slm + lm' (volume) or n*slm + lm' (boundary)
+    uint32_t arow = 0, brow = 0;
+    std::vector<uint16_t> aidx, bidx;  // [row][slm]
+    std::vector<double> aval, bval;    // [row][slm]
+    std::vector<double> massfac;       // [slm]
+    std::vector<double> ddiag;         // [3][slm]  Omega^aa[lm][lm] (kernels.cu:348-353)
+    std::vector<double> bdiag;         // [3][slm]  O_n[lm][lm]      (kernels.cu:490)
+    std::vector<double> rhs;           // [nP] sum kappa V I_p / 4   (kernels.cu:157-163)
+    // dense channel matrices Omega_c[out][in], c = 1..NC-1, and O_n[out][in]
+    std::vector<double> omc;           // [NC-1][slm][slm]
+    std::vector<double> onc;           // [3][slm][slm]
+};
+
+int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
+                      HostOperator &op);
+
+}  // namespace tr

@@ -1,0 +1,48 @@
+"""Workload builders: the meshes, per-tet fields and angular data of the
+benchmark configurations, from the library's own generators.
+
+    slab      [0,1]x[0,1]x[0,4], 12x12x48 cubes (41,472 tets), interior nodes
+              jittered by +-0.15 cell widths, kappa = 1, isotropic scattering
+              albedo 0.5, source kappa*I_p = 1, K = 1 (26-point Lebedev order)
+    ball64    cubes of a 64^3 grid on [-1,1]^3 with centroid r <= 1
+              (824,256 tets), kappa = 50 exp(-r/0.2), I_p = (1 - 0.8 r)^4,
+              no scattering, K = 3 (110-point order)
+    ball96    96^3 grid, same ball, kappa = 1 + 9 U[0,1) per tet, albedo 0.9,
+              Henyey-Greenstein g = 0.6 truncated after P_7, I_p = 1, K = 5
+    cube32    32^3 cubes on [0,1]^3 (196,608 tets), kappa = 0.1 + 8 Gaussian
+              blobs (amplitude log-U[1,100], width U[0.05,0.3]), albedo
+              U[0,0.99], I_p = 1, K = 3
+The Lebedev N-point orders map to the harmonic order K = (degree(N) - 2) // 4.
+"""
+from .api import Angular, Mesh, rng_uniform
+
+
+class Problem:
+    def __init__(self, name, mesh, ang, albedo=0.0, g=0.0, hg_lmax=0):
+        self.name, self.mesh, self.ang = name, mesh, ang
+        self.albedo, self.g, self.hg_lmax = albedo, g, hg_lmax
+
+
+def slab(n=12, seed=1, maxk=1, albedo=0.5, jitter=0.15):
+    m = Mesh.box((n, n, 4 * n), (0, 0, 0), (1, 1, 4), jitter=jitter, seed=seed)
+    m.set_fields(kappa=m.field("const", [1.0]), Ip=m.field("const", [1.0]))
+    return Problem("slab%d" % n, m, Angular(maxk, albedo, 0.0, 0), albedo)
+
+
+def ball(n=64, maxk=3, seed=2, kind="planck"):
+    m = Mesh.box((n, n, n), (-1, -1, -1), (1, 1, 1), ball_radius=1.0, ball_center=(0, 0, 0))
+    if kind == "planck":
+        m.set_fields(kappa=m.field("radial_exp", [50.0, 0.2, 0, 0, 0]),
+                     Ip=m.field("planck", [0.8, 0, 0, 0, 0]))
+        return Problem("ball%d" % n, m, Angular(maxk))
+    m.set_fields(kappa=m.field("uniform", [1.0, 9.0], seed=seed, stream=3), Ip=m.field("const", [1.0]))
+    return Problem("ball%d-hg" % n, m, Angular(maxk, 0.9, 0.6, 7), 0.9, 0.6, 7)
+
+
+def cube(n=32, maxk=3, seed=5, albedo=None):
+    m = Mesh.box((n, n, n), (0, 0, 0), (1, 1, 1))
+    k = m.field("blobs", [0.1, 8, 1.0, 100.0, 0.05, 0.3], seed=seed, stream=4)
+    m.set_fields(kappa=k, Ip=m.field("const", [1.0]))
+    if albedo is None:
+        albedo = 0.99 * rng_uniform(seed, 6, 1)[0]
+    return Problem("cube%d" % n, m, Angular(maxk, albedo, 0.0, 0), albedo)
