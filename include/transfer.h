@@ -203,12 +203,19 @@ const char *tr_last_error(void);
 
 enum {
     TR_SOLVER_REFERENCE = 0, /* main.cpp's PCG + invprec, host-driven, FP64 */
-    TR_SOLVER_PCG_FP64 = 1   /* same algorithm, device-resident scalars     */
+    TR_SOLVER_PCG_FP64 = 1,  /* CG with device-resident scalars and graph-
+                                captured iterations; use_prec 0 = none,
+                                2 = Jacobi (exact diagonal of A)            */
+    TR_SOLVER_MIXED = 2      /* FP64 iterative refinement (true residual
+                                b - A x in FP64) around the FP32 operator's
+                                device CG run to inner_tol (relative);
+                                maxit caps the refinement steps             */
 };
 
 typedef struct tr_solve_params {
     int solver;          /* TR_SOLVER_* */
-    int use_prec;        /* 1: invprec inner CG on the Diag operator */
+    int use_prec;        /* 1: invprec inner CG on the Diag operator
+                            (TR_SOLVER_REFERENCE); 2: Jacobi           */
     int maxit;           /* outer cap (main.cpp:142: 1000) */
     double tol_abs;      /* stop when ||r|| < tol_abs (main.cpp:149: 1e-4) */
     double tol_rel;      /* and/or ||r|| <= tol_rel*||b|| (0 = off) */

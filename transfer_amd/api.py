@@ -23,6 +23,7 @@ vp = C.c_void_p
 
 TR_SOLVER_REFERENCE = 0
 TR_SOLVER_PCG_FP64 = 1
+TR_SOLVER_MIXED = 2
 
 FIELD = dict(const=0, radial_exp=1, uniform=2, lognormal=3, blobs=4, planck=5, step=6)
 

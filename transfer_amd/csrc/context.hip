@@ -17,6 +17,7 @@
 
 #include "internal.h"
 #include "kernels.h"
+#include "solver.h"
 #include "stencil.h"
 
 #define HIPCHECK(x)                                                                   \
@@ -30,23 +31,33 @@ struct tr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     tr::HostOperator op;
-    tr::LaneMap map, bmap;
+    tr::LaneMap map;
     // device operator data (FP64)
-    uint32_t *d_nbr = nullptr, *d_bvert = nullptr, *d_bnbr = nullptr;
+    uint32_t *d_nbr = nullptr, *d_bnbr = nullptr;
+    int32_t *d_bof = nullptr;
     uint32_t *d_perm = nullptr, *d_iperm = nullptr;
     uint16_t *d_aidx = nullptr, *d_bidx = nullptr;
     double *d_coef = nullptr, *d_bcoef = nullptr, *d_aval = nullptr, *d_bval = nullptr;
     double *d_massfac = nullptr, *d_ddiag = nullptr, *d_bdiag = nullptr, *d_rhs = nullptr;
     double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
+    // FP32 copy of the operator data (mixed-precision solves), built lazily;
+    // index arrays are shared with the FP64 set
+    bool have32 = false;
+    tr::LaneMap map32;
+    float *f_coef = nullptr, *f_bcoef = nullptr, *f_aval = nullptr, *f_bval = nullptr;
+    float *f_massfac = nullptr, *f_ddiag = nullptr, *f_bdiag = nullptr;
     std::set<void *> vectors, raw;
     std::vector<void *> owned;
 
-    tr::VolArgs<double> vol() const {
-        tr::VolArgs<double> a;
+    tr::OpArgs<double> args() const {
+        tr::OpArgs<double> a;
         a.nP = op.nP;
         a.slm = op.slm;
         a.W = op.W;
         a.arow = op.arow;
+        a.nB = op.nB;
+        a.WB = op.WB;
+        a.brow = op.brow;
         a.L = map.L;
         a.VPW = map.VPW;
         a.nbr = d_nbr;
@@ -55,22 +66,38 @@ struct tr_ctx {
         a.aval = d_aval;
         a.massfac = d_massfac;
         a.ddiag = d_ddiag;
-        return a;
-    }
-    tr::BndArgs<double> bnd() const {
-        tr::BndArgs<double> a;
-        a.nB = op.nB;
-        a.slm = op.slm;
-        a.WB = op.WB;
-        a.brow = op.brow;
-        a.L = bmap.L;
-        a.VPW = bmap.VPW;
-        a.bvert = d_bvert;
+        a.bof = d_bof;
         a.bnbr = d_bnbr;
         a.bcoef = d_bcoef;
         a.bidx = d_bidx;
         a.bval = d_bval;
         a.bdiag = d_bdiag;
+        return a;
+    }
+    tr::OpArgs<float> args32() const {
+        tr::OpArgs<double> d = args();
+        tr::OpArgs<float> a;
+        a.nP = d.nP;
+        a.slm = d.slm;
+        a.W = d.W;
+        a.arow = d.arow;
+        a.nB = d.nB;
+        a.WB = d.WB;
+        a.brow = d.brow;
+        a.L = map32.L;
+        a.VPW = map32.VPW;
+        a.nbr = d.nbr;
+        a.aidx = d.aidx;
+        a.bof = d.bof;
+        a.bnbr = d.bnbr;
+        a.bidx = d.bidx;
+        a.coef = f_coef;
+        a.aval = f_aval;
+        a.massfac = f_massfac;
+        a.ddiag = f_ddiag;
+        a.bcoef = f_bcoef;
+        a.bval = f_bval;
+        a.bdiag = f_bdiag;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }
@@ -112,14 +139,12 @@ static int upload(tr_ctx *c, T **dst, const std::vector<T> &src) {
 }
 
 static int lhs(tr_ctx *c, const double *f, double *r) {
-    HIPCHECK(launch_lhs_volume<double>(c->vol(), (int)c->op.NC, c->map, f, r, c->stream));
-    HIPCHECK(launch_lhs_boundary<double>(c->bnd(), c->bmap, f, r, c->stream));
+    HIPCHECK(launch_apply<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream));
     return TR_OK;
 }
 
 static int lhs_diag(tr_ctx *c, const double *f, double *r) {
-    HIPCHECK(launch_diag_volume<double>(c->vol(), (int)c->op.NC, c->map, f, r, c->stream));
-    HIPCHECK(launch_diag_boundary<double>(c->bnd(), c->bmap, f, r, c->stream));
+    HIPCHECK(launch_apply_diag<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream));
     return TR_OK;
 }
 
@@ -165,19 +190,18 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
     }
     c->map = choose_map(c->op.slm, 4);
-    c->bmap = c->map;
     int maxlds = 65536;
     hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
-    while (c->map.WPB > 1 && lhs_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds)
+    while (c->map.WPB > 1 && apply_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds)
         c->map.WPB /= 2;
-    if (lhs_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds) {
+    if (apply_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds) {
         tr_ctx_destroy(c);
         return fail(TR_ERR_ARG, "tr_ctx_create: angular order too large for LDS staging");
     }
     const HostOperator &op = c->op;
     std::vector<double> stage;
     if ((rc = upload(c, &c->d_nbr, op.nbr)) || (rc = upload(c, &c->d_coef, op.coef)) ||
-        (rc = upload(c, &c->d_bvert, op.bvert)) || (rc = upload(c, &c->d_bnbr, op.bnbr)) ||
+        (rc = upload(c, &c->d_bof, op.bof)) || (rc = upload(c, &c->d_bnbr, op.bnbr)) ||
         (rc = upload(c, &c->d_bcoef, op.bcoef)) || (rc = upload(c, &c->d_aidx, op.aidx)) ||
         (rc = upload(c, &c->d_aval, op.aval)) || (rc = upload(c, &c->d_bidx, op.bidx)) ||
         (rc = upload(c, &c->d_bval, op.bval)) || (rc = upload(c, &c->d_massfac, op.massfac)) ||
@@ -482,6 +506,195 @@ static int solve_reference(tr_ctx *c, const tr_solve_params *prm, double *f, tr_
 
 }  // namespace tr
 
+namespace tr {
+
+// diag(A)[v][lm] from the assembled data: slot 0 of the volume stencil is
+// the vertex itself; on the boundary the vertex's own boundary slot.
+static std::vector<double> operator_diagonal(const HostOperator &op) {
+    const uint32_t nP = op.nP, slm = op.slm, NC = op.NC;
+    std::vector<double> d((size_t)nP * slm);
+    for (uint32_t v = 0; v < nP; v++)
+        for (uint32_t lm = 0; lm < slm; lm++) {
+            double s = op.massfac[lm] * op.coef[(size_t)0 * nP + v];
+            for (uint32_t c = 1; c < NC; c++)
+                s += op.omc[((size_t)(c - 1) * slm + lm) * slm + lm] * op.coef[(size_t)c * nP + v];
+            int b = op.bof[v];
+            if (b >= 0)
+                for (uint32_t q = 0; q < op.WB; q++)
+                    if (op.bnbr[(size_t)q * op.nB + b] == v) {
+                        for (int n = 0; n < 3; n++)
+                            s += op.onc[((size_t)n * slm + lm) * slm + lm] *
+                                 op.bcoef[((size_t)q * 3 + n) * op.nB + b];
+                        break;
+                    }
+            d[(size_t)v * slm + lm] = s;
+        }
+    return d;
+}
+
+static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
+    Work w(c);
+    const size_t n = c->nint();
+    double *b = w.get(), *r = w.get(), *p = w.get(), *Ap = w.get();
+    double *part = nullptr, *part2 = nullptr, *sc = nullptr, *dinv = nullptr;
+    const uint32_t nb = apply_blocks(c->op.nP, c->map);
+    HIPCHECK(hipMalloc((void **)&part, sizeof(double) * (std::max<size_t>(nb, 3 * 1024) + 2 * 1024 + 16)));
+    w.v.push_back(part);
+    part2 = part + std::max<size_t>(nb, 3 * 1024);
+    sc = part2 + 2 * 1024;
+    if (!Ap) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+    if (prm->use_prec == 2) {
+        std::vector<double> d = operator_diagonal(c->op);
+        for (double &x : d) x = 1.0 / x;
+        dinv = w.get();
+        HIPCHECK(hipMemcpyAsync(dinv, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    } else if (prm->use_prec != 0) {
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none) or 2 (Jacobi)");
+    }
+    int rc;
+    if ((rc = tr_compute_rhs(c, b)) || (rc = axpby(c, f, 0, nullptr, 0))) return rc;
+    CgSpace<double> S;
+    S.a = c->args();
+    S.NC = (int)c->op.NC;
+    S.map = c->map;
+    S.n = n;
+    S.x = f;
+    S.r = r;
+    S.p = p;
+    S.Ap = Ap;
+    S.b = b;
+    S.dinv = dinv;
+    S.part = part;
+    S.part2 = part2;
+    S.sc = sc;
+    CgParams cp;
+    cp.maxit = prm->maxit;
+    cp.tol_abs = prm->tol_abs;
+    cp.tol_rel = prm->tol_rel;
+    cp.chunk = 16;
+    CgResult res;
+    if ((rc = pcg_device<double>(S, cp, c->stream, res))) return rc;
+    if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+    double trn = std::sqrt(dot(c, r, r)), bn = std::sqrt(res.bb);
+    if (st) {
+        st->iters = res.iters;
+        st->preck = 0;
+        st->rnorm = std::sqrt(res.rr);
+        st->bnorm = bn;
+        st->true_rel = bn > 0 ? trn / bn : trn;
+        st->seconds = res.seconds;
+        st->status = res.flag == 1 ? TR_OK : TR_ERR_NOCONV;
+    }
+    return TR_OK;
+}
+
+
+static std::vector<float> to_f32(const std::vector<double> &v) {
+    return std::vector<float>(v.begin(), v.end());
+}
+
+static int ensure_fp32(tr_ctx *c) {
+    if (c->have32) return TR_OK;
+    const HostOperator &op = c->op;
+    c->map32 = choose_map(op.slm, 8);
+    int maxlds = 65536;
+    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    while (c->map32.WPB > 1 && apply_lds_bytes(c->map32, (int)op.NC, op.slm, 4) > (size_t)maxlds)
+        c->map32.WPB /= 2;
+    int rc;
+    if ((rc = upload(c, &c->f_coef, to_f32(op.coef))) || (rc = upload(c, &c->f_bcoef, to_f32(op.bcoef))) ||
+        (rc = upload(c, &c->f_aval, to_f32(op.aval))) || (rc = upload(c, &c->f_bval, to_f32(op.bval))) ||
+        (rc = upload(c, &c->f_massfac, to_f32(op.massfac))) || (rc = upload(c, &c->f_ddiag, to_f32(op.ddiag))) ||
+        (rc = upload(c, &c->f_bdiag, to_f32(op.bdiag))))
+        return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->have32 = true;
+    return TR_OK;
+}
+
+// FP64 iterative refinement around an FP32 device CG:
+//   r = b - A64 x;  d = (A32)^-1 (r / |r|) to inner_tol;  x += |r| d
+static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
+    int rc = ensure_fp32(c);
+    if (rc) return rc;
+    Work w(c);
+    const size_t n = c->nint();
+    double *b = w.get(), *r = w.get(), *Ap = w.get();
+    float *v32 = nullptr;
+    const uint32_t nb = apply_blocks(c->op.nP, c->map32);
+    double *part = nullptr;
+    HIPCHECK(hipMalloc((void **)&part, sizeof(double) * (std::max<size_t>(nb, 3 * 1024) + 2 * 1024 + 16)));
+    w.v.push_back(part);
+    HIPCHECK(hipMalloc((void **)&v32, sizeof(float) * (6 * n + 8)));
+    w.v.push_back((double *)v32);
+    float *d32 = v32, *r32 = v32 + n, *p32 = v32 + 2 * n, *Ap32 = v32 + 3 * n, *b32 = v32 + 4 * n;
+    float *dinv32 = nullptr;
+    if (!Ap) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+    if (prm->use_prec == 2) {
+        std::vector<double> d = operator_diagonal(c->op);
+        std::vector<float> di(d.size());
+        for (size_t i = 0; i < d.size(); i++) di[i] = (float)(1.0 / d[i]);
+        dinv32 = v32 + 5 * n;
+        HIPCHECK(hipMemcpyAsync(dinv32, di.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    } else if (prm->use_prec != 0) {
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none) or 2 (Jacobi)");
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    if ((rc = tr_compute_rhs(c, b)) || (rc = axpby(c, f, 0, nullptr, 0))) return rc;
+    const double bn = std::sqrt(dot(c, b, b));
+    CgSpace<float> S;
+    S.a = c->args32();
+    S.NC = (int)c->op.NC;
+    S.map = c->map32;
+    S.n = n;
+    S.x = d32;
+    S.r = r32;
+    S.p = p32;
+    S.Ap = Ap32;
+    S.b = b32;
+    S.dinv = dinv32;
+    S.part = part;
+    S.part2 = part + std::max<size_t>(nb, 3 * 1024);
+    S.sc = S.part2 + 2 * 1024;
+    CgParams cp;
+    cp.maxit = prm->inner_maxit;
+    cp.tol_abs = 0;
+    cp.tol_rel = prm->inner_tol;
+    cp.chunk = 16;
+    int outer = 0, inner = 0, flag = TR_ERR_NOCONV;
+    double rn = bn;
+    for (; outer <= prm->maxit; outer++) {
+        if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+        rn = std::sqrt(dot(c, r, r));
+        if (!std::isfinite(rn)) break;
+        if (rn <= prm->tol_rel * bn || rn < prm->tol_abs) {
+            flag = TR_OK;
+            break;
+        }
+        if (outer == prm->maxit) break;
+        HIPCHECK(launch_to_f32_scaled(r, b32, 1.0 / rn, n, c->stream));
+        HIPCHECK(hipMemsetAsync(d32, 0, sizeof(float) * n, c->stream));
+        CgResult res;
+        if ((rc = pcg_device<float>(S, cp, c->stream, res))) return rc;
+        inner += res.iters;
+        HIPCHECK(launch_add_f32_scaled(f, d32, rn, n, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->iters = outer;
+        st->preck = inner;
+        st->rnorm = rn;
+        st->bnorm = bn;
+        st->true_rel = bn > 0 ? rn / bn : rn;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->status = flag;
+    }
+    return TR_OK;
+}
+}  // namespace tr
+
 extern "C" {
 
 int tr_solve(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st, double *hist) {
@@ -489,8 +702,11 @@ int tr_solve(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *s
     if (!c->vectors.count(f)) return fail(TR_ERR_ARG, "tr_solve: f must come from tr_alloc_vector");
     switch (prm->solver) {
         case TR_SOLVER_REFERENCE:
-        case TR_SOLVER_PCG_FP64:
             return solve_reference(c, prm, f, st, hist);
+        case TR_SOLVER_PCG_FP64:
+            return solve_device(c, prm, f, st);
+        case TR_SOLVER_MIXED:
+            return solve_mixed(c, prm, f, st);
         default:
             return fail(TR_ERR_ARG, "tr_solve: unknown solver");
     }
@@ -538,9 +754,8 @@ extern "C" int tr_bench_operator(tr_ctx *c, int warmup, int steps, double *out) 
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < steps; i++) {
         HIPCHECK(hipEventRecord(ev[3 * i], c->stream));
-        HIPCHECK(launch_lhs_volume<double>(c->vol(), (int)c->op.NC, c->map, f, r, c->stream));
+        HIPCHECK(launch_apply<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream));
         HIPCHECK(hipEventRecord(ev[3 * i + 1], c->stream));
-        HIPCHECK(launch_lhs_boundary<double>(c->bnd(), c->bmap, f, r, c->stream));
         HIPCHECK(hipEventRecord(ev[3 * i + 2], c->stream));
     }
     HIPCHECK(hipStreamSynchronize(c->stream));

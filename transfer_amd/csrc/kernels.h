@@ -14,9 +14,10 @@ struct LaneMap {
     int J = 1, L = 1, VPW = 1, WPB = 4;
 };
 
+// Device operator data (see stencil.h), precision T.
 template <typename T>
-struct VolArgs {
-    uint32_t nP, slm, W, arow;
+struct OpArgs {
+    uint32_t nP, slm, W, arow, nB, WB, brow;
     int L, VPW;
     const uint32_t *nbr;   // [W][nP]
     const T *coef;         // [W][NC][nP]
@@ -24,13 +25,7 @@ struct VolArgs {
     const T *aval;         // [arow][slm]
     const T *massfac;      // [slm]
     const T *ddiag;        // [3][slm]
-};
-
-template <typename T>
-struct BndArgs {
-    uint32_t nB, slm, WB, brow;
-    int L, VPW;
-    const uint32_t *bvert; // [nB]
+    const int32_t *bof;    // [nP] boundary-stencil row of each vertex, -1 inside
     const uint32_t *bnbr;  // [WB][nB]
     const T *bcoef;        // [WB][3][nB]
     const uint16_t *bidx;  // [brow][slm]
@@ -38,22 +33,18 @@ struct BndArgs {
     const T *bdiag;        // [3][slm]
 };
 
-// r = volume part (overwrite); NC = 7 or 10 channels
+// r = A f (volume and vacuum boundary in one pass).  With part != null each
+// workgroup also writes sum_v f_v . r_v (FP64) to part[blockIdx]; with
+// skip != null the launch does nothing when *skip != 0 (solver loops).
 template <typename T>
-hipError_t launch_lhs_volume(const VolArgs<T> &a, int NC, const LaneMap &m,
-                             const T *f, T *r, hipStream_t s);
-// r += vacuum-boundary part
+hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
+                        double *part, const double *skip, hipStream_t s);
+// r = D f, the angular-diagonal operator (kernels.cu:290-357, 440-493)
 template <typename T>
-hipError_t launch_lhs_boundary(const BndArgs<T> &a, const LaneMap &m, const T *f,
-                               T *r, hipStream_t s);
-template <typename T>
-hipError_t launch_diag_volume(const VolArgs<T> &a, int NC, const LaneMap &m,
-                              const T *f, T *r, hipStream_t s);
-template <typename T>
-hipError_t launch_diag_boundary(const BndArgs<T> &a, const LaneMap &m,
-                                const T *f, T *r, hipStream_t s);
-size_t lhs_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem);
-size_t bnd_lds_bytes(const LaneMap &m, uint32_t slm, size_t elem);
+hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
+                             double *part, const double *skip, hipStream_t s);
+size_t apply_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem);
+uint32_t apply_blocks(uint32_t nP, const LaneMap &m);
 
 // b[v][lm] = lm == 0 ? rhs[v] : 0
 hipError_t launch_rhs(const double *rhs, uint32_t nP, uint32_t slm, double *b,

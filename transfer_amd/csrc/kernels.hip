@@ -16,11 +16,37 @@ namespace tr {
 
 #define TR_WAVE 64
 
+#define TR_RED_BLOCKS 1024
+
+__device__ inline double wave_sum(double s) {
+#pragma unroll
+    for (int off = TR_WAVE / 2; off > 0; off >>= 1) s += __shfl_down(s, off, TR_WAVE);
+    return s;
+}
+
+// Sum over the workgroup (wavefront shuffles, then the wavefronts' totals in
+// a fixed order); the result is valid in thread 0.
+__device__ inline double block_sum(double s) {
+    __shared__ double w[16];
+    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
+    s = wave_sum(s);
+    if (lane == 0) w[wave] = s;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x / TR_WAVE); i++) t += w[i];
+    }
+    return t;
+}
+
 // ---------------------------------------------------------------------------
-// volume part: r = massfac * sum_k M_ik f_k + sum_c Omega_c (sum_k K^c_ik f_k)
-template <typename T, int J, int NC>
-__global__ __launch_bounds__(256) void k_lhs_volume(VolArgs<T> a, const T *__restrict__ f,
-                                                    T *__restrict__ r) {
+// r = massfac * sum_k M_ik f_k + sum_c Omega_c (sum_k K^c_ik f_k)
+//   + sum_n O_n (sum_k B^n_ik f_k)                       (boundary vertices)
+template <typename T, int J, int NC, bool DOT>
+__global__ __launch_bounds__(256) void k_apply(OpArgs<T> a, const T *__restrict__ f,
+                                               T *__restrict__ r, double *__restrict__ part,
+                                               const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
@@ -29,12 +55,17 @@ __global__ __launch_bounds__(256) void k_lhs_volume(VolArgs<T> a, const T *__res
     const uint32_t v = (blockIdx.x * (uint32_t)wpb + wave) * (uint32_t)a.VPW + vsub;
     const bool valid = vsub < a.VPW && v < a.nP;
     const int lm0 = lv * J;
-    const size_t nP = a.nP;
-    T acc[NC][J];
+    const size_t nP = a.nP, nB = a.nB;
+    const int b = valid ? a.bof[v] : -1;
+    T acc[NC][J], x0[J], bacc[3][J];
 #pragma unroll
-    for (int c = 0; c < NC; c++)
+    for (int j = 0; j < J; j++) {
+        x0[j] = 0;
 #pragma unroll
-        for (int j = 0; j < J; j++) acc[c][j] = 0;
+        for (int c = 0; c < NC; c++) acc[c][j] = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) bacc[c][j] = 0;
+    }
     if (valid) {
         for (uint32_t s = 0; s < a.W; s++) {
             const uint32_t k = a.nbr[s * nP + v];
@@ -42,6 +73,10 @@ __global__ __launch_bounds__(256) void k_lhs_volume(VolArgs<T> a, const T *__res
             T x[J];
 #pragma unroll
             for (int j = 0; j < J; j++) x[j] = fk[j];
+            if (s == 0) {
+#pragma unroll
+                for (int j = 0; j < J; j++) x0[j] = x[j];
+            }
             const T *cs = a.coef + (size_t)s * NC * nP + v;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
@@ -50,15 +85,34 @@ __global__ __launch_bounds__(256) void k_lhs_volume(VolArgs<T> a, const T *__res
                 for (int j = 0; j < J; j++) acc[c][j] = fma(cf, x[j], acc[c][j]);
             }
         }
+        if (b >= 0) {
+            for (uint32_t s = 0; s < a.WB; s++) {
+                const uint32_t k = a.bnbr[s * nB + b];
+                const T *fk = f + (size_t)k * a.slm + lm0;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const T cf = a.bcoef[((size_t)s * 3 + c) * nB + b];
+#pragma unroll
+                    for (int j = 0; j < J; j++) bacc[c][j] = fma(cf, fk[j], bacc[c][j]);
+                }
+            }
+        }
     }
-    T *my = lds + (size_t)(wave * a.VPW + vsub) * (NC - 1) * a.slm;
+    T *my = lds + (size_t)(wave * a.VPW + vsub) * (NC - 1 + 3) * a.slm;
     if (valid) {
 #pragma unroll
         for (int c = 1; c < NC; c++)
 #pragma unroll
             for (int j = 0; j < J; j++) my[(c - 1) * a.slm + lm0 + j] = acc[c][j];
+        if (b >= 0) {
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) my[(NC - 1 + c) * a.slm + lm0 + j] = bacc[c][j];
+        }
     }
     __syncthreads();
+    double d = 0;
     if (valid) {
         T R[J];
 #pragma unroll
@@ -68,200 +122,144 @@ __global__ __launch_bounds__(256) void k_lhs_volume(VolArgs<T> a, const T *__res
 #pragma unroll
             for (int j = 0; j < J; j++) R[j] = fma(a.aval[q + j], my[a.aidx[q + j]], R[j]);
         }
-        T *rv = r + (size_t)v * a.slm + lm0;
+        if (b >= 0) {
+            const T *mb = my + (NC - 1) * a.slm;
+            for (uint32_t e = 0; e < a.brow; e++) {
+                const size_t q = (size_t)e * a.slm + lm0;
 #pragma unroll
-        for (int j = 0; j < J; j++) rv[j] = R[j];
-    }
-}
-
-// vacuum boundary: r += sum_n O_n (sum_k B^n_ik f_k) over boundary vertices
-template <typename T, int J>
-__global__ __launch_bounds__(256) void k_lhs_boundary(BndArgs<T> a, const T *__restrict__ f,
-                                                      T *__restrict__ r) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
-    const int wpb = blockDim.x / TR_WAVE;
-    const int vsub = lane / a.L, lv = lane - vsub * a.L;
-    const uint32_t b = (blockIdx.x * (uint32_t)wpb + wave) * (uint32_t)a.VPW + vsub;
-    const bool valid = vsub < a.VPW && b < a.nB;
-    const int lm0 = lv * J;
-    const size_t nB = a.nB;
-    T acc[3][J];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int j = 0; j < J; j++) acc[c][j] = 0;
-    if (valid) {
-        for (uint32_t s = 0; s < a.WB; s++) {
-            const uint32_t k = a.bnbr[s * nB + b];
-            const T *fk = f + (size_t)k * a.slm + lm0;
-            T x[J];
-#pragma unroll
-            for (int j = 0; j < J; j++) x[j] = fk[j];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const T cf = a.bcoef[((size_t)s * 3 + c) * nB + b];
-#pragma unroll
-                for (int j = 0; j < J; j++) acc[c][j] = fma(cf, x[j], acc[c][j]);
+                for (int j = 0; j < J; j++) R[j] = fma(a.bval[q + j], mb[a.bidx[q + j]], R[j]);
             }
         }
-    }
-    T *my = lds + (size_t)(wave * a.VPW + vsub) * 3 * a.slm;
-    if (valid) {
+        T *rv = r + (size_t)v * a.slm + lm0;
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-            for (int j = 0; j < J; j++) my[c * a.slm + lm0 + j] = acc[c][j];
-    }
-    __syncthreads();
-    if (valid) {
-        T R[J];
-#pragma unroll
-        for (int j = 0; j < J; j++) R[j] = 0;
-        for (uint32_t e = 0; e < a.brow; e++) {
-            const size_t q = (size_t)e * a.slm + lm0;
-#pragma unroll
-            for (int j = 0; j < J; j++) R[j] = fma(a.bval[q + j], my[a.bidx[q + j]], R[j]);
+        for (int j = 0; j < J; j++) {
+            rv[j] = R[j];
+            if (DOT) d = fma((double)x0[j], (double)R[j], d);
         }
-        T *rv = r + (size_t)a.bvert[b] * a.slm + lm0;
-#pragma unroll
-        for (int j = 0; j < J; j++) rv[j] += R[j];
+    }
+    if (DOT) {
+        d = block_sum(d);
+        if (threadIdx.x == 0) part[blockIdx.x] = d;
     }
 }
 
-// angular-diagonal operator (kernels.cu:290-357): per lane and slot the
-// coupling massfac*M + sum_a Omega^aa[lm][lm] K^aa
-template <typename T, int J>
-__global__ __launch_bounds__(256) void k_diag_volume(VolArgs<T> a, int NC, const T *__restrict__ f,
-                                                     T *__restrict__ r) {
+// angular-diagonal operator: per lane and slot the coupling
+// massfac*M + sum_a Omega^aa[lm][lm] K^aa (+ sum_n O_n[lm][lm] B^n)
+template <typename T, int J, bool DOT>
+__global__ __launch_bounds__(256) void k_apply_diag(OpArgs<T> a, int NC, const T *__restrict__ f,
+                                                    T *__restrict__ r, double *__restrict__ part,
+                                                    const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
     const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
     const int wpb = blockDim.x / TR_WAVE;
     const int vsub = lane / a.L, lv = lane - vsub * a.L;
     const uint32_t v = (blockIdx.x * (uint32_t)wpb + wave) * (uint32_t)a.VPW + vsub;
-    if (!(vsub < a.VPW && v < a.nP)) return;
+    const bool valid = vsub < a.VPW && v < a.nP;
     const int lm0 = lv * J;
-    const size_t nP = a.nP;
-    T mf[J], d0[J], d1[J], d2[J], acc[J];
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        mf[j] = a.massfac[lm0 + j];
-        d0[j] = a.ddiag[lm0 + j];
-        d1[j] = a.ddiag[a.slm + lm0 + j];
-        d2[j] = a.ddiag[2 * a.slm + lm0 + j];
-        acc[j] = 0;
-    }
-    for (uint32_t s = 0; s < a.W; s++) {
-        const uint32_t k = a.nbr[s * nP + v];
-        const T *fk = f + (size_t)k * a.slm + lm0;
-        const T *cs = a.coef + (size_t)s * NC * nP + v;
-        const T c0 = cs[0], c1 = cs[nP], c2 = cs[2 * nP], c3 = cs[3 * nP];
+    const size_t nP = a.nP, nB = a.nB;
+    double d = 0;
+    if (valid) {
+        T mf[J], d0[J], d1[J], d2[J], acc[J], x0[J];
 #pragma unroll
         for (int j = 0; j < J; j++) {
-            T e = mf[j] * c0 + d0[j] * c1 + d1[j] * c2 + d2[j] * c3;
-            acc[j] = fma(e, fk[j], acc[j]);
+            mf[j] = a.massfac[lm0 + j];
+            d0[j] = a.ddiag[lm0 + j];
+            d1[j] = a.ddiag[a.slm + lm0 + j];
+            d2[j] = a.ddiag[2 * a.slm + lm0 + j];
+            acc[j] = 0;
+            x0[j] = 0;
+        }
+        for (uint32_t s = 0; s < a.W; s++) {
+            const uint32_t k = a.nbr[s * nP + v];
+            const T *fk = f + (size_t)k * a.slm + lm0;
+            const T *cs = a.coef + (size_t)s * NC * nP + v;
+            const T c0 = cs[0], c1 = cs[nP], c2 = cs[2 * nP], c3 = cs[3 * nP];
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                const T x = fk[j];
+                if (s == 0) x0[j] = x;
+                acc[j] = fma(mf[j] * c0 + d0[j] * c1 + d1[j] * c2 + d2[j] * c3, x, acc[j]);
+            }
+        }
+        const int b = a.bof[v];
+        if (b >= 0) {
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                d0[j] = a.bdiag[lm0 + j];
+                d1[j] = a.bdiag[a.slm + lm0 + j];
+                d2[j] = a.bdiag[2 * a.slm + lm0 + j];
+            }
+            for (uint32_t s = 0; s < a.WB; s++) {
+                const uint32_t k = a.bnbr[s * nB + b];
+                const T *fk = f + (size_t)k * a.slm + lm0;
+                const T c0 = a.bcoef[((size_t)s * 3 + 0) * nB + b];
+                const T c1 = a.bcoef[((size_t)s * 3 + 1) * nB + b];
+                const T c2 = a.bcoef[((size_t)s * 3 + 2) * nB + b];
+#pragma unroll
+                for (int j = 0; j < J; j++) acc[j] = fma(d0[j] * c0 + d1[j] * c1 + d2[j] * c2, fk[j], acc[j]);
+            }
+        }
+        T *rv = r + (size_t)v * a.slm + lm0;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            rv[j] = acc[j];
+            if (DOT) d = fma((double)x0[j], (double)acc[j], d);
         }
     }
-    T *rv = r + (size_t)v * a.slm + lm0;
-#pragma unroll
-    for (int j = 0; j < J; j++) rv[j] = acc[j];
-}
-
-// boundary diagonal (kernels.cu:440-493): O_n[lm][lm] only
-template <typename T, int J>
-__global__ __launch_bounds__(256) void k_diag_boundary(BndArgs<T> a, const T *__restrict__ f,
-                                                       T *__restrict__ r) {
-    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
-    const int wpb = blockDim.x / TR_WAVE;
-    const int vsub = lane / a.L, lv = lane - vsub * a.L;
-    const uint32_t b = (blockIdx.x * (uint32_t)wpb + wave) * (uint32_t)a.VPW + vsub;
-    if (!(vsub < a.VPW && b < a.nB)) return;
-    const int lm0 = lv * J;
-    const size_t nB = a.nB;
-    T d0[J], d1[J], d2[J], acc[J];
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        d0[j] = a.bdiag[lm0 + j];
-        d1[j] = a.bdiag[a.slm + lm0 + j];
-        d2[j] = a.bdiag[2 * a.slm + lm0 + j];
-        acc[j] = 0;
+    if (DOT) {
+        d = block_sum(d);
+        if (threadIdx.x == 0) part[blockIdx.x] = d;
     }
-    for (uint32_t s = 0; s < a.WB; s++) {
-        const uint32_t k = a.bnbr[s * nB + b];
-        const T *fk = f + (size_t)k * a.slm + lm0;
-        const T c0 = a.bcoef[((size_t)s * 3 + 0) * nB + b];
-        const T c1 = a.bcoef[((size_t)s * 3 + 1) * nB + b];
-        const T c2 = a.bcoef[((size_t)s * 3 + 2) * nB + b];
-#pragma unroll
-        for (int j = 0; j < J; j++) acc[j] = fma(d0[j] * c0 + d1[j] * c1 + d2[j] * c2, fk[j], acc[j]);
-    }
-    T *rv = r + (size_t)a.bvert[b] * a.slm + lm0;
-#pragma unroll
-    for (int j = 0; j < J; j++) rv[j] += acc[j];
 }
 
 // ---------------------------------------------------------------------------
-size_t lhs_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem) {
-    return (size_t)m.WPB * m.VPW * (NC - 1) * slm * elem;
-}
-size_t bnd_lds_bytes(const LaneMap &m, uint32_t slm, size_t elem) {
-    return (size_t)m.WPB * m.VPW * 3 * slm * elem;
+size_t apply_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem) {
+    return (size_t)m.WPB * m.VPW * (NC - 1 + 3) * slm * elem;
 }
 
-static inline dim3 grid_for(uint32_t n, const LaneMap &m) {
+uint32_t apply_blocks(uint32_t nP, const LaneMap &m) {
     uint32_t per = (uint32_t)(m.WPB * m.VPW);
-    return dim3((n + per - 1) / per);
+    return (nP + per - 1) / per;
 }
 
 #define TR_J_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(11)
 
 template <typename T>
-hipError_t launch_lhs_volume(const VolArgs<T> &a, int NC, const LaneMap &m, const T *f,
-                             T *r, hipStream_t s) {
+hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
+                        double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
-    dim3 g = grid_for(a.nP, m), b(TR_WAVE * m.WPB);
-    size_t sh = lhs_lds_bytes(m, NC, a.slm, sizeof(T));
-#define TR_CASE(JJ)                                                                  \
-    case JJ:                                                                         \
-        if (NC == 7)                                                                 \
-            hipLaunchKernelGGL((k_lhs_volume<T, JJ, 7>), g, b, sh, s, a, f, r);       \
-        else                                                                         \
-            hipLaunchKernelGGL((k_lhs_volume<T, JJ, 10>), g, b, sh, s, a, f, r);      \
+    dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
+    size_t sh = apply_lds_bytes(m, NC, a.slm, sizeof(T));
+#define TR_LAUNCH(JJ, NCC, D) \
+    hipLaunchKernelGGL((k_apply<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip)
+#define TR_CASE(JJ)                                                        \
+    case JJ:                                                               \
+        if (NC == 7) {                                                     \
+            if (part) TR_LAUNCH(JJ, 7, true); else TR_LAUNCH(JJ, 7, false);   \
+        } else {                                                           \
+            if (part) TR_LAUNCH(JJ, 10, true); else TR_LAUNCH(JJ, 10, false); \
+        }                                                                  \
         break;
     switch (m.J) {
         TR_J_CASES(TR_CASE)
         default: return hipErrorInvalidValue;
     }
 #undef TR_CASE
+#undef TR_LAUNCH
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_lhs_boundary(const BndArgs<T> &a, const LaneMap &m, const T *f, T *r,
-                               hipStream_t s) {
-    if (a.nB == 0) return hipSuccess;
-    dim3 g = grid_for(a.nB, m), b(TR_WAVE * m.WPB);
-    size_t sh = bnd_lds_bytes(m, a.slm, sizeof(T));
-#define TR_CASE(JJ)                                                          \
-    case JJ:                                                                 \
-        hipLaunchKernelGGL((k_lhs_boundary<T, JJ>), g, b, sh, s, a, f, r);   \
-        break;
-    switch (m.J) {
-        TR_J_CASES(TR_CASE)
-        default: return hipErrorInvalidValue;
-    }
-#undef TR_CASE
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_diag_volume(const VolArgs<T> &a, int NC, const LaneMap &m, const T *f,
-                              T *r, hipStream_t s) {
+hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
+                             double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
-    dim3 g = grid_for(a.nP, m), b(TR_WAVE * m.WPB);
-#define TR_CASE(JJ)                                                            \
-    case JJ:                                                                   \
-        hipLaunchKernelGGL((k_diag_volume<T, JJ>), g, b, 0, s, a, NC, f, r);   \
+    dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
+#define TR_CASE(JJ)                                                                          \
+    case JJ:                                                                                 \
+        if (part)                                                                            \
+            hipLaunchKernelGGL((k_apply_diag<T, JJ, true>), g, b, 0, s, a, NC, f, r, part, skip);  \
+        else                                                                                 \
+            hipLaunchKernelGGL((k_apply_diag<T, JJ, false>), g, b, 0, s, a, NC, f, r, part, skip); \
         break;
     switch (m.J) {
         TR_J_CASES(TR_CASE)
@@ -271,39 +269,16 @@ hipError_t launch_diag_volume(const VolArgs<T> &a, int NC, const LaneMap &m, con
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_diag_boundary(const BndArgs<T> &a, const LaneMap &m, const T *f, T *r,
-                                hipStream_t s) {
-    if (a.nB == 0) return hipSuccess;
-    dim3 g = grid_for(a.nB, m), b(TR_WAVE * m.WPB);
-#define TR_CASE(JJ)                                                           \
-    case JJ:                                                                  \
-        hipLaunchKernelGGL((k_diag_boundary<T, JJ>), g, b, 0, s, a, f, r);    \
-        break;
-    switch (m.J) {
-        TR_J_CASES(TR_CASE)
-        default: return hipErrorInvalidValue;
-    }
-#undef TR_CASE
-    return hipGetLastError();
-}
-
-template hipError_t launch_lhs_volume<double>(const VolArgs<double> &, int, const LaneMap &,
-                                              const double *, double *, hipStream_t);
-template hipError_t launch_lhs_volume<float>(const VolArgs<float> &, int, const LaneMap &,
-                                             const float *, float *, hipStream_t);
-template hipError_t launch_lhs_boundary<double>(const BndArgs<double> &, const LaneMap &,
-                                                const double *, double *, hipStream_t);
-template hipError_t launch_lhs_boundary<float>(const BndArgs<float> &, const LaneMap &,
-                                               const float *, float *, hipStream_t);
-template hipError_t launch_diag_volume<double>(const VolArgs<double> &, int, const LaneMap &,
-                                               const double *, double *, hipStream_t);
-template hipError_t launch_diag_volume<float>(const VolArgs<float> &, int, const LaneMap &,
-                                              const float *, float *, hipStream_t);
-template hipError_t launch_diag_boundary<double>(const BndArgs<double> &, const LaneMap &,
-                                                 const double *, double *, hipStream_t);
-template hipError_t launch_diag_boundary<float>(const BndArgs<float> &, const LaneMap &,
-                                                const float *, float *, hipStream_t);
+template hipError_t launch_apply<double>(const OpArgs<double> &, int, const LaneMap &, const double *,
+                                         double *, double *, const double *, hipStream_t);
+template hipError_t launch_apply<float>(const OpArgs<float> &, int, const LaneMap &, const float *,
+                                        float *, double *, const double *, hipStream_t);
+template hipError_t launch_apply_diag<double>(const OpArgs<double> &, int, const LaneMap &,
+                                              const double *, double *, double *, const double *,
+                                              hipStream_t);
+template hipError_t launch_apply_diag<float>(const OpArgs<float> &, int, const LaneMap &,
+                                             const float *, float *, double *, const double *,
+                                             hipStream_t);
 
 // ---------------------------------------------------------------------------
 __global__ void k_rhs(const double *__restrict__ rhs, uint32_t nP, uint32_t slm,
@@ -338,31 +313,8 @@ hipError_t launch_axpby(double *x, double wx, const double *y, double wy, size_t
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Deterministic FP64 reductions: per-workgroup partials (wavefront shuffle
-// reduction, then across the workgroup's wavefronts through LDS), summed in a
-// fixed order by one workgroup.
-#define TR_RED_BLOCKS 1024
-
-__device__ inline double wave_sum(double s) {
-#pragma unroll
-    for (int off = TR_WAVE / 2; off > 0; off >>= 1) s += __shfl_down(s, off, TR_WAVE);
-    return s;
-}
-
-__device__ inline double block_sum(double s) {
-    __shared__ double w[16];
-    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
-    s = wave_sum(s);
-    if (lane == 0) w[wave] = s;
-    __syncthreads();
-    double t = 0;
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < (int)(blockDim.x / TR_WAVE); i++) t += w[i];
-    }
-    return t;
-}
-
+// Deterministic FP64 reductions: per-workgroup partials, summed in a fixed
+// order by one workgroup.
 __global__ __launch_bounds__(256) void k_dot_part(const double *__restrict__ x,
                                                   const double *__restrict__ y, size_t n,
                                                   double *__restrict__ part) {

@@ -1,0 +1,59 @@
+// solver.h -- device-resident CG (solver.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace tr {
+
+// slots of the device scalar array
+enum {
+    SC_RZ = 0,
+    SC_PAP = 1,
+    SC_ALPHA = 2,
+    SC_RR = 3,
+    SC_BETA = 4,
+    SC_BB = 5,
+    SC_THR = 6,
+    SC_CONV = 7,  // 0 running, 1 converged, 2 iteration cap, 3 breakdown
+    SC_ITER = 8,
+    SC_N = 16
+};
+
+// Work space of one CG solve of A x = b in precision T.  All device
+// pointers; dinv may be null (no preconditioner).
+template <typename T>
+struct CgSpace {
+    OpArgs<T> a;
+    int NC;
+    LaneMap map;
+    size_t n;          // nP * slm
+    T *x, *r, *p, *Ap;
+    const T *b, *dinv;
+    double *part;      // >= max(apply_blocks, 3*1024) doubles
+    double *part2;     // >= 2*1024 doubles
+    double *sc;        // SC_N doubles
+};
+
+struct CgParams {
+    int maxit = 1000;
+    double tol_abs = 0, tol_rel = 1e-8;
+    int chunk = 16;
+    int diag_op = 0;   // 1: iterate with the angular-diagonal operator
+};
+
+struct CgResult {
+    int iters = 0, flag = 0;
+    double rr = 0, bb = 0, seconds = 0;
+};
+
+// Solves from the x held on entry.
+template <typename T>
+int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResult &res);
+
+// dst = (float)(src * s);  x += s * (double)d
+hipError_t launch_to_f32_scaled(const double *src, float *dst, double s, size_t n, hipStream_t st);
+hipError_t launch_add_f32_scaled(double *x, const float *d, double s, size_t n, hipStream_t st);
+
+}  // namespace tr

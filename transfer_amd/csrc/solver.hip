@@ -1,0 +1,284 @@
+// solver.hip -- device-resident preconditioned CG.
+//
+// The reference drives CG from the host (main.cpp:125-162): every dot
+// product and norm ends in a blocking device-to-host copy of one REAL
+// (CudaContext.cpp:227-250).  Here alpha, beta and the residual norm stay
+// in a small device array; one iteration is five launches
+//   apply A p (+ per-workgroup p.Ap)  ->  alpha = rz / p.Ap
+//   x += alpha p, r -= alpha Ap, z = M r (+ r.r, r.z partials)
+//   beta = rz_new / rz, convergence test  ->  p = z + beta p
+// captured once into a HIP graph of `chunk` iterations; the host reads the
+// convergence flag once per graph launch.  After convergence every kernel
+// of the remaining chunk returns at its first instruction.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+#include "solver.h"
+
+namespace tr {
+
+#define TR_WAVE 64
+#define TR_UPD_BLOCKS 1024
+
+__device__ inline double wave_sum2(double s) {
+#pragma unroll
+    for (int off = TR_WAVE / 2; off > 0; off >>= 1) s += __shfl_down(s, off, TR_WAVE);
+    return s;
+}
+
+// Two workgroup sums at once; results valid in thread 0.
+__device__ inline void block_sum2(double &a, double &b) {
+    __shared__ double w[2][16];
+    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
+    a = wave_sum2(a);
+    b = wave_sum2(b);
+    if (lane == 0) {
+        w[0][wave] = a;
+        w[1][wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0, t = 0;
+        for (int i = 0; i < (int)(blockDim.x / TR_WAVE); i++) {
+            s += w[0][i];
+            t += w[1][i];
+        }
+        a = s;
+        b = t;
+    }
+}
+
+// sum of part[0..n) and part[n2..n2+n) in a fixed order (one workgroup)
+__device__ inline void sum_parts(const double *part, int n, int n2, double &a, double &b) {
+    a = 0;
+    b = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a += part[i];
+        if (n2 >= 0) b += part[n2 + i];
+    }
+    block_sum2(a, b);
+}
+
+__global__ __launch_bounds__(256) void k_fin_alpha(const double *__restrict__ part, int n,
+                                                   double *__restrict__ sc) {
+    if (sc[SC_CONV] != 0.0) return;
+    double pap, dummy;
+    sum_parts(part, n, -1, pap, dummy);
+    if (threadIdx.x == 0) {
+        sc[SC_PAP] = pap;
+        sc[SC_ALPHA] = sc[SC_RZ] / pap;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(T *__restrict__ x, const T *__restrict__ p,
+                                                T *__restrict__ r, const T *__restrict__ Ap,
+                                                const T *__restrict__ dinv, size_t n,
+                                                const double *__restrict__ sc,
+                                                double *__restrict__ part) {
+    if (sc[SC_CONV] != 0.0) return;
+    const T alpha = (T)sc[SC_ALPHA];
+    double rr = 0, rz = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        x[i] = fma(alpha, p[i], x[i]);
+        const T ri = fma(-alpha, Ap[i], r[i]);
+        r[i] = ri;
+        const T zi = dinv ? dinv[i] * ri : ri;
+        rr = fma((double)ri, (double)ri, rr);
+        rz = fma((double)ri, (double)zi, rz);
+    }
+    block_sum2(rr, rz);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = rr;
+        part[TR_UPD_BLOCKS + blockIdx.x] = rz;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fin_beta(const double *__restrict__ part, int n,
+                                                  double *__restrict__ sc, int maxit) {
+    if (sc[SC_CONV] != 0.0) return;
+    double rr, rz;
+    sum_parts(part, n, TR_UPD_BLOCKS, rr, rz);
+    if (threadIdx.x == 0) {
+        sc[SC_RR] = rr;
+        sc[SC_BETA] = rz / sc[SC_RZ];
+        sc[SC_RZ] = rz;
+        sc[SC_ITER] += 1.0;
+        if (!(rr == rr) || !(rz == rz))
+            sc[SC_CONV] = 3.0;  // breakdown: NaN
+        else if (rr <= sc[SC_THR])
+            sc[SC_CONV] = 1.0;  // converged
+        else if (sc[SC_ITER] >= (double)maxit)
+            sc[SC_CONV] = 2.0;  // iteration cap
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pupdate(T *__restrict__ p, const T *__restrict__ r,
+                                                 const T *__restrict__ dinv, size_t n,
+                                                 const double *__restrict__ sc) {
+    if (sc[SC_CONV] != 0.0) return;
+    const T beta = (T)sc[SC_BETA];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const T zi = dinv ? dinv[i] * r[i] : r[i];
+        p[i] = fma(beta, p[i], zi);
+    }
+}
+
+// r = b - r (r holds A x on entry); z = M r; p = z; partials of r.r, r.z, b.b
+template <typename T>
+__global__ __launch_bounds__(256) void k_init(const T *__restrict__ b, T *__restrict__ r,
+                                              T *__restrict__ p, const T *__restrict__ dinv,
+                                              size_t n, double *__restrict__ part) {
+    double rr = 0, rz = 0, bb = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const T ri = b[i] - r[i];
+        r[i] = ri;
+        const T zi = dinv ? dinv[i] * ri : ri;
+        p[i] = zi;
+        rr = fma((double)ri, (double)ri, rr);
+        rz = fma((double)ri, (double)zi, rz);
+        bb = fma((double)b[i], (double)b[i], bb);
+    }
+    block_sum2(rr, rz);
+    double bz = 0;
+    __syncthreads();
+    block_sum2(bb, bz);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = rr;
+        part[TR_UPD_BLOCKS + blockIdx.x] = rz;
+        part[2 * TR_UPD_BLOCKS + blockIdx.x] = bb;
+    }
+}
+
+// sc from the init partials: rr, rz, bb, and the stopping threshold
+__global__ __launch_bounds__(256) void k_fin_init(const double *__restrict__ part, int n,
+                                                  double *__restrict__ sc, double tol_abs,
+                                                  double tol_rel) {
+    double rr, rz, bb, dummy;
+    sum_parts(part, n, TR_UPD_BLOCKS, rr, rz);
+    __syncthreads();
+    sum_parts(part + 2 * TR_UPD_BLOCKS, n, -1, bb, dummy);
+    if (threadIdx.x == 0) {
+        sc[SC_RR] = rr;
+        sc[SC_RZ] = rz;
+        sc[SC_BB] = bb;
+        double thr = tol_abs * tol_abs;
+        if (tol_rel > 0) thr = fmax(thr, tol_rel * tol_rel * bb);
+        sc[SC_THR] = thr;
+        sc[SC_ITER] = 0;
+        sc[SC_CONV] = (rr <= thr) ? 1.0 : 0.0;
+    }
+}
+
+static unsigned upd_grid(size_t n) {
+    return (unsigned)std::min<size_t>(std::max<size_t>((n + 1023) / 1024, 1), TR_UPD_BLOCKS);
+}
+
+template <typename T>
+int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResult &res) {
+    const size_t n = S.n;
+    const unsigned G = upd_grid(n);
+    const uint32_t nb = apply_blocks(S.a.nP, S.map);
+    auto t0 = std::chrono::steady_clock::now();
+    hipError_t e;
+#define HC(x)                                                                               \
+    do {                                                                                    \
+        e = (x);                                                                            \
+        if (e != hipSuccess) return fail(TR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e)); \
+    } while (0)
+    // r = b - A x ; z = M r ; p = z
+    if (prm.diag_op)
+        HC(launch_apply_diag<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
+    else
+        HC(launch_apply<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
+    hipLaunchKernelGGL(k_init<T>, dim3(G), dim3(256), 0, st, S.b, S.r, S.p, S.dinv, n, S.part);
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, st, S.part, (int)G, S.sc, prm.tol_abs, prm.tol_rel);
+    HC(hipGetLastError());
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const int chunk = std::max(1, prm.chunk);
+    HC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < chunk; it++) {
+        if (prm.diag_op)
+            launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
+        else
+            launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, st, S.part, (int)nb, S.sc);
+        hipLaunchKernelGGL(k_update<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap, S.dinv, n, S.sc,
+                           S.part2);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(256), 0, st, S.part2, (int)G, S.sc, prm.maxit);
+        hipLaunchKernelGGL(k_pupdate<T>, dim3(G), dim3(256), 0, st, S.p, S.r, S.dinv, n, S.sc);
+    }
+    e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        return fail(TR_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    double sc[SC_N];
+    int launches = 0;
+    for (;;) {
+        HC(hipMemcpyAsync(sc, S.sc, sizeof(sc), hipMemcpyDeviceToHost, st));
+        HC(hipStreamSynchronize(st));
+        if (sc[SC_CONV] != 0.0) break;
+        if (launches * chunk > prm.maxit + chunk) break;  // defensive: the flag caps earlier
+        HC(hipGraphLaunch(exec, st));
+        launches++;
+    }
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    auto t1 = std::chrono::steady_clock::now();
+    res.iters = (int)sc[SC_ITER];
+    res.rr = sc[SC_RR];
+    res.bb = sc[SC_BB];
+    res.flag = (int)sc[SC_CONV];
+    res.seconds = std::chrono::duration<double>(t1 - t0).count();
+    return TR_OK;
+#undef HC
+}
+
+template int pcg_device<double>(const CgSpace<double> &, const CgParams &, hipStream_t, CgResult &);
+template int pcg_device<float>(const CgSpace<float> &, const CgParams &, hipStream_t, CgResult &);
+
+}  // namespace tr
+
+namespace tr {
+
+__global__ void k_to_f32_scaled(const double *__restrict__ src, float *__restrict__ dst,
+                                double s, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (float)(src[i] * s);
+}
+
+__global__ void k_add_f32_scaled(double *__restrict__ x, const float *__restrict__ d, double s,
+                                 size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        x[i] = fma(s, (double)d[i], x[i]);
+}
+
+hipError_t launch_to_f32_scaled(const double *src, float *dst, double s, size_t n, hipStream_t st) {
+    unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_to_f32_scaled, dim3(g), dim3(256), 0, st, src, dst, s, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_f32_scaled(double *x, const float *d, double s, size_t n, hipStream_t st) {
+    unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_add_f32_scaled, dim3(g), dim3(256), 0, st, x, d, s, n);
+    return hipGetLastError();
+}
+
+}  // namespace tr

@@ -271,6 +271,8 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
         }
     }
     op.nB = (uint32_t)op.bvert.size();
+    op.bof.assign(nP, -1);
+    for (uint32_t b = 0; b < op.nB; b++) op.bof[op.bvert[b]] = (int32_t)b;
     op.WB = 1;
     for (auto &s : bsl) op.WB = std::max<uint32_t>(op.WB, (uint32_t)s.size());
     op.bnbr.assign((size_t)op.WB * std::max<uint32_t>(op.nB, 1), 0);

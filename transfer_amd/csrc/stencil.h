@@ -41,6 +41,7 @@ struct HostOperator {
     // vacuum-boundary stencil over the boundary vertices only
     uint32_t nB = 0, WB = 0;
     std::vector<uint32_t> bvert;  // [nB]
+    std::vector<int32_t> bof;     // [nP] boundary row of each vertex, -1 inside
     std::vector<uint32_t> bnbr;   // [WB][nB]
     std::vector<double> bcoef;    // [WB][3][nB]
     // angular tables: for output lm, the padded list of (channel, lm')
