@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also time a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -72,6 +74,13 @@ def main():
     ctx = ta.HipContext(P.mesh, P.ang)
     nP, nT, slm = P.mesh.nP, P.mesh.nT, P.ang.slm
     print("[bench] %s nP=%d nT=%d nF=%d slm=%d" % (P.name, nP, nT, P.mesh.nF, slm), file=sys.stderr, flush=True)
+    tune = ctx.autotune(0)
+    sched = ctx.schedule(0)
+    print("[bench] autotuned FP64 schedule %s" % sched, file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        cms = ctx.bench_copy(256 << 20, 10)
+        print("[bench] copy 256 MiB: %.3f ms, %.0f GB/s (read+write)" % (cms, 2 * (256 << 20) / cms / 1e6),
+              file=sys.stderr, flush=True)
     wall, vol_ms, bnd_ms, step_ms = ctx.bench_operator(args.warmup, args.steps)
     value = nP * slm * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
@@ -85,22 +94,35 @@ def main():
                config=dict(workload="%s: ball of 64^3-grid cubes, K=3 (slm 28), kappa=50exp(-r/0.2), "
                                     "I_p=(1-0.8r)^4, FP64 operator apply" % P.name,
                            nP=nP, nT=nT, nF=P.mesh.nF, slm=slm))
+    kname = "k_apply%s<double,%d,7,false>" % ("2" if sched["variant"] == 2 else "", sched["J"])
     out["roofline"] = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                            frac=achieved / HBM_PEAK_GBS, traffic=None,
-                           kernel="k_lhs_volume<double,4,7>", kernel_ms=vol_ms,
-                           algorithmic_bytes=vbytes)
+                           kernel=kname, kernel_ms=vol_ms, algorithmic_bytes=vbytes)
+    out["schedule_fp64"] = sched
     if not args.no_solve:
         solves = {}
-        for name, prec in (("pcg_fp64_plain", False),):
+        runs = [("reference_pcg_fp64_host", ta.TR_SOLVER_REFERENCE, 0, {}),
+                ("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
+                ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
+                ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(inner_tol=1e-4, maxit=30))]
+        for name, solver, prec, kw in runs:
+            if solver == ta.TR_SOLVER_MIXED:
+                ctx.autotune(1)
+                out["schedule_fp32"] = ctx.schedule(1)
             d_f = ctx.allocVector()
-            st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_REFERENCE, use_prec=prec, maxit=20000,
-                              tol_abs=0.0, tol_rel=1e-8)
-            solves[name] = dict(seconds=st.seconds, iters=st.iters, preck=st.preck, true_rel=st.true_rel,
+            p = dict(maxit=20000, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000)
+            p.update(kw)
+            st, _ = ctx.solve(d_f, solver=solver, use_prec=prec, **p)
+            solves[name] = dict(seconds=st.seconds, iters=st.iters, inner_iters=st.preck, true_rel=st.true_rel,
                                 converged=st.status == 0)
             ctx.deviceFree(d_f)
-            print("[bench] %s: %.3f s, %d iters, true rel %.2e" % (name, st.seconds, st.iters, st.true_rel),
-                  file=sys.stderr, flush=True)
+            print("[bench] %s: %.4f s, %d iters (%d inner), true rel %.2e" % (name, st.seconds, st.iters, st.preck,
+                                                                            st.true_rel), file=sys.stderr, flush=True)
         out["time_to_1e-8"] = solves
+        plain = min(v["seconds"] for k, v in solves.items() if "fp64" in k and v["converged"])
+        mixed = solves["mixed_fp32_ir_jacobi"]
+        if mixed["converged"]:
+            out["speedup_mixed_vs_best_plain_fp64"] = plain / mixed["seconds"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P)
         print("[bench] cpu baseline %.3g DOF*ordinate/s" % out["cpu_baseline"]["value"], file=sys.stderr, flush=True)

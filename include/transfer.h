@@ -172,6 +172,18 @@ uint32_t tr_ctx_N(const tr_ctx *ctx);          /* N() = nP*aslm :119-121 */
 uint32_t tr_ctx_slm(const tr_ctx *ctx);
 uint32_t tr_ctx_aslm(const tr_ctx *ctx);
 uint32_t tr_ctx_nP(const tr_ctx *ctx);
+/* Autotuner: times every candidate schedule of the operator kernel
+ * (harmonics per lane, wavefronts per workgroup, LDS staging scheme) on
+ * this context's mesh and keeps the fastest.  precision 0 = FP64 operator,
+ * 1 = FP32 operator (mixed solves).  report (may be NULL) gets one line per
+ * candidate.  No reference counterpart (the reference launches fixed
+ * geometry, CudaContext.cpp:160-172). */
+int tr_ctx_autotune(tr_ctx *ctx, int precision, char *report, size_t cap);
+/* The schedule in use: out5 = {J, L, VPW, WPB, variant} */
+int tr_ctx_schedule(const tr_ctx *ctx, int precision, int *out5);
+/* Forces a schedule (J harmonics per lane, WPB wavefronts per workgroup,
+ * variant 1 = whole-wavefront LDS staging, 2 = batched staging). */
+int tr_ctx_set_schedule(tr_ctx *ctx, int precision, int J, int WPB, int variant);
 /* Returns the internal numbering: perm[new] = caller vertex id. */
 int tr_ctx_permutation(const tr_ctx *ctx, uint32_t *perm);
 
@@ -246,6 +258,9 @@ double tr_bench_lhs(tr_ctx *ctx, int reps);
  * (device-synchronised on both sides), out[1] mean volume-kernel ms,
  * out[2] mean boundary-kernel ms, out[3] mean ms per application. */
 int tr_bench_operator(tr_ctx *ctx, int warmup, int steps, double *out4);
+/* A plain device copy of `bytes` timed with HIP events (out[0] = ms per
+ * copy): the calibration kernel for memory-traffic counters. */
+int tr_bench_copy(tr_ctx *ctx, size_t bytes, int reps, double *out1);
 
 /* ------------------------------------------------------------------------ */
 /* Introspection of the assembled operator data (host only, no device work). */

@@ -56,11 +56,18 @@ def test_lhs_lhsdiag_rhs_parity(name):
     assert ctx.N() == op.N()
     f = rvec(op, 1)
     d_f, d_r = ctx.vector_from(f), ctx.allocVector()
+    want = op.lhs(f)
+    default = ctx.schedule()
+    for variant in (1, 2):
+        for J in sorted({default["J"], 1 if op.slm <= 64 else default["J"]}):
+            ctx.set_schedule(0, J=J, WPB=4, variant=variant)
+            ctx.computeLhs(d_f, d_r)
+            r = ctx.copyToHost(d_r)
+            print("[gpu-parity]", name, "J=%d variant=%d lhs rel err %.2e" % (J, variant, rel(r, want)), flush=True)
+            assert rel(r, want) <= TOL
+    ctx.set_schedule(0, J=default["J"], WPB=default["WPB"], variant=default["variant"])
     ctx.computeLhs(d_f, d_r)
     r = ctx.copyToHost(d_r)
-    want = op.lhs(f)
-    print("[gpu-parity]", name, "lhs rel err", rel(r, want), flush=True)
-    assert rel(r, want) <= TOL
     pad = r.reshape(op.nP, op.aslm)[:, op.slm:]
     assert np.all(pad == 0)
     ctx.computeLhsDiag(d_f, d_r)

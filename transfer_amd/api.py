@@ -95,6 +95,9 @@ _sig("tr_ctx_slm", C.c_uint32, vp)
 _sig("tr_ctx_aslm", C.c_uint32, vp)
 _sig("tr_ctx_nP", C.c_uint32, vp)
 _sig("tr_ctx_permutation", C.c_int, vp, u32p)
+_sig("tr_ctx_autotune", C.c_int, vp, C.c_int, C.c_char_p, C.c_size_t)
+_sig("tr_ctx_schedule", C.c_int, vp, C.c_int, C.POINTER(C.c_int))
+_sig("tr_ctx_set_schedule", C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int)
 _sig("tr_device_alloc", vp, vp, C.c_size_t)
 _sig("tr_alloc_vector", vp, vp)
 _sig("tr_device_free", C.c_int, vp, vp)
@@ -114,6 +117,7 @@ _sig("tr_synchronize", C.c_int, vp)
 _sig("tr_solve", C.c_int, vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats), f64p)
 _sig("tr_bench_lhs", C.c_double, vp, C.c_int)
 _sig("tr_bench_operator", C.c_int, vp, C.c_int, C.c_int, f64p)
+_sig("tr_bench_copy", C.c_int, vp, C.c_size_t, C.c_int, f64p)
 _sig("tr_stencil_create", C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), C.POINTER(vp))
 _sig("tr_stencil_destroy", C.c_int, vp)
 _sig("tr_stencil_info", C.c_int, vp, u32p)
@@ -334,6 +338,21 @@ class HipContext:
     def N(self):
         return int(lib.tr_ctx_N(self._h))
 
+    def autotune(self, precision=0):
+        """Times the candidate operator schedules and keeps the fastest;
+        returns the report text."""
+        buf = C.create_string_buffer(16384)
+        check(lib.tr_ctx_autotune(self._h, int(precision), buf, len(buf)))
+        return buf.value.decode()
+
+    def schedule(self, precision=0):
+        out = (C.c_int * 5)()
+        check(lib.tr_ctx_schedule(self._h, int(precision), out))
+        return dict(zip(("J", "L", "VPW", "WPB", "variant"), list(out)))
+
+    def set_schedule(self, precision=0, J=4, WPB=4, variant=2):
+        check(lib.tr_ctx_set_schedule(self._h, int(precision), int(J), int(WPB), int(variant)))
+
     def permutation(self):
         p = np.empty(self.nP, np.uint32)
         check(lib.tr_ctx_permutation(self._h, ptr(p, u32p)))
@@ -420,6 +439,12 @@ class HipContext:
 
     def bench_lhs(self, reps=20):
         return lib.tr_bench_lhs(self._h, reps)
+
+    def bench_copy(self, nbytes, reps=10):
+        """ms per plain device copy of nbytes"""
+        out = np.zeros(1)
+        check(lib.tr_bench_copy(self._h, int(nbytes), int(reps), ptr(out)))
+        return float(out[0])
 
     def bench_operator(self, warmup, steps):
         """(wall s, volume-kernel ms, boundary-kernel ms, ms per application)"""

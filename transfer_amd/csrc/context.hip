@@ -106,6 +106,24 @@ struct tr_ctx {
 
 namespace tr {
 
+// The lane mappings worth trying: J must divide slm with slm/J <= 64 lanes.
+std::vector<LaneMap> map_candidates(uint32_t slm, int maxJ) {
+    static const int Js[] = {1, 2, 3, 4, 5, 6, 7, 11};
+    std::vector<LaneMap> out;
+    for (int J : Js) {
+        if (J > maxJ || slm % J) continue;
+        int L = (int)(slm / J);
+        if (L > 64) continue;
+        LaneMap m;
+        m.J = J;
+        m.L = L;
+        m.VPW = 64 / L;
+        if ((double)m.VPW * L / 64.0 < 0.5) continue;
+        out.push_back(m);
+    }
+    return out;
+}
+
 // Picks the lane mapping: J must divide slm with slm/J <= 64 lanes; prefer
 // the best lane utilisation, then the smaller J (fewer registers).
 LaneMap choose_map(uint32_t slm, int maxJ) {
@@ -126,6 +144,11 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
         }
     }
     return best;
+}
+
+static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem) {
+    return m.variant == 2 ? apply2_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem)
+                          : apply_lds_bytes(m, (int)op.NC, op.slm, elem);
 }
 
 template <typename T>
@@ -190,11 +213,11 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
     }
     c->map = choose_map(c->op.slm, 4);
+    if (c->op.W > 16) c->map.variant = 1;
     int maxlds = 65536;
     hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
-    while (c->map.WPB > 1 && apply_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds)
-        c->map.WPB /= 2;
-    if (apply_lds_bytes(c->map, (int)c->op.NC, c->op.slm, 8) > (size_t)maxlds) {
+    while (c->map.WPB > 1 && lds_need(c->map, c->op, 8) > (size_t)maxlds) c->map.WPB /= 2;
+    if (lds_need(c->map, c->op, 8) > (size_t)maxlds) {
         tr_ctx_destroy(c);
         return fail(TR_ERR_ARG, "tr_ctx_create: angular order too large for LDS staging");
     }
@@ -597,10 +620,10 @@ static int ensure_fp32(tr_ctx *c) {
     if (c->have32) return TR_OK;
     const HostOperator &op = c->op;
     c->map32 = choose_map(op.slm, 8);
+    if (op.W > 16) c->map32.variant = 1;
     int maxlds = 65536;
     hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-    while (c->map32.WPB > 1 && apply_lds_bytes(c->map32, (int)op.NC, op.slm, 4) > (size_t)maxlds)
-        c->map32.WPB /= 2;
+    while (c->map32.WPB > 1 && lds_need(c->map32, op, 4) > (size_t)maxlds) c->map32.WPB /= 2;
     int rc;
     if ((rc = upload(c, &c->f_coef, to_f32(op.coef))) || (rc = upload(c, &c->f_bcoef, to_f32(op.bcoef))) ||
         (rc = upload(c, &c->f_aval, to_f32(op.aval))) || (rc = upload(c, &c->f_bval, to_f32(op.bval))) ||
@@ -774,5 +797,181 @@ extern "C" int tr_bench_operator(tr_ctx *c, int warmup, int steps, double *out) 
     out[1] = vol / steps;
     out[2] = bnd / steps;
     out[3] = all / steps;
+    return TR_OK;
+}
+
+namespace tr {
+__global__ void k_copy_calib(const double4 *__restrict__ src, double4 *__restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace tr
+
+// A plain device copy of `bytes` (rounded down to 32 B), timed with HIP
+// events: the calibration kernel for memory-traffic counters and the
+// achievable-bandwidth reference.  out[0] mean ms per copy.
+extern "C" int tr_bench_copy(tr_ctx *c, size_t bytes, int reps, double *out) {
+    if (!c || !out || reps < 1 || bytes < 32) return fail(TR_ERR_ARG, "tr_bench_copy: bad argument");
+    size_t n = bytes / 32;
+    double4 *a = nullptr, *b = nullptr;
+    HIPCHECK(hipMalloc((void **)&a, n * 32));
+    if (hipMalloc((void **)&b, n * 32) != hipSuccess) {
+        hipFree(a);
+        return fail(TR_ERR_HIP, "tr_bench_copy: hipMalloc failed");
+    }
+    hipMemsetAsync(a, 0, n * 32, c->stream);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    unsigned g = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy_calib, dim3(g), dim3(256), 0, c->stream, a, b, n);
+    hipEventRecord(e0, c->stream);
+    for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_copy_calib, dim3(g), dim3(256), 0, c->stream, a, b, n);
+    hipEventRecord(e1, c->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return fail(TR_ERR_HIP, hipGetErrorString(e));
+    out[0] = ms / reps;
+    return TR_OK;
+}
+
+namespace tr {
+
+// Times every candidate schedule of the FP64 (precision 0) or FP32
+// (precision 1) operator on this context's mesh and keeps the fastest.
+static int autotune(tr_ctx *c, int precision, std::string &report) {
+    const HostOperator &op = c->op;
+    const bool f32 = precision == 1;
+    int rc;
+    if (f32 && (rc = ensure_fp32(c))) return rc;
+    int maxlds = 65536;
+    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    const size_t elem = f32 ? 4 : 8, n = c->nint();
+    void *x = nullptr, *y = nullptr;
+    HIPCHECK(hipMalloc(&x, n * elem));
+    if (hipMalloc(&y, n * elem) != hipSuccess) {
+        hipFree(x);
+        return fail(TR_ERR_HIP, "autotune: out of device memory");
+    }
+    hipMemsetAsync(x, 0, n * elem, c->stream);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    LaneMap best = f32 ? c->map32 : c->map;
+    float bestms = 1e30f;
+    char line[160];
+    for (LaneMap m : map_candidates(op.slm, 11)) {
+        for (int variant : {1, 2}) {
+            if (variant == 2 && op.W > 16) continue;
+            for (int wpb : {1, 2, 4}) {
+                m.variant = variant;
+                m.WPB = wpb;
+                if (lds_need(m, op, elem) > (size_t)maxlds) continue;
+                auto run = [&]() {
+                    if (f32) {
+                        OpArgs<float> a = c->args32();
+                        a.L = m.L;
+                        a.VPW = m.VPW;
+                        return launch_apply<float>(a, (int)op.NC, m, (const float *)x, (float *)y, nullptr,
+                                                   nullptr, c->stream);
+                    }
+                    OpArgs<double> a = c->args();
+                    a.L = m.L;
+                    a.VPW = m.VPW;
+                    return launch_apply<double>(a, (int)op.NC, m, (const double *)x, (double *)y, nullptr,
+                                                nullptr, c->stream);
+                };
+                if (run() != hipSuccess) continue;
+                hipEventRecord(e0, c->stream);
+                const int reps = 5;
+                for (int i = 0; i < reps; i++) run();
+                hipEventRecord(e1, c->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) continue;
+                float ms = 0;
+                hipEventElapsedTime(&ms, e0, e1);
+                ms /= reps;
+                snprintf(line, sizeof line, "%s J=%d L=%d VPW=%d WPB=%d variant=%d: %.2f us\n",
+                         f32 ? "fp32" : "fp64", m.J, m.L, m.VPW, m.WPB, m.variant, ms * 1e3);
+                report += line;
+                if (ms < bestms) {
+                    bestms = ms;
+                    best = m;
+                }
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    snprintf(line, sizeof line, "chosen %s J=%d WPB=%d variant=%d (%.2f us)\n", f32 ? "fp32" : "fp64", best.J,
+             best.WPB, best.variant, bestms * 1e3);
+    report += line;
+    if (f32)
+        c->map32 = best;
+    else
+        c->map = best;
+    return TR_OK;
+}
+
+}  // namespace tr
+
+extern "C" int tr_ctx_autotune(tr_ctx *c, int precision, char *report, size_t cap) {
+    if (!c || precision < 0 || precision > 1) return fail(TR_ERR_ARG, "tr_ctx_autotune: bad argument");
+    std::string rep;
+    int rc = autotune(c, precision, rep);
+    if (report && cap) {
+        std::strncpy(report, rep.c_str(), cap - 1);
+        report[cap - 1] = 0;
+    }
+    return rc;
+}
+
+extern "C" int tr_ctx_schedule(const tr_ctx *c, int precision, int *out5) {
+    if (!c || !out5 || precision < 0 || precision > 1) return fail(TR_ERR_ARG, "tr_ctx_schedule: bad argument");
+    const LaneMap &m = precision ? c->map32 : c->map;
+    out5[0] = m.J;
+    out5[1] = m.L;
+    out5[2] = m.VPW;
+    out5[3] = m.WPB;
+    out5[4] = m.variant;
+    return TR_OK;
+}
+
+extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int variant) {
+    if (!c || precision < 0 || precision > 1) return fail(TR_ERR_ARG, "tr_ctx_set_schedule: bad argument");
+    if (precision == 1) {
+        int rc = ensure_fp32(c);
+        if (rc) return rc;
+    }
+    const HostOperator &op = c->op;
+    if (J < 1 || op.slm % J || (int)(op.slm / J) > 64 || (WPB != 1 && WPB != 2 && WPB != 4) ||
+        (variant != 1 && variant != 2) || (variant == 2 && op.W > 16))
+        return fail(TR_ERR_ARG, "tr_ctx_set_schedule: unsupported schedule");
+    bool ok = false;
+    for (const LaneMap &m : map_candidates(op.slm, 11)) ok = ok || m.J == J;
+    if (!ok) return fail(TR_ERR_ARG, "tr_ctx_set_schedule: J not instantiated for this slm");
+    LaneMap m;
+    m.J = J;
+    m.L = (int)(op.slm / J);
+    m.VPW = 64 / m.L;
+    m.WPB = WPB;
+    m.variant = variant;
+    int maxlds = 65536;
+    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    if (lds_need(m, op, precision ? 4 : 8) > (size_t)maxlds)
+        return fail(TR_ERR_ARG, "tr_ctx_set_schedule: LDS budget exceeded");
+    if (precision)
+        c->map32 = m;
+    else
+        c->map = m;
     return TR_OK;
 }

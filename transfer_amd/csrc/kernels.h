@@ -12,6 +12,7 @@ namespace tr {
 // harmonics.  WPB wavefronts per workgroup.
 struct LaneMap {
     int J = 1, L = 1, VPW = 1, WPB = 4;
+    int variant = 2;  // 1: stage every vertex of a wavefront at once; 2: batched (k_apply2)
 };
 
 // Device operator data (see stencil.h), precision T.
@@ -44,6 +45,8 @@ template <typename T>
 hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                              double *part, const double *skip, hipStream_t s);
 size_t apply_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem);
+size_t apply2_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, uint32_t brow,
+                        size_t elem);
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m);
 
 // b[v][lm] = lm == 0 ? rhs[v] : 0

@@ -143,6 +143,139 @@ __global__ __launch_bounds__(256) void k_apply(OpArgs<T> a, const T *__restrict_
     }
 }
 
+// Workgroup i is dispatched to XCD i % 8; give every XCD a contiguous range of
+// logical blocks so that neighbouring vertices share that XCD's L2.
+__device__ inline uint32_t xcd_block(uint32_t bid, uint32_t nblk) {
+    const uint32_t X = 8, per = nblk / X, rem = nblk % X;
+    const uint32_t xcd = bid % X, k = bid / X;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+#define TR_WMAX 16
+
+// Operator, second schedule.  Gathers: the ELL slots are unrolled to TR_WMAX
+// so every neighbour index is loaded up front and the slab loads can be in
+// flight together.  Angular contraction: instead of staging all of a
+// wavefront's vertices at once, CB = 64/slm vertices at a time are staged in
+// LDS ([vertex][channel][lm]) and contracted with lanes remapped to
+// (vertex, output lm), so a wavefront needs CB*(NC+3)*slm words of LDS, not
+// VPW*(NC+2)*slm; the coupling table sits in LDS, loaded once per workgroup.
+template <typename T, int J, int NC, bool DOT>
+__global__ __launch_bounds__(256) void k_apply2(OpArgs<T> a, const T *__restrict__ f,
+                                                T *__restrict__ r, double *__restrict__ part,
+                                                const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
+    const int wpb = blockDim.x / TR_WAVE;
+    const uint32_t slm = a.slm;
+    const int CH = NC + 3;                       // mass, NC-1 stiffness, 3 boundary
+    const int CB = slm >= TR_WAVE ? 1 : TR_WAVE / (int)slm;
+    // LDS: table values, table indices, then per-wave staging
+    T *tval = reinterpret_cast<T *>(smem_raw);
+    const uint32_t tsz = (a.arow + a.brow) * slm;
+    uint16_t *tidx = reinterpret_cast<uint16_t *>(tval + tsz);
+    T *stage = reinterpret_cast<T *>(smem_raw + ((tsz * (sizeof(T) + 2) + 15) & ~(size_t)15)) +
+               (size_t)wave * CB * CH * slm;
+    for (uint32_t i = threadIdx.x; i < a.arow * slm; i += blockDim.x) {
+        tval[i] = a.aval[i];
+        tidx[i] = a.aidx[i];
+    }
+    for (uint32_t i = threadIdx.x; i < a.brow * slm; i += blockDim.x) {
+        tval[a.arow * slm + i] = a.bval[i];
+        tidx[a.arow * slm + i] = a.bidx[i];
+    }
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int vsub = lane / a.L, lv = lane - vsub * a.L;
+    const uint32_t v0 = (blk * (uint32_t)wpb + wave) * (uint32_t)a.VPW;
+    const uint32_t v = v0 + vsub;
+    const bool valid = vsub < a.VPW && v < a.nP;
+    const int lm0 = lv * J;
+    const size_t nP = a.nP, nB = a.nB;
+    const int b = valid ? a.bof[v] : -1;
+    T acc[NC][J], bacc[3][J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c][j] = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) bacc[c][j] = 0;
+    }
+    if (valid) {
+        uint32_t nb[TR_WMAX];
+#pragma unroll
+        for (int s = 0; s < TR_WMAX; s++) nb[s] = s < (int)a.W ? a.nbr[s * nP + v] : 0;
+#pragma unroll
+        for (int s = 0; s < TR_WMAX; s++) {
+            if (s < (int)a.W) {
+                const T *fk = f + (size_t)nb[s] * slm + lm0;
+                T x[J];
+#pragma unroll
+                for (int j = 0; j < J; j++) x[j] = fk[j];
+                const T *cs = a.coef + (size_t)s * NC * nP + v;
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    const T cf = cs[(size_t)c * nP];
+#pragma unroll
+                    for (int j = 0; j < J; j++) acc[c][j] = fma(cf, x[j], acc[c][j]);
+                }
+            }
+        }
+        if (b >= 0) {
+            for (uint32_t s = 0; s < a.WB; s++) {
+                const uint32_t k = a.bnbr[s * nB + b];
+                const T *fk = f + (size_t)k * slm + lm0;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const T cf = a.bcoef[((size_t)s * 3 + c) * nB + b];
+#pragma unroll
+                    for (int j = 0; j < J; j++) bacc[c][j] = fma(cf, fk[j], bacc[c][j]);
+                }
+            }
+        }
+    }
+    double d = 0;
+    const int nbatch = (a.VPW + CB - 1) / CB;
+    for (int q = 0; q < nbatch; q++) {
+        __syncthreads();
+        const int vb = vsub - q * CB;
+        if (valid && vb >= 0 && vb < CB) {
+            T *my = stage + (size_t)vb * CH * slm;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) my[c * slm + lm0 + j] = acc[c][j];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) my[(NC + c) * slm + lm0 + j] = bacc[c][j];
+        }
+        __syncthreads();
+        // contraction lanes: (vertex cv of this batch, output lm)
+        for (int o = lane; o < CB * (int)slm; o += TR_WAVE) {
+            const int cv = o / (int)slm, lm = o - cv * (int)slm;
+            const uint32_t vv = v0 + q * CB + cv;
+            if (q * CB + cv >= a.VPW || vv >= a.nP) continue;
+            const T *my = stage + (size_t)cv * CH * slm;
+            T R = a.massfac[lm] * my[lm];
+            const T *st = my + slm;  // stiffness channels
+            for (uint32_t e = 0; e < a.arow; e++)
+                R = fma(tval[e * slm + lm], st[tidx[e * slm + lm]], R);
+            if (a.bof[vv] >= 0) {
+                const T *bt = my + NC * slm;
+                for (uint32_t e = 0; e < a.brow; e++)
+                    R = fma(tval[(a.arow + e) * slm + lm], bt[tidx[(a.arow + e) * slm + lm]], R);
+            }
+            r[(size_t)vv * slm + lm] = R;
+            if (DOT) d = fma((double)f[(size_t)vv * slm + lm], (double)R, d);
+        }
+    }
+    if (DOT) {
+        d = block_sum(d);
+        if (threadIdx.x == 0) part[blockIdx.x] = d;
+    }
+}
+
 // angular-diagonal operator: per lane and slot the coupling
 // massfac*M + sum_a Omega^aa[lm][lm] K^aa (+ sum_n O_n[lm][lm] B^n)
 template <typename T, int J, bool DOT>
@@ -217,6 +350,13 @@ size_t apply_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem) {
     return (size_t)m.WPB * m.VPW * (NC - 1 + 3) * slm * elem;
 }
 
+size_t apply2_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, uint32_t brow,
+                        size_t elem) {
+    const int CB = slm >= 64 ? 1 : 64 / (int)slm;
+    size_t table = (((size_t)(arow + brow) * slm * (elem + 2)) + 15) & ~(size_t)15;
+    return table + (size_t)m.WPB * CB * (NC + 3) * slm * elem;
+}
+
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m) {
     uint32_t per = (uint32_t)(m.WPB * m.VPW);
     return (nP + per - 1) / per;
@@ -229,9 +369,16 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
                         double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
     dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
-    size_t sh = apply_lds_bytes(m, NC, a.slm, sizeof(T));
-#define TR_LAUNCH(JJ, NCC, D) \
-    hipLaunchKernelGGL((k_apply<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip)
+    const bool v2 = m.variant == 2 && a.W <= TR_WMAX;
+    size_t sh = v2 ? apply2_lds_bytes(m, NC, a.slm, a.arow, a.brow, sizeof(T))
+                   : apply_lds_bytes(m, NC, a.slm, sizeof(T));
+#define TR_LAUNCH(JJ, NCC, D)                                                          \
+    do {                                                                               \
+        if (v2)                                                                        \
+            hipLaunchKernelGGL((k_apply2<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip); \
+        else                                                                           \
+            hipLaunchKernelGGL((k_apply<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip);  \
+    } while (0)
 #define TR_CASE(JJ)                                                        \
     case JJ:                                                               \
         if (NC == 7) {                                                     \
