@@ -182,7 +182,8 @@ int tr_ctx_autotune(tr_ctx *ctx, int precision, char *report, size_t cap);
 /* The schedule in use: out5 = {J, L, VPW, WPB, variant} */
 int tr_ctx_schedule(const tr_ctx *ctx, int precision, int *out5);
 /* Forces a schedule (J harmonics per lane, WPB wavefronts per workgroup,
- * variant 1 = whole-wavefront LDS staging, 2 = batched staging). */
+ * variant 1 = whole-wavefront LDS staging, 2 = batched staging, 3 = batched
+ * staging with buffer loads and per-slot coefficient records). */
 int tr_ctx_set_schedule(tr_ctx *ctx, int precision, int J, int WPB, int variant);
 /* Returns the internal numbering: perm[new] = caller vertex id. */
 int tr_ctx_permutation(const tr_ctx *ctx, uint32_t *perm);
@@ -198,6 +199,9 @@ int tr_get_host_small(tr_ctx *ctx, const double *src, double *dst);
 int tr_compute_lhs(tr_ctx *ctx, const double *f, double *Af);      /* :160-172 */
 int tr_compute_lhs_diag(tr_ctx *ctx, const double *f, double *Af); /* :174-186 */
 int tr_compute_rhs(tr_ctx *ctx, double *b);                        /* :151-158 */
+/* computeLhs with the operator in FP32 (precision 1; the inner operator of
+ * the mixed solver) or FP64 (precision 0); f and Af stay FP64 vectors. */
+int tr_compute_lhs_prec(tr_ctx *ctx, int precision, const double *f, double *Af);
 int tr_scale(tr_ctx *ctx, double *x, double wx);                   /* :188-196 */
 int tr_add_prod(tr_ctx *ctx, double *x, const double *y, double wy);          /* :198-206 */
 int tr_mul_add(tr_ctx *ctx, double *x, double wx, const double *y);           /* :208-216 */

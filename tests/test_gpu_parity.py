@@ -58,7 +58,7 @@ def test_lhs_lhsdiag_rhs_parity(name):
     d_f, d_r = ctx.vector_from(f), ctx.allocVector()
     want = op.lhs(f)
     default = ctx.schedule()
-    for variant in (1, 2):
+    for variant in (1, 2, 3):
         for J in sorted({default["J"], 1 if op.slm <= 64 else default["J"]}):
             ctx.set_schedule(0, J=J, WPB=4, variant=variant)
             ctx.computeLhs(d_f, d_r)
@@ -76,6 +76,26 @@ def test_lhs_lhsdiag_rhs_parity(name):
     assert rel(ctx.copyToHost(d_r), op.rhs()) <= 1e-14
     for v in (d_f, d_r):
         ctx.deviceFree(v)
+
+
+@pytest.mark.parametrize("name", list(cases()))
+def test_fp32_operator_every_schedule(name):
+    """The FP32 operator (inner operator of the mixed solver) against the FP64
+    oracle: bar 1e-5 relative to max|A f| (FP32 rounding of coefficients,
+    inputs and sums; observed ~1e-7)."""
+    P, ctx, op = setup(name)
+    f = rvec(op, 5)
+    d_f, d_r = ctx.vector_from(f), ctx.allocVector()
+    want = op.lhs(f)
+    J0 = ctx.schedule(1)["J"]
+    for variant in (1, 2, 3):
+        ctx.set_schedule(1, J=J0, WPB=4, variant=variant)
+        ctx.computeLhsPrec(1, d_f, d_r)
+        e = rel(ctx.copyToHost(d_r), want)
+        print("[gpu-parity] %s fp32 J=%d variant=%d rel err %.2e" % (name, J0, variant, e), flush=True)
+        assert e <= 1e-5
+    ctx.deviceFree(d_f)
+    ctx.deviceFree(d_r)
 
 
 def test_copies_blas_reductions():

@@ -107,6 +107,7 @@ _sig("tr_get_host_small", C.c_int, vp, vp, f64p)
 for _n in ("tr_compute_lhs", "tr_compute_lhs_diag"):
     _sig(_n, C.c_int, vp, vp, vp)
 _sig("tr_compute_rhs", C.c_int, vp, vp)
+_sig("tr_compute_lhs_prec", C.c_int, vp, C.c_int, vp, vp)
 _sig("tr_scale", C.c_int, vp, vp, C.c_double)
 _sig("tr_add_prod", C.c_int, vp, vp, vp, C.c_double)
 _sig("tr_mul_add", C.c_int, vp, vp, C.c_double, vp)
@@ -395,6 +396,9 @@ class HipContext:
 
     def computeLhs(self, f, Af):
         check(lib.tr_compute_lhs(self._h, f, Af))
+
+    def computeLhsPrec(self, precision, f, Af):
+        check(lib.tr_compute_lhs_prec(self._h, int(precision), f, Af))
 
     def computeLhsDiag(self, f, Af):
         check(lib.tr_compute_lhs_diag(self._h, f, Af))

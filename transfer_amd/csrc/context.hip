@@ -40,12 +40,16 @@ struct tr_ctx {
     double *d_coef = nullptr, *d_bcoef = nullptr, *d_aval = nullptr, *d_bval = nullptr;
     double *d_massfac = nullptr, *d_ddiag = nullptr, *d_bdiag = nullptr, *d_rhs = nullptr;
     double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
+    double *d_crec = nullptr;   // coefficient records for the buffer-load schedule
+    void *d_tab = nullptr;      // contraction-table records
     // FP32 copy of the operator data (mixed-precision solves), built lazily;
     // index arrays are shared with the FP64 set
     bool have32 = false;
     tr::LaneMap map32;
     float *f_coef = nullptr, *f_bcoef = nullptr, *f_aval = nullptr, *f_bval = nullptr;
     float *f_massfac = nullptr, *f_ddiag = nullptr, *f_bdiag = nullptr;
+    float *f_crec = nullptr;
+    void *f_tab = nullptr;
     std::set<void *> vectors, raw;
     std::vector<void *> owned;
 
@@ -72,6 +76,8 @@ struct tr_ctx {
         a.bidx = d_bidx;
         a.bval = d_bval;
         a.bdiag = d_bdiag;
+        a.crec = d_crec;
+        a.tab = d_tab;
         return a;
     }
     tr::OpArgs<float> args32() const {
@@ -98,6 +104,8 @@ struct tr_ctx {
         a.bcoef = f_bcoef;
         a.bval = f_bval;
         a.bdiag = f_bdiag;
+        a.crec = f_crec;
+        a.tab = f_tab;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }
@@ -147,8 +155,48 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
 }
 
 static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem) {
+    if (m.variant == 3) return apply3_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem);
     return m.variant == 2 ? apply2_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem)
                           : apply_lds_bytes(m, (int)op.NC, op.slm, elem);
+}
+
+// [W][nP][RS] coefficient records from the [W][NC][nP] layout
+template <typename T>
+static std::vector<T> coef_records(const HostOperator &op) {
+    const int RS = rec_stride<T>((int)op.NC);
+    std::vector<T> rec((size_t)op.W * op.nP * RS, (T)0);
+    for (uint32_t s = 0; s < op.W; s++)
+        for (uint32_t v = 0; v < op.nP; v++)
+            for (uint32_t c = 0; c < op.NC; c++)
+                rec[((size_t)s * op.nP + v) * RS + c] = (T)op.coef[((size_t)s * op.NC + c) * op.nP + v];
+    return rec;
+}
+
+// contraction table records {value, byte offset into the staged vertex}:
+// volume entries idx = (c-1)*slm + lm' sit in staging channel c, boundary
+// entries idx = n*slm + lm' in channel NC + n
+template <typename T>
+static std::vector<unsigned char> table_records(const HostOperator &op) {
+    const int TB = tab_rec_bytes<T>();
+    const uint32_t slm = op.slm;
+    std::vector<unsigned char> out((size_t)(op.arow + op.brow) * slm * TB, 0);
+    auto put = [&](size_t row, uint32_t lm, double val, uint32_t off) {
+        unsigned char *p = &out[(row * slm + lm) * TB];
+        T v = (T)val;
+        std::memcpy(p, &v, sizeof(T));
+        std::memcpy(p + sizeof(T), &off, 4);
+    };
+    for (uint32_t e = 0; e < op.arow; e++)
+        for (uint32_t lm = 0; lm < slm; lm++) {
+            size_t q = (size_t)e * slm + lm;
+            put(e, lm, op.aval[q], (uint32_t)((op.aidx[q] + slm) * sizeof(T)));
+        }
+    for (uint32_t e = 0; e < op.brow; e++)
+        for (uint32_t lm = 0; lm < slm; lm++) {
+            size_t q = (size_t)e * slm + lm;
+            put(op.arow + e, lm, op.bval[q], (uint32_t)((op.NC * slm + op.bidx[q]) * sizeof(T)));
+        }
+    return out;
 }
 
 template <typename T>
@@ -214,6 +262,7 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
     }
     c->map = choose_map(c->op.slm, 4);
     if (c->op.W > 16) c->map.variant = 1;
+    if ((double)c->op.W * c->op.nP * rec_stride<double>((int)c->op.NC) * 8 >= 2147483648.0) c->map.variant = 2;
     int maxlds = 65536;
     hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     while (c->map.WPB > 1 && lds_need(c->map, c->op, 8) > (size_t)maxlds) c->map.WPB /= 2;
@@ -230,7 +279,8 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         (rc = upload(c, &c->d_bval, op.bval)) || (rc = upload(c, &c->d_massfac, op.massfac)) ||
         (rc = upload(c, &c->d_ddiag, op.ddiag)) || (rc = upload(c, &c->d_bdiag, op.bdiag)) ||
         (rc = upload(c, &c->d_rhs, op.rhs)) || (rc = upload(c, &c->d_perm, op.perm)) ||
-        (rc = upload(c, &c->d_iperm, op.iperm))) {
+        (rc = upload(c, &c->d_iperm, op.iperm)) || (rc = upload(c, &c->d_crec, coef_records<double>(op))) ||
+        (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op)))) {
         tr_ctx_destroy(c);
         return rc;
     }
@@ -621,6 +671,7 @@ static int ensure_fp32(tr_ctx *c) {
     const HostOperator &op = c->op;
     c->map32 = choose_map(op.slm, 8);
     if (op.W > 16) c->map32.variant = 1;
+    if ((double)op.W * op.nP * rec_stride<float>((int)op.NC) * 4 >= 2147483648.0) c->map32.variant = 2;
     int maxlds = 65536;
     hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     while (c->map32.WPB > 1 && lds_need(c->map32, op, 4) > (size_t)maxlds) c->map32.WPB /= 2;
@@ -628,7 +679,8 @@ static int ensure_fp32(tr_ctx *c) {
     if ((rc = upload(c, &c->f_coef, to_f32(op.coef))) || (rc = upload(c, &c->f_bcoef, to_f32(op.bcoef))) ||
         (rc = upload(c, &c->f_aval, to_f32(op.aval))) || (rc = upload(c, &c->f_bval, to_f32(op.bval))) ||
         (rc = upload(c, &c->f_massfac, to_f32(op.massfac))) || (rc = upload(c, &c->f_ddiag, to_f32(op.ddiag))) ||
-        (rc = upload(c, &c->f_bdiag, to_f32(op.bdiag))))
+        (rc = upload(c, &c->f_bdiag, to_f32(op.bdiag))) || (rc = upload(c, &c->f_crec, coef_records<float>(op))) ||
+        (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
         return rc;
     HIPCHECK(hipStreamSynchronize(c->stream));
     c->have32 = true;
@@ -867,8 +919,8 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
     float bestms = 1e30f;
     char line[160];
     for (LaneMap m : map_candidates(op.slm, 11)) {
-        for (int variant : {1, 2}) {
-            if (variant == 2 && op.W > 16) continue;
+        for (int variant : {1, 2, 3}) {
+            if (variant >= 2 && op.W > 16) continue;
             for (int wpb : {1, 2, 4}) {
                 m.variant = variant;
                 m.WPB = wpb;
@@ -954,7 +1006,7 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
     }
     const HostOperator &op = c->op;
     if (J < 1 || op.slm % J || (int)(op.slm / J) > 64 || (WPB != 1 && WPB != 2 && WPB != 4) ||
-        (variant != 1 && variant != 2) || (variant == 2 && op.W > 16))
+        (variant < 1 || variant > 3) || (variant >= 2 && op.W > 16))
         return fail(TR_ERR_ARG, "tr_ctx_set_schedule: unsupported schedule");
     bool ok = false;
     for (const LaneMap &m : map_candidates(op.slm, 11)) ok = ok || m.J == J;
@@ -973,5 +1025,26 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
         c->map32 = m;
     else
         c->map = m;
+    return TR_OK;
+}
+
+// Af = A f with the FP32 operator (precision 1) or FP64 (precision 0); f and
+// Af are FP64 device vectors, converted around the FP32 application.
+extern "C" int tr_compute_lhs_prec(tr_ctx *c, int precision, const double *f, double *Af) {
+    if (!c || !f || !Af || precision < 0 || precision > 1) return fail(TR_ERR_ARG, "tr_compute_lhs_prec: bad argument");
+    if (precision == 0) return lhs(c, f, Af);
+    int rc = ensure_fp32(c);
+    if (rc) return rc;
+    const size_t n = c->nint();
+    float *v = nullptr;
+    HIPCHECK(hipMalloc((void **)&v, sizeof(float) * 2 * n + 16));
+    hipError_t e = launch_to_f32_scaled(f, v, 1.0, n, c->stream);
+    if (e == hipSuccess)
+        e = launch_apply<float>(c->args32(), (int)c->op.NC, c->map32, v, v + n, nullptr, nullptr, c->stream);
+    if (e == hipSuccess) e = launch_axpby(Af, 0.0, nullptr, 0.0, n, c->stream);
+    if (e == hipSuccess) e = launch_add_f32_scaled(Af, v + n, 1.0, n, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(v);
+    if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("tr_compute_lhs_prec: ") + hipGetErrorString(e));
     return TR_OK;
 }

@@ -12,7 +12,8 @@ namespace tr {
 // harmonics.  WPB wavefronts per workgroup.
 struct LaneMap {
     int J = 1, L = 1, VPW = 1, WPB = 4;
-    int variant = 2;  // 1: stage every vertex of a wavefront at once; 2: batched (k_apply2)
+    int variant = 3;  // 1: stage every vertex of a wavefront at once; 2: batched (k_apply2);
+                      // 3: batched with buffer loads and coefficient records (k_apply3)
 };
 
 // Device operator data (see stencil.h), precision T.
@@ -32,7 +33,21 @@ struct OpArgs {
     const uint16_t *bidx;  // [brow][slm]
     const T *bval;         // [brow][slm]
     const T *bdiag;        // [3][slm]
+    // record layout for the buffer-load schedule (variant 3), may be null
+    const T *crec;         // [W][nP][RS] coefficients of slot s of vertex v, RS = rec_stride<T>(NC)
+    const void *tab;       // [(arow+brow)][slm] {T value; uint32 byte offset into the staged vertex}
 };
+
+// coefficient record stride (elements): NC rounded up to a 16-byte multiple
+template <typename T>
+constexpr int rec_stride(int NC) {
+    return (NC * (int)sizeof(T) + 15) / 16 * (16 / (int)sizeof(T));
+}
+// bytes of one contraction-table record: {T value, uint32 offset} padded to 8 or 16
+template <typename T>
+constexpr int tab_rec_bytes() {
+    return sizeof(T) == 8 ? 16 : 8;
+}
 
 // r = A f (volume and vacuum boundary in one pass).  With part != null each
 // workgroup also writes sum_v f_v . r_v (FP64) to part[blockIdx]; with
@@ -46,6 +61,8 @@ hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const
                              double *part, const double *skip, hipStream_t s);
 size_t apply_lds_bytes(const LaneMap &m, int NC, uint32_t slm, size_t elem);
 size_t apply2_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, uint32_t brow,
+                        size_t elem);
+size_t apply3_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, uint32_t brow,
                         size_t elem);
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m);
 

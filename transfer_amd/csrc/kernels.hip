@@ -276,6 +276,183 @@ __global__ __launch_bounds__(256) void k_apply2(OpArgs<T> a, const T *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Buffer-load schedule.  Every array is read through a buffer resource
+// (SGPR descriptor): the per-slot offset is a scalar, the per-lane offset a
+// loop-invariant 32-bit VGPR, and reads past the end of a buffer return
+// zero, so the slot loop is straight-line code over WT slots.
+#define TR_RSRC_FLAGS 0x00020000
+
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const T *p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), (short)0,
+                                             (int)(bytes > 0xffffffffull ? 0xffffffffull : bytes),
+                                             TR_RSRC_FLAGS);
+}
+
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// N consecutive elements of type T at byte offset voff + soff.  WIDE: the
+// offset is a multiple of 16 bytes, so 16-byte loads may be used.  Loaded
+// vectors are reinterpreted whole (bit-casting single vector elements
+// miscompiles).
+template <typename T, int N, bool WIDE>
+__device__ inline void bload(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff, T *out) {
+    if constexpr (sizeof(T) == 8) {
+        int q = 0;
+        if constexpr (WIDE) {
+#pragma unroll
+            for (; q + 1 < N; q += 2) {
+                f64x2 v = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + q * 8, soff, 0));
+                out[q] = v.x;
+                out[q + 1] = v.y;
+            }
+        }
+#pragma unroll
+        for (; q < N; q++)
+            out[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff + q * 8, soff, 0));
+    } else {
+        int q = 0;
+        if constexpr (WIDE) {
+#pragma unroll
+            for (; q + 3 < N; q += 4) {
+                f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + q * 4, soff, 0));
+                out[q] = v.x;
+                out[q + 1] = v.y;
+                out[q + 2] = v.z;
+                out[q + 3] = v.w;
+            }
+        }
+#pragma unroll
+        for (; q < N; q++)
+            out[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff + q * 4, soff, 0));
+    }
+}
+
+template <typename T, int J, int NC, int WT, bool DOT>
+__global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict__ f,
+                                                T *__restrict__ r, double *__restrict__ part,
+                                                const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int RS = rec_stride<T>(NC), TB = tab_rec_bytes<T>(), CH = NC + 3;
+    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
+    const int wpb = blockDim.x / TR_WAVE;
+    const uint32_t slm = a.slm, nP = a.nP;
+    const int CB = slm >= TR_WAVE ? 1 : TR_WAVE / (int)slm;
+    const uint32_t ntab = (a.arow + a.brow) * slm;
+    // table records, then per-wavefront staging
+    unsigned char *tab = smem_raw;
+    T *stage = reinterpret_cast<T *>(smem_raw + (size_t)ntab * TB) + (size_t)wave * CB * CH * slm;
+    {
+        const i32x2 *src = reinterpret_cast<const i32x2 *>(a.tab);
+        i32x2 *dst = reinterpret_cast<i32x2 *>(tab);
+        for (uint32_t i = threadIdx.x; i < ntab * (TB / 8); i += blockDim.x) dst[i] = src[i];
+    }
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int vsub = lane / a.L, lv = lane - vsub * a.L;
+    const uint32_t v0 = (blk * (uint32_t)wpb + wave) * (uint32_t)a.VPW;
+    const uint32_t v = v0 + vsub;
+    const bool valid = vsub < a.VPW && v < nP;
+    const int lm0 = lv * J;
+    T acc[NC][J];
+#pragma unroll
+    for (int j = 0; j < J; j++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c][j] = 0;
+    if (valid) {
+        const auto frs = rsrc(f, (size_t)nP * slm * sizeof(T));
+        const auto nrs = rsrc(a.nbr, (size_t)a.W * nP * 4);
+        const auto crs = rsrc(a.crec, (size_t)a.W * nP * RS * sizeof(T));
+        uint32_t nb[WT];
+#pragma unroll
+        for (int s = 0; s < WT; s++)
+            nb[s] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrs, v * 4u, (uint32_t)s * nP * 4u, 0);
+        const uint32_t cvoff = v * (uint32_t)(RS * sizeof(T));
+#pragma unroll
+        for (int s = 0; s < WT; s++) {
+            T x[J], cf[RS];
+            // J | slm, so a J whose slab is a multiple of 16 bytes keeps every
+            // lane's offset 16-byte aligned; records are 16-byte multiples
+            bload<T, J, (J * sizeof(T)) % 16 == 0>(frs, (nb[s] * slm + lm0) * (uint32_t)sizeof(T), 0u, x);
+            bload<T, RS, true>(crs, cvoff, (uint32_t)s * nP * (uint32_t)(RS * sizeof(T)), cf);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) acc[c][j] = fma(cf[c], x[j], acc[c][j]);
+        }
+    }
+    // vacuum boundary (few vertices): plain loads
+    T bacc[3][J];
+#pragma unroll
+    for (int j = 0; j < J; j++) bacc[0][j] = bacc[1][j] = bacc[2][j] = 0;
+    const int b = valid ? a.bof[v] : -1;
+    if (b >= 0) {
+        const size_t nB = a.nB;
+        for (uint32_t s = 0; s < a.WB; s++) {
+            const uint32_t k = a.bnbr[s * nB + b];
+            const T *fk = f + (size_t)k * slm + lm0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const T cf = a.bcoef[((size_t)s * 3 + c) * nB + b];
+#pragma unroll
+                for (int j = 0; j < J; j++) bacc[c][j] = fma(cf, fk[j], bacc[c][j]);
+            }
+        }
+    }
+    double d = 0;
+    const int nbatch = (a.VPW + CB - 1) / CB;
+    for (int q = 0; q < nbatch; q++) {
+        __syncthreads();
+        const int vb = vsub - q * CB;
+        if (valid && vb >= 0 && vb < CB) {
+            T *my = stage + (size_t)vb * CH * slm;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) my[c * slm + lm0 + j] = acc[c][j];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) my[(NC + c) * slm + lm0 + j] = bacc[c][j];
+        }
+        __syncthreads();
+        for (int o = lane; o < CB * (int)slm; o += TR_WAVE) {
+            const int cv = o / (int)slm, lm = o - cv * (int)slm;
+            const uint32_t vv = v0 + q * CB + cv;
+            if (q * CB + cv >= a.VPW || vv >= nP) continue;
+            const unsigned char *my = reinterpret_cast<const unsigned char *>(stage + (size_t)cv * CH * slm);
+            T R = a.massfac[lm] * reinterpret_cast<const T *>(my)[lm];
+            const unsigned char *row = tab + (size_t)lm * TB;
+            const uint32_t rstride = slm * TB;
+            for (uint32_t e = 0; e < a.arow; e++) {
+                const T val = *reinterpret_cast<const T *>(row);
+                const uint32_t off = *reinterpret_cast<const uint32_t *>(row + sizeof(T));
+                R = fma(val, *reinterpret_cast<const T *>(my + off), R);
+                row += rstride;
+            }
+            if (a.bof[vv] >= 0) {
+                for (uint32_t e = 0; e < a.brow; e++) {
+                    const T val = *reinterpret_cast<const T *>(row);
+                    const uint32_t off = *reinterpret_cast<const uint32_t *>(row + sizeof(T));
+                    R = fma(val, *reinterpret_cast<const T *>(my + off), R);
+                    row += rstride;
+                }
+            }
+            r[(size_t)vv * slm + lm] = R;
+            if (DOT) d = fma((double)f[(size_t)vv * slm + lm], (double)R, d);
+        }
+    }
+    if (DOT) {
+        d = block_sum(d);
+        if (threadIdx.x == 0) part[blockIdx.x] = d;
+    }
+}
+
 // angular-diagonal operator: per lane and slot the coupling
 // massfac*M + sum_a Omega^aa[lm][lm] K^aa (+ sum_n O_n[lm][lm] B^n)
 template <typename T, int J, bool DOT>
@@ -357,6 +534,13 @@ size_t apply2_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, u
     return table + (size_t)m.WPB * CB * (NC + 3) * slm * elem;
 }
 
+size_t apply3_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, uint32_t brow,
+                        size_t elem) {
+    const int CB = slm >= 64 ? 1 : 64 / (int)slm;
+    size_t table = (size_t)(arow + brow) * slm * (elem == 8 ? 16 : 8);
+    return table + (size_t)m.WPB * CB * (NC + 3) * slm * elem;
+}
+
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m) {
     uint32_t per = (uint32_t)(m.WPB * m.VPW);
     return (nP + per - 1) / per;
@@ -369,15 +553,23 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
                         double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
     dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
-    const bool v2 = m.variant == 2 && a.W <= TR_WMAX;
-    size_t sh = v2 ? apply2_lds_bytes(m, NC, a.slm, a.arow, a.brow, sizeof(T))
-                   : apply_lds_bytes(m, NC, a.slm, sizeof(T));
-#define TR_LAUNCH(JJ, NCC, D)                                                          \
-    do {                                                                               \
-        if (v2)                                                                        \
-            hipLaunchKernelGGL((k_apply2<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip); \
-        else                                                                           \
-            hipLaunchKernelGGL((k_apply<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip);  \
+    // variant 3 needs W = 15 or 16 exactly (the host pads the stencil) so that
+    // every unrolled slot reads inside the arrays
+    const bool v3 = m.variant == 3 && (a.W == 15 || a.W == 16) && a.crec && a.tab;
+    const bool v2 = !v3 && m.variant >= 2 && a.W <= TR_WMAX;
+    size_t sh = v3 ? apply3_lds_bytes(m, NC, a.slm, a.arow, a.brow, sizeof(T))
+                   : v2 ? apply2_lds_bytes(m, NC, a.slm, a.arow, a.brow, sizeof(T))
+                        : apply_lds_bytes(m, NC, a.slm, sizeof(T));
+#define TR_LAUNCH(JJ, NCC, D)                                                                   \
+    do {                                                                                        \
+        if (v3 && a.W == 15)                                                                    \
+            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 15, D>), g, b, sh, s, a, f, r, part, skip);  \
+        else if (v3)                                                                            \
+            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 16, D>), g, b, sh, s, a, f, r, part, skip);  \
+        else if (v2)                                                                            \
+            hipLaunchKernelGGL((k_apply2<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip);      \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_apply<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip);       \
     } while (0)
 #define TR_CASE(JJ)                                                        \
     case JJ:                                                               \

@@ -198,6 +198,10 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
         cnt[i] = 1 + gstart[vo + 1] - gstart[vo];
         W = std::max(W, cnt[i]);
     }
+    // pad to the unrolled slot counts of the buffer-load kernel (15 = the
+    // Kuhn interior degree + 1); extra slots point at the vertex itself with
+    // zero couplings
+    if (W < 15) W = 15;
     op.W = W;
     op.nbr.assign((size_t)W * nP, 0);
     op.coef.assign((size_t)W * NC * nP, 0.0);
