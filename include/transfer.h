@@ -180,11 +180,17 @@ uint32_t tr_ctx_nP(const tr_ctx *ctx);
  * geometry, CudaContext.cpp:160-172). */
 int tr_ctx_autotune(tr_ctx *ctx, int precision, char *report, size_t cap);
 /* The schedule in use: out5 = {J, L, VPW, WPB, variant} */
-int tr_ctx_schedule(const tr_ctx *ctx, int precision, int *out5);
+int tr_ctx_schedule(tr_ctx *ctx, int precision, int *out5);
 /* Forces a schedule (J harmonics per lane, WPB wavefronts per workgroup,
  * variant 1 = whole-wavefront LDS staging, 2 = batched staging, 3 = batched
- * staging with buffer loads and per-slot coefficient records). */
+ * staging with buffer loads and per-slot coefficient records, 4 = angular
+ * contraction on the matrix cores, J ignored). */
 int tr_ctx_set_schedule(tr_ctx *ctx, int precision, int J, int WPB, int variant);
+/* Angular precision of the FP32 operator used by TR_SOLVER_MIXED:
+ * 0 = FP32 throughout, 1 = BF16 matrix-core projection with FP32
+ * accumulation, 3 (default) = BF16 with the hi/lo split (three BF16
+ * products per term, ~16 significant bits). */
+int tr_ctx_set_angular_precision(tr_ctx *ctx, int prec);
 /* Returns the internal numbering: perm[new] = caller vertex id. */
 int tr_ctx_permutation(const tr_ctx *ctx, uint32_t *perm);
 

@@ -65,6 +65,13 @@ def test_lhs_lhsdiag_rhs_parity(name):
             r = ctx.copyToHost(d_r)
             print("[gpu-parity]", name, "J=%d variant=%d lhs rel err %.2e" % (J, variant, rel(r, want)), flush=True)
             assert rel(r, want) <= TOL
+    for wpb in (1, 4, 8):                       # matrix-core schedule, FP64 MFMA
+        if not try_schedule(ctx, 0, J=0, WPB=wpb, variant=4):
+            continue
+        ctx.computeLhs(d_f, d_r)
+        e = rel(ctx.copyToHost(d_r), want)
+        print("[gpu-parity]", name, "variant=4 WPB=%d lhs rel err %.2e" % (wpb, e), flush=True)
+        assert e <= TOL
     ctx.set_schedule(0, J=default["J"], WPB=default["WPB"], variant=default["variant"])
     ctx.computeLhs(d_f, d_r)
     r = ctx.copyToHost(d_r)
@@ -78,6 +85,17 @@ def test_lhs_lhsdiag_rhs_parity(name):
         ctx.deviceFree(v)
 
 
+def try_schedule(ctx, precision, **kw):
+    """Forces a schedule; False where the mesh or harmonic count rules it out
+    (the matrix-core schedule needs ceil(slm/4) in {2, 4, 7} and 15 or 16
+    stencil slots)."""
+    try:
+        ctx.set_schedule(precision, **kw)
+        return True
+    except ta.TransferError:
+        return False
+
+
 @pytest.mark.parametrize("name", list(cases()))
 def test_fp32_operator_every_schedule(name):
     """The FP32 operator (inner operator of the mixed solver) against the FP64
@@ -87,15 +105,53 @@ def test_fp32_operator_every_schedule(name):
     f = rvec(op, 5)
     d_f, d_r = ctx.vector_from(f), ctx.allocVector()
     want = op.lhs(f)
-    J0 = ctx.schedule(1)["J"]
+    # largest instantiated J (harmonics per lane) of the VALU schedules
+    J0 = max(j for j in (1, 2, 3, 4, 5, 6, 7) if op.slm % j == 0 and op.slm // j <= 64)
     for variant in (1, 2, 3):
         ctx.set_schedule(1, J=J0, WPB=4, variant=variant)
         ctx.computeLhsPrec(1, d_f, d_r)
         e = rel(ctx.copyToHost(d_r), want)
         print("[gpu-parity] %s fp32 J=%d variant=%d rel err %.2e" % (name, J0, variant, e), flush=True)
         assert e <= 1e-5
+    ctx.set_angular_precision(0)                      # FP32 MFMA projection
+    try:
+        if try_schedule(ctx, 1, J=0, WPB=4, variant=4):
+            ctx.computeLhsPrec(1, d_f, d_r)
+            e = rel(ctx.copyToHost(d_r), want)
+            print("[gpu-parity] %s fp32 variant=4 (fp32 mfma) rel err %.2e" % (name, e), flush=True)
+            assert e <= 1e-5
+    finally:
+        ctx.set_angular_precision(3)
     ctx.deviceFree(d_f)
     ctx.deviceFree(d_r)
+
+
+@pytest.mark.parametrize("name", list(cases()))
+def test_bf16_angular_projection(name):
+    """The FP32 operator with the angular projection on the BF16 matrix cores
+    (FP32 accumulation) against the FP64 oracle.  Plain BF16 rounds the
+    gathered values and the couplings to 8 significant bits: bar 2e-2
+    relative to max|A f|.  The hi/lo split (three BF16 products per term, the
+    lo*lo term dropped) leaves ~2^-16 per term: bar 2e-4."""
+    P, ctx, op = setup(name)
+    f = rvec(op, 6)
+    d_f, d_r = ctx.vector_from(f), ctx.allocVector()
+    want = op.lhs(f)
+    try:
+        for prec, bar in ((1, 2e-2), (3, 2e-4)):
+            ctx.set_angular_precision(prec)
+            if not try_schedule(ctx, 1, J=0, WPB=4, variant=4):
+                pytest.skip("matrix-core schedule not available for this mesh/order")
+            for wpb in (1, 4, 8):
+                ctx.set_schedule(1, J=0, WPB=wpb, variant=4)
+                ctx.computeLhsPrec(1, d_f, d_r)
+                e = rel(ctx.copyToHost(d_r), want)
+                print("[gpu-parity] %s bf16 prec=%d WPB=%d rel err %.2e" % (name, prec, wpb, e), flush=True)
+                assert e <= bar
+    finally:
+        ctx.set_angular_precision(3)
+        ctx.deviceFree(d_f)
+        ctx.deviceFree(d_r)
 
 
 def test_copies_blas_reductions():

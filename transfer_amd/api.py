@@ -98,6 +98,7 @@ _sig("tr_ctx_permutation", C.c_int, vp, u32p)
 _sig("tr_ctx_autotune", C.c_int, vp, C.c_int, C.c_char_p, C.c_size_t)
 _sig("tr_ctx_schedule", C.c_int, vp, C.c_int, C.POINTER(C.c_int))
 _sig("tr_ctx_set_schedule", C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int)
+_sig("tr_ctx_set_angular_precision", C.c_int, vp, C.c_int)
 _sig("tr_device_alloc", vp, vp, C.c_size_t)
 _sig("tr_alloc_vector", vp, vp)
 _sig("tr_device_free", C.c_int, vp, vp)
@@ -353,6 +354,10 @@ class HipContext:
 
     def set_schedule(self, precision=0, J=4, WPB=4, variant=2):
         check(lib.tr_ctx_set_schedule(self._h, int(precision), int(J), int(WPB), int(variant)))
+
+    def set_angular_precision(self, prec):
+        """0 FP32, 1 BF16 matrix-core projection, 3 BF16 hi/lo split"""
+        check(lib.tr_ctx_set_angular_precision(self._h, int(prec)))
 
     def permutation(self):
         p = np.empty(self.nP, np.uint32)

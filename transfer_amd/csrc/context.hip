@@ -42,6 +42,7 @@ struct tr_ctx {
     double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
     double *d_crec = nullptr;   // coefficient records for the buffer-load schedule
     void *d_tab = nullptr;      // contraction-table records
+    void *d_tab4 = nullptr;     // MFMA B slices (variant 4, FP64)
     // FP32 copy of the operator data (mixed-precision solves), built lazily;
     // index arrays are shared with the FP64 set
     bool have32 = false;
@@ -50,10 +51,13 @@ struct tr_ctx {
     float *f_massfac = nullptr, *f_ddiag = nullptr, *f_bdiag = nullptr;
     float *f_crec = nullptr;
     void *f_tab = nullptr;
+    void *f_tab4 = nullptr, *f_tab4b1 = nullptr, *f_tab4b3 = nullptr;  // variant 4: FP32, BF16, BF16 hi/lo
+    int ang_prec = 3;  // FP32 operator's angular contraction: 0 FP32, 1 BF16, 3 BF16 hi/lo split
     std::set<void *> vectors, raw;
     std::vector<void *> owned;
 
-    tr::OpArgs<double> args() const {
+    tr::OpArgs<double> args() const { return args(map); }
+    tr::OpArgs<double> args(const tr::LaneMap &map) const {
         tr::OpArgs<double> a;
         a.nP = op.nP;
         a.slm = op.slm;
@@ -77,11 +81,13 @@ struct tr_ctx {
         a.bval = d_bval;
         a.bdiag = d_bdiag;
         a.crec = d_crec;
-        a.tab = d_tab;
+        a.tab = map.variant == 4 ? d_tab4 : d_tab;
+        a.nsplit = 1;
         return a;
     }
-    tr::OpArgs<float> args32() const {
-        tr::OpArgs<double> d = args();
+    tr::OpArgs<float> args32() const { return args32(map32); }
+    tr::OpArgs<float> args32(const tr::LaneMap &map32) const {
+        tr::OpArgs<double> d = args(map);
         tr::OpArgs<float> a;
         a.nP = d.nP;
         a.slm = d.slm;
@@ -105,7 +111,8 @@ struct tr_ctx {
         a.bval = f_bval;
         a.bdiag = f_bdiag;
         a.crec = f_crec;
-        a.tab = f_tab;
+        a.tab = map32.variant != 4 ? f_tab : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
+        a.nsplit = ang_prec == 1 ? 1 : 3;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }
@@ -113,6 +120,18 @@ struct tr_ctx {
 };
 
 namespace tr {
+
+// The matrix-core schedule (kernels_mfma.hip): 16 vertices per wavefront.
+LaneMap mfma_map(uint32_t slm, int wpb, int mode) {
+    LaneMap m;
+    m.variant = 4;
+    m.WPB = wpb;
+    m.J = mfma_J(slm);
+    m.L = 4;
+    m.VPW = 16;
+    m.mfma_mode = mode;
+    return m;
+}
 
 // The lane mappings worth trying: J must divide slm with slm/J <= 64 lanes.
 std::vector<LaneMap> map_candidates(uint32_t slm, int maxJ) {
@@ -154,7 +173,8 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
     return best;
 }
 
-static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem) {
+static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem, int nsplit = 3) {
+    if (m.variant == 4) return apply4_lds_bytes(elem == 8 ? 0 : m.mfma_mode, (int)op.NC, op.slm, nsplit) + 64;
     if (m.variant == 3) return apply3_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem);
     return m.variant == 2 ? apply2_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem)
                           : apply_lds_bytes(m, (int)op.NC, op.slm, elem);
@@ -170,6 +190,70 @@ static std::vector<T> coef_records(const HostOperator &op) {
             for (uint32_t c = 0; c < op.NC; c++)
                 rec[((size_t)s * op.nP + v) * RS + c] = (T)op.coef[((size_t)s * op.NC + c) * op.nP + v];
     return rec;
+}
+
+static uint16_t to_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float from_bf16(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+// MFMA B slices of the variant-4 schedule (kernels_mfma.hip): for channel c,
+// K-step st, output tile t and lane l the coupling Mat_c[n][m] with
+// n = 16t + l%16 and m = (l/16)*J + st (FP64/FP32, one value) or
+// m = (l/16)*J + 4st + u, u = 0..3 (BF16, four values); volume channels
+// (mass = diag(massfac), then the NC-1 stiffness channels) first, then the 3
+// boundary channels.  mode 0 FP64, 1 FP32, 2 BF16 (nsplit 3 appends the
+// low parts: hi table, lo table, per section).
+static std::vector<unsigned char> mfma_tables(const HostOperator &op, int mode, int nsplit) {
+    const uint32_t slm = op.slm;
+    const int J = mfma_J(slm), S = mode == 2 ? (J + 3) / 4 : J, NT = (int)((slm + 15) / 16);
+    const int BW = mode == 2 ? 8 : (mode == 0 ? 8 : 4);
+    auto mat = [&](int sec, int c, uint32_t n, uint32_t m) -> double {
+        if (n >= slm || m >= slm) return 0.0;
+        if (sec == 1) return op.onc[((size_t)c * slm + n) * slm + m];
+        if (c == 0) return n == m ? op.massfac[n] : 0.0;
+        return op.omc[((size_t)(c - 1) * slm + n) * slm + m];
+    };
+    std::vector<unsigned char> out;
+    for (int sec = 0; sec < 2; sec++) {
+        const int nch = sec == 0 ? (int)op.NC : 3;
+        const size_t sz = (size_t)nch * S * NT * 64 * BW;
+        for (int part = 0; part < (mode == 2 && nsplit > 1 ? 2 : 1); part++) {
+            std::vector<unsigned char> t(sz, 0);
+            for (int c = 0; c < nch; c++)
+                for (int st = 0; st < S; st++)
+                    for (int tt = 0; tt < NT; tt++)
+                        for (int l = 0; l < 64; l++) {
+                            unsigned char *p = &t[((((size_t)c * S + st) * NT + tt) * 64 + l) * BW];
+                            const uint32_t n = 16u * tt + (l % 16), q = l / 16;
+                            if (mode == 0) {
+                                double v = mat(sec, c, n, q * J + st);
+                                std::memcpy(p, &v, 8);
+                            } else if (mode == 1) {
+                                float v = (float)mat(sec, c, n, q * J + st);
+                                std::memcpy(p, &v, 4);
+                            } else {
+                                uint16_t h[4];
+                                for (int u = 0; u < 4; u++) {
+                                    double v = (4 * st + u < J) ? mat(sec, c, n, q * J + 4 * st + u) : 0.0;
+                                    uint16_t hi = to_bf16((float)v);
+                                    h[u] = part == 0 ? hi : to_bf16((float)(v - (double)from_bf16(hi)));
+                                }
+                                std::memcpy(p, h, 8);
+                            }
+                        }
+            out.insert(out.end(), t.begin(), t.end());
+        }
+    }
+    return out;
 }
 
 // contraction table records {value, byte offset into the staged vertex}:
@@ -280,7 +364,8 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         (rc = upload(c, &c->d_ddiag, op.ddiag)) || (rc = upload(c, &c->d_bdiag, op.bdiag)) ||
         (rc = upload(c, &c->d_rhs, op.rhs)) || (rc = upload(c, &c->d_perm, op.perm)) ||
         (rc = upload(c, &c->d_iperm, op.iperm)) || (rc = upload(c, &c->d_crec, coef_records<double>(op))) ||
-        (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op)))) {
+        (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
+        (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
         tr_ctx_destroy(c);
         return rc;
     }
@@ -670,6 +755,8 @@ static int ensure_fp32(tr_ctx *c) {
     if (c->have32) return TR_OK;
     const HostOperator &op = c->op;
     c->map32 = choose_map(op.slm, 8);
+    if (c->ang_prec != 0 && mfma_supported(op.slm, 1) && (op.W == 15 || op.W == 16))
+        c->map32 = mfma_map(op.slm, 4, 2);
     if (op.W > 16) c->map32.variant = 1;
     if ((double)op.W * op.nP * rec_stride<float>((int)op.NC) * 4 >= 2147483648.0) c->map32.variant = 2;
     int maxlds = 65536;
@@ -681,6 +768,11 @@ static int ensure_fp32(tr_ctx *c) {
         (rc = upload(c, &c->f_massfac, to_f32(op.massfac))) || (rc = upload(c, &c->f_ddiag, to_f32(op.ddiag))) ||
         (rc = upload(c, &c->f_bdiag, to_f32(op.bdiag))) || (rc = upload(c, &c->f_crec, coef_records<float>(op))) ||
         (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
+        return rc;
+    if (mfma_supported(op.slm, 1) &&
+        ((rc = upload(c, (unsigned char **)&c->f_tab4, mfma_tables(op, 1, 1))) ||
+         (rc = upload(c, (unsigned char **)&c->f_tab4b1, mfma_tables(op, 2, 1))) ||
+         (rc = upload(c, (unsigned char **)&c->f_tab4b3, mfma_tables(op, 2, 3)))))
         return rc;
     HIPCHECK(hipStreamSynchronize(c->stream));
     c->have32 = true;
@@ -918,24 +1010,40 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
     LaneMap best = f32 ? c->map32 : c->map;
     float bestms = 1e30f;
     char line[160];
-    for (LaneMap m : map_candidates(op.slm, 11)) {
-        for (int variant : {1, 2, 3}) {
-            if (variant >= 2 && op.W > 16) continue;
-            for (int wpb : {1, 2, 4}) {
-                m.variant = variant;
-                m.WPB = wpb;
-                if (lds_need(m, op, elem) > (size_t)maxlds) continue;
+    // Candidates of equal accuracy only: a BF16 angular projection (FP32
+    // operator with ang_prec 1 or 3) is the matrix-core schedule alone; the
+    // exact schedules (FP64, or FP32 with ang_prec 0) race the VALU variants
+    // against the FP64/FP32 MFMA one.
+    std::vector<LaneMap> cands;
+    const bool bf16 = f32 && c->ang_prec != 0 && mfma_supported(op.slm, 1) && (op.W == 15 || op.W == 16);
+    if (!bf16)
+        for (LaneMap m : map_candidates(op.slm, 11))
+            for (int variant : {1, 2, 3})
+                for (int wpb : {1, 2, 4}) {
+                    if (variant >= 2 && op.W > 16) continue;
+                    m.variant = variant;
+                    m.WPB = wpb;
+                    cands.push_back(m);
+                }
+    if (mfma_supported(op.slm, f32) && (op.W == 15 || op.W == 16))
+        for (int wpb : {1, 2, 4, 8}) {
+            LaneMap m = mfma_map(op.slm, wpb, f32 ? (bf16 ? 2 : 1) : 0);
+            cands.push_back(m);
+        }
+    {
+        for (LaneMap m : cands) {
+            {
+                const int variant = m.variant, wpb = m.WPB;
+                (void)variant;
+                (void)wpb;
+                if (lds_need(m, op, elem, c->ang_prec == 1 ? 1 : 3) > (size_t)maxlds) continue;
                 auto run = [&]() {
                     if (f32) {
-                        OpArgs<float> a = c->args32();
-                        a.L = m.L;
-                        a.VPW = m.VPW;
+                        OpArgs<float> a = c->args32(m);
                         return launch_apply<float>(a, (int)op.NC, m, (const float *)x, (float *)y, nullptr,
                                                    nullptr, c->stream);
                     }
-                    OpArgs<double> a = c->args();
-                    a.L = m.L;
-                    a.VPW = m.VPW;
+                    OpArgs<double> a = c->args(m);
                     return launch_apply<double>(a, (int)op.NC, m, (const double *)x, (double *)y, nullptr,
                                                 nullptr, c->stream);
                 };
@@ -948,8 +1056,9 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
                 float ms = 0;
                 hipEventElapsedTime(&ms, e0, e1);
                 ms /= reps;
-                snprintf(line, sizeof line, "%s J=%d L=%d VPW=%d WPB=%d variant=%d: %.2f us\n",
-                         f32 ? "fp32" : "fp64", m.J, m.L, m.VPW, m.WPB, m.variant, ms * 1e3);
+                snprintf(line, sizeof line, "%s J=%d L=%d VPW=%d WPB=%d variant=%d mfma_mode=%d: %.2f us\n",
+                         f32 ? "fp32" : "fp64", m.J, m.L, m.VPW, m.WPB, m.variant, m.variant == 4 ? m.mfma_mode : 0,
+                         ms * 1e3);
                 report += line;
                 if (ms < bestms) {
                     bestms = ms;
@@ -987,8 +1096,12 @@ extern "C" int tr_ctx_autotune(tr_ctx *c, int precision, char *report, size_t ca
     return rc;
 }
 
-extern "C" int tr_ctx_schedule(const tr_ctx *c, int precision, int *out5) {
+extern "C" int tr_ctx_schedule(tr_ctx *c, int precision, int *out5) {
     if (!c || !out5 || precision < 0 || precision > 1) return fail(TR_ERR_ARG, "tr_ctx_schedule: bad argument");
+    if (precision == 1) {
+        int rc = ensure_fp32(c);
+        if (rc) return rc;
+    }
     const LaneMap &m = precision ? c->map32 : c->map;
     out5[0] = m.J;
     out5[1] = m.L;
@@ -1005,6 +1118,21 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
         if (rc) return rc;
     }
     const HostOperator &op = c->op;
+    if (variant == 4) {
+        if (!mfma_supported(op.slm, precision) || (op.W != 15 && op.W != 16) ||
+            (WPB != 1 && WPB != 2 && WPB != 4 && WPB != 8))
+            return fail(TR_ERR_ARG, "tr_ctx_set_schedule: matrix-core schedule unsupported here");
+        LaneMap m = mfma_map(op.slm, WPB, precision ? (c->ang_prec == 0 ? 1 : 2) : 0);
+        int maxlds = 65536;
+        hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+        if (lds_need(m, op, precision ? 4 : 8, c->ang_prec == 1 ? 1 : 3) > (size_t)maxlds)
+            return fail(TR_ERR_ARG, "tr_ctx_set_schedule: LDS budget exceeded");
+        if (precision)
+            c->map32 = m;
+        else
+            c->map = m;
+        return TR_OK;
+    }
     if (J < 1 || op.slm % J || (int)(op.slm / J) > 64 || (WPB != 1 && WPB != 2 && WPB != 4) ||
         (variant < 1 || variant > 3) || (variant >= 2 && op.W > 16))
         return fail(TR_ERR_ARG, "tr_ctx_set_schedule: unsupported schedule");
@@ -1046,5 +1174,23 @@ extern "C" int tr_compute_lhs_prec(tr_ctx *c, int precision, const double *f, do
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(v);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("tr_compute_lhs_prec: ") + hipGetErrorString(e));
+    return TR_OK;
+}
+
+// Angular precision of the FP32 operator: 0 = FP32 arithmetic throughout,
+// 1 = BF16 matrix-core projection with FP32 accumulation, 3 = the same with
+// the hi/lo split (three BF16 products, ~16 significant bits).
+extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
+    if (!c || (prec != 0 && prec != 1 && prec != 3)) return fail(TR_ERR_ARG, "tr_ctx_set_angular_precision: 0, 1 or 3");
+    c->ang_prec = prec;
+    if (c->have32) {
+        const HostOperator &op = c->op;
+        if (prec != 0 && mfma_supported(op.slm, 1) && (op.W == 15 || op.W == 16))
+            c->map32 = mfma_map(op.slm, 4, 2);
+        else {
+            c->map32 = choose_map(op.slm, 8);
+            if (op.W > 16) c->map32.variant = 1;
+        }
+    }
     return TR_OK;
 }

@@ -13,7 +13,9 @@ namespace tr {
 struct LaneMap {
     int J = 1, L = 1, VPW = 1, WPB = 4;
     int variant = 3;  // 1: stage every vertex of a wavefront at once; 2: batched (k_apply2);
-                      // 3: batched with buffer loads and coefficient records (k_apply3)
+                      // 3: batched with buffer loads and coefficient records (k_apply3);
+                      // 4: angular contraction on the matrix cores (k_apply4)
+    int mfma_mode = 1;  // FP32 operator under variant 4: 1 = FP32 MFMA, 2 = BF16 MFMA
 };
 
 // Device operator data (see stencil.h), precision T.
@@ -35,7 +37,9 @@ struct OpArgs {
     const T *bdiag;        // [3][slm]
     // record layout for the buffer-load schedule (variant 3), may be null
     const T *crec;         // [W][nP][RS] coefficients of slot s of vertex v, RS = rec_stride<T>(NC)
-    const void *tab;       // [(arow+brow)][slm] {T value; uint32 byte offset into the staged vertex}
+    const void *tab;       // variant 3: [(arow+brow)][slm] {T value; uint32 byte offset};
+                           // variant 4: MFMA B slices (volume, then boundary)
+    int nsplit;            // variant 4, BF16: 1 = plain, 3 = hi/lo split
 };
 
 // coefficient record stride (elements): NC rounded up to a 16-byte multiple
@@ -65,6 +69,14 @@ size_t apply2_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, u
 size_t apply3_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, uint32_t brow,
                         size_t elem);
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m);
+
+// variant 4 (kernels_mfma.hip): J = ceil(slm/4) harmonics per lane
+int mfma_J(uint32_t slm);
+bool mfma_supported(uint32_t slm, int mode);
+size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit);
+template <typename T, int MODE>
+hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, const T *f, T *r, double *part,
+                         const double *skip, hipStream_t s);
 
 // b[v][lm] = lm == 0 ? rhs[v] : 0
 hipError_t launch_rhs(const double *rhs, uint32_t nP, uint32_t slm, double *b,

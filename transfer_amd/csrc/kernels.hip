@@ -426,23 +426,31 @@ __global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict
             const uint32_t vv = v0 + q * CB + cv;
             if (q * CB + cv >= a.VPW || vv >= nP) continue;
             const unsigned char *my = reinterpret_cast<const unsigned char *>(stage + (size_t)cv * CH * slm);
-            T R = a.massfac[lm] * reinterpret_cast<const T *>(my)[lm];
+            T R0 = a.massfac[lm] * reinterpret_cast<const T *>(my)[lm], R1 = 0;
             const unsigned char *row = tab + (size_t)lm * TB;
             const uint32_t rstride = slm * TB;
-            for (uint32_t e = 0; e < a.arow; e++) {
-                const T val = *reinterpret_cast<const T *>(row);
-                const uint32_t off = *reinterpret_cast<const uint32_t *>(row + sizeof(T));
-                R = fma(val, *reinterpret_cast<const T *>(my + off), R);
-                row += rstride;
-            }
-            if (a.bof[vv] >= 0) {
-                for (uint32_t e = 0; e < a.brow; e++) {
-                    const T val = *reinterpret_cast<const T *>(row);
-                    const uint32_t off = *reinterpret_cast<const uint32_t *>(row + sizeof(T));
-                    R = fma(val, *reinterpret_cast<const T *>(my + off), R);
-                    row += rstride;
+            // rows are padded to multiples of 8 entries (zero couplings):
+            // issue eight table reads, then eight staged reads, then the FMAs
+            const uint32_t nent = a.arow + (a.bof[vv] >= 0 ? a.brow : 0);
+            for (uint32_t e = 0; e < nent; e += 8) {
+                T val[8];
+                uint32_t off[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const unsigned char *rr = row + (e + u) * rstride;
+                    val[u] = *reinterpret_cast<const T *>(rr);
+                    off[u] = *reinterpret_cast<const uint32_t *>(rr + sizeof(T));
+                }
+                T t[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) t[u] = *reinterpret_cast<const T *>(my + off[u]);
+#pragma unroll
+                for (int u = 0; u < 8; u += 2) {
+                    R0 = fma(val[u], t[u], R0);
+                    R1 = fma(val[u + 1], t[u + 1], R1);
                 }
             }
+            const T R = R0 + R1;
             r[(size_t)vv * slm + lm] = R;
             if (DOT) d = fma((double)f[(size_t)vv * slm + lm], (double)R, d);
         }
@@ -552,6 +560,14 @@ template <typename T>
 hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                         double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
+    if (m.variant == 4) {
+        if constexpr (sizeof(T) == 8)
+            return launch_apply4<T, 0>(a, NC, m.WPB, f, r, part, skip, s);
+        else if (m.mfma_mode == 2)
+            return launch_apply4<T, 2>(a, NC, m.WPB, f, r, part, skip, s);
+        else
+            return launch_apply4<T, 1>(a, NC, m.WPB, f, r, part, skip, s);
+    }
     dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
     // variant 3 needs W = 15 or 16 exactly (the host pads the stencil) so that
     // every unrolled slot reads inside the arrays

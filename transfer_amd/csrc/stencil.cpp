@@ -167,7 +167,9 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
                 }
         rows = 0;
         for (auto &r : R) rows = std::max<uint32_t>(rows, (uint32_t)r.size());
-        rows = std::max<uint32_t>(rows, 1);
+        // padded to a multiple of 8 entries (zero couplings) for the unrolled
+        // contraction loop
+        rows = std::max<uint32_t>((rows + 7) / 8 * 8, 8);
         idx.assign((size_t)rows * slm, 0);
         val.assign((size_t)rows * slm, 0.0);
         for (uint32_t o = 0; o < slm; o++)
