@@ -34,6 +34,24 @@ def volume_kernel_bytes(nT, nP, slm, p):
     return nT * (16 + 10 * p) + 2 * nP * slm * p
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    factor the same counter reads on a plain copy of known size); None when
+    no summary covers this kernel."""
+    fn = os.path.join(ROOT, "profiles", "traffic_%s.txt" % workload)
+    if not os.path.exists(fn):
+        return None
+    for line in open(fn):
+        parts = line.split("|")
+        if len(parts) >= 3 and parts[0].strip() == kernel:
+            try:
+                return float(parts[-1])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(P, seconds=12.0):
     import numpy as np
     import oracle
@@ -81,24 +99,32 @@ def main():
         cms = ctx.bench_copy(256 << 20, 10)
         print("[bench] copy 256 MiB: %.3f ms, %.0f GB/s (read+write)" % (cms, 2 * (256 << 20) / cms / 1e6),
               file=sys.stderr, flush=True)
-    wall, vol_ms, bnd_ms, step_ms = ctx.bench_operator(args.warmup, args.steps)
+    wall, k_ms, k_min, k_max = ctx.bench_operator(args.warmup, args.steps, precision=0)
     value = nP * slm * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
     vbytes = volume_kernel_bytes(nT, nP, slm, 8)
-    achieved = vbytes / (vol_ms * 1e-3) / 1e9
-    print("[bench] operator %.1f us/step (volume %.1f us, boundary %.1f us)" % (ms_per_step * 1e3, vol_ms * 1e3,
-                                                                              bnd_ms * 1e3), file=sys.stderr, flush=True)
+    achieved = vbytes / (k_ms * 1e-3) / 1e9
+    kname = ctx.kernel_name(0)
+    print("[bench] FP64 operator %.1f us/step wall, kernel %s %.1f us (min %.1f, max %.1f)"
+          % (ms_per_step * 1e3, kname, k_ms * 1e3, k_min * 1e3, k_max * 1e3), file=sys.stderr, flush=True)
     out = dict(metric=METRIC, value=value, unit="DOF*ordinate/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="f64", data="synthetic",
                config=dict(workload="%s: ball of 64^3-grid cubes, K=3 (slm 28), kappa=50exp(-r/0.2), "
                                     "I_p=(1-0.8r)^4, FP64 operator apply" % P.name,
                            nP=nP, nT=nT, nF=P.mesh.nF, slm=slm))
-    kname = "k_apply%s<double,%d,7,false>" % ("2" if sched["variant"] == 2 else "", sched["J"])
     out["roofline"] = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=achieved / HBM_PEAK_GBS, traffic=None,
-                           kernel=kname, kernel_ms=vol_ms, algorithmic_bytes=vbytes)
+                           frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(kname, P.name),
+                           kernel=kname, kernel_ms=k_ms, algorithmic_bytes=vbytes)
     out["schedule_fp64"] = sched
+    # the FP32 operator of the mixed solver (BF16 matrix-core projection)
+    tune32 = ctx.autotune(1)
+    w32, k32, _, _ = ctx.bench_operator(args.warmup, args.steps, precision=1)
+    b32 = volume_kernel_bytes(nT, nP, slm, 4)
+    out["operator_fp32"] = dict(kernel=ctx.kernel_name(1), kernel_ms=k32, dof_ordinate_per_s=nP * slm * args.steps / w32,
+                                achieved_gbs=b32 / (k32 * 1e-3) / 1e9, algorithmic_bytes=b32,
+                                schedule=ctx.schedule(1))
+    print("[bench] FP32 operator kernel %s %.1f us" % (ctx.kernel_name(1), k32 * 1e3), file=sys.stderr, flush=True)
     if not args.no_solve:
         solves = {}
         runs = [("reference_pcg_fp64_host", ta.TR_SOLVER_REFERENCE, 0, {}),
@@ -107,7 +133,6 @@ def main():
                 ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(inner_tol=1e-4, maxit=30))]
         for name, solver, prec, kw in runs:
             if solver == ta.TR_SOLVER_MIXED:
-                ctx.autotune(1)
                 out["schedule_fp32"] = ctx.schedule(1)
             d_f = ctx.allocVector()
             p = dict(maxit=20000, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000)

@@ -263,11 +263,15 @@ int tr_solve(tr_ctx *ctx, const tr_solve_params *prm, double *f,
 
 /* Seconds per operator application (median over reps), for benchmarking. */
 double tr_bench_lhs(tr_ctx *ctx, int reps);
-/* Times `steps` operator applications after `warmup` untimed ones, with HIP
- * events on the context's stream.  out[0] wall seconds of the timed loop
- * (device-synchronised on both sides), out[1] mean volume-kernel ms,
- * out[2] mean boundary-kernel ms, out[3] mean ms per application. */
-int tr_bench_operator(tr_ctx *ctx, int warmup, int steps, double *out4);
+/* Times `steps` operator applications (precision 0 FP64, 1 the FP32
+ * operator of the mixed solver) after `warmup` untimed ones, with HIP events
+ * on the context's stream around each launch.  out[0] wall seconds of the
+ * timed loop (device-synchronised on both sides), out[1] mean, out[2] min,
+ * out[3] max kernel ms. */
+int tr_bench_operator(tr_ctx *ctx, int precision, int warmup, int steps, double *out4);
+/* Name of the HIP kernel of the current operator schedule, as profilers
+ * print it (without the namespace and argument list). */
+int tr_ctx_kernel_name(tr_ctx *ctx, int precision, char *buf, size_t cap);
 /* A plain device copy of `bytes` timed with HIP events (out[0] = ms per
  * copy): the calibration kernel for memory-traffic counters. */
 int tr_bench_copy(tr_ctx *ctx, size_t bytes, int reps, double *out1);

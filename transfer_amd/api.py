@@ -118,7 +118,8 @@ _sig("tr_dot", C.c_double, vp, vp, vp)
 _sig("tr_synchronize", C.c_int, vp)
 _sig("tr_solve", C.c_int, vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats), f64p)
 _sig("tr_bench_lhs", C.c_double, vp, C.c_int)
-_sig("tr_bench_operator", C.c_int, vp, C.c_int, C.c_int, f64p)
+_sig("tr_bench_operator", C.c_int, vp, C.c_int, C.c_int, C.c_int, f64p)
+_sig("tr_ctx_kernel_name", C.c_int, vp, C.c_int, C.c_char_p, C.c_size_t)
 _sig("tr_bench_copy", C.c_int, vp, C.c_size_t, C.c_int, f64p)
 _sig("tr_stencil_create", C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), C.POINTER(vp))
 _sig("tr_stencil_destroy", C.c_int, vp)
@@ -455,11 +456,16 @@ class HipContext:
         check(lib.tr_bench_copy(self._h, int(nbytes), int(reps), ptr(out)))
         return float(out[0])
 
-    def bench_operator(self, warmup, steps):
-        """(wall s, volume-kernel ms, boundary-kernel ms, ms per application)"""
+    def bench_operator(self, warmup, steps, precision=0):
+        """(wall s, mean, min, max kernel ms) of the FP64 (0) or FP32 (1) operator"""
         out = np.zeros(4)
-        check(lib.tr_bench_operator(self._h, warmup, steps, ptr(out)))
+        check(lib.tr_bench_operator(self._h, int(precision), warmup, steps, ptr(out)))
         return tuple(out)
+
+    def kernel_name(self, precision=0):
+        buf = C.create_string_buffer(256)
+        check(lib.tr_ctx_kernel_name(self._h, int(precision), buf, len(buf)))
+        return buf.value.decode()
 
     def close(self):
         if getattr(self, "_h", None):

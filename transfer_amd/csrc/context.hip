@@ -41,6 +41,7 @@ struct tr_ctx {
     double *d_massfac = nullptr, *d_ddiag = nullptr, *d_bdiag = nullptr, *d_rhs = nullptr;
     double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
     double *d_crec = nullptr;   // coefficient records for the buffer-load schedule
+    double *d_bcrec = nullptr;  // boundary coefficient records (variant 4)
     void *d_tab = nullptr;      // contraction-table records
     void *d_tab4 = nullptr;     // MFMA B slices (variant 4, FP64)
     // FP32 copy of the operator data (mixed-precision solves), built lazily;
@@ -49,7 +50,7 @@ struct tr_ctx {
     tr::LaneMap map32;
     float *f_coef = nullptr, *f_bcoef = nullptr, *f_aval = nullptr, *f_bval = nullptr;
     float *f_massfac = nullptr, *f_ddiag = nullptr, *f_bdiag = nullptr;
-    float *f_crec = nullptr;
+    float *f_crec = nullptr, *f_bcrec = nullptr;
     void *f_tab = nullptr;
     void *f_tab4 = nullptr, *f_tab4b1 = nullptr, *f_tab4b3 = nullptr;  // variant 4: FP32, BF16, BF16 hi/lo
     int ang_prec = 3;  // FP32 operator's angular contraction: 0 FP32, 1 BF16, 3 BF16 hi/lo split
@@ -83,6 +84,7 @@ struct tr_ctx {
         a.crec = d_crec;
         a.tab = map.variant == 4 ? d_tab4 : d_tab;
         a.nsplit = 1;
+        a.bcrec = d_bcrec;
         return a;
     }
     tr::OpArgs<float> args32() const { return args32(map32); }
@@ -113,6 +115,7 @@ struct tr_ctx {
         a.crec = f_crec;
         a.tab = map32.variant != 4 ? f_tab : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
         a.nsplit = ang_prec == 1 ? 1 : 3;
+        a.bcrec = f_bcrec;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }
@@ -173,6 +176,31 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
     return best;
 }
 
+// Whether the matrix-core schedule covers this operator: ceil(slm/4) in
+// {2, 4, 7}, exactly 15 or 16 volume slots, at most TR_WBMAX boundary slots.
+static bool v4_ok(const HostOperator &op) {
+    return mfma_supported(op.slm, 0) && (op.W == 15 || op.W == 16) && op.WB <= TR_WBMAX &&
+           (op.NC == 7 || op.NC == 10);
+}
+
+// The matrix-core schedule of this context with its persistent grid sized
+// from the kernel's occupancy.
+static LaneMap mfma_map_ctx(const tr_ctx *c, int wpb, int mode) {
+    LaneMap m = mfma_map(c->op.slm, wpb, mode);
+    const int NC = (int)c->op.NC;
+    for (bool dot : {false, true}) {
+        uint32_t g;
+        if (mode == 0)
+            g = apply4_grid<double, 0>(c->args(m), NC, wpb, dot);
+        else if (mode == 1)
+            g = apply4_grid<float, 1>(c->args32(m), NC, wpb, dot);
+        else
+            g = apply4_grid<float, 2>(c->args32(m), NC, wpb, dot);
+        (dot ? m.grid_dot : m.grid) = g;
+    }
+    return m;
+}
+
 static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem, int nsplit = 3) {
     if (m.variant == 4) return apply4_lds_bytes(elem == 8 ? 0 : m.mfma_mode, (int)op.NC, op.slm, nsplit) + 64;
     if (m.variant == 3) return apply3_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem);
@@ -189,6 +217,18 @@ static std::vector<T> coef_records(const HostOperator &op) {
         for (uint32_t v = 0; v < op.nP; v++)
             for (uint32_t c = 0; c < op.NC; c++)
                 rec[((size_t)s * op.nP + v) * RS + c] = (T)op.coef[((size_t)s * op.NC + c) * op.nP + v];
+    return rec;
+}
+
+// [WB][nB][4] boundary coefficient records (3 channels, one pad) from the
+// [WB][3][nB] layout
+template <typename T>
+static std::vector<T> bcoef_records(const HostOperator &op) {
+    std::vector<T> rec((size_t)op.WB * op.nB * 4, (T)0);
+    for (uint32_t s = 0; s < op.WB; s++)
+        for (uint32_t b = 0; b < op.nB; b++)
+            for (int n = 0; n < 3; n++)
+                rec[((size_t)s * op.nB + b) * 4 + n] = (T)op.bcoef[((size_t)s * 3 + n) * op.nB + b];
     return rec;
 }
 
@@ -364,6 +404,7 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         (rc = upload(c, &c->d_ddiag, op.ddiag)) || (rc = upload(c, &c->d_bdiag, op.bdiag)) ||
         (rc = upload(c, &c->d_rhs, op.rhs)) || (rc = upload(c, &c->d_perm, op.perm)) ||
         (rc = upload(c, &c->d_iperm, op.iperm)) || (rc = upload(c, &c->d_crec, coef_records<double>(op))) ||
+        (rc = upload(c, &c->d_bcrec, bcoef_records<double>(op))) ||
         (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
         (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
         tr_ctx_destroy(c);
@@ -755,8 +796,8 @@ static int ensure_fp32(tr_ctx *c) {
     if (c->have32) return TR_OK;
     const HostOperator &op = c->op;
     c->map32 = choose_map(op.slm, 8);
-    if (c->ang_prec != 0 && mfma_supported(op.slm, 1) && (op.W == 15 || op.W == 16))
-        c->map32 = mfma_map(op.slm, 4, 2);
+    if (c->ang_prec != 0 && v4_ok(op))
+        c->map32 = mfma_map_ctx(c, 4, 2);
     if (op.W > 16) c->map32.variant = 1;
     if ((double)op.W * op.nP * rec_stride<float>((int)op.NC) * 4 >= 2147483648.0) c->map32.variant = 2;
     int maxlds = 65536;
@@ -767,6 +808,7 @@ static int ensure_fp32(tr_ctx *c) {
         (rc = upload(c, &c->f_aval, to_f32(op.aval))) || (rc = upload(c, &c->f_bval, to_f32(op.bval))) ||
         (rc = upload(c, &c->f_massfac, to_f32(op.massfac))) || (rc = upload(c, &c->f_ddiag, to_f32(op.ddiag))) ||
         (rc = upload(c, &c->f_bdiag, to_f32(op.bdiag))) || (rc = upload(c, &c->f_crec, coef_records<float>(op))) ||
+        (rc = upload(c, &c->f_bcrec, bcoef_records<float>(op))) ||
         (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
         return rc;
     if (mfma_supported(op.slm, 1) &&
@@ -904,43 +946,75 @@ double tr_bench_lhs(tr_ctx *c, int reps) {
 
 }  // extern "C"
 
-extern "C" int tr_bench_operator(tr_ctx *c, int warmup, int steps, double *out) {
-    if (!c || !out || steps < 1 || warmup < 0) return fail(TR_ERR_ARG, "tr_bench_operator: bad argument");
+extern "C" int tr_bench_operator(tr_ctx *c, int precision, int warmup, int steps, double *out) {
+    if (!c || !out || steps < 1 || warmup < 0 || precision < 0 || precision > 1)
+        return fail(TR_ERR_ARG, "tr_bench_operator: bad argument");
+    int rc;
+    if (precision == 1 && (rc = ensure_fp32(c))) return rc;
     Work w(c);
     double *f = w.get(), *r = w.get();
     if (!r) return fail(TR_ERR_HIP, "tr_bench_operator: out of device memory");
-    // a non-trivial input: f = rhs, then one application
+    // a non-trivial input: f = rhs (converted to FP32 for precision 1, in
+    // the first half of the FP64 buffer f; the result goes to r)
     HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, f, c->stream));
-    for (int i = 0; i < warmup; i++) {
-        int rc = lhs(c, f, r);
-        if (rc) return rc;
-    }
-    std::vector<hipEvent_t> ev(3 * (size_t)steps);
+    float *f32 = reinterpret_cast<float *>(r), *r32 = f32 + c->nint();
+    if (precision == 1) HIPCHECK(launch_to_f32_scaled(f, f32, 1.0, c->nint(), c->stream));
+    auto apply = [&]() -> hipError_t {
+        if (precision == 1)
+            return launch_apply<float>(c->args32(), (int)c->op.NC, c->map32, f32, r32, nullptr, nullptr, c->stream);
+        return launch_apply<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream);
+    };
+    for (int i = 0; i < warmup; i++) HIPCHECK(apply());
+    std::vector<hipEvent_t> ev(2 * (size_t)steps);
     for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipStreamSynchronize(c->stream));
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < steps; i++) {
-        HIPCHECK(hipEventRecord(ev[3 * i], c->stream));
-        HIPCHECK(launch_apply<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream));
-        HIPCHECK(hipEventRecord(ev[3 * i + 1], c->stream));
-        HIPCHECK(hipEventRecord(ev[3 * i + 2], c->stream));
+        HIPCHECK(hipEventRecord(ev[2 * i], c->stream));
+        HIPCHECK(apply());
+        HIPCHECK(hipEventRecord(ev[2 * i + 1], c->stream));
     }
     HIPCHECK(hipStreamSynchronize(c->stream));
     auto t1 = std::chrono::steady_clock::now();
-    double vol = 0, bnd = 0, all = 0;
+    double sum = 0, mn = 1e30, mx = 0;
     for (int i = 0; i < steps; i++) {
-        float a = 0, b = 0;
-        HIPCHECK(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
-        HIPCHECK(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
-        vol += a;
-        bnd += b;
-        all += a + b;
+        float a = 0;
+        HIPCHECK(hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]));
+        sum += a;
+        mn = std::min(mn, (double)a);
+        mx = std::max(mx, (double)a);
     }
     for (auto &e : ev) hipEventDestroy(e);
     out[0] = std::chrono::duration<double>(t1 - t0).count();
-    out[1] = vol / steps;
-    out[2] = bnd / steps;
-    out[3] = all / steps;
+    out[1] = sum / steps;
+    out[2] = mn;
+    out[3] = mx;
+    return TR_OK;
+}
+
+// HIP kernel name (as a profiler prints it) of the operator schedule in use.
+extern "C" int tr_ctx_kernel_name(tr_ctx *c, int precision, char *buf, size_t cap) {
+    if (!c || !buf || cap < 2 || precision < 0 || precision > 1) return fail(TR_ERR_ARG, "tr_ctx_kernel_name: bad argument");
+    if (precision == 1) {
+        int rc = ensure_fp32(c);
+        if (rc) return rc;
+    }
+    const LaneMap &m = precision ? c->map32 : c->map;
+    const char *T = precision ? "float" : "double";
+    const int NC = (int)c->op.NC, W = (int)c->op.W;
+    std::string name;
+    char tmp[160];
+    if (m.variant == 4)
+        snprintf(tmp, sizeof tmp, "k_apply4<%s, %d, %d, %d, %d, false>", T, precision ? m.mfma_mode : 0, m.J, NC, W);
+    else if (m.variant == 3 && (W == 15 || W == 16))
+        snprintf(tmp, sizeof tmp, "k_apply3<%s, %d, %d, %d, false>", T, m.J, NC, W);
+    else if (m.variant >= 2 && W <= TR_WMAX)
+        snprintf(tmp, sizeof tmp, "k_apply2<%s, %d, %d, false>", T, m.J, NC);
+    else
+        snprintf(tmp, sizeof tmp, "k_apply<%s, %d, %d, false>", T, m.J, NC);
+    name = tmp;
+    std::strncpy(buf, name.c_str(), cap - 1);
+    buf[cap - 1] = 0;
     return TR_OK;
 }
 
@@ -1015,7 +1089,7 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
     // exact schedules (FP64, or FP32 with ang_prec 0) race the VALU variants
     // against the FP64/FP32 MFMA one.
     std::vector<LaneMap> cands;
-    const bool bf16 = f32 && c->ang_prec != 0 && mfma_supported(op.slm, 1) && (op.W == 15 || op.W == 16);
+    const bool bf16 = f32 && c->ang_prec != 0 && v4_ok(op);
     if (!bf16)
         for (LaneMap m : map_candidates(op.slm, 11))
             for (int variant : {1, 2, 3})
@@ -1025,9 +1099,9 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
                     m.WPB = wpb;
                     cands.push_back(m);
                 }
-    if (mfma_supported(op.slm, f32) && (op.W == 15 || op.W == 16))
+    if (v4_ok(op))
         for (int wpb : {1, 2, 4, 8}) {
-            LaneMap m = mfma_map(op.slm, wpb, f32 ? (bf16 ? 2 : 1) : 0);
+            LaneMap m = mfma_map_ctx(c, wpb, f32 ? (bf16 ? 2 : 1) : 0);
             cands.push_back(m);
         }
     {
@@ -1119,10 +1193,9 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
     }
     const HostOperator &op = c->op;
     if (variant == 4) {
-        if (!mfma_supported(op.slm, precision) || (op.W != 15 && op.W != 16) ||
-            (WPB != 1 && WPB != 2 && WPB != 4 && WPB != 8))
+        if (!v4_ok(op) || (WPB != 1 && WPB != 2 && WPB != 4 && WPB != 8))
             return fail(TR_ERR_ARG, "tr_ctx_set_schedule: matrix-core schedule unsupported here");
-        LaneMap m = mfma_map(op.slm, WPB, precision ? (c->ang_prec == 0 ? 1 : 2) : 0);
+        LaneMap m = mfma_map_ctx(c, WPB, precision ? (c->ang_prec == 0 ? 1 : 2) : 0);
         int maxlds = 65536;
         hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
         if (lds_need(m, op, precision ? 4 : 8, c->ang_prec == 1 ? 1 : 3) > (size_t)maxlds)
@@ -1185,8 +1258,8 @@ extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
     c->ang_prec = prec;
     if (c->have32) {
         const HostOperator &op = c->op;
-        if (prec != 0 && mfma_supported(op.slm, 1) && (op.W == 15 || op.W == 16))
-            c->map32 = mfma_map(op.slm, 4, 2);
+        if (prec != 0 && v4_ok(op))
+            c->map32 = mfma_map_ctx(c, 4, 2);
         else {
             c->map32 = choose_map(op.slm, 8);
             if (op.W > 16) c->map32.variant = 1;

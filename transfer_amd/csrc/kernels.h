@@ -5,6 +5,11 @@
 
 #include <cstdint>
 
+// most ELL slots the batched schedules (variants 2-4) unroll
+#define TR_WMAX 16
+// most boundary-stencil slots the matrix-core schedule (variant 4) unrolls
+#define TR_WBMAX 12
+
 namespace tr {
 
 // Lane mapping of the vertex kernels: a wavefront of 64 lanes holds VPW
@@ -16,6 +21,8 @@ struct LaneMap {
                       // 3: batched with buffer loads and coefficient records (k_apply3);
                       // 4: angular contraction on the matrix cores (k_apply4)
     int mfma_mode = 1;  // FP32 operator under variant 4: 1 = FP32 MFMA, 2 = BF16 MFMA
+    uint32_t grid = 0;      // variant 4: workgroups of the persistent grid (0 = one per tile)
+    uint32_t grid_dot = 0;  // the same for the launch with the fused dot product
 };
 
 // Device operator data (see stencil.h), precision T.
@@ -40,6 +47,7 @@ struct OpArgs {
     const void *tab;       // variant 3: [(arow+brow)][slm] {T value; uint32 byte offset};
                            // variant 4: MFMA B slices (volume, then boundary)
     int nsplit;            // variant 4, BF16: 1 = plain, 3 = hi/lo split
+    const T *bcrec;        // variant 4: [WB][nB][4] boundary coefficient records (3 channels + pad)
 };
 
 // coefficient record stride (elements): NC rounded up to a 16-byte multiple
@@ -68,6 +76,8 @@ size_t apply2_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, u
                         size_t elem);
 size_t apply3_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, uint32_t brow,
                         size_t elem);
+// workgroups (= dot-product partials) of an operator launch with the fused
+// dot product
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m);
 
 // variant 4 (kernels_mfma.hip): J = ceil(slm/4) harmonics per lane
@@ -75,8 +85,12 @@ int mfma_J(uint32_t slm);
 bool mfma_supported(uint32_t slm, int mode);
 size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit);
 template <typename T, int MODE>
-hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, const T *f, T *r, double *part,
+hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, const T *f, T *r, double *part,
                          const double *skip, hipStream_t s);
+// workgroups of the persistent variant-4 grid for this operator and WPB,
+// for the launch without or with the fused dot product
+template <typename T, int MODE>
+uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, bool dot);
 
 // b[v][lm] = lm == 0 ? rhs[v] : 0
 hipError_t launch_rhs(const double *rhs, uint32_t nP, uint32_t slm, double *b,

@@ -10,6 +10,8 @@
 // angular couplings lane by lane from there.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace tr {
@@ -151,7 +153,6 @@ __device__ inline uint32_t xcd_block(uint32_t bid, uint32_t nblk) {
     return xcd * per + (xcd < rem ? xcd : rem) + k;
 }
 
-#define TR_WMAX 16
 
 // Operator, second schedule.  Gathers: the ELL slots are unrolled to TR_WMAX
 // so every neighbour index is loaded up front and the slab loads can be in
@@ -551,7 +552,8 @@ size_t apply3_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, u
 
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m) {
     uint32_t per = (uint32_t)(m.WPB * m.VPW);
-    return (nP + per - 1) / per;
+    const uint32_t tiles = (nP + per - 1) / per;
+    return (m.variant == 4 && m.grid_dot) ? std::min(m.grid_dot, tiles) : tiles;
 }
 
 #define TR_J_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(11)
@@ -562,11 +564,11 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
     if (a.nP == 0) return hipSuccess;
     if (m.variant == 4) {
         if constexpr (sizeof(T) == 8)
-            return launch_apply4<T, 0>(a, NC, m.WPB, f, r, part, skip, s);
+            return launch_apply4<T, 0>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
         else if (m.mfma_mode == 2)
-            return launch_apply4<T, 2>(a, NC, m.WPB, f, r, part, skip, s);
+            return launch_apply4<T, 2>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
         else
-            return launch_apply4<T, 1>(a, NC, m.WPB, f, r, part, skip, s);
+            return launch_apply4<T, 1>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
     }
     dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
     // variant 3 needs W = 15 or 16 exactly (the host pads the stencil) so that

@@ -21,6 +21,7 @@
 //   A_hi B_hi + A_lo B_hi + A_hi B_lo for ~16 significant bits.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "kernels.h"
@@ -89,102 +90,16 @@ __device__ inline float bf16_to_f(short h) {
     return __builtin_bit_cast(float, ((unsigned)(unsigned short)h) << 16);
 }
 
-// MODE: 0 = FP64 (T = double), 1 = FP32 MFMA, 2 = BF16 MFMA (T = float)
-template <typename T, int MODE, int J, int NC, int WT, bool DOT>
-__global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict__ f,
-                                                T *__restrict__ r, double *__restrict__ part,
-                                                const double *__restrict__ skip) {
-    if (skip && *skip != 0.0) return;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    constexpr int RS = rec_stride<T>(NC);
-    constexpr int S = (MODE == 2) ? (J + 3) / 4 : J;         // K-steps per channel
-    constexpr int BW = (MODE == 2) ? 8 : (int)sizeof(T);      // B bytes per lane and step
-    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
-    const int wpb = blockDim.x / TR_WAVE;
-    const uint32_t slm = a.slm, nP = a.nP;
-    const int NT = (int)((slm + 15) / 16);
-    const int nsplit = a.nsplit;
-    // stage the volume B slices: NC channels x S steps x NT tiles x 64 lanes
-    const uint32_t nvol = (uint32_t)NC * S * NT * TR_WAVE * BW * (MODE == 2 && nsplit > 1 ? 2 : 1);
-    {
-        const i32x4m *src = reinterpret_cast<const i32x4m *>(a.tab);
-        i32x4m *dst = reinterpret_cast<i32x4m *>(smem_raw);
-        for (uint32_t i = threadIdx.x; i < nvol / 16; i += blockDim.x) dst[i] = src[i];
-    }
-    __syncthreads();
-    const uint32_t blk = xcd_block4(blockIdx.x, gridDim.x);
-    const int vloc = lane & 15, q = lane >> 4;
-    const uint32_t v0 = (blk * (uint32_t)wpb + wave) * 16u;
-    const uint32_t v = v0 + vloc;
-    const bool valid = v < nP;
-    const int lm0 = q * J;
-    T acc[NC][J];
+// One channel group of the angular contraction: D[t] += A_c B_c over the
+// NCH channels of acc (lane layout as above), B slices at B (hi) and B + lo
+// (lo part of the BF16 split).
+template <typename T, int MODE, int J, int NCH, typename ACC4, int NTMAX>
+__device__ inline void contract(const T (&acc)[NCH][J], const unsigned char *B, size_t lo, int NT, int lane,
+                                int nsplit, ACC4 (&D)[NTMAX]) {
+    constexpr int S = (MODE == 2) ? (J + 3) / 4 : J;
+    constexpr int BW = (MODE == 2) ? 8 : (int)sizeof(T);
 #pragma unroll
-    for (int c = 0; c < NC; c++)
-#pragma unroll
-        for (int j = 0; j < J; j++) acc[c][j] = 0;
-    if (valid) {
-        const auto frs = rsrc4(f, (size_t)nP * slm * sizeof(T));
-        const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
-        const auto crs = rsrc4(a.crec, (size_t)a.W * nP * RS * sizeof(T));
-        uint32_t nb[WT];
-#pragma unroll
-        for (int s = 0; s < WT; s++)
-            nb[s] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrs, v * 4u, (uint32_t)s * nP * 4u, 0);
-        const uint32_t cvoff = v * (uint32_t)(RS * sizeof(T));
-        const uint32_t cstride = nP * (uint32_t)(RS * sizeof(T));
-        // two slots in flight: slot s+1's loads are issued before slot s's
-        // FMAs.  Left alone, the optimiser issues every slot's loads first and
-        // sinks all FMAs below them, which holds W slots of data live and
-        // spills; empty volatile asm statements (which keep their relative
-        // order) pin each slot's FMAs before the next slot's load offsets.
-        T xa[J], ca[RS], xb[J], cb[RS];
-        auto load = [&](int s, T *x, T *cf) {
-            uint32_t base = (nb[s] * slm + lm0) * (uint32_t)sizeof(T);
-            uint32_t cv = cvoff;
-            asm volatile("" : "+v"(base), "+v"(cv));
-#pragma unroll
-            for (int j = 0; j < J; j++) x[j] = bload1<T>(frs, base + j * (uint32_t)sizeof(T), 0u);
-            brec<T, RS>(crs, cv, (uint32_t)s * cstride, cf);
-        };
-        auto accum = [&](const T *x, const T *cf) {
-#pragma unroll
-            for (int c = 0; c < NC; c++)
-#pragma unroll
-                for (int j = 0; j < J; j++) acc[c][j] = fma(cf[c], x[j], acc[c][j]);
-#pragma unroll
-            for (int c = 0; c < NC; c++)
-#pragma unroll
-                for (int j = 0; j < J; j++) asm volatile("" : "+v"(acc[c][j]));
-        };
-        load(0, xa, ca);
-#pragma unroll
-        for (int s = 0; s < WT; s += 2) {
-            if (s + 1 < WT) load(s + 1, xb, cb);
-            __builtin_amdgcn_sched_barrier(0);
-            accum(xa, ca);
-            __builtin_amdgcn_sched_barrier(0);
-            if (s + 2 < WT) load(s + 2, xa, ca);
-            __builtin_amdgcn_sched_barrier(0);
-            if (s + 1 < WT) accum(xb, cb);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        // harmonics past slm (the last chunk) carry neighbouring data: zero them
-#pragma unroll
-        for (int j = 0; j < J; j++)
-            if (lm0 + j >= (int)slm)
-#pragma unroll
-                for (int c = 0; c < NC; c++) acc[c][j] = 0;
-    }
-    // ---- angular contraction on the matrix cores
-    constexpr int NTMAX = (J * 4 + 15) / 16;
-    typedef typename std::conditional<MODE == 0, d4v, f4v>::type acc4;
-    acc4 D[NTMAX];
-#pragma unroll
-    for (int t = 0; t < NTMAX; t++) D[t] = (acc4){0, 0, 0, 0};
-    const unsigned char *Bv = smem_raw;
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
+    for (int c = 0; c < NCH; c++) {
 #pragma unroll
         for (int st = 0; st < S; st++) {
 #pragma unroll
@@ -192,13 +107,13 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
                 if (t >= NT) break;
                 const size_t bo = ((((size_t)c * S + st) * NT + t) * TR_WAVE + lane) * BW;
                 if constexpr (MODE == 0) {
-                    const double bv = *reinterpret_cast<const double *>(Bv + bo);
+                    const double bv = *reinterpret_cast<const double *>(B + bo);
                     D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[c][st], bv, D[t], 0, 0, 0);
                 } else if constexpr (MODE == 1) {
-                    const float bv = *reinterpret_cast<const float *>(Bv + bo);
+                    const float bv = *reinterpret_cast<const float *>(B + bo);
                     D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(acc[c][st], bv, D[t], 0, 0, 0);
                 } else {
-                    s4v ah, al, bh = *reinterpret_cast<const s4v *>(Bv + bo);
+                    s4v ah, al, bh = *reinterpret_cast<const s4v *>(B + bo);
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const float x = (4 * st + u < J) ? (float)acc[c][4 * st + u] : 0.0f;
@@ -207,7 +122,7 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
                     }
                     D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, D[t], 0, 0, 0);
                     if (nsplit > 1) {
-                        const s4v bl = *reinterpret_cast<const s4v *>(Bv + (size_t)NC * S * NT * TR_WAVE * BW + bo);
+                        const s4v bl = *reinterpret_cast<const s4v *>(B + lo + bo);
                         D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, D[t], 0, 0, 0);
                         D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
                     }
@@ -218,77 +133,195 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
         // MFMAs would hold every slice in registers at once
         __builtin_amdgcn_sched_barrier(0);
     }
-    // ---- vacuum boundary: only wavefronts that hold boundary vertices
-    const int b = valid ? a.bof[v] : -1;
-    if (__any(b >= 0)) {
-        T bacc[3][J];
+}
+
+// Neighbour indices of `nslot` slots (nslot <= NS, wavefront-uniform): slot
+// s at nrs (voff, s*nstride).
+template <int NS>
+__device__ inline void load_idx(uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t nrs, uint32_t voff, uint32_t nstride,
+                                int nslot) {
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+    for (int s = 0; s < NS; s++)
+        nb[s] = s < nslot ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrs, voff, (uint32_t)s * nstride, 0) : 0u;
+}
+
+// Pipelined ELL gather of one lane: acc[c][j] += sum_s cf_s[c] f[nb_s][lm0+j]
+// over `nslot` slots (nslot <= NS, a wavefront-uniform count), coefficient
+// record (RS elements) of slot s at crs (coff, s*cstride).  DEPTH slots are
+// in flight: slot s+DEPTH's loads are issued right after slot s's FMAs, so a
+// wavefront waits for one memory round trip per DEPTH slots.  Left alone, the
+// optimiser issues every slot's loads first and sinks all FMAs below them,
+// which holds NS slots of data live and spills; empty volatile asm
+// statements (which keep their relative order) pin each slot's FMAs before
+// the next load offsets are formed.
+template <typename T, int J, int NCH, int RS, int NS, int DEPTH>
+__device__ inline void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
+                              __amdgpu_buffer_rsrc_t crs, uint32_t coff, uint32_t cstride, int nslot, uint32_t slm,
+                              int lm0) {
+    T xs[DEPTH][J], cs[DEPTH][RS];
+    auto load = [&](int s, T *x, T *cf) {
+        uint32_t base = (nb[s] * slm + lm0) * (uint32_t)sizeof(T);
+        uint32_t cv = coff;
+        asm volatile("" : "+v"(base), "+v"(cv));
 #pragma unroll
-            for (int j = 0; j < J; j++) bacc[c][j] = 0;
-        if (b >= 0) {
-            const size_t nB = a.nB;
-            for (uint32_t s = 0; s < a.WB; s++) {
-                const uint32_t k = a.bnbr[s * nB + b];
-                const T *fk = f + (size_t)k * slm + lm0;
-                T x[J];
+        for (int j = 0; j < J; j++) x[j] = bload1<T>(frs, base + j * (uint32_t)sizeof(T), 0u);
+        brec<T, RS>(crs, cv, (uint32_t)s * cstride, cf);
+    };
+    auto accum = [&](const T *x, const T *cf) {
 #pragma unroll
-                for (int j = 0; j < J; j++) x[j] = (lm0 + j < (int)slm) ? fk[j] : (T)0;
+        for (int c = 0; c < NCH; c++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const T cf = a.bcoef[((size_t)s * 3 + c) * nB + b];
+            for (int j = 0; j < J; j++) acc[c][j] = fma(cf[c], x[j], acc[c][j]);
 #pragma unroll
-                    for (int j = 0; j < J; j++) bacc[c][j] = fma(cf, x[j], bacc[c][j]);
-                }
-            }
-        }
-        // boundary B slices come from global memory (few wavefronts read them)
-        const unsigned char *Bb = reinterpret_cast<const unsigned char *>(a.tab) + nvol;
+        for (int c = 0; c < NCH; c++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
+            for (int j = 0; j < J; j++) asm volatile("" : "+v"(acc[c][j]));
+    };
 #pragma unroll
-            for (int st = 0; st < S; st++) {
+    for (int d = 0; d < DEPTH; d++)
+        if (d < NS && d < nslot) load(d, xs[d], cs[d]);
 #pragma unroll
-                for (int t = 0; t < NTMAX; t++) {
-                    if (t >= NT) break;
-                    const size_t bo = ((((size_t)c * S + st) * NT + t) * TR_WAVE + lane) * BW;
-                    if constexpr (MODE == 0) {
-                        D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(bacc[c][st], *reinterpret_cast<const double *>(Bb + bo), D[t], 0, 0, 0);
-                    } else if constexpr (MODE == 1) {
-                        D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bacc[c][st], *reinterpret_cast<const float *>(Bb + bo), D[t], 0, 0, 0);
-                    } else {
-                        s4v ah, al, bh = *reinterpret_cast<const s4v *>(Bb + bo);
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const float x = (4 * st + u < J) ? (float)bacc[c][4 * st + u] : 0.0f;
-                            ah[u] = bf16_rne(x);
-                            al[u] = bf16_rne(x - bf16_to_f(ah[u]));
-                        }
-                        D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, D[t], 0, 0, 0);
-                        if (nsplit > 1) {
-                            const s4v bl = *reinterpret_cast<const s4v *>(Bb + (size_t)3 * S * NT * TR_WAVE * BW + bo);
-                            D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, D[t], 0, 0, 0);
-                            D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
-                        }
-                    }
-                }
-            }
-            __builtin_amdgcn_sched_barrier(0);
+    for (int s = 0; s < NS; s++) {
+        if (s < nslot) {
+            accum(xs[s % DEPTH], cs[s % DEPTH]);
+            if (s + DEPTH < NS && s + DEPTH < nslot) load(s + DEPTH, xs[s % DEPTH], cs[s % DEPTH]);
         }
     }
-    // ---- write: FP64 D holds rows (l/16) + 4i, FP32 rows 4(l/16) + i; column l % 16
+    // harmonics past slm (the last chunk) carry neighbouring data: zero them
+#pragma unroll
+    for (int j = 0; j < J; j++)
+        if (lm0 + j >= (int)slm)
+#pragma unroll
+            for (int c = 0; c < NCH; c++) acc[c][j] = 0;
+}
+
+// MODE: 0 = FP64 (T = double), 1 = FP32 MFMA, 2 = BF16 MFMA (T = float).
+// A persistent grid: every workgroup stages the B slices in LDS once
+// and then walks vertex tiles (WPB wavefronts x 16 vertices).  With a grid
+// that is a multiple of the 8 XCDs, the workgroups of one XCD (block ids
+// congruent mod 8) share a contiguous range of tiles, so neighbouring
+// vertices' data stay in that XCD's L2.
+template <typename T, int MODE, int J, int NC, int WT, bool DOT>
+__global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict__ f,
+                                                T *__restrict__ r, double *__restrict__ part,
+                                                const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int RS = rec_stride<T>(NC);
+    constexpr int S = (MODE == 2) ? (J + 3) / 4 : J;
+    constexpr int BW = (MODE == 2) ? 8 : (int)sizeof(T);
+    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
+    const int wpb = blockDim.x / TR_WAVE;
+    const uint32_t slm = a.slm, nP = a.nP;
+    const int NT = (int)((slm + 15) / 16);
+    const int nsplit = a.nsplit;
+    const size_t vsec = (size_t)NC * S * NT * TR_WAVE * BW;  // one part of the volume B table
+    const size_t bsec = (size_t)3 * S * NT * TR_WAVE * BW;   // one part of the boundary B table
+    const int parts = (MODE == 2 && nsplit > 1) ? 2 : 1;
+    {
+        // volume then boundary slices, every part: the whole table
+        const uint32_t nall = (uint32_t)((vsec + bsec) * parts);
+        const i32x4m *src = reinterpret_cast<const i32x4m *>(a.tab);
+        i32x4m *dst = reinterpret_cast<i32x4m *>(smem_raw);
+        for (uint32_t i = threadIdx.x; i < nall / 16; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const auto frs = rsrc4(f, (size_t)nP * slm * sizeof(T));
+    const auto brs = rsrc4(a.bof, (size_t)nP * 4);
+    const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
+    const auto crs = rsrc4(a.crec, (size_t)a.W * nP * RS * sizeof(T));
+    const auto bnrs = rsrc4(a.bnbr, (size_t)a.WB * a.nB * 4);
+    const auto bcrs = rsrc4(a.bcrec, (size_t)a.WB * a.nB * 4 * sizeof(T));
+    const int vloc = lane & 15, q = lane >> 4, lm0 = q * J;
+    constexpr int NTMAX = (J * 4 + 15) / 16;
+    typedef typename std::conditional<MODE == 0, d4v, f4v>::type acc4;
+    // register budget: 2 wavefronts per SIMD for FP64 at J = 7, 3 otherwise
+    constexpr int BUDGET = (sizeof(T) == 8 && J >= 7) ? 240 : 148;
+    constexpr int ACCR = NC * J * (int)sizeof(T) / 4, SLOTR = (J + RS) * (int)sizeof(T) / 4;
+    constexpr int DEPTH = std::max(2, std::min(6, (BUDGET - ACCR - 20) / SLOTR));
+    constexpr int BACCR = 3 * J * (int)sizeof(T) / 4, BSLOTR = (J + 4) * (int)sizeof(T) / 4;
+    constexpr int DEPTHB = std::max(2, std::min(4, (BUDGET - BACCR - 8 * NTMAX - 30) / BSLOTR));
+
+    const uint32_t ntiles = (nP + 16u * wpb - 1) / (16u * wpb);
+    const uint32_t G = gridDim.x, X = 8;
+    uint32_t t0 = blockIdx.x, t1 = ntiles, tstep = G;
+    if (G % X == 0 && G <= ntiles) {
+        const uint32_t xcd = blockIdx.x % X, per = ntiles / X, rem = ntiles % X;
+        const uint32_t lo = xcd * per + (xcd < rem ? xcd : rem);
+        t0 = lo + blockIdx.x / X;
+        t1 = lo + per + (xcd < rem ? 1u : 0u);
+        tstep = G / X;
+    }
+    // neighbour indices and boundary row of a tile's vertex (this lane's)
+    auto tile_idx = [&](uint32_t tile, uint32_t (&nb)[WT], int &b) {
+        uint32_t v = (tile * (uint32_t)wpb + wave) * 16u + vloc;
+        asm volatile("" : "+v"(v));
+        if (v < nP) {
+            load_idx<WT>(nb, nrs, v * 4u, nP * 4u, WT);
+            b = (int)__builtin_amdgcn_raw_buffer_load_b32(brs, v * 4u, 0u, 0);
+        } else {
+#pragma unroll
+            for (int s = 0; s < WT; s++) nb[s] = 0;
+            b = -1;
+        }
+    };
+    uint32_t nbc[WT];
+    int bcur = -1;
+    if (t0 < t1) tile_idx(t0, nbc, bcur);
     double d = 0;
+    for (uint32_t tile = t0; tile < t1; tile += tstep) {
+        const uint32_t v0 = (tile * (uint32_t)wpb + wave) * 16u;
+        const uint32_t v = v0 + vloc;
+        const bool valid = v < nP;
+        // ---- volume: gather, then the contraction on the matrix cores
+        T acc[NC][J];
 #pragma unroll
-    for (int t = 0; t < NTMAX; t++) {
-        if (t >= NT) break;
-        const uint32_t n = 16u * t + (uint32_t)vloc;
+        for (int c = 0; c < NC; c++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint32_t vr = v0 + (MODE == 0 ? (uint32_t)(q + 4 * i) : (uint32_t)(4 * q + i));
-            if (n < slm && vr < nP) {
-                const T val = (T)D[t][i];
-                r[(size_t)vr * slm + n] = val;
-                if (DOT) d = fma((double)f[(size_t)vr * slm + n], (double)val, d);
+            for (int j = 0; j < J; j++) acc[c][j] = 0;
+        if (valid)
+            gather<T, J, NC, RS, WT, DEPTH>(acc, nbc, frs, crs, v * (uint32_t)(RS * sizeof(T)),
+                                             nP * (uint32_t)(RS * sizeof(T)), WT, slm, lm0);
+        // the next tile's indices load while this tile contracts
+        const int b = bcur;
+        if (tile + tstep < t1) tile_idx(tile + tstep, nbc, bcur);
+        acc4 D[NTMAX];
+#pragma unroll
+        for (int t = 0; t < NTMAX; t++) D[t] = (acc4){0, 0, 0, 0};
+        // the B slices are the same for every tile: an opaque zero offset
+        // keeps the optimiser from hoisting all of their loads out of the
+        // tile loop into registers
+        uint32_t boff = 0;
+        asm volatile("" : "+v"(boff));
+        contract<T, MODE, J, NC>(acc, smem_raw + boff, vsec, NT, lane, nsplit, D);
+        // ---- vacuum boundary: only wavefronts that hold boundary vertices
+        if (__any(b >= 0)) {
+            T bacc[3][J];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) bacc[c][j] = 0;
+            if (b >= 0) {
+                uint32_t bn[TR_WBMAX];
+                load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, (int)a.WB);
+                gather<T, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, bcrs, (uint32_t)b * (uint32_t)(4 * sizeof(T)),
+                                                     a.nB * (uint32_t)(4 * sizeof(T)), (int)a.WB, slm, lm0);
+            }
+            contract<T, MODE, J, 3>(bacc, smem_raw + vsec * parts + boff, bsec, NT, lane, nsplit, D);
+        }
+        // ---- write: FP64 D holds rows (l/16) + 4i, FP32 rows 4(l/16) + i; column l % 16
+#pragma unroll
+        for (int t = 0; t < NTMAX; t++) {
+            if (t >= NT) break;
+            const uint32_t n = 16u * t + (uint32_t)vloc;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t vr = v0 + (MODE == 0 ? (uint32_t)(q + 4 * i) : (uint32_t)(4 * q + i));
+                if (n < slm && vr < nP) {
+                    const T val = (T)D[t][i];
+                    r[(size_t)vr * slm + n] = val;
+                    if (DOT) d = fma((double)f[(size_t)vr * slm + n], (double)val, d);
+                }
             }
         }
     }
@@ -314,59 +347,81 @@ bool mfma_supported(uint32_t slm, int precision_mode) {
     return J == 2 || J == 4 || J == 7;
 }
 
-// bytes of the volume B table staged in LDS
+// bytes of the B tables (volume and boundary slices) staged in LDS
 size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit) {
     const int J = mfma_J(slm), S = mode == 2 ? (J + 3) / 4 : J, BW = mode == 2 ? 8 : (mode == 0 ? 8 : 4);
     const int NT = (int)((slm + 15) / 16);
-    return (size_t)NC * S * NT * 64 * BW * (mode == 2 && nsplit > 1 ? 2 : 1);
+    return (size_t)(NC + 3) * S * NT * 64 * BW * (mode == 2 && nsplit > 1 ? 2 : 1);
+}
+
+// The kernel instantiation for (J, NC, W) with DOT = D, or null.
+template <typename T, int MODE, bool D>
+static const void *apply4_kernel(int J, int NC, uint32_t W) {
+#define TR_K4(JJ, NCC, WW) reinterpret_cast<const void *>(&k_apply4<T, MODE, JJ, NCC, WW, D>)
+#define TR_K4W(JJ, NCC) (W == 15 ? TR_K4(JJ, NCC, 15) : TR_K4(JJ, NCC, 16))
+#define TR_K4N(JJ) (NC == 7 ? TR_K4W(JJ, 7) : TR_K4W(JJ, 10))
+    if ((W != 15 && W != 16) || (NC != 7 && NC != 10)) return nullptr;
+    if (J == 2) return TR_K4N(2);
+    if (J == 4) return TR_K4N(4);
+    if (J == 7) return TR_K4N(7);
+    return nullptr;
+#undef TR_K4N
+#undef TR_K4W
+#undef TR_K4
+}
+
+// Persistent grid of the variant-4 operator (the kernel without or with the
+// fused dot product): the workgroups resident on the device at once
+// (occupancy x CUs), rounded down to a multiple of the 8 XCDs, at most one
+// per tile.
+template <typename T, int MODE>
+uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, bool dot) {
+    const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
+    const void *k = dot ? apply4_kernel<T, MODE, true>(mfma_J(a.slm), NC, a.W)
+                        : apply4_kernel<T, MODE, false>(mfma_J(a.slm), NC, a.W);
+    if (!k || ntiles == 0) return ntiles;
+    const size_t sh = apply4_lds_bytes(MODE, NC, a.slm, a.nsplit);
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, TR_WAVE * wpb, sh) != hipSuccess || nb < 1 ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        return ntiles;
+    }
+    uint32_t g = (uint32_t)nb * (uint32_t)cus;
+    if (g >= 8) g -= g % 8;
+    return std::min(g, ntiles);
 }
 
 template <typename T, int MODE>
-hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, const T *f, T *r, double *part,
+hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, const T *f, T *r, double *part,
                          const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
-    const uint32_t per = 16u * (uint32_t)wpb;
-    dim3 g((a.nP + per - 1) / per), b(TR_WAVE * wpb);
+    const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
+    // the unrolled slot loops read exactly W = 15 or 16 volume slots and at
+    // most TR_WBMAX boundary slots; coefficient records and B slices must be
+    // present
+    if ((a.W != 15 && a.W != 16) || a.WB > TR_WBMAX || !a.crec || !a.tab || (a.nB && !a.bcrec) || wpb < 1 ||
+        wpb > 8)
+        return hipErrorInvalidValue;
+    dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
     const size_t sh = apply4_lds_bytes(MODE, NC, a.slm, a.nsplit);
-    const int J = mfma_J(a.slm);
-    // the unrolled slot loop reads exactly W = 15 or 16 slots; coefficient
-    // records and B slices must be present
-    if ((a.W != 15 && a.W != 16) || !a.crec || !a.tab || wpb < 1 || wpb > 8) return hipErrorInvalidValue;
-#define TR_L4(JJ, NCC, WW, D) \
-    hipLaunchKernelGGL((k_apply4<T, MODE, JJ, NCC, WW, D>), g, b, sh, s, a, f, r, part, skip)
-#define TR_L4W(JJ, NCC, D)                        \
-    do {                                          \
-        if (a.W == 15) TR_L4(JJ, NCC, 15, D);     \
-        else TR_L4(JJ, NCC, 16, D);               \
-    } while (0)
-#define TR_L4N(JJ, D)                             \
-    do {                                          \
-        if (NC == 7) TR_L4W(JJ, 7, D);            \
-        else TR_L4W(JJ, 10, D);                   \
-    } while (0)
-#define TR_L4J(D)                                 \
-    do {                                          \
-        if (J == 2) TR_L4N(2, D);                 \
-        else if (J == 4) TR_L4N(4, D);            \
-        else if (J == 7) TR_L4N(7, D);            \
-        else return hipErrorInvalidValue;         \
-    } while (0)
-    if (part)
-        TR_L4J(true);
-    else
-        TR_L4J(false);
-#undef TR_L4J
-#undef TR_L4N
-#undef TR_L4W
-#undef TR_L4
-    return hipGetLastError();
+    const void *k = part ? apply4_kernel<T, MODE, true>(mfma_J(a.slm), NC, a.W)
+                         : apply4_kernel<T, MODE, false>(mfma_J(a.slm), NC, a.W);
+    if (!k) return hipErrorInvalidValue;
+    OpArgs<T> aa = a;
+    void *args[] = {&aa, &f, &r, &part, &skip};
+    return hipLaunchKernel(k, g, b, args, sh, s);
 }
 
-template hipError_t launch_apply4<double, 0>(const OpArgs<double> &, int, int, const double *, double *,
+template hipError_t launch_apply4<double, 0>(const OpArgs<double> &, int, int, uint32_t, const double *, double *,
                                              double *, const double *, hipStream_t);
-template hipError_t launch_apply4<float, 1>(const OpArgs<float> &, int, int, const float *, float *,
+template hipError_t launch_apply4<float, 1>(const OpArgs<float> &, int, int, uint32_t, const float *, float *,
                                             double *, const double *, hipStream_t);
-template hipError_t launch_apply4<float, 2>(const OpArgs<float> &, int, int, const float *, float *,
+template hipError_t launch_apply4<float, 2>(const OpArgs<float> &, int, int, uint32_t, const float *, float *,
                                             double *, const double *, hipStream_t);
+template uint32_t apply4_grid<double, 0>(const OpArgs<double> &, int, int, bool);
+template uint32_t apply4_grid<float, 1>(const OpArgs<float> &, int, int, bool);
+template uint32_t apply4_grid<float, 2>(const OpArgs<float> &, int, int, bool);
 
 }  // namespace tr
