@@ -27,11 +27,12 @@ METRIC = "time-to-1e-8 residual & DOF·ordinate/s, learned+mixed vs plain FP64, 
 HBM_PEAK_GBS = 8000.0
 
 
-def volume_kernel_bytes(nT, nP, slm, p):
-    """Algorithmic bytes of one operator application's volume part: per tet
-    4 u32 indices + 1/(kappa V) + 3 stored s rows (16 + 10p), and f read /
-    r written once (2 nP slm p)."""
-    return nT * (16 + 10 * p) + 2 * nP * slm * p
+def operator_bytes(nT, nP, nF, slm, p):
+    """Algorithmic bytes of one operator application (the tet-centric
+    minimum): per tet 4 u32 indices + 1/(kappa V) + 3 stored s rows
+    (16 + 10p), f read and r written once (2 nP slm p), per boundary face
+    3 u32 indices + its area vector's axis weight (12 + p)."""
+    return nT * (16 + 10 * p) + 2 * nP * slm * p + nF * (12 + p)
 
 
 def measured_traffic(kernel, workload):
@@ -102,7 +103,7 @@ def main():
     wall, k_ms, k_min, k_max = ctx.bench_operator(args.warmup, args.steps, precision=0)
     value = nP * slm * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
-    vbytes = volume_kernel_bytes(nT, nP, slm, 8)
+    vbytes = operator_bytes(nT, nP, P.mesh.nF, slm, 8)
     achieved = vbytes / (k_ms * 1e-3) / 1e9
     kname = ctx.kernel_name(0)
     print("[bench] FP64 operator %.1f us/step wall, kernel %s %.1f us (min %.1f, max %.1f)"
@@ -120,7 +121,7 @@ def main():
     # the FP32 operator of the mixed solver (BF16 matrix-core projection)
     tune32 = ctx.autotune(1)
     w32, k32, _, _ = ctx.bench_operator(args.warmup, args.steps, precision=1)
-    b32 = volume_kernel_bytes(nT, nP, slm, 4)
+    b32 = operator_bytes(nT, nP, P.mesh.nF, slm, 4)
     out["operator_fp32"] = dict(kernel=ctx.kernel_name(1), kernel_ms=k32, dof_ordinate_per_s=nP * slm * args.steps / w32,
                                 achieved_gbs=b32 / (k32 * 1e-3) / 1e9, algorithmic_bytes=b32,
                                 schedule=ctx.schedule(1))
