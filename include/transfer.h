@@ -208,6 +208,13 @@ int tr_compute_rhs(tr_ctx *ctx, double *b);                        /* :151-158 *
 /* computeLhs with the operator in FP32 (precision 1; the inner operator of
  * the mixed solver) or FP64 (precision 0); f and Af stay FP64 vectors. */
 int tr_compute_lhs_prec(tr_ctx *ctx, int precision, const double *f, double *Af);
+/* MeshData::ComputeMoments (MeshData.cpp:96-116) of the device vector f:
+ * device arrays U[nP] = 4 pi f[00] and T[3i+j][nP], caller's vertex order. */
+int tr_compute_moments(tr_ctx *ctx, const double *f, double *U, double *T);
+/* MeshData::ComputeFlux (MeshData.cpp:118-136): device W[j][nT] from the
+ * device T of tr_compute_moments and the mesh view the context was built
+ * from. */
+int tr_compute_flux(tr_ctx *ctx, const tr_mesh_view *mesh, const double *T, double *W);
 int tr_scale(tr_ctx *ctx, double *x, double wx);                   /* :188-196 */
 int tr_add_prod(tr_ctx *ctx, double *x, const double *y, double wy);          /* :198-206 */
 int tr_mul_add(tr_ctx *ctx, double *x, double wx, const double *y);           /* :208-216 */
@@ -287,6 +294,11 @@ int tr_stencil_create(const tr_mesh_view *mesh, const tr_angular_view *ang,
 int tr_stencil_destroy(tr_stencil *s);
 /* info: nP, slm, NC, W, nB, WB, arow, brow */
 int tr_stencil_info(const tr_stencil *s, uint32_t *info8);
+/* Specular reflection (extension; faces of kind 1 on axis-aligned planes):
+ * the entries v*slm + lm of the internal [v][slm] numbering that vanish,
+ * the harmonics odd under a reflecting plane's mirror at its vertices.
+ * *count receives the number; list (may be NULL) receives the entries. */
+int tr_stencil_mask(const tr_stencil *s, uint32_t *count, uint32_t *list);
 /* Any pointer may be NULL.  perm[nP]; nbr[W][nP]; coef[W][NC][nP];
  * bvert[nB]; bnbr[WB][nB]; bcoef[WB][3][nB]; omc[NC-1][slm][slm];
  * onc[3][slm][slm]; massfac[slm]; rhs[nP] (all in the internal numbering) */

@@ -69,7 +69,7 @@ class Problem(C.Structure):
                 ("facepos", C.c_void_p), ("bnd", C.c_void_p), ("slm", C.c_uint32),
                 ("aslm", C.c_uint32), ("omega", C.c_void_p), ("omega_pos", C.c_void_p),
                 ("Ox", C.c_void_p), ("Oy", C.c_void_p), ("Oz", C.c_void_p),
-                ("massfac", C.c_void_p), ("nthreads", C.c_int)]
+                ("massfac", C.c_void_p), ("nthreads", C.c_int), ("face_kind", C.c_void_p)]
 
 
 class PcgParams(C.Structure):
@@ -147,7 +147,8 @@ class OracleProblem:
                           _p(ar["tetpos"]), _p(ar["mesh"]), _p(ar["facestart"]), _p(ar["faceidx"]),
                           _p(ar["facepos"]), _p(ar["bnd"]), s, a, _p(self.omega_d), _p(self.pos_d),
                           _p(self.Ox_d), _p(self.Oy_d), _p(self.Oz_d),
-                          _p(self.massfac) if self.massfac is not None else None, nthreads)
+                          _p(self.massfac) if self.massfac is not None else None, nthreads,
+                          _p(ar["face_kind"]) if "face_kind" in ar and len(ar["face_kind"]) else None)
 
     def N(self):
         return self.nP * self.aslm

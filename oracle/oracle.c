@@ -13,7 +13,7 @@
  *                      440-493 (surfacePartDiag)
  *   BLAS-1/reductions  kernels.cu:12-115, CudaContext.cpp:188-250
  *   PCG / invprec      main.cpp:125-162, 223-248
- *   moments / flux     MeshData.cpp:153-193
+ *   moments / flux     MeshData.cpp:96-136
  *   step analytic      main.cpp:498-515, E1.cpp:136-159 (with a correct E_n)
  *
  * Extensions beyond the reference (scattering, relative stopping rule) are
@@ -437,6 +437,7 @@ void or_surface_part(const or_problem *pr, const double *f, double *r) {
             for (or_idx lm = 0; lm < aslm; lm++) sum[lm] = 0;
             for (or_idx j = lo; j < hi; j++) {
                 or_idx local = pr->facepos[j];
+                if (pr->face_kind && pr->face_kind[pr->faceidx[j]] != 0) continue;  /* extension */
                 const or_face *tr = &pr->bnd[pr->faceidx[j]];
                 double surf;
                 const double *On = pick_axis(pr, tr, &surf);
@@ -472,6 +473,7 @@ void or_surface_part_diag(const or_problem *pr, const double *f, double *r) {
             double sum = 0;
             for (or_idx j = lo; j < hi; j++) {
                 or_idx local = pr->facepos[j];
+                if (pr->face_kind && pr->face_kind[pr->faceidx[j]] != 0) continue;  /* extension */
                 const or_face *tr = &pr->bnd[pr->faceidx[j]];
                 double surf;
                 const double *On = pick_axis(pr, tr, &surf);
@@ -640,7 +642,7 @@ int or_pcg(const or_problem *pr, const or_pcg_params *prm, double *f,
 }
 
 /* ===================================================================== */
-/*  Post-processing (MeshData.cpp:153-193)                                */
+/*  Post-processing (MeshData.cpp:96-136)                                 */
 /* ===================================================================== */
 
 void or_compute_moments(const or_problem *pr, const double *omega_host,

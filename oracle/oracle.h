@@ -69,6 +69,11 @@ typedef struct or_problem {
     const double *Ox, *Oy, *Oz; /* [aslm][aslm] padded */
     const double *massfac;   /* [slm] extension, NULL = 1 */
     int nthreads;            /* OpenMP threads, <=0: default */
+    const unsigned char *face_kind; /* [nF] extension: 0 vacuum (the reference's
+                                       only kind), 1 specular reflection, which
+                                       is the natural condition of the
+                                       even-parity form (no boundary term);
+                                       NULL = all vacuum */
 } or_problem;
 
 void or_volume_part(const or_problem *pr, const double *f, double *r);

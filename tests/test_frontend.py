@@ -160,21 +160,7 @@ def test_stencil_assembly_matches_oracle(case, rng):
     a = P.ang
     st = ta.Stencil(P.mesh.view, a.view)
     arr = P.mesh.arrays()
-    if case == "reflective":
-        # the oracle has no face kinds: drop the reflective faces from its CSR
-        keep = arr["face_kind"] == 0
-        newid = -np.ones(len(keep), np.int64)
-        newid[keep] = np.arange(keep.sum())
-        fs, fi, fp = arr["facestart"], arr["faceidx"], arr["facepos"]
-        nfs, nfi, nfp = [0], [], []
-        for v in range(len(fs) - 1):
-            for j in range(fs[v], fs[v + 1]):
-                if keep[fi[j]]:
-                    nfi.append(newid[fi[j]])
-                    nfp.append(fp[j])
-            nfs.append(len(nfi))
-        arr.update(bnd=arr["bnd"][keep], facestart=np.array(nfs, np.uint32),
-                   faceidx=np.array(nfi, np.uint32), facepos=np.array(nfp, np.uint32))
+    # reflective faces (face_kind 1) carry no boundary term, in the oracle too
     op = oracle.OracleProblem(arr, ang_dict(a))
     f = np.zeros((op.nP, op.aslm))
     f[:, :op.slm] = rng.standard_normal((op.nP, op.slm))

@@ -109,6 +109,11 @@ for _n in ("tr_compute_lhs", "tr_compute_lhs_diag"):
     _sig(_n, C.c_int, vp, vp, vp)
 _sig("tr_compute_rhs", C.c_int, vp, vp)
 _sig("tr_compute_lhs_prec", C.c_int, vp, C.c_int, vp, vp)
+_sig("tr_compute_moments", C.c_int, vp, vp, vp, vp)
+_sig("tr_compute_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
+_sig("tr_expn", C.c_double, C.c_int, C.c_double)
+_sig("tr_step_analytic", C.c_int, C.c_double, f64p)
+_sig("tr_slab_analytic_U", C.c_double, C.c_double, C.c_double)
 _sig("tr_scale", C.c_int, vp, vp, C.c_double)
 _sig("tr_add_prod", C.c_int, vp, vp, vp, C.c_double)
 _sig("tr_mul_add", C.c_int, vp, vp, C.c_double, vp)
@@ -124,6 +129,7 @@ _sig("tr_bench_copy", C.c_int, vp, C.c_size_t, C.c_int, f64p)
 _sig("tr_stencil_create", C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), C.POINTER(vp))
 _sig("tr_stencil_destroy", C.c_int, vp)
 _sig("tr_stencil_info", C.c_int, vp, u32p)
+_sig("tr_stencil_mask", C.c_int, vp, u32p, u32p)
 _sig("tr_stencil_export", C.c_int, vp, u32p, u32p, f64p, u32p, u32p, f64p, f64p, f64p, f64p, f64p)
 
 
@@ -311,6 +317,11 @@ class Stencil:
         self.onc = np.empty((3, slm, slm))
         self.massfac = np.empty(slm)
         self.rhs = np.empty(nP)
+        cnt = np.zeros(1, np.uint32)
+        check(lib.tr_stencil_mask(self._h, ptr(cnt, u32p), None))
+        self.mask = np.empty(int(cnt[0]), np.uint32)
+        if len(self.mask):
+            check(lib.tr_stencil_mask(self._h, ptr(cnt, u32p), ptr(self.mask, u32p)))
         check(lib.tr_stencil_export(self._h, ptr(self.perm, u32p), ptr(self.nbr, u32p), ptr(self.coef),
                                     ptr(self.bvert, u32p), ptr(self.bnbr, u32p), ptr(self.bcoef),
                                     ptr(self.omc), ptr(self.onc), ptr(self.massfac), ptr(self.rhs)))
@@ -395,6 +406,30 @@ class HipContext:
         self.copyToDev(v, np.ascontiguousarray(host, np.float64).reshape(-1))
         return v
 
+    def moments(self, f):
+        """ComputeMoments of the device vector f: host U[nP], T[9][nP] (T_ij at
+        row 3i+j), caller's vertex order."""
+        nP = self.nP
+        dU, dT = self.deviceAlloc(8 * nP), self.deviceAlloc(8 * 9 * nP)
+        try:
+            check(lib.tr_compute_moments(self._h, f, dU, dT))
+            return self.copyToHost(dU, 8 * nP), self.copyToHost(dT, 8 * 9 * nP).reshape(9, nP)
+        finally:
+            self.deviceFree(dU)
+            self.deviceFree(dT)
+
+    def flux(self, mesh, T):
+        """ComputeFlux from host T[9][nP]: host W[3][nT]."""
+        nP, nT = self.nP, mesh.nT
+        dT, dW = self.deviceAlloc(8 * 9 * nP), self.deviceAlloc(8 * 3 * nT)
+        try:
+            self.copyToDev(dT, np.ascontiguousarray(T, np.float64).reshape(-1))
+            check(lib.tr_compute_flux(self._h, C.byref(mesh.view), dT, dW))
+            return self.copyToHost(dW, 8 * 3 * nT).reshape(3, nT)
+        finally:
+            self.deviceFree(dT)
+            self.deviceFree(dW)
+
     def getHostSmall(self, src):
         out = np.empty(self.nP * self.slm)
         check(lib.tr_get_host_small(self._h, src, ptr(out)))
@@ -474,3 +509,21 @@ class HipContext:
 
     def __del__(self):
         self.close()
+
+
+def expn(n, x):
+    """E_n(x), the library's exponential integral."""
+    return lib.tr_expn(int(n), float(x))
+
+
+def step_analytic(x):
+    """(U, W, T_xx) of the step problem (main.cpp:498-515, correct E_n)."""
+    out = np.zeros(3)
+    check(lib.tr_step_analytic(float(x), ptr(out)))
+    return tuple(out)
+
+
+def slab_analytic_U(x, L):
+    """Scalar flux of the purely absorbing slab [-L, L], kappa = 1, I_p = 1 on
+    x < 0, vacuum at both ends."""
+    return lib.tr_slab_analytic_U(float(x), float(L))

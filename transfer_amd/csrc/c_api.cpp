@@ -307,6 +307,13 @@ int tr_stencil_destroy(tr_stencil *s) {
     return TR_OK;
 }
 
+int tr_stencil_mask(const tr_stencil *s, uint32_t *count, uint32_t *list) {
+    if (!s || !count) return fail(TR_ERR_ARG, "tr_stencil_mask: null argument");
+    *count = (uint32_t)s->op.mask.size();
+    if (list && !s->op.mask.empty()) std::memcpy(list, s->op.mask.data(), sizeof(uint32_t) * s->op.mask.size());
+    return TR_OK;
+}
+
 int tr_stencil_info(const tr_stencil *s, uint32_t *info) {
     if (!s || !info) return fail(TR_ERR_ARG, "tr_stencil_info: null argument");
     const HostOperator &o = s->op;

@@ -42,6 +42,9 @@ struct tr_ctx {
     double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
     double *d_crec = nullptr;   // coefficient records for the buffer-load schedule
     double *d_bcrec = nullptr;  // boundary coefficient records (variant 4)
+    uint32_t *d_mask = nullptr; // specular reflection: vanishing entries
+    double *d_mom_val = nullptr;   // moments table [3][slm]
+    uint32_t *d_mom_pos = nullptr;
     void *d_tab = nullptr;      // contraction-table records
     void *d_tab4 = nullptr;     // MFMA B slices (variant 4, FP64)
     // FP32 copy of the operator data (mixed-precision solves), built lazily;
@@ -85,6 +88,8 @@ struct tr_ctx {
         a.tab = map.variant == 4 ? d_tab4 : d_tab;
         a.nsplit = 1;
         a.bcrec = d_bcrec;
+        a.mask = d_mask;
+        a.nmask = (uint32_t)op.mask.size();
         return a;
     }
     tr::OpArgs<float> args32() const { return args32(map32); }
@@ -116,6 +121,8 @@ struct tr_ctx {
         a.tab = map32.variant != 4 ? f_tab : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
         a.nsplit = ang_prec == 1 ? 1 : 3;
         a.bcrec = f_bcrec;
+        a.mask = d.mask;
+        a.nmask = d.nmask;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }
@@ -405,6 +412,8 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         (rc = upload(c, &c->d_rhs, op.rhs)) || (rc = upload(c, &c->d_perm, op.perm)) ||
         (rc = upload(c, &c->d_iperm, op.iperm)) || (rc = upload(c, &c->d_crec, coef_records<double>(op))) ||
         (rc = upload(c, &c->d_bcrec, bcoef_records<double>(op))) ||
+        (!op.mask.empty() && (rc = upload(c, &c->d_mask, op.mask))) ||
+        (rc = upload(c, &c->d_mom_val, op.mom_val)) || (rc = upload(c, &c->d_mom_pos, op.mom_pos)) ||
         (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
         (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
         tr_ctx_destroy(c);
@@ -549,6 +558,31 @@ int tr_compute_lhs_diag(tr_ctx *c, const double *f, double *Af) {
 int tr_compute_rhs(tr_ctx *c, double *b) {
     if (!c || !b) return fail(TR_ERR_ARG, "tr_compute_rhs: null argument");
     HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, b, c->stream));
+    return TR_OK;
+}
+
+// MeshData::ComputeMoments (MeshData.cpp:96-116) of a device vector: U[nP]
+// and T[3i+j][nP] (device arrays) in the caller's vertex order.
+int tr_compute_moments(tr_ctx *c, const double *f, double *U, double *T) {
+    if (!c || !f || !U || !T) return fail(TR_ERR_ARG, "tr_compute_moments: null argument");
+    HIPCHECK(launch_moments(f, c->d_perm, c->d_mom_val, c->d_mom_pos, c->op.nP, c->op.slm, U, T, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return TR_OK;
+}
+
+// MeshData::ComputeFlux (MeshData.cpp:118-136): W[j][nT] (device) from the
+// device moments T of tr_compute_moments and the mesh the context was built
+// from (its tets are uploaded for the call).
+int tr_compute_flux(tr_ctx *c, const tr_mesh_view *mesh, const double *T, double *W) {
+    if (!c || !mesh || !mesh->mesh || !T || !W) return fail(TR_ERR_ARG, "tr_compute_flux: null argument");
+    if (mesh->nP != c->op.nP) return fail(TR_ERR_ARG, "tr_compute_flux: mesh does not match the context");
+    tr_tetrahedron *d = nullptr;
+    HIPCHECK(hipMalloc((void **)&d, sizeof(tr_tetrahedron) * std::max<uint32_t>(mesh->nT, 1)));
+    hipError_t e = hipMemcpyAsync(d, mesh->mesh, sizeof(tr_tetrahedron) * mesh->nT, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_flux(d, mesh->nT, mesh->nP, T, W, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("tr_compute_flux: ") + hipGetErrorString(e));
     return TR_OK;
 }
 
@@ -736,7 +770,7 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     const size_t n = c->nint();
     double *b = w.get(), *r = w.get(), *p = w.get(), *Ap = w.get();
     double *part = nullptr, *part2 = nullptr, *sc = nullptr, *dinv = nullptr;
-    const uint32_t nb = apply_blocks(c->op.nP, c->map);
+    const uint32_t nb = apply_tiles(c->op.nP, c->map);
     HIPCHECK(hipMalloc((void **)&part, sizeof(double) * (std::max<size_t>(nb, 3 * 1024) + 2 * 1024 + 16)));
     w.v.push_back(part);
     part2 = part + std::max<size_t>(nb, 3 * 1024);
@@ -830,7 +864,7 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     const size_t n = c->nint();
     double *b = w.get(), *r = w.get(), *Ap = w.get();
     float *v32 = nullptr;
-    const uint32_t nb = apply_blocks(c->op.nP, c->map32);
+    const uint32_t nb = apply_tiles(c->op.nP, c->map32);
     double *part = nullptr;
     HIPCHECK(hipMalloc((void **)&part, sizeof(double) * (std::max<size_t>(nb, 3 * 1024) + 2 * 1024 + 16)));
     w.v.push_back(part);

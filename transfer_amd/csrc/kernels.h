@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "transfer.h"
+
 // most ELL slots the batched schedules (variants 2-4) unroll
 #define TR_WMAX 16
 // most boundary-stencil slots the matrix-core schedule (variant 4) unrolls
@@ -48,6 +50,8 @@ struct OpArgs {
                            // variant 4: MFMA B slices (volume, then boundary)
     int nsplit;            // variant 4, BF16: 1 = plain, 3 = hi/lo split
     const T *bcrec;        // variant 4: [WB][nB][4] boundary coefficient records (3 channels + pad)
+    const uint32_t *mask;  // specular reflection: entries of r that vanish, may be null
+    uint32_t nmask;
 };
 
 // coefficient record stride (elements): NC rounded up to a 16-byte multiple
@@ -79,6 +83,9 @@ size_t apply3_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, u
 // workgroups (= dot-product partials) of an operator launch with the fused
 // dot product
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m);
+// vertex tiles (VPW x WPB vertices) of a lane map: the grid of the
+// non-persistent kernels (k_apply_diag) and an upper bound of apply_blocks
+uint32_t apply_tiles(uint32_t nP, const LaneMap &m);
 
 // variant 4 (kernels_mfma.hip): J = ceil(slm/4) harmonics per lane
 int mfma_J(uint32_t slm);
@@ -91,6 +98,12 @@ hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, con
 // for the launch without or with the fused dot product
 template <typename T, int MODE>
 uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, bool dot);
+
+// moments U, T and flux W of a solution (postproc.hip)
+hipError_t launch_moments(const double *f, const uint32_t *perm, const double *mval, const uint32_t *mpos,
+                          uint32_t nP, uint32_t slm, double *U, double *T, hipStream_t s);
+hipError_t launch_flux(const tr_tetrahedron *mesh, uint32_t nT, uint32_t nP, const double *T, double *W,
+                       hipStream_t s);
 
 // b[v][lm] = lm == 0 ? rhs[v] : 0
 hipError_t launch_rhs(const double *rhs, uint32_t nP, uint32_t slm, double *b,

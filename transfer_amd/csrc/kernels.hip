@@ -550,10 +550,30 @@ size_t apply3_lds_bytes(const LaneMap &m, int NC, uint32_t slm, uint32_t arow, u
     return table + (size_t)m.WPB * CB * (NC + 3) * slm * elem;
 }
 
-uint32_t apply_blocks(uint32_t nP, const LaneMap &m) {
+uint32_t apply_tiles(uint32_t nP, const LaneMap &m) {
     uint32_t per = (uint32_t)(m.WPB * m.VPW);
-    const uint32_t tiles = (nP + per - 1) / per;
+    return (nP + per - 1) / per;
+}
+
+uint32_t apply_blocks(uint32_t nP, const LaneMap &m) {
+    const uint32_t tiles = apply_tiles(nP, m);
     return (m.variant == 4 && m.grid_dot) ? std::min(m.grid_dot, tiles) : tiles;
+}
+
+// Specular reflection: the listed entries of r vanish (see stencil.h).
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_mask(T *__restrict__ r, const uint32_t *__restrict__ mask,
+                                                   uint32_t n, const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) r[mask[i]] = (T)0;
+}
+
+template <typename T>
+static hipError_t zero_mask(const OpArgs<T> &a, T *r, const double *skip, hipStream_t s) {
+    if (!a.nmask) return hipSuccess;
+    const uint32_t g = std::min<uint32_t>((a.nmask + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero_mask<T>, dim3(g), dim3(256), 0, s, r, a.mask, a.nmask, skip);
+    return hipGetLastError();
 }
 
 #define TR_J_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(11)
@@ -563,12 +583,14 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
                         double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
     if (m.variant == 4) {
+        hipError_t e;
         if constexpr (sizeof(T) == 8)
-            return launch_apply4<T, 0>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+            e = launch_apply4<T, 0>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
         else if (m.mfma_mode == 2)
-            return launch_apply4<T, 2>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+            e = launch_apply4<T, 2>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
         else
-            return launch_apply4<T, 1>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+            e = launch_apply4<T, 1>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+        return e != hipSuccess ? e : zero_mask(a, r, skip, s);
     }
     dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
     // variant 3 needs W = 15 or 16 exactly (the host pads the stencil) so that
@@ -603,14 +625,15 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
     }
 #undef TR_CASE
 #undef TR_LAUNCH
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : zero_mask(a, r, skip, s);
 }
 
 template <typename T>
 hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                              double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
-    dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
+    dim3 g(apply_tiles(a.nP, m)), b(TR_WAVE * m.WPB);
 #define TR_CASE(JJ)                                                                          \
     case JJ:                                                                                 \
         if (part)                                                                            \
@@ -623,7 +646,8 @@ hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const
         default: return hipErrorInvalidValue;
     }
 #undef TR_CASE
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : zero_mask(a, r, skip, s);
 }
 
 template hipError_t launch_apply<double>(const OpArgs<double> &, int, const LaneMap &, const double *,

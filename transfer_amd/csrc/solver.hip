@@ -187,7 +187,9 @@ template <typename T>
 int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResult &res) {
     const size_t n = S.n;
     const unsigned G = upd_grid(n);
-    const uint32_t nb = apply_blocks(S.a.nP, S.map);
+    // dot partials written by the operator launch (the angular-diagonal
+    // kernel is never persistent)
+    const uint32_t nb = prm.diag_op ? apply_tiles(S.a.nP, S.map) : apply_blocks(S.a.nP, S.map);
     auto t0 = std::chrono::steady_clock::now();
     hipError_t e;
 #define HC(x)                                                                               \

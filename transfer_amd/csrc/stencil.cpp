@@ -73,6 +73,20 @@ static void rcm_order(uint32_t nP, const std::vector<uint32_t> &start,
     perm.assign(order.rbegin(), order.rend());
 }
 
+unsigned mirror_odd(int l, int m) {
+    (void)l;  // l is even: the z-mirror parity (-1)^(l+|m|) depends on |m| only
+    const int am = m < 0 ? -m : m;
+    unsigned bits = 0;
+    // x -> -x: phi -> pi - phi; cos(m(pi - phi)) = (-1)^m cos(m phi),
+    // sin(k(pi - phi)) = -(-1)^k sin(k phi)
+    if (m >= 0 ? (am & 1) : !(am & 1)) bits |= 1u;
+    // y -> -y: phi -> -phi flips the sine terms
+    if (m < 0) bits |= 2u;
+    // z -> -z: cos(theta) -> -cos(theta), P_l^|m| picks up (-1)^(l+|m|)
+    if (am & 1) bits |= 4u;
+    return bits;
+}
+
 int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
                       HostOperator &op) {
     const uint32_t nP = mv.nP, slm = av.slm;
@@ -288,6 +302,43 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
             op.bnbr[(size_t)s * op.nB + b] = s < bsl[b].size() ? bsl[b][s] : op.bvert[b];
             for (int n = 0; n < 3; n++)
                 op.bcoef[((size_t)s * 3 + n) * op.nB + b] = s < bsl[b].size() ? bco[b][3 * s + n] : 0.0;
+        }
+
+    // ---- specular reflection (extension).  In the even-parity weak form a
+    // reflecting plane is not obtained by dropping its boundary term alone:
+    // the trial and test functions must also be symmetric under the plane's
+    // mirror there, i.e. the harmonics odd under that mirror vanish at the
+    // plane's vertices.  Only axis-aligned planes are supported.
+    op.mask.clear();
+    {
+        std::vector<uint8_t> refl(nP, 0);
+        for (uint32_t f = 0; f < mv.nF; f++) {
+            if (!mv.face_kind || mv.face_kind[f] != 1) continue;
+            const tr_face &F = mv.bnd[f];
+            int axis = 0;
+            for (int a = 1; a < 3; a++)
+                if (std::fabs(F.s[a]) > std::fabs(F.s[axis])) axis = a;
+            for (int a = 0; a < 3; a++)
+                if (a != axis && std::fabs(F.s[a]) > 1e-9 * std::fabs(F.s[axis]))
+                    return fail(TR_ERR_ARG, "reflective boundary faces must lie in axis-aligned planes");
+            for (int k = 0; k < 3; k++) refl[op.iperm[F.p[k]]] |= (uint8_t)(1u << axis);
+        }
+        std::vector<uint8_t> odd;
+        for (int l = 0; odd.size() < slm; l += 2)
+            for (int m = -l; m <= l; m++) odd.push_back((uint8_t)mirror_odd(l, m));
+        for (uint32_t i = 0; i < nP; i++)
+            if (refl[i])
+                for (uint32_t n = 0; n < slm; n++)
+                    if (refl[i] & odd[n]) op.mask.push_back(i * slm + n);
+    }
+
+    // ---- moments table: omega[i][lm][0] and its b index (MeshData.cpp:108-113)
+    op.mom_val.assign(3 * (size_t)slm, 0.0);
+    op.mom_pos.assign(3 * (size_t)slm, 0);
+    for (int i = 0; i < 3; i++)
+        for (uint32_t lm = 0; lm < slm; lm++) {
+            op.mom_val[(size_t)i * slm + lm] = av.omega[(size_t)i * slm * slm + (size_t)lm * slm];
+            op.mom_pos[(size_t)i * slm + lm] = av.omega_pos[(size_t)i * slm * slm + (size_t)lm * slm];
         }
 
     // ---- right-hand side (kernels.cu:157-163, same summation order)

@@ -56,7 +56,20 @@ struct HostOperator {
     // dense channel matrices Omega_c[out][in], c = 1..NC-1, and O_n[out][in]
     std::vector<double> omc;           // [NC-1][slm][slm]
     std::vector<double> onc;           // [3][slm][slm]
+    // specular reflection on axis-aligned planes (extension): entries
+    // v*slm + lm of the internal [v][slm] layout that vanish, i.e. the
+    // harmonics odd under the mirror of a reflective plane through v
+    std::vector<uint32_t> mask;
+    // first column of the reference's compressed angular data (omega[i][lm][0],
+    // omega_pos[i][lm][0], i = 0..2) for the moments T_ij (MeshData.cpp:96-116)
+    std::vector<double> mom_val;    // [3][slm]
+    std::vector<uint32_t> mom_pos;  // [3][slm]
 };
+
+// Bit a of mirror_odd(l, m) is set when the real harmonic Y_lm (cos(m phi)
+// for m >= 0, sin(|m| phi) for m < 0, polar axis z, phi from x) changes sign
+// under the mirror x_a -> -x_a.
+unsigned mirror_odd(int l, int m);
 
 int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
                       HostOperator &op);

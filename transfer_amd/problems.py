@@ -12,6 +12,10 @@ benchmark configurations, from the library's own generators.
     cube32    32^3 cubes on [0,1]^3 (196,608 tets), kappa = 0.1 + 8 Gaussian
               blobs (amplitude log-U[1,100], width U[0.05,0.3]), albedo
               U[0,0.99], I_p = 1, K = 3
+    milne     the step problem of milne.geo as a 1-D slab: [-L,L] x [-h,h]^2,
+              kappa = 1, I_p = 1 for x < 0 and 0 for x > 0, vacuum at
+              x = +-L, specular reflection on the four side planes (so the
+              solution is the exact 1-D one, U(x) = 2 pi (E2 ... ) terms)
 The Lebedev N-point orders map to the harmonic order K = (degree(N) - 2) // 4.
 """
 from .api import Angular, Mesh, rng_uniform
@@ -46,3 +50,15 @@ def cube(n=32, maxk=3, seed=5, albedo=None):
     if albedo is None:
         albedo = 0.99 * rng_uniform(seed, 6, 1)[0]
     return Problem("cube%d" % n, m, Angular(maxk, albedo, 0.0, 0), albedo)
+
+
+def milne(n=32, maxk=3, L=1.0, ncross=2):
+    """Step problem on a slab: 2n cells across [-L, L], ncross cells of the
+    same width across each side direction."""
+    h = L / n * ncross / 2.0
+    m = Mesh.box((2 * n, ncross, ncross), (-L, -h, -h), (L, h, h))
+    m.set_fields(kappa=m.field("const", [1.0]), Ip=m.field("step", [0, 0.0, 1.0, 0.0]))
+    for axis in (1, 2):
+        for value in (-h, h):
+            m.set_face_kind_plane(axis, value, 1)
+    return Problem("milne%d" % n, m, Angular(maxk))
