@@ -268,6 +268,37 @@ typedef struct tr_solve_stats {
 int tr_solve(tr_ctx *ctx, const tr_solve_params *prm, double *f,
              tr_solve_stats *st, double *hist);
 
+/* ------------------------------------------------------------------------ */
+/* Batches (extension: the reference solves one mesh per run).               */
+/* ------------------------------------------------------------------------ */
+
+/* A context over nprob independent problems with the same angular data:
+ * the block-diagonal union of their operators, problem p with isotropic
+ * scattering albedo[p] (NULL = none; the angular view must carry no
+ * scattering of its own).  Every tr_ctx call works on the union: vectors
+ * are [union vertex][aslm] with problem p's vertices at voff[p] ..
+ * voff[p+1] (tr_batch_info) in the order of `meshes`. */
+int tr_batch_create(int device, int nprob, const tr_mesh_view *meshes,
+                    const tr_angular_view *ang, const double *albedo, int flags,
+                    tr_ctx **out);
+/* Number of problems and their vertex offsets voff[nprob+1] (may be NULL). */
+int tr_batch_info(const tr_ctx *ctx, uint32_t *nprob, uint32_t *voff);
+
+typedef struct tr_batch_stats {
+    int iters;           /* CG iterations (FP64 solver) or refinement steps */
+    int inner_iters;     /* FP32 CG iterations (mixed solver) */
+    double true_rel;     /* ||b - A f|| / ||b|| of the problem, FP64 operator */
+    int status;          /* 0 converged, TR_ERR_NOCONV otherwise */
+} tr_batch_stats;
+
+/* Solves every problem of a batch from f = 0 to its own stopping test
+ * (TR_SOLVER_PCG_FP64 or TR_SOLVER_MIXED; use_prec 0 or 2 = Jacobi) with
+ * per-problem CG scalars; one operator launch per iteration covers all
+ * problems.  total: aggregate (iters = max, preck = sum of inner, true_rel =
+ * worst problem); per: [nprob] or NULL. */
+int tr_batch_solve(tr_ctx *ctx, const tr_solve_params *prm, double *f,
+                   tr_solve_stats *total, tr_batch_stats *per);
+
 /* Seconds per operator application (median over reps), for benchmarking. */
 double tr_bench_lhs(tr_ctx *ctx, int reps);
 /* Times `steps` operator applications (precision 0 FP64, 1 the FP32

@@ -63,6 +63,10 @@ class SolveStats(C.Structure):
                 ("status", C.c_int)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("inner_iters", C.c_int), ("true_rel", C.c_double), ("status", C.c_int)]
+
+
 def _sig(name, res, *args):
     f = getattr(lib, name)
     f.restype = res
@@ -112,6 +116,10 @@ _sig("tr_compute_lhs_prec", C.c_int, vp, C.c_int, vp, vp)
 _sig("tr_compute_moments", C.c_int, vp, vp, vp, vp)
 _sig("tr_compute_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
 _sig("tr_expn", C.c_double, C.c_int, C.c_double)
+_sig("tr_batch_create", C.c_int, C.c_int, C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), f64p, C.c_int,
+     C.POINTER(vp))
+_sig("tr_batch_info", C.c_int, vp, u32p, u32p)
+_sig("tr_batch_solve", C.c_int, vp, C.POINTER(SolveParams), vp, C.POINTER(SolveStats), C.POINTER(BatchStats))
 _sig("tr_step_analytic", C.c_int, C.c_double, f64p)
 _sig("tr_slab_analytic_U", C.c_double, C.c_double, C.c_double)
 _sig("tr_scale", C.c_int, vp, vp, C.c_double)
@@ -345,6 +353,9 @@ class HipContext:
         av = ang_view if ang_view is not None else ang.view
         check(lib.tr_ctx_create(int(device), C.byref(mv), C.byref(av), 0, C.byref(self._h)))
         self._keep = (mesh, ang, mv, av)
+        self._sizes()
+
+    def _sizes(self):
         self.nP = int(lib.tr_ctx_nP(self._h))
         self.slm = int(lib.tr_ctx_slm(self._h))
         self.aslm = int(lib.tr_ctx_aslm(self._h))
@@ -527,3 +538,33 @@ def slab_analytic_U(x, L):
     """Scalar flux of the purely absorbing slab [-L, L], kappa = 1, I_p = 1 on
     x < 0, vacuum at both ends."""
     return lib.tr_slab_analytic_U(float(x), float(L))
+
+
+class Batch(HipContext):
+    """Independent problems (meshes with the same angular data, per-problem
+    isotropic albedo) on one context: the union operator plus per-problem CG
+    (tr_batch_create / tr_batch_solve).  Problem p owns vertices
+    voff[p] .. voff[p+1] of the union vectors."""
+
+    def __init__(self, meshes, ang, albedo=None, device=0):
+        self._h = vp()
+        views = (MeshView * len(meshes))(*[m.view for m in meshes])
+        alb = None if albedo is None else np.ascontiguousarray(albedo, np.float64)
+        check(lib.tr_batch_create(int(device), len(meshes), views, C.byref(ang.view), ptr(alb), 0,
+                                  C.byref(self._h)))
+        self._keep = (meshes, ang, views, alb)
+        self._sizes()
+        n = np.zeros(1, np.uint32)
+        check(lib.tr_batch_info(self._h, ptr(n, u32p), None))
+        self.nprob = int(n[0])
+        self.voff = np.zeros(self.nprob + 1, np.uint32)
+        check(lib.tr_batch_info(self._h, ptr(n, u32p), ptr(self.voff, u32p)))
+
+    def solve_batch(self, f, solver=TR_SOLVER_MIXED, use_prec=2, maxit=1000, tol_abs=0.0, tol_rel=1e-8,
+                    inner_maxit=20000, inner_tol=1e-4):
+        prm = SolveParams(solver, int(use_prec), maxit, tol_abs, tol_rel, inner_maxit, inner_tol, 0)
+        st = SolveStats()
+        per = (BatchStats * self.nprob)()
+        check(lib.tr_batch_solve(self._h, C.byref(prm), f, C.byref(st), per))
+        return st, [dict(iters=p.iters, inner_iters=p.inner_iters, true_rel=p.true_rel, status=p.status)
+                    for p in per]

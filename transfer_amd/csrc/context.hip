@@ -15,119 +15,11 @@
 #include <set>
 #include <vector>
 
+#include "ctx.h"
 #include "internal.h"
 #include "kernels.h"
 #include "solver.h"
 #include "stencil.h"
-
-#define HIPCHECK(x)                                                                   \
-    do {                                                                              \
-        hipError_t e_ = (x);                                                          \
-        if (e_ != hipSuccess)                                                         \
-            return tr::fail(TR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-struct tr_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    tr::HostOperator op;
-    tr::LaneMap map;
-    // device operator data (FP64)
-    uint32_t *d_nbr = nullptr, *d_bnbr = nullptr;
-    int32_t *d_bof = nullptr;
-    uint32_t *d_perm = nullptr, *d_iperm = nullptr;
-    uint16_t *d_aidx = nullptr, *d_bidx = nullptr;
-    double *d_coef = nullptr, *d_bcoef = nullptr, *d_aval = nullptr, *d_bval = nullptr;
-    double *d_massfac = nullptr, *d_ddiag = nullptr, *d_bdiag = nullptr, *d_rhs = nullptr;
-    double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
-    double *d_crec = nullptr;   // coefficient records for the buffer-load schedule
-    double *d_bcrec = nullptr;  // boundary coefficient records (variant 4)
-    uint32_t *d_mask = nullptr; // specular reflection: vanishing entries
-    double *d_mom_val = nullptr;   // moments table [3][slm]
-    uint32_t *d_mom_pos = nullptr;
-    void *d_tab = nullptr;      // contraction-table records
-    void *d_tab4 = nullptr;     // MFMA B slices (variant 4, FP64)
-    // FP32 copy of the operator data (mixed-precision solves), built lazily;
-    // index arrays are shared with the FP64 set
-    bool have32 = false;
-    tr::LaneMap map32;
-    float *f_coef = nullptr, *f_bcoef = nullptr, *f_aval = nullptr, *f_bval = nullptr;
-    float *f_massfac = nullptr, *f_ddiag = nullptr, *f_bdiag = nullptr;
-    float *f_crec = nullptr, *f_bcrec = nullptr;
-    void *f_tab = nullptr;
-    void *f_tab4 = nullptr, *f_tab4b1 = nullptr, *f_tab4b3 = nullptr;  // variant 4: FP32, BF16, BF16 hi/lo
-    int ang_prec = 3;  // FP32 operator's angular contraction: 0 FP32, 1 BF16, 3 BF16 hi/lo split
-    std::set<void *> vectors, raw;
-    std::vector<void *> owned;
-
-    tr::OpArgs<double> args() const { return args(map); }
-    tr::OpArgs<double> args(const tr::LaneMap &map) const {
-        tr::OpArgs<double> a;
-        a.nP = op.nP;
-        a.slm = op.slm;
-        a.W = op.W;
-        a.arow = op.arow;
-        a.nB = op.nB;
-        a.WB = op.WB;
-        a.brow = op.brow;
-        a.L = map.L;
-        a.VPW = map.VPW;
-        a.nbr = d_nbr;
-        a.coef = d_coef;
-        a.aidx = d_aidx;
-        a.aval = d_aval;
-        a.massfac = d_massfac;
-        a.ddiag = d_ddiag;
-        a.bof = d_bof;
-        a.bnbr = d_bnbr;
-        a.bcoef = d_bcoef;
-        a.bidx = d_bidx;
-        a.bval = d_bval;
-        a.bdiag = d_bdiag;
-        a.crec = d_crec;
-        a.tab = map.variant == 4 ? d_tab4 : d_tab;
-        a.nsplit = 1;
-        a.bcrec = d_bcrec;
-        a.mask = d_mask;
-        a.nmask = (uint32_t)op.mask.size();
-        return a;
-    }
-    tr::OpArgs<float> args32() const { return args32(map32); }
-    tr::OpArgs<float> args32(const tr::LaneMap &map32) const {
-        tr::OpArgs<double> d = args(map);
-        tr::OpArgs<float> a;
-        a.nP = d.nP;
-        a.slm = d.slm;
-        a.W = d.W;
-        a.arow = d.arow;
-        a.nB = d.nB;
-        a.WB = d.WB;
-        a.brow = d.brow;
-        a.L = map32.L;
-        a.VPW = map32.VPW;
-        a.nbr = d.nbr;
-        a.aidx = d.aidx;
-        a.bof = d.bof;
-        a.bnbr = d.bnbr;
-        a.bidx = d.bidx;
-        a.coef = f_coef;
-        a.aval = f_aval;
-        a.massfac = f_massfac;
-        a.ddiag = f_ddiag;
-        a.bcoef = f_bcoef;
-        a.bval = f_bval;
-        a.bdiag = f_bdiag;
-        a.crec = f_crec;
-        a.tab = map32.variant != 4 ? f_tab : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
-        a.nsplit = ang_prec == 1 ? 1 : 3;
-        a.bcrec = f_bcrec;
-        a.mask = d.mask;
-        a.nmask = d.nmask;
-        return a;
-    }
-    size_t nint() const { return (size_t)op.nP * op.slm; }
-    size_t next() const { return (size_t)op.nP * op.aslm; }
-};
 
 namespace tr {
 
@@ -186,6 +78,12 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
 // Whether the matrix-core schedule covers this operator: ceil(slm/4) in
 // {2, 4, 7}, exactly 15 or 16 volume slots, at most TR_WBMAX boundary slots.
 static bool v4_ok(const HostOperator &op) {
+    // 32-bit buffer offsets: the vector, the neighbour table and one slot's
+    // coefficient records (FP64 sizes) each stay below 4 GiB
+    const double lim = 4294967295.0;
+    if ((double)op.nP * op.slm * 8 >= lim || (double)op.W * op.nP * 4 >= lim ||
+        (double)op.nP * rec_stride<double>((int)op.NC) * 8 >= lim)
+        return false;
     return mfma_supported(op.slm, 0) && (op.W == 15 || op.W == 16) && op.WB <= TR_WBMAX &&
            (op.NC == 7 || op.NC == 10);
 }
@@ -330,15 +228,6 @@ static std::vector<unsigned char> table_records(const HostOperator &op) {
     return out;
 }
 
-template <typename T>
-static int upload(tr_ctx *c, T **dst, const std::vector<T> &src) {
-    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
-    HIPCHECK(hipMalloc((void **)dst, bytes));
-    c->owned.push_back(*dst);
-    if (!src.empty())
-        HIPCHECK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    return TR_OK;
-}
 
 static int lhs(tr_ctx *c, const double *f, double *r) {
     HIPCHECK(launch_apply<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream));
@@ -385,6 +274,15 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         delete c;
         return rc;
     }
+    if ((rc = ctx_init(c, device))) return rc;
+    *out = c;
+    return TR_OK;
+}
+
+}  // extern "C"
+
+int tr::ctx_init(tr_ctx *c, int device) {
+    int rc;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -414,6 +312,7 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         (rc = upload(c, &c->d_bcrec, bcoef_records<double>(op))) ||
         (!op.mask.empty() && (rc = upload(c, &c->d_mask, op.mask))) ||
         (rc = upload(c, &c->d_mom_val, op.mom_val)) || (rc = upload(c, &c->d_mom_pos, op.mom_pos)) ||
+        (!op.mf0.empty() && (rc = upload(c, &c->d_mf0, op.mf0))) ||
         (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
         (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
         tr_ctx_destroy(c);
@@ -433,9 +332,10 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
         tr_ctx_destroy(c);
         return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
     }
-    *out = c;
     return TR_OK;
 }
+
+extern "C" {
 
 int tr_ctx_destroy(tr_ctx *c) {
     if (!c) return TR_OK;
@@ -743,12 +643,13 @@ namespace tr {
 
 // diag(A)[v][lm] from the assembled data: slot 0 of the volume stencil is
 // the vertex itself; on the boundary the vertex's own boundary slot.
-static std::vector<double> operator_diagonal(const HostOperator &op) {
+std::vector<double> operator_diagonal(const HostOperator &op) {
     const uint32_t nP = op.nP, slm = op.slm, NC = op.NC;
     std::vector<double> d((size_t)nP * slm);
     for (uint32_t v = 0; v < nP; v++)
         for (uint32_t lm = 0; lm < slm; lm++) {
-            double s = op.massfac[lm] * op.coef[(size_t)0 * nP + v];
+            const double mfl = (!op.mf0.empty() && lm == 0) ? op.massfac[0] * op.mf0[v] : op.massfac[lm];
+            double s = mfl * op.coef[(size_t)0 * nP + v];
             for (uint32_t c = 1; c < NC; c++)
                 s += op.omc[((size_t)(c - 1) * slm + lm) * slm + lm] * op.coef[(size_t)c * nP + v];
             int b = op.bof[v];
@@ -826,7 +727,7 @@ static std::vector<float> to_f32(const std::vector<double> &v) {
     return std::vector<float>(v.begin(), v.end());
 }
 
-static int ensure_fp32(tr_ctx *c) {
+int ensure_fp32(tr_ctx *c) {
     if (c->have32) return TR_OK;
     const HostOperator &op = c->op;
     c->map32 = choose_map(op.slm, 8);
@@ -843,6 +744,7 @@ static int ensure_fp32(tr_ctx *c) {
         (rc = upload(c, &c->f_massfac, to_f32(op.massfac))) || (rc = upload(c, &c->f_ddiag, to_f32(op.ddiag))) ||
         (rc = upload(c, &c->f_bdiag, to_f32(op.bdiag))) || (rc = upload(c, &c->f_crec, coef_records<float>(op))) ||
         (rc = upload(c, &c->f_bcrec, bcoef_records<float>(op))) ||
+        (!op.mf0.empty() && (rc = upload(c, &c->f_mf0, to_f32(op.mf0)))) ||
         (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
         return rc;
     if (mfma_supported(op.slm, 1) &&

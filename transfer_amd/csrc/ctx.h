@@ -1,0 +1,153 @@
+// ctx.h -- tr_ctx, the device context behind the C ABI (context.hip), and
+// the helpers the batch entry points (batch.hip) share with it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+#include "stencil.h"
+
+#define HIPCHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            return tr::fail(TR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct tr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    tr::HostOperator op;
+    tr::LaneMap map;
+    // device operator data (FP64)
+    uint32_t *d_nbr = nullptr, *d_bnbr = nullptr;
+    int32_t *d_bof = nullptr;
+    uint32_t *d_perm = nullptr, *d_iperm = nullptr;
+    uint16_t *d_aidx = nullptr, *d_bidx = nullptr;
+    double *d_coef = nullptr, *d_bcoef = nullptr, *d_aval = nullptr, *d_bval = nullptr;
+    double *d_massfac = nullptr, *d_ddiag = nullptr, *d_bdiag = nullptr, *d_rhs = nullptr;
+    double *d_part = nullptr, *d_red = nullptr, *d_stage = nullptr;
+    double *d_crec = nullptr;   // coefficient records for the buffer-load schedule
+    double *d_bcrec = nullptr;  // boundary coefficient records (variant 4)
+    uint32_t *d_mask = nullptr; // specular reflection: vanishing entries
+    double *d_mf0 = nullptr;       // [nP] lm = 0 mass factor per vertex (batches), or null
+    float *f_mf0 = nullptr;
+    double *d_mom_val = nullptr;   // moments table [3][slm]
+    uint32_t *d_mom_pos = nullptr;
+    void *d_tab = nullptr;      // contraction-table records
+    void *d_tab4 = nullptr;     // MFMA B slices (variant 4, FP64)
+    // FP32 copy of the operator data (mixed-precision solves), built lazily;
+    // index arrays are shared with the FP64 set
+    bool have32 = false;
+    tr::LaneMap map32;
+    float *f_coef = nullptr, *f_bcoef = nullptr, *f_aval = nullptr, *f_bval = nullptr;
+    float *f_massfac = nullptr, *f_ddiag = nullptr, *f_bdiag = nullptr;
+    float *f_crec = nullptr, *f_bcrec = nullptr;
+    void *f_tab = nullptr;
+    void *f_tab4 = nullptr, *f_tab4b1 = nullptr, *f_tab4b3 = nullptr;  // variant 4: FP32, BF16, BF16 hi/lo
+    int ang_prec = 3;  // FP32 operator's angular contraction: 0 FP32, 1 BF16, 3 BF16 hi/lo split
+    std::set<void *> vectors, raw;
+    // batches: internal vertex range of problem p = voff[p] .. voff[p+1]
+    // (empty for a single problem)
+    std::vector<uint32_t> voff;
+    std::vector<void *> owned;
+
+    tr::OpArgs<double> args() const { return args(map); }
+    tr::OpArgs<double> args(const tr::LaneMap &map) const {
+        tr::OpArgs<double> a;
+        a.nP = op.nP;
+        a.slm = op.slm;
+        a.W = op.W;
+        a.arow = op.arow;
+        a.nB = op.nB;
+        a.WB = op.WB;
+        a.brow = op.brow;
+        a.L = map.L;
+        a.VPW = map.VPW;
+        a.nbr = d_nbr;
+        a.coef = d_coef;
+        a.aidx = d_aidx;
+        a.aval = d_aval;
+        a.massfac = d_massfac;
+        a.ddiag = d_ddiag;
+        a.bof = d_bof;
+        a.bnbr = d_bnbr;
+        a.bcoef = d_bcoef;
+        a.bidx = d_bidx;
+        a.bval = d_bval;
+        a.bdiag = d_bdiag;
+        a.crec = d_crec;
+        a.tab = map.variant == 4 ? d_tab4 : d_tab;
+        a.nsplit = 1;
+        a.bcrec = d_bcrec;
+        a.mask = d_mask;
+        a.nmask = (uint32_t)op.mask.size();
+        a.mf0 = d_mf0;
+        return a;
+    }
+    tr::OpArgs<float> args32() const { return args32(map32); }
+    tr::OpArgs<float> args32(const tr::LaneMap &map32) const {
+        tr::OpArgs<double> d = args(map);
+        tr::OpArgs<float> a;
+        a.nP = d.nP;
+        a.slm = d.slm;
+        a.W = d.W;
+        a.arow = d.arow;
+        a.nB = d.nB;
+        a.WB = d.WB;
+        a.brow = d.brow;
+        a.L = map32.L;
+        a.VPW = map32.VPW;
+        a.nbr = d.nbr;
+        a.aidx = d.aidx;
+        a.bof = d.bof;
+        a.bnbr = d.bnbr;
+        a.bidx = d.bidx;
+        a.coef = f_coef;
+        a.aval = f_aval;
+        a.massfac = f_massfac;
+        a.ddiag = f_ddiag;
+        a.bcoef = f_bcoef;
+        a.bval = f_bval;
+        a.bdiag = f_bdiag;
+        a.crec = f_crec;
+        a.tab = map32.variant != 4 ? f_tab : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
+        a.nsplit = ang_prec == 1 ? 1 : 3;
+        a.bcrec = f_bcrec;
+        a.mask = d.mask;
+        a.nmask = d.nmask;
+        a.mf0 = f_mf0;
+        return a;
+    }
+    size_t nint() const { return (size_t)op.nP * op.slm; }
+    size_t next() const { return (size_t)op.nP * op.aslm; }
+};
+
+namespace tr {
+
+// Uploads a host vector into a new device allocation owned by the context.
+template <typename T>
+int upload(tr_ctx *c, T **dst, const std::vector<T> &src) {
+    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    HIPCHECK(hipMalloc((void **)dst, bytes));
+    c->owned.push_back(*dst);
+    if (!src.empty())
+        HIPCHECK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return TR_OK;
+}
+
+// Second half of context creation: schedules and device data for the
+// operator already in c->op (the context is destroyed on failure).
+int ctx_init(tr_ctx *c, int device);
+// Lazily builds the FP32 operator data of the mixed solver.
+int ensure_fp32(tr_ctx *c);
+// diag(A) per entry of the internal [v][slm] layout (Jacobi preconditioner)
+std::vector<double> operator_diagonal(const HostOperator &op);
+
+}  // namespace tr

@@ -52,6 +52,7 @@ struct OpArgs {
     const T *bcrec;        // variant 4: [WB][nB][4] boundary coefficient records (3 channels + pad)
     const uint32_t *mask;  // specular reflection: entries of r that vanish, may be null
     uint32_t nmask;
+    const T *mf0;          // [nP] per-vertex factor of the lm = 0 mass term (batches), may be null
 };
 
 // coefficient record stride (elements): NC rounded up to a 16-byte multiple

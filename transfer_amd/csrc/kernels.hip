@@ -118,7 +118,8 @@ __global__ __launch_bounds__(256) void k_apply(OpArgs<T> a, const T *__restrict_
     if (valid) {
         T R[J];
 #pragma unroll
-        for (int j = 0; j < J; j++) R[j] = a.massfac[lm0 + j] * acc[0][j];
+        for (int j = 0; j < J; j++)
+            R[j] = ((a.mf0 && lm0 + j == 0) ? a.mf0[v] * a.massfac[0] : a.massfac[lm0 + j]) * acc[0][j];
         for (uint32_t e = 0; e < a.arow; e++) {
             const size_t q = (size_t)e * a.slm + lm0;
 #pragma unroll
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(256) void k_apply2(OpArgs<T> a, const T *__restrict
             const uint32_t vv = v0 + q * CB + cv;
             if (q * CB + cv >= a.VPW || vv >= a.nP) continue;
             const T *my = stage + (size_t)cv * CH * slm;
-            T R = a.massfac[lm] * my[lm];
+            T R = ((a.mf0 && lm == 0) ? a.mf0[vv] * a.massfac[0] : a.massfac[lm]) * my[lm];
             const T *st = my + slm;  // stiffness channels
             for (uint32_t e = 0; e < a.arow; e++)
                 R = fma(tval[e * slm + lm], st[tidx[e * slm + lm]], R);
@@ -427,7 +428,8 @@ __global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict
             const uint32_t vv = v0 + q * CB + cv;
             if (q * CB + cv >= a.VPW || vv >= nP) continue;
             const unsigned char *my = reinterpret_cast<const unsigned char *>(stage + (size_t)cv * CH * slm);
-            T R0 = a.massfac[lm] * reinterpret_cast<const T *>(my)[lm], R1 = 0;
+            T R0 = ((a.mf0 && lm == 0) ? a.mf0[vv] * a.massfac[0] : a.massfac[lm]) * reinterpret_cast<const T *>(my)[lm],
+              R1 = 0;
             const unsigned char *row = tab + (size_t)lm * TB;
             const uint32_t rstride = slm * TB;
             // rows are padded to multiples of 8 entries (zero couplings):
@@ -482,6 +484,7 @@ __global__ __launch_bounds__(256) void k_apply_diag(OpArgs<T> a, int NC, const T
 #pragma unroll
         for (int j = 0; j < J; j++) {
             mf[j] = a.massfac[lm0 + j];
+            if (a.mf0 && lm0 + j == 0) mf[j] *= a.mf0[v];
             d0[j] = a.ddiag[lm0 + j];
             d1[j] = a.ddiag[a.slm + lm0 + j];
             d2[j] = a.ddiag[2 * a.slm + lm0 + j];

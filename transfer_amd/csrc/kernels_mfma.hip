@@ -147,7 +147,9 @@ __device__ inline void load_idx(uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t nrs, 
 
 // Pipelined ELL gather of one lane: acc[c][j] += sum_s cf_s[c] f[nb_s][lm0+j]
 // over `nslot` slots (nslot <= NS, a wavefront-uniform count), coefficient
-// record (RS elements) of slot s at crs (coff, s*cstride).  DEPTH slots are
+// record (RS elements) of slot s at byte coff of the slot's array
+// cbase + s*cstride (one buffer descriptor per slot, so only a slot's array,
+// not all W of them, has to fit the 32-bit buffer offsets).  DEPTH slots are
 // in flight: slot s+DEPTH's loads are issued right after slot s's FMAs, so a
 // wavefront waits for one memory round trip per DEPTH slots.  Left alone, the
 // optimiser issues every slot's loads first and sinks all FMAs below them,
@@ -156,8 +158,7 @@ __device__ inline void load_idx(uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t nrs, 
 // the next load offsets are formed.
 template <typename T, int J, int NCH, int RS, int NS, int DEPTH>
 __device__ inline void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
-                              __amdgpu_buffer_rsrc_t crs, uint32_t coff, uint32_t cstride, int nslot, uint32_t slm,
-                              int lm0) {
+                              const T *cbase, size_t cstride, uint32_t coff, int nslot, uint32_t slm, int lm0) {
     T xs[DEPTH][J], cs[DEPTH][RS];
     auto load = [&](int s, T *x, T *cf) {
         uint32_t base = (nb[s] * slm + lm0) * (uint32_t)sizeof(T);
@@ -165,7 +166,7 @@ __device__ inline void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdg
         asm volatile("" : "+v"(base), "+v"(cv));
 #pragma unroll
         for (int j = 0; j < J; j++) x[j] = bload1<T>(frs, base + j * (uint32_t)sizeof(T), 0u);
-        brec<T, RS>(crs, cv, (uint32_t)s * cstride, cf);
+        brec<T, RS>(rsrc4(cbase + (size_t)s * cstride, cstride * sizeof(T)), cv, 0u, cf);
     };
     auto accum = [&](const T *x, const T *cf) {
 #pragma unroll
@@ -229,9 +230,9 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     const auto frs = rsrc4(f, (size_t)nP * slm * sizeof(T));
     const auto brs = rsrc4(a.bof, (size_t)nP * 4);
     const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
-    const auto crs = rsrc4(a.crec, (size_t)a.W * nP * RS * sizeof(T));
+
     const auto bnrs = rsrc4(a.bnbr, (size_t)a.WB * a.nB * 4);
-    const auto bcrs = rsrc4(a.bcrec, (size_t)a.WB * a.nB * 4 * sizeof(T));
+
     const int vloc = lane & 15, q = lane >> 4, lm0 = q * J;
     constexpr int NTMAX = (J * 4 + 15) / 16;
     typedef typename std::conditional<MODE == 0, d4v, f4v>::type acc4;
@@ -280,8 +281,11 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
 #pragma unroll
             for (int j = 0; j < J; j++) acc[c][j] = 0;
         if (valid)
-            gather<T, J, NC, RS, WT, DEPTH>(acc, nbc, frs, crs, v * (uint32_t)(RS * sizeof(T)),
-                                             nP * (uint32_t)(RS * sizeof(T)), WT, slm, lm0);
+            gather<T, J, NC, RS, WT, DEPTH>(acc, nbc, frs, a.crec, (size_t)nP * RS, v * (uint32_t)(RS * sizeof(T)),
+                                             WT, slm, lm0);
+        // per-vertex factor of the mass channel at lm = 0 (isotropic albedo
+        // per problem in a batch: 1 - albedo, with massfac[0] = 1)
+        if (a.mf0 && valid && q == 0) acc[0][0] *= a.mf0[v];
         // the next tile's indices load while this tile contracts
         const int b = bcur;
         if (tile + tstep < t1) tile_idx(tile + tstep, nbc, bcur);
@@ -304,8 +308,8 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
             if (b >= 0) {
                 uint32_t bn[TR_WBMAX];
                 load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, (int)a.WB);
-                gather<T, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, bcrs, (uint32_t)b * (uint32_t)(4 * sizeof(T)),
-                                                     a.nB * (uint32_t)(4 * sizeof(T)), (int)a.WB, slm, lm0);
+                gather<T, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, a.bcrec, (size_t)a.nB * 4,
+                                                     (uint32_t)b * (uint32_t)(4 * sizeof(T)), (int)a.WB, slm, lm0);
             }
             contract<T, MODE, J, 3>(bacc, smem_raw + vsec * parts + boff, bsec, NT, lane, nsplit, D);
         }

@@ -356,3 +356,93 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
 }
 
 }  // namespace tr
+
+namespace tr {
+
+int concat_operators(const std::vector<HostOperator> &ops, const std::vector<double> &albedo, HostOperator &u,
+                     std::vector<uint32_t> &voff) {
+    if (ops.empty() || albedo.size() != ops.size()) return fail(TR_ERR_ARG, "concat_operators: no problems");
+    const HostOperator &o0 = ops[0];
+    const uint32_t slm = o0.slm, NC = o0.NC;
+    uint64_t nP = 0, nB = 0;
+    uint32_t W = 0, WB = 1;
+    for (const HostOperator &o : ops) {
+        if (o.slm != slm || o.NC != NC || o.arow != o0.arow || o.brow != o0.brow || o.omc != o0.omc ||
+            o.onc != o0.onc || o.massfac != o0.massfac || !o.mf0.empty())
+            return fail(TR_ERR_ARG, "concat_operators: the problems' angular data differ");
+        nP += o.nP;
+        nB += o.nB;
+        W = std::max(W, o.W);
+        WB = std::max(WB, o.WB);
+    }
+    if (nP >= (1ull << 32) || (double)nP * slm >= 4294967296.0)
+        return fail(TR_ERR_ARG, "concat_operators: batch too large for 32-bit vertex ids");
+    u = HostOperator();
+    u.nP = (uint32_t)nP;
+    u.slm = slm;
+    u.aslm = o0.aslm;
+    u.NC = NC;
+    u.W = W;
+    u.WB = WB;
+    u.nB = (uint32_t)nB;
+    u.arow = o0.arow;
+    u.brow = o0.brow;
+    u.aidx = o0.aidx;
+    u.bidx = o0.bidx;
+    u.aval = o0.aval;
+    u.bval = o0.bval;
+    u.massfac = o0.massfac;
+    u.ddiag = o0.ddiag;
+    u.bdiag = o0.bdiag;
+    u.omc = o0.omc;
+    u.onc = o0.onc;
+    u.mom_val = o0.mom_val;
+    u.mom_pos = o0.mom_pos;
+    u.perm.resize(nP);
+    u.iperm.resize(nP);
+    u.nbr.assign((size_t)W * nP, 0);
+    u.coef.assign((size_t)W * NC * nP, 0.0);
+    u.bof.assign(nP, -1);
+    u.bnbr.assign((size_t)WB * std::max<uint64_t>(nB, 1), 0);
+    u.bcoef.assign((size_t)WB * 3 * std::max<uint64_t>(nB, 1), 0.0);
+    u.rhs.resize(nP);
+    u.mf0.resize(nP);
+    voff.assign(ops.size() + 1, 0);
+    uint32_t v0 = 0, b0 = 0;
+    for (size_t p = 0; p < ops.size(); p++) {
+        const HostOperator &o = ops[p];
+        voff[p] = v0;
+        for (uint32_t i = 0; i < o.nP; i++) {
+            const uint32_t g = v0 + i;
+            u.perm[g] = v0 + o.perm[i];
+            u.iperm[v0 + o.perm[i]] = g;
+            u.rhs[g] = o.rhs[i];
+            u.mf0[g] = 1.0 - albedo[p];
+            u.bof[g] = o.bof[i] >= 0 ? (int32_t)(b0 + (uint32_t)o.bof[i]) : -1;
+            for (uint32_t s = 0; s < W; s++) {
+                // slots past a problem's own W repeat the vertex with zero couplings
+                u.nbr[(size_t)s * nP + g] = s < o.W ? v0 + o.nbr[(size_t)s * o.nP + i] : g;
+                if (s < o.W)
+                    for (uint32_t c = 0; c < NC; c++)
+                        u.coef[((size_t)s * NC + c) * nP + g] = o.coef[((size_t)s * NC + c) * o.nP + i];
+            }
+        }
+        for (uint32_t b = 0; b < o.nB; b++) {
+            const uint32_t gb = b0 + b;
+            u.bvert.push_back(v0 + o.bvert[b]);
+            for (uint32_t s = 0; s < WB; s++) {
+                u.bnbr[(size_t)s * nB + gb] = s < o.WB ? v0 + o.bnbr[(size_t)s * o.nB + b] : v0 + o.bvert[b];
+                if (s < o.WB)
+                    for (int n = 0; n < 3; n++)
+                        u.bcoef[((size_t)s * 3 + n) * nB + gb] = o.bcoef[((size_t)s * 3 + n) * o.nB + b];
+            }
+        }
+        for (uint32_t e : o.mask) u.mask.push_back(v0 * slm + e);
+        v0 += o.nP;
+        b0 += o.nB;
+    }
+    voff[ops.size()] = v0;
+    return TR_OK;
+}
+
+}  // namespace tr

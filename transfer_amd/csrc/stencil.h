@@ -64,7 +64,17 @@ struct HostOperator {
     // omega_pos[i][lm][0], i = 0..2) for the moments T_ij (MeshData.cpp:96-116)
     std::vector<double> mom_val;    // [3][slm]
     std::vector<uint32_t> mom_pos;  // [3][slm]
+    // batches: per-vertex factor of the lm = 0 mass term (1 - albedo of the
+    // vertex's problem, isotropic scattering); empty outside batches
+    std::vector<double> mf0;
 };
+
+// The block-diagonal union of independent operators with identical angular
+// data (the problems of a batch), vertices of problem p at
+// voff[p] .. voff[p+1] in the union's internal numbering and caller ids
+// shifted by the same offsets.  mf0 gets 1 - albedo[p] per vertex.
+int concat_operators(const std::vector<HostOperator> &ops, const std::vector<double> &albedo, HostOperator &u,
+                     std::vector<uint32_t> &voff);
 
 // Bit a of mirror_odd(l, m) is set when the real harmonic Y_lm (cos(m phi)
 // for m >= 0, sin(|m| phi) for m < 0, polar axis z, phi from x) changes sign

@@ -12,6 +12,9 @@ benchmark configurations, from the library's own generators.
     cube32    32^3 cubes on [0,1]^3 (196,608 tets), kappa = 0.1 + 8 Gaussian
               blobs (amplitude log-U[1,100], width U[0.05,0.3]), albedo
               U[0,0.99], I_p = 1, K = 3
+    batch     256 slabs n x n x 4n (n ~ U{8..32} from the seed), per-tet
+              ln kappa ~ N(0, 1), albedo ~ U[0, 0.95] per problem, I_p = 1,
+              K = 2 (50-point order); `count` and `nrange` shrink it for tests
     milne     the step problem of milne.geo as a 1-D slab: [-L,L] x [-h,h]^2,
               kappa = 1, I_p = 1 for x < 0 and 0 for x > 0, vacuum at
               x = +-L, specular reflection on the four side planes (so the
@@ -62,3 +65,24 @@ def milne(n=32, maxk=3, L=1.0, ncross=2):
         for value in (-h, h):
             m.set_face_kind_plane(axis, value, 1)
     return Problem("milne%d" % n, m, Angular(maxk))
+
+
+class BatchProblem:
+    def __init__(self, name, meshes, ang, albedo):
+        self.name, self.meshes, self.ang, self.albedo = name, meshes, ang, albedo
+
+
+def batch(count=256, nrange=(8, 32), maxk=2, seed=7):
+    """Independent slab problems: problem p is n x n x 4n cubes on
+    [0,1]^2 x [0,4] (n from the seed), ln kappa ~ N(0, 1) per tet."""
+    u = rng_uniform(seed, 20, 2 * count)
+    lo, hi = nrange
+    meshes, albedo = [], []
+    for p in range(count):
+        n = lo + min(int(u[2 * p] * (hi - lo + 1)), hi - lo)
+        m = Mesh.box((n, n, 4 * n), (0, 0, 0), (1, 1, 4))
+        m.set_fields(kappa=m.field("lognormal", [0.0, 1.0], seed=seed, stream=1000 + p),
+                     Ip=m.field("const", [1.0]))
+        meshes.append(m)
+        albedo.append(0.95 * u[2 * p + 1])
+    return BatchProblem("batch%d" % count, meshes, Angular(maxk), albedo)
