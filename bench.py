@@ -72,6 +72,52 @@ def cpu_baseline(P, seconds=12.0):
                        "vertices) on the full ball64 mesh, %.1f s" % (n, dt))
 
 
+def bench_batch(count, steps, warmup):
+    """The batching workload: `count` independent slabs (n ~ U{8..32}),
+    log-normal opacity, per-problem albedo, K = 2, solved together to a 1e-8
+    relative FP64 residual per problem."""
+    import transfer_amd as ta
+    t0 = time.perf_counter()
+    def progress(p):
+        if (p + 1) % 32 == 0 or p + 1 == count:
+            print("[bench-batch] %d/%d meshes, %.1f s" % (p + 1, count, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    B = ta.problems.batch(count, progress=progress)
+    t1 = time.perf_counter()
+    ctx = ta.Batch(B.meshes, B.ang, B.albedo)
+    t2 = time.perf_counter()
+    nP, slm = ctx.nP, ctx.slm
+    nT = sum(m.nT for m in B.meshes)
+    print("[bench-batch] %d problems, %d vertices, %d tets; meshes %.1f s, assembly+upload %.1f s"
+          % (count, nP, nT, t1 - t0, t2 - t1), file=sys.stderr, flush=True)
+    ctx.autotune(0)
+    ctx.autotune(1)
+    out = dict(metric="batched time-to-1e-8 residual & DOF*ordinate/s", unit="DOF*ordinate/s", n_gpus=1,
+               steps=steps, warmup=warmup, higher_is_better=True, data="synthetic", dtype="f32+f64",
+               config=dict(workload="batch%d: slabs n x n x 4n, n~U{8..32}, ln kappa~N(0,1), albedo~U[0,0.95], K=2"
+                                    % count, nP=nP, nT=nT, slm=slm, problems=count))
+    for prec, key in ((0, "operator_fp64"), (1, "operator_fp32")):
+        w, k, _, _ = ctx.bench_operator(warmup, steps, precision=prec)
+        out[key] = dict(kernel=ctx.kernel_name(prec), kernel_ms=k, dof_ordinate_per_s=nP * slm * steps / w)
+        print("[bench-batch] %s %s %.3f ms" % (key, ctx.kernel_name(prec), k), file=sys.stderr, flush=True)
+    out["value"] = out["operator_fp32"]["dof_ordinate_per_s"]
+    out["ms_per_step"] = out["operator_fp32"]["kernel_ms"]
+    solves = {}
+    for name, solver in (("batched_mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED),
+                         ("batched_pcg_fp64_jacobi", ta.TR_SOLVER_PCG_FP64)):
+        d_x = ctx.allocVector()
+        st, per = ctx.solve_batch(d_x, solver=solver, use_prec=2, maxit=20000 if solver != ta.TR_SOLVER_MIXED else 30)
+        solves[name] = dict(seconds=st.seconds, max_iters=st.iters, inner_iters_total=st.preck,
+                            worst_true_rel=st.true_rel, converged=st.status == 0,
+                            problems_converged=sum(p["status"] == 0 for p in per))
+        print("[bench-batch] %s: %.3f s, worst true rel %.2e, %d/%d converged"
+              % (name, st.seconds, st.true_rel, solves[name]["problems_converged"], count),
+              file=sys.stderr, flush=True)
+        ctx.deviceFree(d_x)
+    out["time_to_1e-8"] = solves
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,11 +128,16 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time a plain 256 MiB device copy (counter calibration)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="run the batching workload with this many problems instead (256 = the full one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
               "(the multi-GPU sharding is a design note, not an implementation)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        bench_batch(args.batch, args.steps, args.warmup)
+        return
 
     import transfer_amd as ta
     P = ta.problems.ball(args.n)

@@ -72,9 +72,10 @@ class BatchProblem:
         self.name, self.meshes, self.ang, self.albedo = name, meshes, ang, albedo
 
 
-def batch(count=256, nrange=(8, 32), maxk=2, seed=7):
+def batch(count=256, nrange=(8, 32), maxk=2, seed=7, progress=None):
     """Independent slab problems: problem p is n x n x 4n cubes on
-    [0,1]^2 x [0,4] (n from the seed), ln kappa ~ N(0, 1) per tet."""
+    [0,1]^2 x [0,4] (n from the seed), ln kappa ~ N(0, 1) per tet.
+    progress(p) is called after each problem's mesh when given."""
     u = rng_uniform(seed, 20, 2 * count)
     lo, hi = nrange
     meshes, albedo = [], []
@@ -85,4 +86,6 @@ def batch(count=256, nrange=(8, 32), maxk=2, seed=7):
                      Ip=m.field("const", [1.0]))
         meshes.append(m)
         albedo.append(0.95 * u[2 * p + 1])
+        if progress:
+            progress(p)
     return BatchProblem("batch%d" % count, meshes, Angular(maxk), albedo)
