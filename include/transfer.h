@@ -251,6 +251,9 @@ typedef struct tr_solve_params {
     int inner_maxit;     /* main.cpp:232: 1000 */
     double inner_tol;    /* main.cpp:239: 1e-8 absolute */
     int record;          /* leading iterations whose (alpha, beta, ||r||) to record */
+    int x0;              /* TR_SOLVER_PCG_FP64 / MIXED: start from the contents of f
+                            (e.g. the learned guess) instead of 0; the solve
+                            falls back to 0 when ||b - A f|| >= ||b|| */
 } tr_solve_params;
 
 typedef struct tr_solve_stats {
@@ -261,6 +264,7 @@ typedef struct tr_solve_stats {
     double true_rel;     /* ||b - A f|| / ||b|| with the FP64 operator */
     double seconds;      /* wall time of the solve, device-synchronised */
     int status;          /* 0 converged, TR_ERR_NOCONV otherwise */
+    int fallback;        /* 1: the x0 guess was rejected and the solve started from 0 */
 } tr_solve_stats;
 
 /* Solves A f = b, b = computeRhs, from f = 0.  f is a device vector.
@@ -298,6 +302,43 @@ typedef struct tr_batch_stats {
  * worst problem); per: [nprob] or NULL. */
 int tr_batch_solve(tr_ctx *ctx, const tr_solve_params *prm, double *f,
                    tr_solve_stats *total, tr_batch_stats *per);
+
+/* ------------------------------------------------------------------------ */
+/* Learned initial guess (extension): a per-vertex MLP, 8 features, two     */
+/* hidden layers of 64 (SiLU), sigmoid output = U / U_inf with              */
+/* U_inf = 4 pi I_p / (1 - albedo); BF16 weights, FP32 accumulation.        */
+/* ------------------------------------------------------------------------ */
+#define TR_MLP_FEATURES 8
+#define TR_MLP_PARAMS 4801   /* W1[64][8] b1[64] W2[64][64] b2[64] w3[64] b3 */
+
+typedef struct tr_mlp tr_mlp;
+/* Per-vertex features (caller's order) of a mesh with coordinates and its
+ * albedo: feat[nP][8] (see features.cpp for the list). */
+int tr_mlp_features(const tr_mesh_view *mesh, double albedo, float *feat);
+/* params: TR_MLP_PARAMS FP32 values, or NULL for a seeded initialisation. */
+int tr_mlp_create(int device, const float *params, uint64_t seed, tr_mlp **out);
+int tr_mlp_destroy(tr_mlp *m);
+/* Device buffers owned by the network (freed with it) and copies between
+ * them and the host (to_host 0: host -> device). */
+int tr_mlp_alloc(tr_mlp *m, size_t bytes, void **out);
+int tr_mlp_copy(tr_mlp *m, void *dst, const void *src, size_t bytes, int to_host);
+/* Feature normalisation xh = (x - mu) / sigma (8 values each). */
+int tr_mlp_set_norm(tr_mlp *m, const float *mu, const float *sigma);
+int tr_mlp_get_params(tr_mlp *m, float *params);
+/* y[n] (device) from device features feat[n][8]. */
+int tr_mlp_predict(tr_mlp *m, const float *feat, uint32_t n, float *y);
+/* Mean squared error of y against target[n] (device arrays) and its
+ * gradient (host, TR_MLP_PARAMS floats) with respect to the BF16 weights. */
+int tr_mlp_grad(tr_mlp *m, const float *feat, const float *target, uint32_t n,
+                float *loss, float *grad);
+/* `steps` Adam steps on minibatches of `batch` samples drawn from the n
+ * device samples; loss_hist[steps] (host, may be NULL). */
+int tr_mlp_train(tr_mlp *m, const float *feat, const float *target, uint32_t n,
+                 int steps, uint32_t batch, float lr, uint64_t seed, float *loss_hist);
+/* f (device vector) = the guess with scalar radiation U = 4 pi scale y,
+ * i.e. f[v][00] = scale * y[v], other harmonics 0; y[nP] is a device array
+ * in the caller's vertex order (scale = I_p / (1 - albedo) for the MLP). */
+int tr_ctx_scalar_guess(tr_ctx *ctx, const float *y, double scale, double *f);
 
 /* Seconds per operator application (median over reps), for benchmarking. */
 double tr_bench_lhs(tr_ctx *ctx, int reps);

@@ -1,0 +1,78 @@
+"""GPU: the learned initial guess.  The HIP forward pass and gradients
+against the FP64 oracle (oracle/mlp.py, itself pinned by finite
+differences); training on solutions the solver generates; and the gain of
+the learned guess over the plain start on held-out problems.
+
+Bars: predictions within 1e-4 of the oracle (FP32 accumulation, BF16 weights
+in both); gradient within 1e-3 relative (norm); training lowers the loss;
+on held-out seeds the learned guess predicts U/U_inf better than the
+training mean and lowers the initial FP64 residual below ||b||."""
+import numpy as np
+import pytest
+
+import transfer_amd as ta
+from oracle import mlp as omlp
+from transfer_amd import learned
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_forward_and_gradient_match_oracle():
+    rng = np.random.default_rng(4)
+    p = (omlp.init_params(5) + 0.05 * rng.standard_normal(omlp.NPARAM)).astype(np.float32)
+    net = ta.MLP(params=p)
+    mu = rng.standard_normal(8).astype(np.float32) * 0.1
+    sd = rng.uniform(0.5, 2.0, 8).astype(np.float32)
+    net.set_norm(mu, sd)
+    n = 1000
+    x = rng.standard_normal((n, 8)).astype(np.float32)
+    t = rng.uniform(0.05, 0.95, n).astype(np.float32)
+    d_x, d_t, d_y = net.upload(x), net.upload(t), net.alloc(n)
+    net.predict(d_x, n, d_y)
+    y = net.download(d_y, n)
+    y_or, _ = omlp.forward(p.astype(np.float64), x, mu, sd)
+    print("[gpu-mlp] forward max err %.2e" % np.abs(y - y_or).max(), flush=True)
+    assert np.abs(y - y_or).max() <= 1e-4
+    loss, g = net.grad(d_x, d_t, n)
+    l_or, g_or = omlp.loss_and_grad(p.astype(np.float64), x, t, mu, sd)
+    e = np.linalg.norm(g - g_or) / np.linalg.norm(g_or)
+    print("[gpu-mlp] loss %.6f vs %.6f, gradient rel err %.2e" % (loss, l_or, e), flush=True)
+    assert abs(loss - l_or) <= 1e-5 * max(1.0, l_or) and e <= 1e-3
+
+
+def test_learned_guess_on_held_out_problems():
+    n, K = 10, 3
+    print("[gpu-mlp] generating training data", flush=True)
+    feat, targ, _ = learned.generate(list(range(24)), n=n, maxk=K, chunk=12)
+    g = learned.LearnedGuess(seed=3)
+    hist = g.fit(feat, targ, steps=600, batch=8192, lr=3e-3)
+    print("[gpu-mlp] loss %.5f -> %.5f (%d samples)" % (hist[:10].mean(), hist[-10:].mean(), len(targ)), flush=True)
+    assert hist[-10:].mean() < 0.5 * hist[:10].mean()
+    base_it, learn_it, base_t, learn_t, pe, me = [], [], [], [], [], []
+    for seed in range(100, 106):
+        P = ta.problems.cube(n, maxk=K, seed=seed)
+        ctx = ta.HipContext(P.mesh, P.ang)
+        kw = dict(solver=ta.TR_SOLVER_MIXED, use_prec=2, maxit=30, tol_abs=0.0, tol_rel=1e-8,
+                  inner_maxit=20000, inner_tol=1e-4)
+        d0 = ctx.allocVector()
+        s0, _ = ctx.solve(d0, **kw)
+        U, _ = ctx.moments(d0)
+        t = U * (1 - P.albedo) / (4 * np.pi)
+        y = g.predict(ta.mlp_features(P.mesh, P.albedo))
+        pe.append(np.abs(y - t).mean())
+        me.append(np.abs(targ.mean() - t).mean())
+        d1 = g.write_guess(ctx, P.mesh, P.albedo)
+        s1, _ = ctx.solve(d1, x0=True, **kw)
+        assert s0.status == 0 and s1.status == 0 and s1.true_rel <= 1e-8
+        base_it.append(s0.preck)
+        learn_it.append(s1.preck)
+        base_t.append(s0.seconds)
+        learn_t.append(s1.seconds)
+        print("[gpu-mlp] seed %d albedo %.2f: inner iters %d -> %d, %.4f s -> %.4f s, fallback %d, "
+              "|y-t| %.4f (mean predictor %.4f)" % (seed, P.albedo, s0.preck, s1.preck, s0.seconds, s1.seconds,
+                                                    s1.fallback, pe[-1], me[-1]), flush=True)
+        for v in (d0, d1):
+            ctx.deviceFree(v)
+    print("[gpu-mlp] held-out mean inner iterations %.1f -> %.1f, time %.4f -> %.4f s" % (
+        np.mean(base_it), np.mean(learn_it), np.mean(base_t), np.mean(learn_t)), flush=True)
+    assert np.mean(pe) < np.mean(me)

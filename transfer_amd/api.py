@@ -19,6 +19,7 @@ lib = C.CDLL(LIB_PATH)
 u32p = C.POINTER(C.c_uint32)
 f64p = C.POINTER(C.c_double)
 u8p = C.POINTER(C.c_uint8)
+f32p = C.POINTER(C.c_float)
 vp = C.c_void_p
 
 TR_SOLVER_REFERENCE = 0
@@ -54,13 +55,13 @@ class AngularView(C.Structure):
 class SolveParams(C.Structure):
     _fields_ = [("solver", C.c_int), ("use_prec", C.c_int), ("maxit", C.c_int),
                 ("tol_abs", C.c_double), ("tol_rel", C.c_double),
-                ("inner_maxit", C.c_int), ("inner_tol", C.c_double), ("record", C.c_int)]
+                ("inner_maxit", C.c_int), ("inner_tol", C.c_double), ("record", C.c_int), ("x0", C.c_int)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("preck", C.c_int), ("rnorm", C.c_double),
                 ("bnorm", C.c_double), ("true_rel", C.c_double), ("seconds", C.c_double),
-                ("status", C.c_int)]
+                ("status", C.c_int), ("fallback", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -116,6 +117,17 @@ _sig("tr_compute_lhs_prec", C.c_int, vp, C.c_int, vp, vp)
 _sig("tr_compute_moments", C.c_int, vp, vp, vp, vp)
 _sig("tr_compute_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
 _sig("tr_expn", C.c_double, C.c_int, C.c_double)
+_sig("tr_mlp_features", C.c_int, C.POINTER(MeshView), C.c_double, f32p)
+_sig("tr_mlp_create", C.c_int, C.c_int, f32p, C.c_uint64, C.POINTER(vp))
+_sig("tr_mlp_destroy", C.c_int, vp)
+_sig("tr_mlp_set_norm", C.c_int, vp, f32p, f32p)
+_sig("tr_mlp_alloc", C.c_int, vp, C.c_size_t, C.POINTER(vp))
+_sig("tr_mlp_copy", C.c_int, vp, vp, vp, C.c_size_t, C.c_int)
+_sig("tr_mlp_get_params", C.c_int, vp, f32p)
+_sig("tr_mlp_predict", C.c_int, vp, vp, C.c_uint32, vp)
+_sig("tr_mlp_grad", C.c_int, vp, vp, vp, C.c_uint32, f32p, f32p)
+_sig("tr_mlp_train", C.c_int, vp, vp, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_float, C.c_uint64, f32p)
+_sig("tr_ctx_scalar_guess", C.c_int, vp, vp, C.c_double, vp)
 _sig("tr_batch_create", C.c_int, C.c_int, C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), f64p, C.c_int,
      C.POINTER(vp))
 _sig("tr_batch_info", C.c_int, vp, u32p, u32p)
@@ -417,6 +429,10 @@ class HipContext:
         self.copyToDev(v, np.ascontiguousarray(host, np.float64).reshape(-1))
         return v
 
+    def scalar_guess(self, y, scale, f):
+        """f[v][00] = scale * y[v] (device y in caller order), the rest 0."""
+        check(lib.tr_ctx_scalar_guess(self._h, y, float(scale), f))
+
     def moments(self, f):
         """ComputeMoments of the device vector f: host U[nP], T[9][nP] (T_ij at
         row 3i+j), caller's vertex order."""
@@ -486,8 +502,8 @@ class HipContext:
         check(lib.tr_synchronize(self._h))
 
     def solve(self, f, solver=TR_SOLVER_REFERENCE, use_prec=False, maxit=1000, tol_abs=1e-4,
-              tol_rel=0.0, inner_maxit=1000, inner_tol=1e-8, record=0):
-        prm = SolveParams(solver, int(use_prec), maxit, tol_abs, tol_rel, inner_maxit, inner_tol, record)
+              tol_rel=0.0, inner_maxit=1000, inner_tol=1e-8, record=0, x0=False):
+        prm = SolveParams(solver, int(use_prec), maxit, tol_abs, tol_rel, inner_maxit, inner_tol, record, int(x0))
         st = SolveStats()
         hist = np.zeros(3 * max(record, 1))
         check(lib.tr_solve(self._h, C.byref(prm), f, C.byref(st), ptr(hist)))
@@ -568,3 +584,76 @@ class Batch(HipContext):
         check(lib.tr_batch_solve(self._h, C.byref(prm), f, C.byref(st), per))
         return st, [dict(iters=p.iters, inner_iters=p.inner_iters, true_rel=p.true_rel, status=p.status)
                     for p in per]
+
+
+MLP_FEATURES, MLP_PARAMS = 8, 4801
+
+
+def mlp_features(mesh, albedo):
+    """Per-vertex features of the learned guess: float32 [nP][8]."""
+    out = np.empty((mesh.nP, MLP_FEATURES), np.float32)
+    check(lib.tr_mlp_features(C.byref(mesh.view), float(albedo), ptr(out, f32p)))
+    return out
+
+
+class MLP:
+    """The learned initial guess: 8 -> 64 -> 64 -> 1 (SiLU, sigmoid), BF16
+    weights with FP32 accumulation, trained with Adam by the library's HIP
+    kernels.  Device arrays are passed as raw addresses (e.g. from
+    HipContext.deviceAlloc)."""
+
+    def __init__(self, params=None, seed=1, device=0):
+        self._h = vp()
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        check(lib.tr_mlp_create(int(device), ptr(p, f32p), int(seed), C.byref(self._h)))
+
+    def upload(self, host):
+        """A device copy (owned by the network) of a float32 host array."""
+        host = np.ascontiguousarray(host, np.float32)
+        p = vp()
+        check(lib.tr_mlp_alloc(self._h, host.nbytes, C.byref(p)))
+        check(lib.tr_mlp_copy(self._h, p, host.ctypes.data, host.nbytes, 0))
+        return p.value
+
+    def alloc(self, n):
+        p = vp()
+        check(lib.tr_mlp_alloc(self._h, 4 * int(n), C.byref(p)))
+        return p.value
+
+    def download(self, dev, n):
+        out = np.empty(int(n), np.float32)
+        check(lib.tr_mlp_copy(self._h, out.ctypes.data, dev, out.nbytes, 1))
+        return out
+
+    def set_norm(self, mu, sigma):
+        mu = np.ascontiguousarray(mu, np.float32)
+        sigma = np.ascontiguousarray(sigma, np.float32)
+        check(lib.tr_mlp_set_norm(self._h, ptr(mu, f32p), ptr(sigma, f32p)))
+
+    def params(self):
+        out = np.empty(MLP_PARAMS, np.float32)
+        check(lib.tr_mlp_get_params(self._h, ptr(out, f32p)))
+        return out
+
+    def predict(self, feat, n, y):
+        check(lib.tr_mlp_predict(self._h, feat, int(n), y))
+
+    def grad(self, feat, target, n):
+        loss = np.zeros(1, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        check(lib.tr_mlp_grad(self._h, feat, target, int(n), ptr(loss, f32p), ptr(g, f32p)))
+        return float(loss[0]), g
+
+    def train(self, feat, target, n, steps, batch=65536, lr=1e-3, seed=1):
+        hist = np.zeros(max(steps, 1), np.float32)
+        check(lib.tr_mlp_train(self._h, feat, target, int(n), int(steps), int(batch), float(lr), int(seed),
+                               ptr(hist, f32p)))
+        return hist[:steps]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.tr_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -461,6 +461,27 @@ int tr_compute_rhs(tr_ctx *c, double *b) {
     return TR_OK;
 }
 
+namespace tr {
+__global__ void k_scalar_guess(const float *__restrict__ y, const uint32_t *__restrict__ perm, uint32_t nP,
+                               uint32_t slm, double scale, double *__restrict__ f) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)nP * slm;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t v = i / slm;
+        f[i] = (i - v * slm) == 0 ? scale * (double)y[perm[v]] : 0.0;
+    }
+}
+}  // namespace tr
+
+// f = the guess f[v][00] = scale * y[v] (y device, caller's order), the
+// learned initial guess of the solvers (with tr_solve_params.x0).
+int tr_ctx_scalar_guess(tr_ctx *c, const float *y, double scale, double *f) {
+    if (!c || !y || !f) return fail(TR_ERR_ARG, "tr_ctx_scalar_guess: null argument");
+    hipLaunchKernelGGL(k_scalar_guess, dim3(1024), dim3(256), 0, c->stream, y, c->d_perm, c->op.nP, c->op.slm, scale,
+                       f);
+    HIPCHECK(hipGetLastError());
+    return TR_OK;
+}
+
 // MeshData::ComputeMoments (MeshData.cpp:96-116) of a device vector: U[nP]
 // and T[3i+j][nP] (device arrays) in the caller's vertex order.
 int tr_compute_moments(tr_ctx *c, const double *f, double *U, double *T) {
@@ -666,6 +687,21 @@ std::vector<double> operator_diagonal(const HostOperator &op) {
     return d;
 }
 
+// Starting point of the device solvers: 0, or the caller's f (prm->x0)
+// unless its residual is no smaller than b's (then 0, reported as a
+// fallback).  r is scratch.
+static int start_point(tr_ctx *c, const tr_solve_params *prm, double *f, const double *b, double *r,
+                       tr_solve_stats *st) {
+    if (st) st->fallback = 0;
+    if (prm->x0) {
+        int rc = lhs(c, f, r);
+        if (rc || (rc = axpby(c, r, -1.0, b, 1.0))) return rc;
+        if (std::sqrt(dot(c, r, r)) < std::sqrt(dot(c, b, b))) return TR_OK;
+        if (st) st->fallback = 1;
+    }
+    return axpby(c, f, 0, nullptr, 0);
+}
+
 static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
     Work w(c);
     const size_t n = c->nint();
@@ -686,7 +722,7 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
         return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none) or 2 (Jacobi)");
     }
     int rc;
-    if ((rc = tr_compute_rhs(c, b)) || (rc = axpby(c, f, 0, nullptr, 0))) return rc;
+    if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
     CgSpace<double> S;
     S.a = c->args();
     S.NC = (int)c->op.NC;
@@ -786,7 +822,7 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none) or 2 (Jacobi)");
     }
     auto t0 = std::chrono::steady_clock::now();
-    if ((rc = tr_compute_rhs(c, b)) || (rc = axpby(c, f, 0, nullptr, 0))) return rc;
+    if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
     const double bn = std::sqrt(dot(c, b, b));
     CgSpace<float> S;
     S.a = c->args32();

@@ -1,0 +1,75 @@
+"""The learned initial guess end to end: solver-generated training data,
+training on the device, and the guess a solve starts from.
+
+    data = generate(seeds, n=32, maxk=3)          # cube problems, one batched solve per chunk
+    g = LearnedGuess(); g.fit(data)               # Adam on the device
+    st = g.solve(mesh, albedo, ang)               # guess + mixed solve, plain fallback
+
+The target of vertex v is U_v / U_inf,v with U_inf = 4 pi I_p / (1 - albedo)
+(I_p = 1 in the cube problems), so the network's sigmoid output maps to
+f[v][00] = y_v I_p / (1 - albedo)."""
+import numpy as np
+
+from . import problems
+from .api import MLP, Angular, Batch, HipContext, mlp_features, TR_SOLVER_MIXED
+
+FOUR_PI = 4.0 * np.pi
+
+
+def cube_problems(seeds, n=32, maxk=3):
+    return [problems.cube(n, maxk=maxk, seed=int(s)) for s in seeds]
+
+
+def generate(seeds, n=32, maxk=3, chunk=64, inner_tol=1e-4, progress=None):
+    """Features and targets of the cube problems of `seeds`, solved to a 1e-8
+    FP64 residual with the batched mixed solver.  Returns (feat [N][8],
+    target [N], per-problem vertex counts)."""
+    feats, targs, counts = [], [], []
+    for c0 in range(0, len(seeds), chunk):
+        P = cube_problems(seeds[c0:c0 + chunk], n, maxk)
+        B = Batch([p.mesh for p in P], Angular(maxk), [p.albedo for p in P])
+        B.autotune(0)
+        B.autotune(1)
+        d_x = B.allocVector()
+        st, per = B.solve_batch(d_x, solver=TR_SOLVER_MIXED, use_prec=2, tol_rel=1e-8, inner_tol=inner_tol, maxit=30)
+        U, _ = B.moments(d_x)
+        B.deviceFree(d_x)
+        for k, p in enumerate(P):
+            lo, hi = int(B.voff[k]), int(B.voff[k + 1])
+            feats.append(mlp_features(p.mesh, p.albedo))
+            targs.append(np.clip(U[lo:hi] * (1.0 - p.albedo) / FOUR_PI, 0.0, 1.0).astype(np.float32))
+            counts.append(hi - lo)
+        if progress:
+            progress(min(c0 + chunk, len(seeds)))
+        del B
+    return np.concatenate(feats), np.concatenate(targs), counts
+
+
+class LearnedGuess:
+    def __init__(self, seed=1, params=None):
+        self.net = MLP(params=params, seed=seed)
+        self.mu = np.zeros(8, np.float32)
+        self.sd = np.ones(8, np.float32)
+
+    def fit(self, feat, target, steps=2000, batch=16384, lr=2e-3, seed=1):
+        self.mu = feat.mean(0).astype(np.float32)
+        self.sd = np.maximum(feat.std(0), 1e-3).astype(np.float32)
+        self.net.set_norm(self.mu, self.sd)
+        d_f, d_t = self.net.upload(feat), self.net.upload(target)
+        return self.net.train(d_f, d_t, len(target), steps, batch, lr, seed)
+
+    def predict(self, feat):
+        d_f = self.net.upload(feat)
+        d_y = self.net.alloc(len(feat))
+        self.net.predict(d_f, len(feat), d_y)
+        return self.net.download(d_y, len(feat))
+
+    def write_guess(self, ctx, mesh, albedo, Ip=1.0):
+        """ctx's device vector holding the learned guess."""
+        y = self.predict(mlp_features(mesh, albedo))
+        d_y = ctx.deviceAlloc(4 * len(y))
+        ctx.copyToDev(d_y, y)
+        f = ctx.allocVector()
+        ctx.scalar_guess(d_y, Ip / (1.0 - albedo), f)
+        ctx.deviceFree(d_y)
+        return f
