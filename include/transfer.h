@@ -186,6 +186,12 @@ int tr_ctx_schedule(tr_ctx *ctx, int precision, int *out5);
  * staging with buffer loads and per-slot coefficient records, 4 = angular
  * contraction on the matrix cores, J ignored). */
 int tr_ctx_set_schedule(tr_ctx *ctx, int precision, int J, int WPB, int variant);
+/* Multilevel preconditioner (use_prec 3 of the device solvers, extension):
+ * Galerkin coarse operators of the same Kronecker form under aggregation of
+ * `agg` consecutive vertices, coarsened until at most `coarsest` vertices
+ * remain; symmetric V-cycle with one damped-Jacobi step per side and `nu`
+ * on the coarsest level (defaults 3000, 16, 2). */
+int tr_ctx_set_multilevel(tr_ctx *ctx, uint32_t coarsest, uint32_t agg, int nu);
 /* Angular precision of the FP32 operator used by TR_SOLVER_MIXED:
  * 0 = FP32 throughout, 1 = BF16 matrix-core projection with FP32
  * accumulation, 3 (default) = BF16 with the hi/lo split (three BF16
@@ -366,6 +372,11 @@ int tr_stencil_create(const tr_mesh_view *mesh, const tr_angular_view *ang,
 int tr_stencil_destroy(tr_stencil *s);
 /* info: nP, slm, NC, W, nB, WB, arow, brow */
 int tr_stencil_info(const tr_stencil *s, uint32_t *info8);
+/* The Galerkin coarse operator of the multilevel preconditioner: vertices
+ * aggregated in runs of `size` consecutive internal ids (aggregate I =
+ * agg[I] .. agg[I+1]-1, agg has ceil(nP/size)+1 entries, may be NULL).
+ * The result's perm is the identity over aggregates. */
+int tr_stencil_coarsen(const tr_stencil *s, uint32_t size, tr_stencil **out, uint32_t *agg);
 /* Specular reflection (extension; faces of kind 1 on axis-aligned planes):
  * the entries v*slm + lm of the internal [v][slm] numbering that vanish,
  * the harmonics odd under a reflecting plane's mirror at its vertices.

@@ -1,5 +1,6 @@
-"""Device-resident CG (FP64) and FP64-refined mixed-precision solves against
-the oracle's FP64 PCG solution on the same inputs.
+"""Device-resident CG (FP64) and FP64-refined mixed-precision solves, with
+no, Jacobi (2) or multilevel (3) preconditioning, against the oracle's FP64
+PCG solution on the same inputs.
 
 Bars: the FP64 true residual ||b - A x|| / ||b|| of every returned solution
 meets the requested tolerance (checked on the GPU with the FP64 operator and
@@ -29,7 +30,8 @@ def build(name):
 
 @pytest.mark.parametrize("name", ["slab8", "ball16", "cube8", "ball12-hg"])
 @pytest.mark.parametrize("solver,prec", [(ta.TR_SOLVER_PCG_FP64, 0), (ta.TR_SOLVER_PCG_FP64, 2),
-                                         (ta.TR_SOLVER_MIXED, 2)])
+                                         (ta.TR_SOLVER_PCG_FP64, 3), (ta.TR_SOLVER_MIXED, 2),
+                                         (ta.TR_SOLVER_MIXED, 3)])
 def test_solver_reaches_fp64_residual(name, solver, prec):
     print("[gpu-solvers]", name, solver, prec, flush=True)
     P = build(name)
@@ -51,3 +53,22 @@ def test_solver_reaches_fp64_residual(name, solver, prec):
     assert st.true_rel <= tol * 1.01 and res <= tol * 1.5
     assert err <= 1e-6
     ctx.close()
+
+
+def test_multilevel_hierarchy_shapes():
+    """Every hierarchy shape gives a converged solve with the same solution
+    (the preconditioner changes the path, not the answer)."""
+    P = ta.problems.cube(10, seed=2)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    ref = None
+    for coarsest, agg, nu in ((64, 8, 2), (500, 16, 4), (100000, 8, 2)):
+        ctx.set_multilevel(coarsest, agg, nu)
+        d = ctx.allocVector()
+        st, _ = ctx.solve(d, solver=ta.TR_SOLVER_PCG_FP64, use_prec=3, maxit=5000, tol_abs=0.0, tol_rel=1e-10)
+        x = ctx.copyToHost(d)
+        ctx.deviceFree(d)
+        print("[gpu-solvers] multilevel %d/%d/%d: %d iters" % (coarsest, agg, nu, st.iters), flush=True)
+        assert st.status == 0 and st.true_rel <= 1.01e-10
+        if ref is None:
+            ref = x
+        assert np.abs(x - ref).max() <= 1e-8 * np.abs(ref).max()

@@ -115,6 +115,7 @@ for _n in ("tr_compute_lhs", "tr_compute_lhs_diag"):
 _sig("tr_compute_rhs", C.c_int, vp, vp)
 _sig("tr_compute_lhs_prec", C.c_int, vp, C.c_int, vp, vp)
 _sig("tr_compute_moments", C.c_int, vp, vp, vp, vp)
+_sig("tr_ctx_set_multilevel", C.c_int, vp, C.c_uint32, C.c_uint32, C.c_int)
 _sig("tr_compute_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
 _sig("tr_expn", C.c_double, C.c_int, C.c_double)
 _sig("tr_mlp_features", C.c_int, C.POINTER(MeshView), C.c_double, f32p)
@@ -150,6 +151,7 @@ _sig("tr_stencil_create", C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), 
 _sig("tr_stencil_destroy", C.c_int, vp)
 _sig("tr_stencil_info", C.c_int, vp, u32p)
 _sig("tr_stencil_mask", C.c_int, vp, u32p, u32p)
+_sig("tr_stencil_coarsen", C.c_int, vp, C.c_uint32, C.POINTER(vp), u32p)
 _sig("tr_stencil_export", C.c_int, vp, u32p, u32p, f64p, u32p, u32p, f64p, f64p, f64p, f64p, f64p)
 
 
@@ -320,9 +322,12 @@ class Config:
 class Stencil:
     """The assembled operator data (introspection for tests)."""
 
-    def __init__(self, mesh_view, ang_view):
+    def __init__(self, mesh_view, ang_view, _handle=None):
         self._h = vp()
-        check(lib.tr_stencil_create(C.byref(mesh_view), C.byref(ang_view), C.byref(self._h)))
+        if _handle is not None:
+            self._h = _handle
+        else:
+            check(lib.tr_stencil_create(C.byref(mesh_view), C.byref(ang_view), C.byref(self._h)))
         info = np.zeros(8, np.uint32)
         check(lib.tr_stencil_info(self._h, ptr(info, u32p)))
         nP, slm, NC, W, nB, WB, arow, brow = [int(x) for x in info]
@@ -345,6 +350,13 @@ class Stencil:
         check(lib.tr_stencil_export(self._h, ptr(self.perm, u32p), ptr(self.nbr, u32p), ptr(self.coef),
                                     ptr(self.bvert, u32p), ptr(self.bnbr, u32p), ptr(self.bcoef),
                                     ptr(self.omc), ptr(self.onc), ptr(self.massfac), ptr(self.rhs)))
+
+    def coarsen(self, size=8):
+        """(coarse Stencil, aggregate starts) of the multilevel preconditioner."""
+        h = vp()
+        agg = np.zeros((self.nP + size - 1) // size + 1, np.uint32)
+        check(lib.tr_stencil_coarsen(self._h, int(size), C.byref(h), ptr(agg, u32p)))
+        return Stencil(None, None, _handle=h), agg
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -428,6 +440,9 @@ class HipContext:
         v = self.allocVector()
         self.copyToDev(v, np.ascontiguousarray(host, np.float64).reshape(-1))
         return v
+
+    def set_multilevel(self, coarsest=2048, agg=8, nu=2):
+        check(lib.tr_ctx_set_multilevel(self._h, int(coarsest), int(agg), int(nu)))
 
     def scalar_guess(self, y, scale, f):
         """f[v][00] = scale * y[v] (device y in caller order), the rest 0."""

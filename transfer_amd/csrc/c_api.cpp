@@ -302,6 +302,20 @@ int tr_stencil_create(const tr_mesh_view *mesh, const tr_angular_view *ang, tr_s
     return TR_OK;
 }
 
+int tr_stencil_coarsen(const tr_stencil *s, uint32_t size, tr_stencil **out, uint32_t *agg) {
+    if (!s || !out) return fail(TR_ERR_ARG, "tr_stencil_coarsen: null argument");
+    tr_stencil *c = new tr_stencil();
+    std::vector<uint32_t> a, cv;
+    int rc = coarsen_operator(s->op, {}, size, c->op, a, cv);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    if (agg) std::memcpy(agg, a.data(), sizeof(uint32_t) * a.size());
+    *out = c;
+    return TR_OK;
+}
+
 int tr_stencil_destroy(tr_stencil *s) {
     delete s;
     return TR_OK;

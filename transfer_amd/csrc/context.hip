@@ -137,6 +137,8 @@ static std::vector<T> bcoef_records(const HostOperator &op) {
     return rec;
 }
 
+
+
 static uint16_t to_bf16(float x) {
     uint32_t u;
     std::memcpy(&u, &x, 4);
@@ -284,7 +286,7 @@ int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *a
 int tr::ctx_init(tr_ctx *c, int device) {
     int rc;
     hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess && !c->stream) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
         return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
@@ -340,10 +342,11 @@ extern "C" {
 int tr_ctx_destroy(tr_ctx *c) {
     if (!c) return TR_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
+    destroy_multilevel(c->mg);
     for (void *p : c->owned) hipFree(p);
     for (void *p : c->vectors) hipFree(p);
     for (void *p : c->raw) hipFree(p);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return TR_OK;
 }
@@ -713,17 +716,24 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     part2 = part + std::max<size_t>(nb, 3 * 1024);
     sc = part2 + 2 * 1024;
     if (!Ap) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+    int rc;
     if (prm->use_prec == 2) {
         std::vector<double> d = operator_diagonal(c->op);
         for (double &x : d) x = 1.0 / x;
         dinv = w.get();
         HIPCHECK(hipMemcpyAsync(dinv, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    } else if (prm->use_prec == 3) {
+        if ((rc = build_multilevel(c, false, c->mg_coarsest, c->mg_agg))) return rc;
     } else if (prm->use_prec != 0) {
-        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none) or 2 (Jacobi)");
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none), 2 (Jacobi) or 3 (multilevel)");
     }
-    int rc;
     if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
     CgSpace<double> S;
+    if (prm->use_prec == 3) {
+        S.z = w.get();
+        if (!S.z) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+        S.prec = [c](const double *rr, double *z, const double *skip) { return multilevel_apply64(c, rr, z, skip); };
+    }
     S.a = c->args();
     S.NC = (int)c->op.NC;
     S.map = c->map;
@@ -806,7 +816,7 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     double *part = nullptr;
     HIPCHECK(hipMalloc((void **)&part, sizeof(double) * (std::max<size_t>(nb, 3 * 1024) + 2 * 1024 + 16)));
     w.v.push_back(part);
-    HIPCHECK(hipMalloc((void **)&v32, sizeof(float) * (6 * n + 8)));
+    HIPCHECK(hipMalloc((void **)&v32, sizeof(float) * (7 * n + 8)));
     w.v.push_back((double *)v32);
     float *d32 = v32, *r32 = v32 + n, *p32 = v32 + 2 * n, *Ap32 = v32 + 3 * n, *b32 = v32 + 4 * n;
     float *dinv32 = nullptr;
@@ -818,13 +828,19 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         dinv32 = v32 + 5 * n;
         HIPCHECK(hipMemcpyAsync(dinv32, di.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
+    } else if (prm->use_prec == 3) {
+        if ((rc = build_multilevel(c, true, c->mg_coarsest, c->mg_agg))) return rc;
     } else if (prm->use_prec != 0) {
-        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none) or 2 (Jacobi)");
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none), 2 (Jacobi) or 3 (multilevel)");
     }
     auto t0 = std::chrono::steady_clock::now();
     if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
     const double bn = std::sqrt(dot(c, b, b));
     CgSpace<float> S;
+    if (prm->use_prec == 3) {
+        S.z = v32 + 6 * n;
+        S.prec = [c](const float *rr, float *z, const double *skip) { return multilevel_apply32(c, rr, z, skip); };
+    }
     S.a = c->args32();
     S.NC = (int)c->op.NC;
     S.map = c->map32;
@@ -1219,6 +1235,23 @@ extern "C" int tr_compute_lhs_prec(tr_ctx *c, int precision, const double *f, do
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(v);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("tr_compute_lhs_prec: ") + hipGetErrorString(e));
+    return TR_OK;
+}
+
+// Shape of the multilevel preconditioner (use_prec 3): levels are coarsened
+// until at most `coarsest` vertices remain, by aggregates of `agg` vertices,
+// and the coarsest level takes `nu` damped-Jacobi steps.  Drops a hierarchy
+// already built.
+extern "C" int tr_ctx_set_multilevel(tr_ctx *c, uint32_t coarsest, uint32_t agg, int nu) {
+    if (!c || agg < 2 || nu < 1 || nu > 64) return fail(TR_ERR_ARG, "tr_ctx_set_multilevel: bad argument");
+    c->mg_coarsest = coarsest;
+    c->mg_agg = agg;
+    c->mg_nu = nu;
+    if (c->mg) {
+        hipStreamSynchronize(c->stream);
+        destroy_multilevel(c->mg);
+        c->mg = nullptr;
+    }
     return TR_OK;
 }
 

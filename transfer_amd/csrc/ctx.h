@@ -13,6 +13,10 @@
 #include "kernels.h"
 #include "stencil.h"
 
+namespace tr {
+struct Multilevel;
+}
+
 #define HIPCHECK(x)                                                                   \
     do {                                                                              \
         hipError_t e_ = (x);                                                          \
@@ -56,6 +60,10 @@ struct tr_ctx {
     // batches: internal vertex range of problem p = voff[p] .. voff[p+1]
     // (empty for a single problem)
     std::vector<uint32_t> voff;
+    bool own_stream = true;        // false for the levels of a multilevel hierarchy
+    tr::Multilevel *mg = nullptr;  // multilevel preconditioner (use_prec 3), built on first use
+    uint32_t mg_coarsest = 3000, mg_agg = 16; // hierarchy shape: coarsest-level size, aggregate size
+    int mg_nu = 2;                            // damped-Jacobi steps on the coarsest level
     std::vector<void *> owned;
 
     tr::OpArgs<double> args() const { return args(map); }
@@ -149,5 +157,12 @@ int ctx_init(tr_ctx *c, int device);
 int ensure_fp32(tr_ctx *c);
 // diag(A) per entry of the internal [v][slm] layout (Jacobi preconditioner)
 std::vector<double> operator_diagonal(const HostOperator &op);
+
+// multilevel preconditioner (mg.hip)
+int build_multilevel(tr_ctx *c, bool f32, uint32_t min_coarse, uint32_t size);
+void destroy_multilevel(Multilevel *mg);
+hipError_t multilevel_apply64(tr_ctx *c, const double *r, double *z, const double *skip);
+hipError_t multilevel_apply32(tr_ctx *c, const float *r, float *z, const double *skip);
+int multilevel_levels(const tr_ctx *c);
 
 }  // namespace tr

@@ -72,6 +72,10 @@ constexpr int tab_rec_bytes() {
 template <typename T>
 hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                         double *part, const double *skip, hipStream_t s);
+// r[mask[i]] = 0 for the entries that specular reflection removes (no-op
+// without reflecting planes)
+template <typename T>
+hipError_t launch_zero_mask(const OpArgs<T> &a, T *r, const double *skip, hipStream_t s);
 // r = D f, the angular-diagonal operator (kernels.cu:290-357, 440-493)
 template <typename T>
 hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,

@@ -572,7 +572,7 @@ __global__ __launch_bounds__(256) void k_zero_mask(T *__restrict__ r, const uint
 }
 
 template <typename T>
-static hipError_t zero_mask(const OpArgs<T> &a, T *r, const double *skip, hipStream_t s) {
+hipError_t launch_zero_mask(const OpArgs<T> &a, T *r, const double *skip, hipStream_t s) {
     if (!a.nmask) return hipSuccess;
     const uint32_t g = std::min<uint32_t>((a.nmask + 255) / 256, 1024);
     hipLaunchKernelGGL(k_zero_mask<T>, dim3(g), dim3(256), 0, s, r, a.mask, a.nmask, skip);
@@ -593,7 +593,7 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
             e = launch_apply4<T, 2>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
         else
             e = launch_apply4<T, 1>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
-        return e != hipSuccess ? e : zero_mask(a, r, skip, s);
+        return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
     }
     dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
     // variant 3 needs W = 15 or 16 exactly (the host pads the stencil) so that
@@ -629,7 +629,7 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
 #undef TR_CASE
 #undef TR_LAUNCH
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : zero_mask(a, r, skip, s);
+    return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
 }
 
 template <typename T>
@@ -650,9 +650,11 @@ hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const
     }
 #undef TR_CASE
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : zero_mask(a, r, skip, s);
+    return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
 }
 
+template hipError_t launch_zero_mask<double>(const OpArgs<double> &, double *, const double *, hipStream_t);
+template hipError_t launch_zero_mask<float>(const OpArgs<float> &, float *, const double *, hipStream_t);
 template hipError_t launch_apply<double>(const OpArgs<double> &, int, const LaneMap &, const double *,
                                          double *, double *, const double *, hipStream_t);
 template hipError_t launch_apply<float>(const OpArgs<float> &, int, const LaneMap &, const float *,

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     constexpr int NTMAX = (J * 4 + 15) / 16;
     typedef typename std::conditional<MODE == 0, d4v, f4v>::type acc4;
     // register budget: 2 wavefronts per SIMD for FP64 at J = 7, 3 otherwise
-    constexpr int BUDGET = (sizeof(T) == 8 && J >= 7) ? 240 : 148;
+    constexpr int BUDGET = (sizeof(T) == 8 && J >= 7) ? 230 : 148;
     constexpr int ACCR = NC * J * (int)sizeof(T) / 4, SLOTR = (J + RS) * (int)sizeof(T) / 4;
     constexpr int DEPTH = std::max(2, std::min(6, (BUDGET - ACCR - 20) / SLOTR));
     constexpr int BACCR = 3 * J * (int)sizeof(T) / 4, BSLOTR = (J + 4) * (int)sizeof(T) / 4;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include "kernels.h"
 
 namespace tr {
@@ -34,6 +36,12 @@ struct CgSpace {
     double *part;      // >= max(apply_blocks, 3*1024) doubles
     double *part2;     // >= 2*1024 doubles
     double *sc;        // SC_N doubles
+    // general preconditioner z = M r (launches on the solve's stream, no
+    // host synchronisation: it is captured into the iteration graph; its
+    // kernels return at once when *skip != 0); when set, dinv is ignored
+    // and z (n entries) holds M r
+    std::function<hipError_t(const T *r, T *z, const double *skip)> prec;
+    T *z = nullptr;
 };
 
 struct CgParams {

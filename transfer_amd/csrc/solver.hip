@@ -159,6 +159,31 @@ __global__ __launch_bounds__(256) void k_init(const T *__restrict__ b, T *__rest
     }
 }
 
+// r.z partials (into part, the r.z slots of k_update / k_init), and p = z
+// when p is given
+template <typename T>
+__global__ __launch_bounds__(256) void k_rz(const T *__restrict__ r, const T *__restrict__ z, T *__restrict__ p,
+                                            size_t n, double *__restrict__ part, const double *__restrict__ sc) {
+    if (sc && sc[SC_CONV] != 0.0) return;
+    double rz = 0, dummy = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        rz = fma((double)r[i], (double)z[i], rz);
+        if (p) p[i] = z[i];
+    }
+    block_sum2(rz, dummy);
+    if (threadIdx.x == 0) part[blockIdx.x] = rz;
+}
+
+// p = z + beta p
+template <typename T>
+__global__ __launch_bounds__(256) void k_pupdate_z(T *__restrict__ p, const T *__restrict__ z, size_t n,
+                                                   const double *__restrict__ sc) {
+    if (sc[SC_CONV] != 0.0) return;
+    const T beta = (T)sc[SC_BETA];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = fma(beta, p[i], z[i]);
+}
+
 // sc from the init partials: rr, rz, bb, and the stopping threshold
 __global__ __launch_bounds__(256) void k_fin_init(const double *__restrict__ part, int n,
                                                   double *__restrict__ sc, double tol_abs,
@@ -202,7 +227,12 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
         HC(launch_apply_diag<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
     else
         HC(launch_apply<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
-    hipLaunchKernelGGL(k_init<T>, dim3(G), dim3(256), 0, st, S.b, S.r, S.p, S.dinv, n, S.part);
+    const bool gen = (bool)S.prec;
+    hipLaunchKernelGGL(k_init<T>, dim3(G), dim3(256), 0, st, S.b, S.r, S.p, gen ? nullptr : S.dinv, n, S.part);
+    if (gen) {
+        HC(S.prec(S.r, S.z, nullptr));
+        hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, S.p, n, S.part + TR_UPD_BLOCKS, nullptr);
+    }
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, st, S.part, (int)G, S.sc, prm.tol_abs, prm.tol_rel);
     HC(hipGetLastError());
 
@@ -216,10 +246,18 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
         else
             launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
         hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, st, S.part, (int)nb, S.sc);
-        hipLaunchKernelGGL(k_update<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap, S.dinv, n, S.sc,
-                           S.part2);
+        hipLaunchKernelGGL(k_update<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap, gen ? nullptr : S.dinv, n,
+                           S.sc, S.part2);
+        if (gen) {
+            S.prec(S.r, S.z, S.sc + SC_CONV);
+            hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, (T *)nullptr, n,
+                               S.part2 + TR_UPD_BLOCKS, S.sc);
+        }
         hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(256), 0, st, S.part2, (int)G, S.sc, prm.maxit);
-        hipLaunchKernelGGL(k_pupdate<T>, dim3(G), dim3(256), 0, st, S.p, S.r, S.dinv, n, S.sc);
+        if (gen)
+            hipLaunchKernelGGL(k_pupdate_z<T>, dim3(G), dim3(256), 0, st, S.p, S.z, n, S.sc);
+        else
+            hipLaunchKernelGGL(k_pupdate<T>, dim3(G), dim3(256), 0, st, S.p, S.r, S.dinv, n, S.sc);
     }
     e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));

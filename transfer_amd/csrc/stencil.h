@@ -73,6 +73,13 @@ struct HostOperator {
 // data (the problems of a batch), vertices of problem p at
 // voff[p] .. voff[p+1] in the union's internal numbering and caller ids
 // shifted by the same offsets.  mf0 gets 1 - albedo[p] per vertex.
+// Galerkin coarse operator under aggregation of runs of `size` consecutive
+// vertices (per problem range voff, or all vertices when voff is empty):
+// agg[I] .. agg[I+1] are aggregate I's fine vertices, cvoff the coarse
+// problem ranges (coarsen.cpp).
+int coarsen_operator(const HostOperator &f, const std::vector<uint32_t> &voff, uint32_t size, HostOperator &c,
+                     std::vector<uint32_t> &agg, std::vector<uint32_t> &cvoff);
+
 int concat_operators(const std::vector<HostOperator> &ops, const std::vector<double> &albedo, HostOperator &u,
                      std::vector<uint32_t> &voff);
 
