@@ -137,7 +137,32 @@ static std::vector<T> bcoef_records(const HostOperator &op) {
     return rec;
 }
 
-
+// Order in which the matrix-core schedule walks its wavefront tiles (16
+// consecutive vertices each): tiles holding vacuum-boundary vertices do the
+// boundary gather as well, about twice the work of the others, so they are
+// spread evenly through the order (each XCD's contiguous share of the
+// persistent grid's work gets its share of them); both kinds keep their
+// (Morton) order, so neighbouring entries stay spatially close.
+static std::vector<uint32_t> tile_order(const HostOperator &op) {
+    const uint32_t nt = (op.nP + 15) / 16;
+    std::vector<uint32_t> bt, it, out;
+    for (uint32_t t = 0; t < nt; t++) {
+        bool b = false;
+        for (uint32_t v = 16 * t; v < std::min(op.nP, 16 * t + 16); v++) b = b || op.bof[v] >= 0;
+        (b ? bt : it).push_back(t);
+    }
+    out.reserve(nt);
+    size_t ib = 0, ii = 0;
+    for (uint32_t k = 0; k < nt; k++) {
+        // boundary tiles due after k + 1 entries: floor((k + 1) nb / nt)
+        const size_t due = (size_t)(k + 1) * bt.size() / nt;
+        if (ib < due || ii == it.size())
+            out.push_back(bt[ib++]);
+        else
+            out.push_back(it[ii++]);
+    }
+    return out;
+}
 
 static uint16_t to_bf16(float x) {
     uint32_t u;
@@ -311,7 +336,7 @@ int tr::ctx_init(tr_ctx *c, int device) {
         (rc = upload(c, &c->d_ddiag, op.ddiag)) || (rc = upload(c, &c->d_bdiag, op.bdiag)) ||
         (rc = upload(c, &c->d_rhs, op.rhs)) || (rc = upload(c, &c->d_perm, op.perm)) ||
         (rc = upload(c, &c->d_iperm, op.iperm)) || (rc = upload(c, &c->d_crec, coef_records<double>(op))) ||
-        (rc = upload(c, &c->d_bcrec, bcoef_records<double>(op))) ||
+        (rc = upload(c, &c->d_bcrec, bcoef_records<double>(op))) || (rc = upload(c, &c->d_tiles, tile_order(op))) ||
         (!op.mask.empty() && (rc = upload(c, &c->d_mask, op.mask))) ||
         (rc = upload(c, &c->d_mom_val, op.mom_val)) || (rc = upload(c, &c->d_mom_pos, op.mom_pos)) ||
         (!op.mf0.empty() && (rc = upload(c, &c->d_mf0, op.mf0))) ||

@@ -40,6 +40,7 @@ struct tr_ctx {
     double *d_crec = nullptr;   // coefficient records for the buffer-load schedule
     double *d_bcrec = nullptr;  // boundary coefficient records (variant 4)
     uint32_t *d_mask = nullptr; // specular reflection: vanishing entries
+    uint32_t *d_tiles = nullptr; // variant 4: order of the 16-vertex wavefront tiles
     double *d_mf0 = nullptr;       // [nP] lm = 0 mass factor per vertex (batches), or null
     float *f_mf0 = nullptr;
     double *d_mom_val = nullptr;   // moments table [3][slm]
@@ -97,6 +98,7 @@ struct tr_ctx {
         a.mask = d_mask;
         a.nmask = (uint32_t)op.mask.size();
         a.mf0 = d_mf0;
+        a.tiles = d_tiles;
         return a;
     }
     tr::OpArgs<float> args32() const { return args32(map32); }
@@ -131,6 +133,7 @@ struct tr_ctx {
         a.mask = d.mask;
         a.nmask = d.nmask;
         a.mf0 = f_mf0;
+        a.tiles = d.tiles;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }

@@ -53,6 +53,8 @@ struct OpArgs {
     const uint32_t *mask;  // specular reflection: entries of r that vanish, may be null
     uint32_t nmask;
     const T *mf0;          // [nP] per-vertex factor of the lm = 0 mass term (batches), may be null
+    const uint32_t *tiles; // variant 4: [ceil(nP/16)] a permutation of the 16-vertex wavefront
+                           // tiles, the order of the walk; null = natural order
 };
 
 // coefficient record stride (elements): NC rounded up to a 16-byte multiple

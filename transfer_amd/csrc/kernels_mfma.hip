@@ -253,9 +253,15 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
         t1 = lo + per + (xcd < rem ? 1u : 0u);
         tstep = G / X;
     }
-    // neighbour indices and boundary row of a tile's vertex (this lane's)
-    auto tile_idx = [&](uint32_t tile, uint32_t (&nb)[WT], int &b) {
-        uint32_t v = (tile * (uint32_t)wpb + wave) * 16u + vloc;
+    // first vertex of this wavefront's share of a tile (a.tiles: the order of
+    // the 16-vertex wavefront tiles), the neighbour indices and the boundary
+    // row of this lane's vertex
+    const uint32_t nwt = (nP + 15u) / 16u;
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
+    auto tile_idx = [&](uint32_t tile, uint32_t &vb, uint32_t (&nb)[WT], int &b) {
+        const uint32_t e = tile * (uint32_t)wpb + wv;
+        vb = 16u * ((a.tiles && e < nwt) ? a.tiles[e] : e);
+        uint32_t v = vb + vloc;
         asm volatile("" : "+v"(v));
         if (v < nP) {
             load_idx<WT>(nb, nrs, v * 4u, nP * 4u, WT);
@@ -266,12 +272,12 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
             b = -1;
         }
     };
-    uint32_t nbc[WT];
+    uint32_t nbc[WT], vcur = nP;
     int bcur = -1;
-    if (t0 < t1) tile_idx(t0, nbc, bcur);
+    if (t0 < t1) tile_idx(t0, vcur, nbc, bcur);
     double d = 0;
     for (uint32_t tile = t0; tile < t1; tile += tstep) {
-        const uint32_t v0 = (tile * (uint32_t)wpb + wave) * 16u;
+        const uint32_t v0 = vcur;
         const uint32_t v = v0 + vloc;
         const bool valid = v < nP;
         // ---- volume: gather, then the contraction on the matrix cores
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
         if (a.mf0 && valid && q == 0) acc[0][0] *= a.mf0[v];
         // the next tile's indices load while this tile contracts
         const int b = bcur;
-        if (tile + tstep < t1) tile_idx(tile + tstep, nbc, bcur);
+        if (tile + tstep < t1) tile_idx(tile + tstep, vcur, nbc, bcur);
         acc4 D[NTMAX];
 #pragma unroll
         for (int t = 0; t < NTMAX; t++) D[t] = (acc4){0, 0, 0, 0};

@@ -121,6 +121,16 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
     } else {
         rcm_order(nP, gstart, gadj, op.perm);
     }
+    // vertices on vacuum faces first, each group in the order above: their
+    // wavefronts take the boundary couplings, and packing them together
+    // leaves every other wavefront free of boundary work
+    {
+        std::vector<uint8_t> onb(nP, 0);
+        for (uint32_t f = 0; f < mv.nF; f++)
+            if (!mv.face_kind || mv.face_kind[f] == 0)
+                for (int k = 0; k < 3; k++) onb[mv.bnd[f].p[k]] = 1;
+        std::stable_partition(op.perm.begin(), op.perm.end(), [&](uint32_t v) { return onb[v] != 0; });
+    }
     op.iperm.resize(nP);
     for (uint32_t i = 0; i < nP; i++) op.iperm[op.perm[i]] = i;
 
