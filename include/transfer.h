@@ -243,8 +243,9 @@ enum {
                                 2 = Jacobi (exact diagonal of A)            */
     TR_SOLVER_MIXED = 2      /* FP64 iterative refinement (true residual
                                 b - A x in FP64) around the FP32 operator's
-                                device CG run to inner_tol (relative);
-                                maxit caps the refinement steps             */
+                                device CG run to inner_tol (relative), or
+                                to the reduction still needed when that is
+                                less; maxit caps the refinement steps       */
 };
 
 typedef struct tr_solve_params {

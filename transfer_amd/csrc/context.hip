@@ -28,7 +28,7 @@ LaneMap mfma_map(uint32_t slm, int wpb, int mode) {
     LaneMap m;
     m.variant = 4;
     m.WPB = wpb;
-    m.J = mfma_J(slm);
+    m.J = mfma_J(slm, mode == 0 ? 8 : 4);
     m.L = 4;
     m.VPW = 16;
     m.mfma_mode = mode;
@@ -78,11 +78,12 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
 // Whether the matrix-core schedule covers this operator: ceil(slm/4) in
 // {2, 4, 7}, exactly 15 or 16 volume slots, at most TR_WBMAX boundary slots.
 static bool v4_ok(const HostOperator &op) {
-    // 32-bit buffer offsets: the vector, the neighbour table and one slot's
+    // 32-bit buffer offsets: the vector, the neighbour table and the
     // coefficient records (FP64 sizes) each stay below 4 GiB
     const double lim = 4294967295.0;
     if ((double)op.nP * op.slm * 8 >= lim || (double)op.W * op.nP * 4 >= lim ||
-        (double)op.nP * rec_stride<double>((int)op.NC) * 8 >= lim)
+        (double)op.W * op.nP * rec_stride<double>((int)op.NC) * 8 >= lim ||
+        (double)op.WB * op.nB * 4 * 8 >= lim)
         return false;
     return mfma_supported(op.slm, 0) && (op.W == 15 || op.W == 16) && op.WB <= TR_WBMAX &&
            (op.NC == 7 || op.NC == 10);
@@ -179,14 +180,16 @@ static float from_bf16(uint16_t h) {
 
 // MFMA B slices of the variant-4 schedule (kernels_mfma.hip): for channel c,
 // K-step st, output tile t and lane l the coupling Mat_c[n][m] with
-// n = 16t + l%16 and m = (l/16)*J + st (FP64/FP32, one value) or
-// m = (l/16)*J + 4st + u, u = 0..3 (BF16, four values); volume channels
+// n = 16t + l%16 and m = mfma_kharm(mode, J, l/16, st) (FP64/FP32, one value) or
+// m = mfma_harm(E, l/16, 4st + u), u = 0..3 (BF16, four values), E = 2 for
+// FP64 and 4 for FP32 (kernels.h); volume channels
 // (mass = diag(massfac), then the NC-1 stiffness channels) first, then the 3
 // boundary channels.  mode 0 FP64, 1 FP32, 2 BF16 (nsplit 3 appends the
 // low parts: hi table, lo table, per section).
 static std::vector<unsigned char> mfma_tables(const HostOperator &op, int mode, int nsplit) {
     const uint32_t slm = op.slm;
-    const int J = mfma_J(slm), S = mode == 2 ? (J + 3) / 4 : J, NT = (int)((slm + 15) / 16);
+    const int E = mode == 0 ? 2 : 4, J = mfma_J(slm, mode == 0 ? 8 : 4);
+    const int S = mfma_steps(mode, J), NT = (int)((slm + 15) / 16);
     const int BW = mode == 2 ? 8 : (mode == 0 ? 8 : 4);
     auto mat = [&](int sec, int c, uint32_t n, uint32_t m) -> double {
         if (n >= slm || m >= slm) return 0.0;
@@ -207,15 +210,15 @@ static std::vector<unsigned char> mfma_tables(const HostOperator &op, int mode, 
                             unsigned char *p = &t[((((size_t)c * S + st) * NT + tt) * 64 + l) * BW];
                             const uint32_t n = 16u * tt + (l % 16), q = l / 16;
                             if (mode == 0) {
-                                double v = mat(sec, c, n, q * J + st);
+                                double v = mat(sec, c, n, mfma_kharm(0, J, (int)q, st));
                                 std::memcpy(p, &v, 8);
                             } else if (mode == 1) {
-                                float v = (float)mat(sec, c, n, q * J + st);
+                                float v = (float)mat(sec, c, n, mfma_kharm(1, J, (int)q, st));
                                 std::memcpy(p, &v, 4);
                             } else {
                                 uint16_t h[4];
                                 for (int u = 0; u < 4; u++) {
-                                    double v = (4 * st + u < J) ? mat(sec, c, n, q * J + 4 * st + u) : 0.0;
+                                    double v = (4 * st + u < J) ? mat(sec, c, n, mfma_harm(E, (int)q, 4 * st + u)) : 0.0;
                                     uint16_t hi = to_bf16((float)v);
                                     h[u] = part == 0 ? hi : to_bf16((float)(v - (double)from_bf16(hi)));
                                 }
@@ -895,6 +898,13 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
             break;
         }
         if (outer == prm->maxit) break;
+        // the inner solve reduces its residual by inner_tol, or by what is
+        // still needed (with a margin: the FP32 recursion overstates the
+        // true reduction) when that is less
+        {
+            const double goal = std::max(prm->tol_rel * bn, prm->tol_abs);
+            cp.tol_rel = std::max(prm->inner_tol, std::min(0.1, 0.3 * goal / rn));
+        }
         HIPCHECK(launch_to_f32_scaled(r, b32, 1.0 / rn, n, c->stream));
         HIPCHECK(hipMemsetAsync(d32, 0, sizeof(float) * n, c->stream));
         CgResult res;

@@ -95,7 +95,26 @@ uint32_t apply_blocks(uint32_t nP, const LaneMap &m);
 uint32_t apply_tiles(uint32_t nP, const LaneMap &m);
 
 // variant 4 (kernels_mfma.hip): J = ceil(slm/4) harmonics per lane
-int mfma_J(uint32_t slm);
+// Variant 4: the four chunk lanes q = 0..3 of a vertex each hold J harmonics
+// of its row; register j of lane q holds harmonic mfma_harm(E, q, j), E the
+// elements of a 16-byte piece (2 FP64, 4 FP32): the lane's pieces are the
+// row's pieces 4k + q, so a vertex's four lanes read 64 contiguous bytes
+// per load.  J = E * ceil(slm / 4E).
+__host__ __device__ constexpr uint32_t mfma_harm(int E, int q, int j) {
+    return (uint32_t)(4 * E * (j / E) + E * q + j % E);
+}
+// K steps of the contraction (mode 0 FP64, 1 FP32, 2 BF16): one register
+// per step, four for BF16; FP64 at J = 8 (slm <= 28) folds registers 6 and 7
+// of chunks 0 and 1 (harmonics 24..27; the rest are past slm) into one step.
+__host__ __device__ constexpr int mfma_steps(int mode, int J) {
+    return mode == 2 ? (J + 3) / 4 : (mode == 0 && J == 8 ? 7 : J);
+}
+// harmonic of K index q in step st (modes 0 and 1)
+__host__ __device__ constexpr uint32_t mfma_kharm(int mode, int J, int q, int st) {
+    return (mode == 0 && J == 8 && st == 6) ? (uint32_t)(24 + q) : mfma_harm(mode == 0 ? 2 : 4, q, st);
+}
+int mfma_J(uint32_t slm, int elem_bytes);
+// mode 0 FP64, 1 FP32, 2 BF16
 bool mfma_supported(uint32_t slm, int mode);
 size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit);
 template <typename T, int MODE>

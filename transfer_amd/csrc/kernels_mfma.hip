@@ -1,19 +1,25 @@
 // kernels_mfma.hip -- the operator with its angular contraction on the matrix
 // cores (schedule variant 4).
 //
-// Lane layout: a wavefront holds 16 vertices; lane l owns vertex l % 16 and
-// the harmonic chunk q = l / 16, i.e. harmonics q*J .. q*J+J-1 with
-// J = ceil(slm / 4).  Phase 1 (as in the other schedules) gathers the ELL
-// neighbours and accumulates the spatial channels T_c[v][lm] in registers.
-// Phase 2 is
+// A wavefront holds 16 vertices, four lanes (chunks q = 0..3) per vertex.
+// Phase 1 gathers the ELL neighbours and accumulates the spatial channels
+// T_c[v][m] in registers, register j of chunk q holding harmonic
+// mfma_harm(E, q, j) (kernels.h).  In this phase the four lanes of a vertex
+// are adjacent (lane 4v + q): each lane reads 16-byte pieces of the
+// neighbour row, the four together 64 contiguous bytes, so the L1 cache
+// serves a vertex's share of a load with one tag lookup rather than four
+// (the row gathers, not HBM bandwidth, bound this kernel otherwise); each
+// lane also loads only a quarter of the slot's coefficient record and the
+// quad shares it through DPP broadcasts.  Phase 2 is
 //   R[v][n] = sum_c sum_m T_c[v][m] Mat_c[n][m]
 // (c = mass (Mat = diag(massfac)), the NC-1 stiffness channels and, for
 // wavefronts holding boundary vertices, the 3 boundary channels), a
-// [16 vertices x K] x [K x 16 outputs] product per output tile.  With the
-// K index ordered as (channel, j, chunk q), the accumulator acc[c][j] of
-// lane l is exactly the A operand of one MFMA step (A[v][q]), so the
-// channel sums never leave the registers.  B slices (4 x 16 couplings per
-// step and tile) are staged in LDS once per workgroup.
+// [16 vertices x K] x [K x 16 outputs] product per output tile, on the
+// matrix cores.  The accumulators move once per tile to the MFMA operand
+// layout (lane v + 16q, v = lane % 16 the A row), where acc[c][j] of lane l
+// is exactly the A operand of one MFMA step, so the channel sums never
+// leave the registers.  B slices (4 x 16 couplings per step and tile) are
+// staged in LDS once per workgroup.
 //   FP64 operator: v_mfma_f64_16x16x4f64, K = 4 per step (one j).
 //   FP32 operator: v_mfma_f32_16x16x4f32 (exact FP32), or BF16 MFMA
 //   v_mfma_f32_16x16x16bf16_1k with FP32 accumulation, K = 16 per step
@@ -81,30 +87,22 @@ __device__ inline void brec(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t s
     }
 }
 
-__device__ inline short bf16_rne(float x) {
-    unsigned u = __builtin_bit_cast(unsigned, x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
-__device__ inline float bf16_to_f(short h) {
-    return __builtin_bit_cast(float, ((unsigned)(unsigned short)h) << 16);
-}
+typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 
 // One channel group of the angular contraction: D[t] += A_c B_c over the
 // NCH channels of acc (lane layout as above), B slices at B (hi) and B + lo
 // (lo part of the BF16 split).
-template <typename T, int MODE, int J, int NCH, typename ACC4, int NTMAX>
-__device__ inline void contract(const T (&acc)[NCH][J], const unsigned char *B, size_t lo, int NT, int lane,
-                                int nsplit, ACC4 (&D)[NTMAX]) {
-    constexpr int S = (MODE == 2) ? (J + 3) / 4 : J;
+template <typename T, int MODE, int J, int NCH, typename ACC4, int NT>
+__device__ __forceinline__ void contract(const T (&acc)[NCH][J], const unsigned char *B, size_t lo, int lane,
+                                         int nsplit, ACC4 (&D)[NT]) {
+    constexpr int S = mfma_steps(MODE, J);
     constexpr int BW = (MODE == 2) ? 8 : (int)sizeof(T);
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
 #pragma unroll
         for (int st = 0; st < S; st++) {
 #pragma unroll
-            for (int t = 0; t < NTMAX; t++) {
-                if (t >= NT) break;
+            for (int t = 0; t < NT; t++) {
                 const size_t bo = ((((size_t)c * S + st) * NT + t) * TR_WAVE + lane) * BW;
                 if constexpr (MODE == 0) {
                     const double bv = *reinterpret_cast<const double *>(B + bo);
@@ -113,15 +111,17 @@ __device__ inline void contract(const T (&acc)[NCH][J], const unsigned char *B, 
                     const float bv = *reinterpret_cast<const float *>(B + bo);
                     D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(acc[c][st], bv, D[t], 0, 0, 0);
                 } else {
-                    s4v ah, al, bh = *reinterpret_cast<const s4v *>(B + bo);
+                    // A operand: registers 4st .. 4st+3, rounded to BF16 (the
+                    // hardware conversion rounds to nearest even) and, for the
+                    // split, the BF16 residual x - hi
+                    f4v x;
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const float x = (4 * st + u < J) ? (float)acc[c][4 * st + u] : 0.0f;
-                        ah[u] = bf16_rne(x);
-                        al[u] = bf16_rne(x - bf16_to_f(ah[u]));
-                    }
+                    for (int u = 0; u < 4; u++) x[u] = (4 * st + u < J) ? (float)acc[c][4 * st + u] : 0.0f;
+                    const bf4v hb = __builtin_convertvector(x, bf4v);
+                    const s4v ah = __builtin_bit_cast(s4v, hb), bh = *reinterpret_cast<const s4v *>(B + bo);
                     D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, D[t], 0, 0, 0);
                     if (nsplit > 1) {
+                        const s4v al = __builtin_bit_cast(s4v, __builtin_convertvector(x - __builtin_convertvector(hb, f4v), bf4v));
                         const s4v bl = *reinterpret_cast<const s4v *>(B + lo + bo);
                         D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, D[t], 0, 0, 0);
                         D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
@@ -140,39 +140,71 @@ __device__ inline void contract(const T (&acc)[NCH][J], const unsigned char *B, 
 template <int NS>
 __device__ inline void load_idx(uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t nrs, uint32_t voff, uint32_t nstride,
                                 int nslot) {
+    asm volatile("" : "+s"(nstride));
 #pragma unroll
     for (int s = 0; s < NS; s++)
         nb[s] = s < nslot ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrs, voff, (uint32_t)s * nstride, 0) : 0u;
 }
 
-// Pipelined ELL gather of one lane: acc[c][j] += sum_s cf_s[c] f[nb_s][lm0+j]
-// over `nslot` slots (nslot <= NS, a wavefront-uniform count), coefficient
-// record (RS elements) of slot s at byte coff of the slot's array
-// cbase + s*cstride (one buffer descriptor per slot, so only a slot's array,
-// not all W of them, has to fit the 32-bit buffer offsets).  DEPTH slots are
-// in flight: slot s+DEPTH's loads are issued right after slot s's FMAs, so a
-// wavefront waits for one memory round trip per DEPTH slots.  Left alone, the
-// optimiser issues every slot's loads first and sinks all FMAs below them,
-// which holds NS slots of data live and spills; empty volatile asm
-// statements (which keep their relative order) pin each slot's FMAs before
-// the next load offsets are formed.
+// Broadcast of lane `src` of each quad (4 adjacent lanes) to the quad.
+__device__ inline int quad_bcast32(int x, int src) {
+    switch (src) {
+        case 0: return __builtin_amdgcn_mov_dpp(x, 0x00, 0xf, 0xf, false);
+        case 1: return __builtin_amdgcn_mov_dpp(x, 0x55, 0xf, 0xf, false);
+        case 2: return __builtin_amdgcn_mov_dpp(x, 0xaa, 0xf, 0xf, false);
+        default: return __builtin_amdgcn_mov_dpp(x, 0xff, 0xf, 0xf, false);
+    }
+}
+__device__ inline float quad_bcast(float x, int src) {
+    return __builtin_bit_cast(float, quad_bcast32(__builtin_bit_cast(int, x), src));
+}
+__device__ inline double quad_bcast(double x, int src) {
+    i32x2m v = __builtin_bit_cast(i32x2m, x);
+    v.x = quad_bcast32(v.x, src);
+    v.y = quad_bcast32(v.y, src);
+    return __builtin_bit_cast(double, v);
+}
+
+// Pipelined ELL gather of one lane (chunk q of its vertex, the vertex's four
+// lanes adjacent): acc[c][j] += sum_s cf_s[c] f[nb_s][mfma_harm(E, q, j)]
+// over `nslot` slots (nslot <= NS, a wavefront-uniform count).  The
+// coefficient record (RS elements) of slot s sits at byte coff + s*cstride
+// of the record array (one buffer descriptor: per-slot descriptors, 4 SGPRs
+// each, get hoisted out of the tile loop and spilled); lane q loads elements
+// q*PE .. q*PE+PE-1 of it and channel c comes from lane c/PE of the quad.
+// Reads past the last record (PE*4 > RS) fall outside the descriptor's
+// range, which returns zeros.  DEPTH slots are in flight: slot s+DEPTH's loads are issued
+// right after slot s's FMAs, so a wavefront waits for one memory round trip
+// per DEPTH slots.  Left alone, the optimiser issues every slot's loads
+// first and sinks all FMAs below them, which holds NS slots of data live and
+// spills; empty volatile asm statements (which keep their relative order)
+// pin each slot's FMAs before the next load offsets are formed.
 template <typename T, int J, int NCH, int RS, int NS, int DEPTH>
-__device__ inline void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
-                              const T *cbase, size_t cstride, uint32_t coff, int nslot, uint32_t slm, int lm0) {
-    T xs[DEPTH][J], cs[DEPTH][RS];
-    auto load = [&](int s, T *x, T *cf) {
-        uint32_t base = (nb[s] * slm + lm0) * (uint32_t)sizeof(T);
-        uint32_t cv = coff;
+__device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
+                                       __amdgpu_buffer_rsrc_t crs, uint32_t cstride, uint32_t coff, int nslot,
+                                       uint32_t slm, int q) {
+    constexpr int E = 16 / (int)sizeof(T), PE = (RS + 3) / 4;
+    // opaque here, so that the per-slot products and tests are formed where
+    // used instead of hoisted out of the tile loop (kept live, they spill)
+    asm volatile("" : "+s"(cstride), "+s"(nslot), "+s"(slm));
+    T xs[DEPTH][J], cs[DEPTH][PE];
+    auto load = [&](int s, T *x, T *cf) __attribute__((always_inline)) {
+        uint32_t base = nb[s] * slm * (uint32_t)sizeof(T) + 16u * (uint32_t)q;
+        uint32_t cv = coff + (uint32_t)(q * PE * (int)sizeof(T)) + (uint32_t)s * cstride;
         asm volatile("" : "+v"(base), "+v"(cv));
 #pragma unroll
-        for (int j = 0; j < J; j++) x[j] = bload1<T>(frs, base + j * (uint32_t)sizeof(T), 0u);
-        brec<T, RS>(rsrc4(cbase + (size_t)s * cstride, cstride * sizeof(T)), cv, 0u, cf);
+        for (int j = 0; j < J; j++)
+            x[j] = bload1<T>(frs, base + (uint32_t)((64 * (j / E) + (j % E) * (int)sizeof(T))), 0u);
+#pragma unroll
+        for (int e = 0; e < PE; e++) cf[e] = bload1<T>(crs, cv + (uint32_t)(e * (int)sizeof(T)), 0u);
     };
-    auto accum = [&](const T *x, const T *cf) {
+    auto accum = [&](const T *x, const T *cf) __attribute__((always_inline)) {
 #pragma unroll
-        for (int c = 0; c < NCH; c++)
+        for (int c = 0; c < NCH; c++) {
+            const T w = quad_bcast(cf[c % PE], c / PE);
 #pragma unroll
-            for (int j = 0; j < J; j++) acc[c][j] = fma(cf[c], x[j], acc[c][j]);
+            for (int j = 0; j < J; j++) acc[c][j] = fma(w, x[j], acc[c][j]);
+        }
 #pragma unroll
         for (int c = 0; c < NCH; c++)
 #pragma unroll
@@ -188,12 +220,45 @@ __device__ inline void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdg
             if (s + DEPTH < NS && s + DEPTH < nslot) load(s + DEPTH, xs[s % DEPTH], cs[s % DEPTH]);
         }
     }
-    // harmonics past slm (the last chunk) carry neighbouring data: zero them
+    // harmonics past slm (padding of the last pieces) read neighbouring data:
+    // zero them
 #pragma unroll
     for (int j = 0; j < J; j++)
-        if (lm0 + j >= (int)slm)
+        if (mfma_harm(E, q, j) >= slm)
 #pragma unroll
             for (int c = 0; c < NCH; c++) acc[c][j] = 0;
+}
+
+__device__ inline double bperm(int addr, double x) {
+    i32x2m v = __builtin_bit_cast(i32x2m, x);
+    v.x = __builtin_amdgcn_ds_bpermute(addr, v.x);
+    v.y = __builtin_amdgcn_ds_bpermute(addr, v.y);
+    return __builtin_bit_cast(double, v);
+}
+__device__ inline float bperm(int addr, float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, x)));
+}
+
+// Moves the accumulators from the gather layout (lane 4v + q) to the MFMA
+// operand layout (lane v + 16q).  MODE 0 at J = 8: register 6 of chunk q
+// becomes harmonic 24 + q (register 6 + q%2 of chunk q/2), see mfma_steps.
+template <typename T, int MODE, int J, int NCH>
+__device__ __forceinline__ void to_mfma_layout(T (&acc)[NCH][J], int lane) {
+    constexpr bool MERGE = MODE == 0 && J == 8;
+    const int src = 4 * (4 * (lane & 15) + (lane >> 4));
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+#pragma unroll
+        for (int j = 0; j < (MERGE ? 6 : J); j++) acc[c][j] = bperm(src, acc[c][j]);
+    if constexpr (MERGE) {
+        const int src2 = 4 * (4 * (lane & 15) + (lane >> 5));
+        const bool odd = (lane >> 4) & 1;
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const T a6 = bperm(src2, acc[c][6]), a7 = bperm(src2, acc[c][7]);
+            acc[c][6] = odd ? a7 : a6;
+        }
+    }
 }
 
 // MODE: 0 = FP64 (T = double), 1 = FP32 MFMA, 2 = BF16 MFMA (T = float).
@@ -209,12 +274,14 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     if (skip && *skip != 0.0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int RS = rec_stride<T>(NC);
-    constexpr int S = (MODE == 2) ? (J + 3) / 4 : J;
+    constexpr int S = mfma_steps(MODE, J);
     constexpr int BW = (MODE == 2) ? 8 : (int)sizeof(T);
     const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
     const int wpb = blockDim.x / TR_WAVE;
     const uint32_t slm = a.slm, nP = a.nP;
-    const int NT = (int)((slm + 15) / 16);
+    // output tiles of 16 harmonics: ceil(slm / 16), which the accepted
+    // (J, slm) pairs make J / 4 rounded up
+    constexpr int NT = (J * 4 + 15) / 16;
     const int nsplit = a.nsplit;
     const size_t vsec = (size_t)NC * S * NT * TR_WAVE * BW;  // one part of the volume B table
     const size_t bsec = (size_t)3 * S * NT * TR_WAVE * BW;   // one part of the boundary B table
@@ -232,16 +299,24 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
 
     const auto bnrs = rsrc4(a.bnbr, (size_t)a.WB * a.nB * 4);
+    const auto crs = rsrc4(a.crec, (size_t)a.W * nP * RS * sizeof(T));
+    const auto bcrs = rsrc4(a.bcrec, (size_t)a.WB * a.nB * 4 * sizeof(T));
 
-    const int vloc = lane & 15, q = lane >> 4, lm0 = q * J;
-    constexpr int NTMAX = (J * 4 + 15) / 16;
+    // MFMA layout (contraction and write): vertex vloc, chunk q; gather
+    // layout: vertex gv, chunk gq (the vertex's four lanes adjacent)
+    const int vloc = lane & 15, q = lane >> 4;
+    const int gv = lane >> 2, gq = lane & 3;
     typedef typename std::conditional<MODE == 0, d4v, f4v>::type acc4;
-    // register budget: 2 wavefronts per SIMD for FP64 at J = 7, 3 otherwise
-    constexpr int BUDGET = (sizeof(T) == 8 && J >= 7) ? 230 : 148;
-    constexpr int ACCR = NC * J * (int)sizeof(T) / 4, SLOTR = (J + RS) * (int)sizeof(T) / 4;
-    constexpr int DEPTH = std::max(2, std::min(6, (BUDGET - ACCR - 20) / SLOTR));
-    constexpr int BACCR = 3 * J * (int)sizeof(T) / 4, BSLOTR = (J + 4) * (int)sizeof(T) / 4;
-    constexpr int DEPTHB = std::max(2, std::min(4, (BUDGET - BACCR - 8 * NTMAX - 30) / BSLOTR));
+    // register budget per lane: 2 wavefronts per SIMD for FP64 (256 VGPRs),
+    // 3 for FP32 (168); OVH covers the indices, the MFMA accumulators and
+    // addresses
+    constexpr int BUDGET = sizeof(T) == 8 ? 256 : 168;
+    constexpr int OVH = WT + 8 * NT + (MODE == 2 ? 36 : 28);
+    constexpr int PE = (RS + 3) / 4;
+    constexpr int ACCR = NC * J * (int)sizeof(T) / 4, SLOTR = (J + PE) * (int)sizeof(T) / 4;
+    constexpr int DEPTH = std::max(2, std::min(6, (BUDGET - ACCR - OVH) / SLOTR));
+    constexpr int BACCR = 3 * J * (int)sizeof(T) / 4, BSLOTR = (J + 1) * (int)sizeof(T) / 4;
+    constexpr int DEPTHB = std::max(2, std::min(4, (BUDGET - BACCR - OVH - 8) / BSLOTR));
 
     const uint32_t ntiles = (nP + 16u * wpb - 1) / (16u * wpb);
     const uint32_t G = gridDim.x, X = 8;
@@ -255,13 +330,13 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     }
     // first vertex of this wavefront's share of a tile (a.tiles: the order of
     // the 16-vertex wavefront tiles), the neighbour indices and the boundary
-    // row of this lane's vertex
+    // row of this lane's vertex (gather layout)
     const uint32_t nwt = (nP + 15u) / 16u;
     const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
     auto tile_idx = [&](uint32_t tile, uint32_t &vb, uint32_t (&nb)[WT], int &b) {
         const uint32_t e = tile * (uint32_t)wpb + wv;
         vb = 16u * ((a.tiles && e < nwt) ? a.tiles[e] : e);
-        uint32_t v = vb + vloc;
+        uint32_t v = vb + gv;
         asm volatile("" : "+v"(v));
         if (v < nP) {
             load_idx<WT>(nb, nrs, v * 4u, nP * 4u, WT);
@@ -278,7 +353,7 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     double d = 0;
     for (uint32_t tile = t0; tile < t1; tile += tstep) {
         const uint32_t v0 = vcur;
-        const uint32_t v = v0 + vloc;
+        const uint32_t v = v0 + gv;
         const bool valid = v < nP;
         // ---- volume: gather, then the contraction on the matrix cores
         T acc[NC][J];
@@ -287,23 +362,25 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
 #pragma unroll
             for (int j = 0; j < J; j++) acc[c][j] = 0;
         if (valid)
-            gather<T, J, NC, RS, WT, DEPTH>(acc, nbc, frs, a.crec, (size_t)nP * RS, v * (uint32_t)(RS * sizeof(T)),
-                                             WT, slm, lm0);
+            gather<T, J, NC, RS, WT, DEPTH>(acc, nbc, frs, crs, nP * (uint32_t)(RS * sizeof(T)),
+                                             v * (uint32_t)(RS * sizeof(T)), WT, slm, gq);
         // per-vertex factor of the mass channel at lm = 0 (isotropic albedo
-        // per problem in a batch: 1 - albedo, with massfac[0] = 1)
-        if (a.mf0 && valid && q == 0) acc[0][0] *= a.mf0[v];
+        // per problem in a batch: 1 - albedo, with massfac[0] = 1); harmonic
+        // 0 is register 0 of chunk 0
+        if (a.mf0 && valid && gq == 0) acc[0][0] *= a.mf0[v];
+        to_mfma_layout<T, MODE, J, NC>(acc, lane);
         // the next tile's indices load while this tile contracts
         const int b = bcur;
         if (tile + tstep < t1) tile_idx(tile + tstep, vcur, nbc, bcur);
-        acc4 D[NTMAX];
+        acc4 D[NT];
 #pragma unroll
-        for (int t = 0; t < NTMAX; t++) D[t] = (acc4){0, 0, 0, 0};
+        for (int t = 0; t < NT; t++) D[t] = (acc4){0, 0, 0, 0};
         // the B slices are the same for every tile: an opaque zero offset
         // keeps the optimiser from hoisting all of their loads out of the
         // tile loop into registers
         uint32_t boff = 0;
         asm volatile("" : "+v"(boff));
-        contract<T, MODE, J, NC>(acc, smem_raw + boff, vsec, NT, lane, nsplit, D);
+        contract<T, MODE, J, NC>(acc, smem_raw + boff, vsec, lane, nsplit, D);
         // ---- vacuum boundary: only wavefronts that hold boundary vertices
         if (__any(b >= 0)) {
             T bacc[3][J];
@@ -314,15 +391,15 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
             if (b >= 0) {
                 uint32_t bn[TR_WBMAX];
                 load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, (int)a.WB);
-                gather<T, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, a.bcrec, (size_t)a.nB * 4,
-                                                     (uint32_t)b * (uint32_t)(4 * sizeof(T)), (int)a.WB, slm, lm0);
+                gather<T, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, bcrs, a.nB * (uint32_t)(4 * sizeof(T)),
+                                                     (uint32_t)b * (uint32_t)(4 * sizeof(T)), (int)a.WB, slm, gq);
             }
-            contract<T, MODE, J, 3>(bacc, smem_raw + vsec * parts + boff, bsec, NT, lane, nsplit, D);
+            to_mfma_layout<T, MODE, J, 3>(bacc, lane);
+            contract<T, MODE, J, 3>(bacc, smem_raw + vsec * parts + boff, bsec, lane, nsplit, D);
         }
         // ---- write: FP64 D holds rows (l/16) + 4i, FP32 rows 4(l/16) + i; column l % 16
 #pragma unroll
-        for (int t = 0; t < NTMAX; t++) {
-            if (t >= NT) break;
+        for (int t = 0; t < NT; t++) {
             const uint32_t n = 16u * t + (uint32_t)vloc;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -336,9 +413,12 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
         }
     }
     if (DOT) {
-        __shared__ double w[8];
+        // the B tables are done with: their LDS takes the wavefront sums (a
+        // separate array would push two FP64 workgroups past the LDS of a CU)
+        double *w = reinterpret_cast<double *>(smem_raw);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, TR_WAVE);
+        __syncthreads();
         if (lane == 0) w[wave] = d;
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -349,17 +429,22 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     }
 }
 
-int mfma_J(uint32_t slm) { return (int)((slm + 3) / 4); }
+int mfma_J(uint32_t slm, int elem_bytes) {
+    const int E = 16 / elem_bytes;
+    return E * (int)((slm + 4 * E - 1) / (4 * E));
+}
 
-bool mfma_supported(uint32_t slm, int precision_mode) {
-    const int J = mfma_J(slm);
-    (void)precision_mode;
-    return J == 2 || J == 4 || J == 7;
+bool mfma_supported(uint32_t slm, int mode) {
+    const int J = mfma_J(slm, mode == 0 ? 8 : 4);
+    // the kernels take ceil(slm / 16) output tiles to be ceil(4J / 16); the
+    // FP64 contraction at J = 8 takes slm <= 28 (mfma_steps)
+    return (J == 2 || J == 4 || J == 8) && (slm + 15) / 16 == (uint32_t)(4 * J + 15) / 16 &&
+           !(mode == 0 && J == 8 && slm > 28);
 }
 
 // bytes of the B tables (volume and boundary slices) staged in LDS
 size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit) {
-    const int J = mfma_J(slm), S = mode == 2 ? (J + 3) / 4 : J, BW = mode == 2 ? 8 : (mode == 0 ? 8 : 4);
+    const int J = mfma_J(slm, mode == 0 ? 8 : 4), S = mfma_steps(mode, J), BW = mode == 2 ? 8 : (mode == 0 ? 8 : 4);
     const int NT = (int)((slm + 15) / 16);
     return (size_t)(NC + 3) * S * NT * 64 * BW * (mode == 2 && nsplit > 1 ? 2 : 1);
 }
@@ -371,9 +456,10 @@ static const void *apply4_kernel(int J, int NC, uint32_t W) {
 #define TR_K4W(JJ, NCC) (W == 15 ? TR_K4(JJ, NCC, 15) : TR_K4(JJ, NCC, 16))
 #define TR_K4N(JJ) (NC == 7 ? TR_K4W(JJ, 7) : TR_K4W(JJ, 10))
     if ((W != 15 && W != 16) || (NC != 7 && NC != 10)) return nullptr;
-    if (J == 2) return TR_K4N(2);
+    if constexpr (sizeof(T) == 8)
+        if (J == 2) return TR_K4N(2);
     if (J == 4) return TR_K4N(4);
-    if (J == 7) return TR_K4N(7);
+    if (J == 8) return TR_K4N(8);
     return nullptr;
 #undef TR_K4N
 #undef TR_K4W
@@ -387,8 +473,8 @@ static const void *apply4_kernel(int J, int NC, uint32_t W) {
 template <typename T, int MODE>
 uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, bool dot) {
     const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
-    const void *k = dot ? apply4_kernel<T, MODE, true>(mfma_J(a.slm), NC, a.W)
-                        : apply4_kernel<T, MODE, false>(mfma_J(a.slm), NC, a.W);
+    const void *k = dot ? apply4_kernel<T, MODE, true>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W)
+                        : apply4_kernel<T, MODE, false>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W);
     if (!k || ntiles == 0) return ntiles;
     const size_t sh = apply4_lds_bytes(MODE, NC, a.slm, a.nsplit);
     int nb = 0, dev = 0, cus = 0;
@@ -416,8 +502,8 @@ hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, con
         return hipErrorInvalidValue;
     dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
     const size_t sh = apply4_lds_bytes(MODE, NC, a.slm, a.nsplit);
-    const void *k = part ? apply4_kernel<T, MODE, true>(mfma_J(a.slm), NC, a.W)
-                         : apply4_kernel<T, MODE, false>(mfma_J(a.slm), NC, a.W);
+    const void *k = part ? apply4_kernel<T, MODE, true>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W)
+                         : apply4_kernel<T, MODE, false>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W);
     if (!k) return hipErrorInvalidValue;
     OpArgs<T> aa = a;
     void *args[] = {&aa, &f, &r, &part, &skip};

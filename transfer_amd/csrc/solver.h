@@ -20,6 +20,7 @@ enum {
     SC_THR = 6,
     SC_CONV = 7,  // 0 running, 1 converged, 2 iteration cap, 3 breakdown
     SC_ITER = 8,
+    SC_RZ2 = 9,   // r.z of odd iterations (SC_RZ holds it for even ones)
     SC_N = 16
 };
 

@@ -3,13 +3,19 @@
 // The reference drives CG from the host (main.cpp:125-162): every dot
 // product and norm ends in a blocking device-to-host copy of one REAL
 // (CudaContext.cpp:227-250).  Here alpha, beta and the residual norm stay
-// in a small device array; one iteration is five launches
-//   apply A p (+ per-workgroup p.Ap)  ->  alpha = rz / p.Ap
-//   x += alpha p, r -= alpha Ap, z = M r (+ r.r, r.z partials)
-//   beta = rz_new / rz, convergence test  ->  p = z + beta p
-// captured once into a HIP graph of `chunk` iterations; the host reads the
-// convergence flag once per graph launch.  After convergence every kernel
-// of the remaining chunk returns at its first instruction.
+// in a small device array; one iteration is three launches
+//   apply A p (+ per-workgroup p.Ap)
+//   alpha = rz / p.Ap; x += alpha p, r -= alpha Ap, z = M r (+ r.r, r.z partials)
+//   beta = rz_new / rz, convergence test; p = z + beta p
+// (with a general preconditioner z = M r runs between the last two, with
+// its own r.z pass).  Every workgroup of the two vector launches sums the
+// previous launch's partials itself, in one fixed order, so all agree on
+// alpha and beta without a separate reduction launch; r.z alternates
+// between two scalar slots by iteration parity, so workgroup 0 can record
+// the new value while the others still read the old one.  The iterations
+// are captured once into a HIP graph of `chunk` (even) iterations; the host
+// reads the convergence flag once per graph launch.  After convergence
+// every kernel of the remaining chunk returns at its first instruction.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -64,25 +70,30 @@ __device__ inline void sum_parts(const double *part, int n, int n2, double &a, d
     block_sum2(a, b);
 }
 
-__global__ __launch_bounds__(256) void k_fin_alpha(const double *__restrict__ part, int n,
-                                                   double *__restrict__ sc) {
-    if (sc[SC_CONV] != 0.0) return;
-    double pap, dummy;
-    sum_parts(part, n, -1, pap, dummy);
-    if (threadIdx.x == 0) {
-        sc[SC_PAP] = pap;
-        sc[SC_ALPHA] = sc[SC_RZ] / pap;
-    }
-}
-
+// x += alpha p, r -= alpha Ap, z = M r and the r.r, r.z partials, alpha =
+// rz / p.Ap from the nb p.Ap partials of the operator launch; workgroup 0
+// records p.Ap and alpha
 template <typename T>
-__global__ __launch_bounds__(256) void k_update(T *__restrict__ x, const T *__restrict__ p,
-                                                T *__restrict__ r, const T *__restrict__ Ap,
-                                                const T *__restrict__ dinv, size_t n,
-                                                const double *__restrict__ sc,
-                                                double *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T *__restrict__ p,
+                                                      T *__restrict__ r, const T *__restrict__ Ap,
+                                                      const T *__restrict__ dinv, size_t n,
+                                                      double *__restrict__ sc, const double *__restrict__ pap_part,
+                                                      int nb, int rzs, double *__restrict__ part) {
     if (sc[SC_CONV] != 0.0) return;
-    const T alpha = (T)sc[SC_ALPHA];
+    __shared__ double al_sh;
+    {
+        double pap, dummy;
+        sum_parts(pap_part, nb, -1, pap, dummy);
+        if (threadIdx.x == 0) {
+            al_sh = sc[rzs] / pap;
+            if (blockIdx.x == 0) {
+                sc[SC_PAP] = pap;
+                sc[SC_ALPHA] = al_sh;
+            }
+        }
+        __syncthreads();
+    }
+    const T alpha = (T)al_sh;
     double rr = 0, rz = 0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
@@ -100,34 +111,41 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, const T *__re
     }
 }
 
-__global__ __launch_bounds__(256) void k_fin_beta(const double *__restrict__ part, int n,
-                                                  double *__restrict__ sc, int maxit) {
-    if (sc[SC_CONV] != 0.0) return;
-    double rr, rz;
-    sum_parts(part, n, TR_UPD_BLOCKS, rr, rz);
-    if (threadIdx.x == 0) {
-        sc[SC_RR] = rr;
-        sc[SC_BETA] = rz / sc[SC_RZ];
-        sc[SC_RZ] = rz;
-        sc[SC_ITER] += 1.0;
-        if (!(rr == rr) || !(rz == rz))
-            sc[SC_CONV] = 3.0;  // breakdown: NaN
-        else if (rr <= sc[SC_THR])
-            sc[SC_CONV] = 1.0;  // converged
-        else if (sc[SC_ITER] >= (double)maxit)
-            sc[SC_CONV] = 2.0;  // iteration cap
-    }
-}
-
+// beta = rz_new / rz from the G r.r / r.z partials (rz in slot rz_old, the
+// new value into slot rz_new by workgroup 0, with the iteration count and
+// the convergence flag); p = z + beta p, z = M r (dinv) or z given
 template <typename T>
-__global__ __launch_bounds__(256) void k_pupdate(T *__restrict__ p, const T *__restrict__ r,
-                                                 const T *__restrict__ dinv, size_t n,
-                                                 const double *__restrict__ sc) {
+__global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const T *__restrict__ r,
+                                                       const T *__restrict__ dinv, const T *__restrict__ z,
+                                                       size_t n, double *__restrict__ sc,
+                                                       const double *__restrict__ part, int G, int rz_old,
+                                                       int rz_new, int maxit) {
     if (sc[SC_CONV] != 0.0) return;
-    const T beta = (T)sc[SC_BETA];
+    __shared__ double be_sh;
+    {
+        double rr, rz;
+        sum_parts(part, G, TR_UPD_BLOCKS, rr, rz);
+        if (threadIdx.x == 0) {
+            be_sh = rz / sc[rz_old];
+            if (blockIdx.x == 0) {
+                sc[SC_RR] = rr;
+                sc[SC_BETA] = be_sh;
+                sc[rz_new] = rz;
+                sc[SC_ITER] += 1.0;
+                if (!(rr == rr) || !(rz == rz))
+                    sc[SC_CONV] = 3.0;  // breakdown: NaN
+                else if (rr <= sc[SC_THR])
+                    sc[SC_CONV] = 1.0;  // converged
+                else if (sc[SC_ITER] >= (double)maxit)
+                    sc[SC_CONV] = 2.0;  // iteration cap
+            }
+        }
+        __syncthreads();
+    }
+    const T beta = (T)be_sh;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        const T zi = dinv ? dinv[i] * r[i] : r[i];
+        const T zi = z ? z[i] : (dinv ? dinv[i] * r[i] : r[i]);
         p[i] = fma(beta, p[i], zi);
     }
 }
@@ -172,16 +190,6 @@ __global__ __launch_bounds__(256) void k_rz(const T *__restrict__ r, const T *__
     }
     block_sum2(rz, dummy);
     if (threadIdx.x == 0) part[blockIdx.x] = rz;
-}
-
-// p = z + beta p
-template <typename T>
-__global__ __launch_bounds__(256) void k_pupdate_z(T *__restrict__ p, const T *__restrict__ z, size_t n,
-                                                   const double *__restrict__ sc) {
-    if (sc[SC_CONV] != 0.0) return;
-    const T beta = (T)sc[SC_BETA];
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = fma(beta, p[i], z[i]);
 }
 
 // sc from the init partials: rr, rz, bb, and the stopping threshold
@@ -238,26 +246,25 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    const int chunk = std::max(1, prm.chunk);
+    // an even number of iterations per graph: iteration k reads r.z from
+    // slot k % 2 and writes slot (k + 1) % 2, and every launch starts at 0
+    const int chunk = std::max(2, prm.chunk + (prm.chunk & 1));
     HC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int it = 0; it < chunk; it++) {
+        const int rz_old = (it & 1) ? SC_RZ2 : SC_RZ, rz_new = (it & 1) ? SC_RZ : SC_RZ2;
         if (prm.diag_op)
             launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
         else
             launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
-        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, st, S.part, (int)nb, S.sc);
-        hipLaunchKernelGGL(k_update<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap, gen ? nullptr : S.dinv, n,
-                           S.sc, S.part2);
+        hipLaunchKernelGGL(k_update_fused<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap,
+                           gen ? nullptr : S.dinv, n, S.sc, S.part, (int)nb, rz_old, S.part2);
         if (gen) {
             S.prec(S.r, S.z, S.sc + SC_CONV);
             hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, (T *)nullptr, n,
                                S.part2 + TR_UPD_BLOCKS, S.sc);
         }
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(256), 0, st, S.part2, (int)G, S.sc, prm.maxit);
-        if (gen)
-            hipLaunchKernelGGL(k_pupdate_z<T>, dim3(G), dim3(256), 0, st, S.p, S.z, n, S.sc);
-        else
-            hipLaunchKernelGGL(k_pupdate<T>, dim3(G), dim3(256), 0, st, S.p, S.r, S.dinv, n, S.sc);
+        hipLaunchKernelGGL(k_pupdate_fused<T>, dim3(G), dim3(256), 0, st, S.p, S.r, gen ? nullptr : S.dinv,
+                           gen ? (const T *)S.z : nullptr, n, S.sc, S.part2, (int)G, rz_old, rz_new, prm.maxit);
     }
     e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
