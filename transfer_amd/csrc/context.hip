@@ -126,11 +126,13 @@ static std::vector<T> coef_records(const HostOperator &op) {
     return rec;
 }
 
-// [WB][nB][4] boundary coefficient records (3 channels, one pad) from the
-// [WB][3][nB] layout
+// [WBP][nB][4] boundary coefficient records (3 channels, one pad) from the
+// [WB][3][nB] layout, WBP = max(WB, TR_WBMAX): the matrix-core schedule
+// walks TR_WBMAX slots, the ones past WB with zero coefficients
 template <typename T>
 static std::vector<T> bcoef_records(const HostOperator &op) {
-    std::vector<T> rec((size_t)op.WB * op.nB * 4, (T)0);
+    const uint32_t WBP = std::max<uint32_t>(op.WB, TR_WBMAX);
+    std::vector<T> rec((size_t)WBP * op.nB * 4, (T)0);
     for (uint32_t s = 0; s < op.WB; s++)
         for (uint32_t b = 0; b < op.nB; b++)
             for (int n = 0; n < 3; n++)
@@ -186,6 +188,16 @@ static float from_bf16(uint16_t h) {
 // (mass = diag(massfac), then the NC-1 stiffness channels) first, then the 3
 // boundary channels.  mode 0 FP64, 1 FP32, 2 BF16 (nsplit 3 appends the
 // low parts: hi table, lo table, per section).
+// [WBP][nB] boundary neighbours, WBP as in bcoef_records: the extra slots
+// point at the boundary vertex itself
+static std::vector<uint32_t> bnbr_padded(const HostOperator &op) {
+    const uint32_t WBP = std::max<uint32_t>(op.WB, TR_WBMAX);
+    std::vector<uint32_t> out((size_t)WBP * op.nB);
+    for (uint32_t s = 0; s < WBP; s++)
+        for (uint32_t b = 0; b < op.nB; b++) out[(size_t)s * op.nB + b] = s < op.WB ? op.bnbr[(size_t)s * op.nB + b] : op.bvert[b];
+    return out;
+}
+
 static std::vector<unsigned char> mfma_tables(const HostOperator &op, int mode, int nsplit) {
     const uint32_t slm = op.slm;
     const int E = mode == 0 ? 2 : 4, J = mfma_J(slm, mode == 0 ? 8 : 4);
@@ -332,7 +344,7 @@ int tr::ctx_init(tr_ctx *c, int device) {
     const HostOperator &op = c->op;
     std::vector<double> stage;
     if ((rc = upload(c, &c->d_nbr, op.nbr)) || (rc = upload(c, &c->d_coef, op.coef)) ||
-        (rc = upload(c, &c->d_bof, op.bof)) || (rc = upload(c, &c->d_bnbr, op.bnbr)) ||
+        (rc = upload(c, &c->d_bof, op.bof)) || (rc = upload(c, &c->d_bnbr, bnbr_padded(op))) ||
         (rc = upload(c, &c->d_bcoef, op.bcoef)) || (rc = upload(c, &c->d_aidx, op.aidx)) ||
         (rc = upload(c, &c->d_aval, op.aval)) || (rc = upload(c, &c->d_bidx, op.bidx)) ||
         (rc = upload(c, &c->d_bval, op.bval)) || (rc = upload(c, &c->d_massfac, op.massfac)) ||

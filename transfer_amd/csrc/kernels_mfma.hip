@@ -186,7 +186,7 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
     constexpr int E = 16 / (int)sizeof(T), PE = (RS + 3) / 4;
     // opaque here, so that the per-slot products and tests are formed where
     // used instead of hoisted out of the tile loop (kept live, they spill)
-    asm volatile("" : "+s"(cstride), "+s"(nslot), "+s"(slm));
+    asm volatile("" : "+s"(cstride), "+s"(slm));
     T xs[DEPTH][J], cs[DEPTH][PE];
     auto load = [&](int s, T *x, T *cf) __attribute__((always_inline)) {
         uint32_t base = nb[s] * slm * (uint32_t)sizeof(T) + 16u * (uint32_t)q;
@@ -298,9 +298,9 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     const auto brs = rsrc4(a.bof, (size_t)nP * 4);
     const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
 
-    const auto bnrs = rsrc4(a.bnbr, (size_t)a.WB * a.nB * 4);
+    const auto bnrs = rsrc4(a.bnbr, (size_t)TR_WBMAX * a.nB * 4);
     const auto crs = rsrc4(a.crec, (size_t)a.W * nP * RS * sizeof(T));
-    const auto bcrs = rsrc4(a.bcrec, (size_t)a.WB * a.nB * 4 * sizeof(T));
+    const auto bcrs = rsrc4(a.bcrec, (size_t)TR_WBMAX * a.nB * 4 * sizeof(T));
 
     // MFMA layout (contraction and write): vertex vloc, chunk q; gather
     // layout: vertex gv, chunk gq (the vertex's four lanes adjacent)
@@ -390,9 +390,13 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
                 for (int j = 0; j < J; j++) bacc[c][j] = 0;
             if (b >= 0) {
                 uint32_t bn[TR_WBMAX];
-                load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, (int)a.WB);
+                // all TR_WBMAX slots (the host pads the boundary stencil with
+                // zero-coefficient slots): a fixed count keeps the pipelined
+                // gather free of branches, whose merges would make the
+                // compiler wait for every load in flight
+                load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, TR_WBMAX);
                 gather<T, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, bcrs, a.nB * (uint32_t)(4 * sizeof(T)),
-                                                     (uint32_t)b * (uint32_t)(4 * sizeof(T)), (int)a.WB, slm, gq);
+                                                     (uint32_t)b * (uint32_t)(4 * sizeof(T)), TR_WBMAX, slm, gq);
             }
             to_mfma_layout<T, MODE, J, 3>(bacc, lane);
             contract<T, MODE, J, 3>(bacc, smem_raw + vsec * parts + boff, bsec, lane, nsplit, D);
