@@ -135,12 +135,20 @@ __device__ __forceinline__ void contract(const T (&acc)[NCH][J], const unsigned 
     }
 }
 
+// A wavefront-uniform value the optimiser cannot see through (so it is not
+// hoisted out of loops and kept live): laundered through a VGPR, back in an
+// SGPR with readfirstlane.
+__device__ inline uint32_t opaque_uniform(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+
 // Neighbour indices of `nslot` slots (nslot <= NS, wavefront-uniform): slot
 // s at nrs (voff, s*nstride).
 template <int NS>
 __device__ inline void load_idx(uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t nrs, uint32_t voff, uint32_t nstride,
                                 int nslot) {
-    asm volatile("" : "+s"(nstride));
+    nstride = opaque_uniform(nstride);
 #pragma unroll
     for (int s = 0; s < NS; s++)
         nb[s] = s < nslot ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(nrs, voff, (uint32_t)s * nstride, 0) : 0u;
@@ -186,7 +194,8 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
     constexpr int E = 16 / (int)sizeof(T), PE = (RS + 3) / 4;
     // opaque here, so that the per-slot products and tests are formed where
     // used instead of hoisted out of the tile loop (kept live, they spill)
-    asm volatile("" : "+s"(cstride), "+s"(slm));
+    cstride = opaque_uniform(cstride);
+    slm = opaque_uniform(slm);
     T xs[DEPTH][J], cs[DEPTH][PE];
     auto load = [&](int s, T *x, T *cf) __attribute__((always_inline)) {
         uint32_t base = nb[s] * slm * (uint32_t)sizeof(T) + 16u * (uint32_t)q;
@@ -309,9 +318,10 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
     typedef typename std::conditional<MODE == 0, d4v, f4v>::type acc4;
     // register budget per lane: 2 wavefronts per SIMD for FP64 (256 VGPRs),
     // 3 for FP32 (168); OVH covers the indices, the MFMA accumulators and
-    // addresses
+    // addresses (fitted to the compiler's allocation; 5 slots in flight at
+    // NC = 7, J = 8 time best, 6 no better)
     constexpr int BUDGET = sizeof(T) == 8 ? 256 : 168;
-    constexpr int OVH = WT + 8 * NT + (MODE == 2 ? 36 : 28);
+    constexpr int OVH = WT + 8 * NT + (sizeof(T) == 8 ? 12 : (MODE == 2 ? 26 : 22));
     constexpr int PE = (RS + 3) / 4;
     constexpr int ACCR = NC * J * (int)sizeof(T) / 4, SLOTR = (J + PE) * (int)sizeof(T) / 4;
     constexpr int DEPTH = std::max(2, std::min(6, (BUDGET - ACCR - OVH) / SLOTR));
