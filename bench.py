@@ -9,10 +9,13 @@ the 110-point Lebedev order), FP64.
 A step is one application of the operator A (computeLhs: volume + vacuum
 boundary kernels) to a device-resident vector.  value = DOF*ordinate/s =
 nP * slm * steps / wall seconds of the timed loop (device-synchronised on
-both sides).  Also reported: time to a 1e-8 relative FP64 residual of the
-plain FP64 PCG (the reference algorithm, main.cpp:125-162), the roofline of
-the volume kernel (HIP events on its stream) and the oracle timed on the
-host cores.
+both sides).  Also reported: the roofline of the operator kernel (HIP
+events on its stream), the FP32 operator of the mixed solver, the time to
+a 1e-8 relative FP64 residual of every solver (the plain FP64 PCG is the
+reference algorithm, main.cpp:125-162, run on the device; the learned
+initial guess feeds the mixed solver) and the oracle timed on the host
+cores.  --batch N runs the batching workload, --c3 the K = 5 scattering
+workload, each as its own JSON line.
 """
 import argparse
 import json
@@ -118,6 +121,87 @@ def bench_batch(count, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
+def bench_c3(steps, warmup):
+    """The high-order scattering workload: the 96^3-grid staircase ball
+    (2,780,400 tets, 485,385 vertices), K = 5 (66 harmonics, the 194-point
+    Lebedev order), Henyey-Greenstein g = 0.6 truncated to P_7 with albedo
+    0.9, kappa = 1 + 9 U[0,1) per tet, I_p = 1.  Reports both operators, the
+    FP32 operator's error against the FP64 one on a random vector, and the
+    solves to a 1e-8 relative FP64 residual."""
+    import numpy as np
+    import transfer_amd as ta
+    t0 = time.perf_counter()
+    P = ta.problems.ball(96, maxk=5, kind="hg")
+    ctx = ta.HipContext(P.mesh, P.ang)
+    nP, nT, slm = P.mesh.nP, P.mesh.nT, P.ang.slm
+    print("[bench-c3] %s nP=%d nT=%d slm=%d, setup %.1f s" % (P.name, nP, nT, slm, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    ctx.autotune(0)
+    ctx.autotune(1)
+    out = dict(metric="C3 operator DOF*ordinate/s and time-to-1e-8", unit="DOF*ordinate/s", n_gpus=1, steps=steps,
+               warmup=warmup, higher_is_better=True, data="synthetic", dtype="f64",
+               config=dict(workload="ball96-hg: ball of 96^3-grid cubes, K=5 (slm 66), HG g=0.6 P7, albedo 0.9, "
+                                    "kappa=1+9U", nP=nP, nT=nT, nF=P.mesh.nF, slm=slm))
+    for prec, key, p in ((0, "operator_fp64", 8), (1, "operator_fp32", 4)):
+        w, k, _, _ = ctx.bench_operator(warmup, steps, precision=prec)
+        b = operator_bytes(nT, nP, P.mesh.nF, slm, p)
+        out[key] = dict(kernel=ctx.kernel_name(prec), kernel_ms=k, dof_ordinate_per_s=nP * slm * steps / w,
+                        algorithmic_bytes=b, achieved_gbs=b / (k * 1e-3) / 1e9, schedule=ctx.schedule(prec))
+        print("[bench-c3] %s %s %.1f us" % (key, ctx.kernel_name(prec), k * 1e3), file=sys.stderr, flush=True)
+    out["value"] = out["operator_fp64"]["dof_ordinate_per_s"]
+    out["ms_per_step"] = out["operator_fp64"]["kernel_ms"]
+    f, a64, a32 = ctx.allocVector(), ctx.allocVector(), ctx.allocVector()
+    ctx.copyToDev(f, np.random.default_rng(1).standard_normal(ctx.N()))
+    ctx.computeLhs(f, a64)
+    ctx.computeLhsPrec(1, f, a32)
+    A64, A32 = ctx.copyToHost(a64), ctx.copyToHost(a32)
+    out["operator_fp32"]["rel_error_vs_fp64"] = float(np.linalg.norm(A32 - A64) / np.linalg.norm(A64))
+    for v in (f, a64, a32):
+        ctx.deviceFree(v)
+    solves = {}
+    for name, solver, prec, kw in (("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
+                                   ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
+                                   ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(inner_tol=1e-5, maxit=30))):
+        d = ctx.allocVector()
+        p = dict(maxit=5000, tol_abs=0.0, tol_rel=1e-8, inner_maxit=5000)
+        p.update(kw)
+        st, _ = ctx.solve(d, solver=solver, use_prec=prec, **p)
+        solves[name] = dict(seconds=st.seconds, iters=st.iters, inner_iters=st.preck, true_rel=st.true_rel,
+                            converged=st.status == 0)
+        print("[bench-c3] %s: %.3f s, %d iters (%d inner), true rel %.2e" % (name, st.seconds, st.iters, st.preck,
+                                                                           st.true_rel), file=sys.stderr, flush=True)
+        ctx.deviceFree(d)
+    out["time_to_1e-8"] = solves
+    out["speedup_mixed_vs_plain_fp64"] = solves["pcg_fp64_device"]["seconds"] / solves["mixed_fp32_ir_jacobi"]["seconds"]
+    print(json.dumps(out), flush=True)
+
+
+def learned_mixed(ctx, P, seeds=48, n=12):
+    """The learned initial guess in front of the mixed solver on the bench
+    mesh.  The network (8 -> 64 -> 64 -> 1, BF16 weights) is trained first on
+    solver-generated cube problems (that time is reported, not counted);
+    the timed solve includes building the guess (features on the host, the
+    network on the device).  The solver falls back to a zero start when the
+    guess's residual is not below ||b||."""
+    import transfer_amd as ta
+    from transfer_amd import learned
+    t0 = time.perf_counter()
+    feat, targ, _ = learned.generate(list(range(seeds)), n=n, maxk=P.ang.maxk, chunk=24)
+    g = learned.LearnedGuess(seed=3)
+    hist = g.fit(feat, targ, steps=800, batch=8192, lr=3e-3)
+    train_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    d_f = g.write_guess(ctx, P.mesh, P.albedo, Ip=None)
+    ctx.synchronize()
+    guess_s = time.perf_counter() - t1
+    st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_MIXED, use_prec=2, x0=True, maxit=30, tol_abs=0.0, tol_rel=1e-8,
+                      inner_maxit=20000, inner_tol=1e-5)
+    ctx.deviceFree(d_f)
+    return dict(seconds=guess_s + st.seconds, guess_seconds=guess_s, solve_seconds=st.seconds, iters=st.iters,
+                inner_iters=st.preck, true_rel=st.true_rel, converged=st.status == 0, fallback=st.fallback,
+                training_seconds=train_s, training_samples=int(len(targ)), training_loss=float(hist[-10:].mean()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +212,8 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time a plain 256 MiB device copy (counter calibration)")
+    ap.add_argument("--c3", action="store_true", help="run the K = 5 scattering workload (ball96-hg) instead")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
                     help="run the batching workload with this many problems instead (256 = the full one)")
     args = ap.parse_args()
@@ -137,6 +223,9 @@ def main():
         sys.exit(2)
     if args.batch:
         bench_batch(args.batch, args.steps, args.warmup)
+        return
+    if args.c3:
+        bench_c3(min(args.steps, 50), min(args.warmup, 5))
         return
 
     import transfer_amd as ta
@@ -197,11 +286,25 @@ def main():
             ctx.deviceFree(d_f)
             print("[bench] %s: %.4f s, %d iters (%d inner), true rel %.2e" % (name, st.seconds, st.iters, st.preck,
                                                                             st.true_rel), file=sys.stderr, flush=True)
+        if not args.no_learned:
+            lm = learned_mixed(ctx, P)
+            solves["learned_mixed_fp32_ir_jacobi"] = lm
+            print("[bench] learned_mixed_fp32_ir_jacobi: %.4f s (guess %.4f s, solve %.4f s), %d inner, "
+                  "fallback %d, true rel %.2e; training %.1f s on %d samples"
+                  % (lm["seconds"], lm["guess_seconds"], lm["solve_seconds"], lm["inner_iters"], lm["fallback"],
+                     lm["true_rel"], lm["training_seconds"], lm["training_samples"]), file=sys.stderr, flush=True)
         out["time_to_1e-8"] = solves
-        plain = min(v["seconds"] for k, v in solves.items() if k.startswith(("pcg_fp64", "reference")) and v["converged"])
-        mixed = min((v["seconds"], k) for k, v in solves.items() if k.startswith("mixed") and v["converged"])
-        out["speedup_mixed_vs_best_plain_fp64"] = plain / mixed[0]
+        # the plain FP64 GPU solver: the reference's method (main.cpp's PCG
+        # as configured) run on the device
+        plain = solves["pcg_fp64_device"]["seconds"]
+        best_plain = min(v["seconds"] for k, v in solves.items() if k.startswith(("pcg_fp64", "reference"))
+                         and v["converged"])
+        mixed = min((v["seconds"], k) for k, v in solves.items() if "mixed" in k and v["converged"])
+        out["speedup_vs_plain_fp64"] = plain / mixed[0]
+        out["speedup_vs_best_plain_fp64"] = best_plain / mixed[0]
         out["best_mixed"] = mixed[1]
+        if "learned_mixed_fp32_ir_jacobi" in solves:
+            out["speedup_learned_mixed_vs_plain_fp64"] = plain / solves["learned_mixed_fp32_ir_jacobi"]["seconds"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P)
         print("[bench] cpu baseline %.3g DOF*ordinate/s" % out["cpu_baseline"]["value"], file=sys.stderr, flush=True)

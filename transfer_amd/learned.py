@@ -53,7 +53,11 @@ class LearnedGuess:
 
     def fit(self, feat, target, steps=2000, batch=16384, lr=2e-3, seed=1):
         self.mu = feat.mean(0).astype(np.float32)
-        self.sd = np.maximum(feat.std(0), 1e-3).astype(np.float32)
+        # a feature constant over the training set carries no information and
+        # its first-layer weights get no gradient: a huge scale makes the
+        # network ignore it wherever it is applied
+        sd = feat.std(0)
+        self.sd = np.where(sd < 1e-6, 1e30, np.maximum(sd, 1e-3)).astype(np.float32)
         self.net.set_norm(self.mu, self.sd)
         d_f, d_t = self.net.upload(feat), self.net.upload(target)
         return self.net.train(d_f, d_t, len(target), steps, batch, lr, seed)
@@ -65,8 +69,14 @@ class LearnedGuess:
         return self.net.download(d_y, len(feat))
 
     def write_guess(self, ctx, mesh, albedo, Ip=1.0):
-        """ctx's device vector holding the learned guess."""
-        y = self.predict(mlp_features(mesh, albedo))
+        """ctx's device vector holding the learned guess.  Ip: the source
+        scale I_p of U_inf = 4 pi I_p / (1 - albedo), or None for each
+        vertex's own (feature 7 holds log I_p per vertex)."""
+        feat = mlp_features(mesh, albedo)
+        y = self.predict(feat)
+        if Ip is None:
+            y = (y * np.exp(feat[:, 7])).astype(np.float32)
+            Ip = 1.0
         d_y = ctx.deviceAlloc(4 * len(y))
         ctx.copyToDev(d_y, y)
         f = ctx.allocVector()
