@@ -346,6 +346,17 @@ int tr_mlp_train(tr_mlp *m, const float *feat, const float *target, uint32_t n,
  * i.e. f[v][00] = scale * y[v], other harmonics 0; y[nP] is a device array
  * in the caller's vertex order (scale = I_p / (1 - albedo) for the MLP). */
 int tr_ctx_scalar_guess(tr_ctx *ctx, const float *y, double scale, double *f);
+/* The features of tr_mlp_features computed on the device from the
+ * context's data (contexts built from meshes with coordinates; batches use
+ * each problem's albedo): feat[nP][8] device, caller's vertex order.  The
+ * optical depth is the same shortest-path value, by parallel Bellman-Ford
+ * sweeps; *sweeps (may be NULL) receives their number. */
+int tr_ctx_mlp_features(tr_ctx *ctx, double albedo, float *feat, int *sweeps);
+/* f (device vector) = the learned guess: device features, the network's
+ * prediction and f[v][00] = y I_p / (1 - albedo), with I_p = ip_scale or,
+ * when ip_scale <= 0, each vertex's own source (exp of feature 7). */
+int tr_ctx_learned_guess(tr_ctx *ctx, tr_mlp *m, double albedo, double ip_scale,
+                         double *f);
 
 /* Seconds per operator application (median over reps), for benchmarking. */
 double tr_bench_lhs(tr_ctx *ctx, int reps);

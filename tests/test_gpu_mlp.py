@@ -76,3 +76,37 @@ def test_learned_guess_on_held_out_problems():
     print("[gpu-mlp] held-out mean inner iterations %.1f -> %.1f, time %.4f -> %.4f s" % (
         np.mean(base_it), np.mean(learn_it), np.mean(base_t), np.mean(learn_t)), flush=True)
     assert np.mean(pe) < np.mean(me)
+
+
+@pytest.mark.parametrize("case", ["cube", "ball-planck", "batch"])
+def test_device_features_match_host(case):
+    """tr_ctx_mlp_features against tr_mlp_features: the same eight features
+    (the optical depth by Bellman-Ford sweeps on the device, by Dijkstra on
+    the host), to float32 rounding of log/exp evaluated by two libraries."""
+    if case == "batch":
+        P = [ta.problems.cube(6, maxk=2, seed=s) for s in (3, 8)]
+        ctx = ta.Batch([p.mesh for p in P], ta.Angular(2), [p.albedo for p in P])
+        want = np.concatenate([ta.mlp_features(p.mesh, p.albedo) for p in P])
+        albedo = 0.0
+    else:
+        P = ta.problems.cube(10, maxk=2, seed=8) if case == "cube" else ta.problems.ball(16, maxk=2)
+        ctx = ta.HipContext(P.mesh, P.ang)
+        albedo = P.albedo
+        want = ta.mlp_features(P.mesh, albedo)
+    got, sweeps = ctx.mlp_features(albedo)
+    err = np.abs(got - want) / np.maximum(1.0, np.abs(want))
+    print("[gpu-mlp] %s: device features max rel err %.2e per feature %s, %d sweeps" % (
+        case, err.max(), np.array2string(err.max(0), precision=1), sweeps), flush=True)
+    assert err.max() <= 1e-5
+
+
+def test_device_guess_matches_host_guess():
+    P = ta.problems.ball(12, maxk=2)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    g = learned.LearnedGuess(seed=5)
+    d_dev = g.write_guess(ctx, P.mesh, P.albedo, Ip=None)
+    d_host = g.write_guess_host(ctx, P.mesh, P.albedo, Ip=None)
+    a, b = ctx.copyToHost(d_dev), ctx.copyToHost(d_host)
+    err = np.abs(a - b).max() / np.abs(b).max()
+    print("[gpu-mlp] device guess vs host guess: max rel err %.2e" % err, flush=True)
+    assert err <= 1e-5

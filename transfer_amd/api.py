@@ -129,6 +129,8 @@ _sig("tr_mlp_predict", C.c_int, vp, vp, C.c_uint32, vp)
 _sig("tr_mlp_grad", C.c_int, vp, vp, vp, C.c_uint32, f32p, f32p)
 _sig("tr_mlp_train", C.c_int, vp, vp, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_float, C.c_uint64, f32p)
 _sig("tr_ctx_scalar_guess", C.c_int, vp, vp, C.c_double, vp)
+_sig("tr_ctx_mlp_features", C.c_int, vp, C.c_double, vp, C.POINTER(C.c_int))
+_sig("tr_ctx_learned_guess", C.c_int, vp, vp, C.c_double, C.c_double, vp)
 _sig("tr_batch_create", C.c_int, C.c_int, C.c_int, C.POINTER(MeshView), C.POINTER(AngularView), f64p, C.c_int,
      C.POINTER(vp))
 _sig("tr_batch_info", C.c_int, vp, u32p, u32p)
@@ -447,6 +449,21 @@ class HipContext:
     def scalar_guess(self, y, scale, f):
         """f[v][00] = scale * y[v] (device y in caller order), the rest 0."""
         check(lib.tr_ctx_scalar_guess(self._h, y, float(scale), f))
+
+    def mlp_features(self, albedo):
+        """(features [nP][8] float32 in the caller's order, sweeps) computed on
+        the device (tr_ctx_mlp_features)."""
+        d = self.deviceAlloc(4 * 8 * self.nP)
+        sw = C.c_int(0)
+        check(lib.tr_ctx_mlp_features(self._h, float(albedo), d, C.byref(sw)))
+        out = self.copyToHost(d, 4 * 8 * self.nP, np.float32).reshape(self.nP, 8)
+        self.deviceFree(d)
+        return out, sw.value
+
+    def learned_guess(self, net, albedo, ip_scale, f):
+        """f = the learned initial guess (tr_ctx_learned_guess); net is an MLP,
+        ip_scale <= 0 takes each vertex's own source."""
+        check(lib.tr_ctx_learned_guess(self._h, net._h, float(albedo), float(ip_scale), f))
 
     def moments(self, f):
         """ComputeMoments of the device vector f: host U[nP], T[9][nP] (T_ij at

@@ -355,6 +355,8 @@ int tr::ctx_init(tr_ctx *c, int device) {
         (!op.mask.empty() && (rc = upload(c, &c->d_mask, op.mask))) ||
         (rc = upload(c, &c->d_mom_val, op.mom_val)) || (rc = upload(c, &c->d_mom_pos, op.mom_pos)) ||
         (!op.mf0.empty() && (rc = upload(c, &c->d_mf0, op.mf0))) ||
+        (!op.vsum.empty() && (rc = upload(c, &c->d_vsum, op.vsum))) ||
+        (!op.xyz.empty() && (rc = upload(c, &c->d_xyz, op.xyz))) ||
         (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
         (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
         tr_ctx_destroy(c);

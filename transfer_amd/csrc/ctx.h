@@ -43,6 +43,9 @@ struct tr_ctx {
     uint32_t *d_tiles = nullptr; // variant 4: order of the 16-vertex wavefront tiles
     double *d_mf0 = nullptr;       // [nP] lm = 0 mass factor per vertex (batches), or null
     float *f_mf0 = nullptr;
+    double *d_vsum = nullptr, *d_xyz = nullptr;  // learned-guess feature inputs [3][nP] each, or null
+    double *d_feat_tmp = nullptr;  // [2][nP] kappa and optical depth + 8 flags (guess.hip), on first use
+    float *d_guess_tmp = nullptr;  // [9][nP] features and prediction (guess.hip), on first use
     double *d_mom_val = nullptr;   // moments table [3][slm]
     uint32_t *d_mom_pos = nullptr;
     void *d_tab = nullptr;      // contraction-table records

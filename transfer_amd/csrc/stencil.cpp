@@ -362,6 +362,39 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
         }
         op.rhs[i] = sum;
     }
+
+    // ---- per-vertex element sums and coordinates for the learned guess's
+    // features (the same sums as tr_mlp_features)
+    op.vsum.clear();
+    op.xyz.clear();
+    if (mv.coords) {
+        op.vsum.assign((size_t)3 * nP, 0.0);
+        op.xyz.resize((size_t)3 * nP);
+        const double *X = mv.coords;
+        for (uint32_t i = 0; i < nP; i++)
+            for (int k = 0; k < 3; k++) op.xyz[(size_t)k * nP + i] = X[3 * (size_t)op.perm[i] + k];
+        for (uint32_t t = 0; t < mv.nT; t++) {
+            const tr_tetrahedron &T = mv.mesh[t];
+            const double *a = X + 3 * (size_t)T.p[0], *b = X + 3 * (size_t)T.p[1], *c = X + 3 * (size_t)T.p[2],
+                         *d = X + 3 * (size_t)T.p[3];
+            double u[3], v[3], w[3];
+            for (int k = 0; k < 3; k++) {
+                u[k] = b[k] - a[k];
+                v[k] = c[k] - a[k];
+                w[k] = d[k] - a[k];
+            }
+            const double V = std::fabs(u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
+                                       u[2] * (v[0] * w[1] - v[1] * w[0])) / 6.0;
+            if (!(V > 0)) return fail(TR_ERR_ARG, "assemble_operator: degenerate tetrahedron");
+            const double kap = T.kappa_volume / V;
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = op.iperm[T.p[k]];
+                op.vsum[i] += V;
+                op.vsum[(size_t)nP + i] += kap * V;
+                op.vsum[(size_t)2 * nP + i] += T.I_p * kap * V;
+            }
+        }
+    }
     return TR_OK;
 }
 
@@ -417,6 +450,12 @@ int concat_operators(const std::vector<HostOperator> &ops, const std::vector<dou
     u.bcoef.assign((size_t)WB * 3 * std::max<uint64_t>(nB, 1), 0.0);
     u.rhs.resize(nP);
     u.mf0.resize(nP);
+    bool feats = true;
+    for (const HostOperator &o : ops) feats = feats && !o.vsum.empty() && !o.xyz.empty();
+    if (feats) {
+        u.vsum.resize((size_t)3 * nP);
+        u.xyz.resize((size_t)3 * nP);
+    }
     voff.assign(ops.size() + 1, 0);
     uint32_t v0 = 0, b0 = 0;
     for (size_t p = 0; p < ops.size(); p++) {
@@ -429,6 +468,11 @@ int concat_operators(const std::vector<HostOperator> &ops, const std::vector<dou
             u.rhs[g] = o.rhs[i];
             u.mf0[g] = 1.0 - albedo[p];
             u.bof[g] = o.bof[i] >= 0 ? (int32_t)(b0 + (uint32_t)o.bof[i]) : -1;
+            if (feats)
+                for (int k = 0; k < 3; k++) {
+                    u.vsum[(size_t)k * nP + g] = o.vsum[(size_t)k * o.nP + i];
+                    u.xyz[(size_t)k * nP + g] = o.xyz[(size_t)k * o.nP + i];
+                }
             for (uint32_t s = 0; s < W; s++) {
                 // slots past a problem's own W repeat the vertex with zero couplings
                 u.nbr[(size_t)s * nP + g] = s < o.W ? v0 + o.nbr[(size_t)s * o.nP + i] : g;

@@ -67,6 +67,11 @@ struct HostOperator {
     // batches: per-vertex factor of the lm = 0 mass term (1 - albedo of the
     // vertex's problem, isotropic scattering); empty outside batches
     std::vector<double> mf0;
+    // inputs of the learned guess's per-vertex features, in the new
+    // numbering (empty when the mesh has no coordinates, and on coarse
+    // levels): vsum[0..2][nP] = sums over the incident tets of V, kappa V
+    // and kappa V I_p; xyz[3][nP] = the vertex coordinates
+    std::vector<double> vsum, xyz;
 };
 
 // The block-diagonal union of independent operators with identical angular

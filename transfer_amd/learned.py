@@ -69,9 +69,17 @@ class LearnedGuess:
         return self.net.download(d_y, len(feat))
 
     def write_guess(self, ctx, mesh, albedo, Ip=1.0):
-        """ctx's device vector holding the learned guess.  Ip: the source
-        scale I_p of U_inf = 4 pi I_p / (1 - albedo), or None for each
+        """ctx's device vector holding the learned guess, built on the device
+        (features, prediction and guess: tr_ctx_learned_guess).  Ip: the
+        source scale I_p of U_inf = 4 pi I_p / (1 - albedo), or None for each
         vertex's own (feature 7 holds log I_p per vertex)."""
+        f = ctx.allocVector()
+        ctx.learned_guess(self.net, albedo, -1.0 if Ip is None else Ip, f)
+        return f
+
+    def write_guess_host(self, ctx, mesh, albedo, Ip=1.0):
+        """write_guess with the features computed on the host
+        (tr_mlp_features) and the prediction copied through the host."""
         feat = mlp_features(mesh, albedo)
         y = self.predict(feat)
         if Ip is None:
