@@ -192,6 +192,12 @@ int tr_ctx_set_schedule(tr_ctx *ctx, int precision, int J, int WPB, int variant)
  * remain; symmetric V-cycle with one damped-Jacobi step per side and `nu`
  * on the coarsest level (defaults 3000, 16, 2). */
 int tr_ctx_set_multilevel(tr_ctx *ctx, uint32_t coarsest, uint32_t agg, int nu);
+/* Chebyshev polynomial preconditioner (use_prec 4 of the device solvers,
+ * extension): `degree` steps of the Chebyshev iteration on D^-1 A z = D^-1 r
+ * over [lambda_max / ratio, lambda_max] (lambda_max of D^-1 A from a power
+ * iteration), each step one operator launch with the update fused into its
+ * write; defaults 4, 100. */
+int tr_ctx_set_chebyshev(tr_ctx *ctx, int degree, double ratio);
 /* Angular precision of the FP32 operator used by TR_SOLVER_MIXED:
  * 0 = FP32 throughout, 1 = BF16 matrix-core projection with FP32
  * accumulation, 3 (default) = BF16 with the hi/lo split (three BF16
@@ -240,7 +246,8 @@ enum {
     TR_SOLVER_REFERENCE = 0, /* main.cpp's PCG + invprec, host-driven, FP64 */
     TR_SOLVER_PCG_FP64 = 1,  /* CG with device-resident scalars and graph-
                                 captured iterations; use_prec 0 = none,
-                                2 = Jacobi (exact diagonal of A)            */
+                                2 = Jacobi (exact diagonal of A),
+                                3 = multilevel, 4 = Chebyshev polynomial   */
     TR_SOLVER_MIXED = 2      /* FP64 iterative refinement (true residual
                                 b - A x in FP64) around the FP32 operator's
                                 device CG run to inner_tol (relative), or

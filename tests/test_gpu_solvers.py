@@ -1,6 +1,6 @@
 """Device-resident CG (FP64) and FP64-refined mixed-precision solves, with
-no, Jacobi (2) or multilevel (3) preconditioning, against the oracle's FP64
-PCG solution on the same inputs.
+no, Jacobi (2), multilevel (3) or Chebyshev polynomial (4) preconditioning,
+against the oracle's FP64 PCG solution on the same inputs.
 
 Bars: the FP64 true residual ||b - A x|| / ||b|| of every returned solution
 meets the requested tolerance (checked on the GPU with the FP64 operator and
@@ -30,8 +30,9 @@ def build(name):
 
 @pytest.mark.parametrize("name", ["slab8", "ball16", "cube8", "ball12-hg"])
 @pytest.mark.parametrize("solver,prec", [(ta.TR_SOLVER_PCG_FP64, 0), (ta.TR_SOLVER_PCG_FP64, 2),
-                                         (ta.TR_SOLVER_PCG_FP64, 3), (ta.TR_SOLVER_MIXED, 2),
-                                         (ta.TR_SOLVER_MIXED, 3)])
+                                         (ta.TR_SOLVER_PCG_FP64, 3), (ta.TR_SOLVER_PCG_FP64, 4),
+                                         (ta.TR_SOLVER_MIXED, 2), (ta.TR_SOLVER_MIXED, 3),
+                                         (ta.TR_SOLVER_MIXED, 4)])
 def test_solver_reaches_fp64_residual(name, solver, prec):
     print("[gpu-solvers]", name, solver, prec, flush=True)
     P = build(name)
@@ -72,3 +73,40 @@ def test_multilevel_hierarchy_shapes():
         if ref is None:
             ref = x
         assert np.abs(x - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case", ["fused", "unfused", "reflective"])
+def test_chebyshev_preconditioner(case):
+    """The Chebyshev polynomial preconditioner at several degrees, with its
+    steps fused into the matrix-core operator's write ("fused") or run as
+    operator + elementwise pass on a VALU schedule ("unfused"), and on a
+    problem with reflecting planes (vanishing entries): every solve reaches
+    the FP64 residual, higher degrees take fewer outer iterations, and all
+    return the same solution."""
+    if case == "reflective":
+        P = ta.problems.milne(n=12, maxk=2)
+    else:
+        P = ta.problems.cube(10, seed=4)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    if case == "unfused":
+        ctx.set_schedule(0, J=4, WPB=4, variant=3)
+        ctx.set_schedule(1, J=4, WPB=4, variant=3)
+    ref, its = None, []
+    for solver in (ta.TR_SOLVER_PCG_FP64, ta.TR_SOLVER_MIXED):
+        for deg in (1, 3, 6):
+            ctx.set_chebyshev(deg, 100.0)
+            d = ctx.allocVector()
+            st, _ = ctx.solve(d, solver=solver, use_prec=4, maxit=20000 if solver != ta.TR_SOLVER_MIXED else 30,
+                              tol_abs=0.0, tol_rel=1e-10, inner_maxit=20000, inner_tol=1e-4)
+            x = ctx.copyToHost(d)
+            ctx.deviceFree(d)
+            n_it = st.iters if solver == ta.TR_SOLVER_PCG_FP64 else st.preck
+            print("[gpu-solvers] chebyshev %s solver=%d degree %d: %d iterations, true rel %.2e" % (
+                case, solver, deg, n_it, st.true_rel), flush=True)
+            assert st.status == 0 and st.true_rel <= 1.01e-10
+            if solver == ta.TR_SOLVER_PCG_FP64:
+                its.append(n_it)
+            if ref is None:
+                ref = x
+            assert np.abs(x - ref).max() <= 1e-7 * np.abs(ref).max()
+    assert its[0] > its[1] > its[2]

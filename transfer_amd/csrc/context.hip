@@ -94,15 +94,15 @@ static bool v4_ok(const HostOperator &op) {
 static LaneMap mfma_map_ctx(const tr_ctx *c, int wpb, int mode) {
     LaneMap m = mfma_map(c->op.slm, wpb, mode);
     const int NC = (int)c->op.NC;
-    for (bool dot : {false, true}) {
+    for (int epi : {0, 1, 2}) {
         uint32_t g;
         if (mode == 0)
-            g = apply4_grid<double, 0>(c->args(m), NC, wpb, dot);
+            g = apply4_grid<double, 0>(c->args(m), NC, wpb, epi);
         else if (mode == 1)
-            g = apply4_grid<float, 1>(c->args32(m), NC, wpb, dot);
+            g = apply4_grid<float, 1>(c->args32(m), NC, wpb, epi);
         else
-            g = apply4_grid<float, 2>(c->args32(m), NC, wpb, dot);
-        (dot ? m.grid_dot : m.grid) = g;
+            g = apply4_grid<float, 2>(c->args32(m), NC, wpb, epi);
+        (epi == 0 ? m.grid : (epi == 1 ? m.grid_dot : m.grid_cheb)) = g;
     }
     return m;
 }
@@ -747,6 +747,87 @@ static int start_point(tr_ctx *c, const tr_solve_params *prm, double *f, const d
     return axpby(c, f, 0, nullptr, 0);
 }
 
+// Chebyshev polynomial preconditioner (use_prec 4): z = the deg-th iterate
+// of the Chebyshev iteration on D^-1 A z = D^-1 r from z = 0 over the
+// interval [lambda_max / ratio, lambda_max] of D^-1 A's spectrum (Saad,
+// Iterative Methods for Sparse Linear Systems, Alg. 12.1, in its three-term
+// form).  Its residual polynomial is below 1 in magnitude on the interval
+// and in (0, 1) below it, so M = p(D^-1 A) D^-1 is symmetric positive
+// definite for any spectrum inside (0, lambda_max]: a fixed CG
+// preconditioner.  lambda_max comes from an FP64 power iteration times 1.1.
+// Each of the deg - 1 steps is one operator launch with the update fused
+// into its write (matrix-core schedule).
+static int cheb_lambda(tr_ctx *c, const std::vector<double> &dinv) {
+    if (c->cheb_lmax > 0) return TR_OK;
+    const size_t n = c->nint();
+    double *d = nullptr, *x = nullptr, *y = nullptr;
+    HIPCHECK(hipMalloc((void **)&d, sizeof(double) * 3 * n));
+    x = d + n;
+    y = x + n;
+    double lam = 0;
+    int rc = TR_OK;
+    if (hipMemcpyAsync(d, dinv.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(TR_ERR_HIP, "chebyshev: upload failed");
+    if (!rc) rc = lambda_max(c, d, x, y, lam);
+    hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (rc) return rc;
+    c->cheb_lmax = 1.1 * lam;
+    return TR_OK;
+}
+
+// z = s dinv r, the first Chebyshev iterate from z = 0
+template <typename T>
+__global__ __launch_bounds__(256) void k_cheb_first(const T *__restrict__ r, const T *__restrict__ dinv, T s,
+                                                    T *__restrict__ z, size_t n, const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        z[i] = s * dinv[i] * r[i];
+}
+
+template <typename T>
+struct ChebPrec {
+    OpArgs<T> a;
+    int NC = 7;
+    LaneMap map;
+    const T *dinv = nullptr;
+    T *zb = nullptr, *tmp = nullptr;
+    size_t n = 0;
+    double inv_theta = 0;
+    std::vector<double> om, ga;  // steps 1 .. deg-1
+    void setup(int deg, double lmax, double ratio) {
+        const double lmin = lmax / ratio, th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sig = th / de;
+        double rho = 1.0 / sig;
+        inv_theta = 1.0 / th;
+        om.clear();
+        ga.clear();
+        for (int k = 1; k < deg; k++) {
+            const double rn = 1.0 / (2.0 * sig - rho);
+            om.push_back(rn * rho);
+            ga.push_back(2.0 * rn / de);
+            rho = rn;
+        }
+    }
+    hipError_t apply(const T *r, T *z, const double *skip, hipStream_t st) const {
+        const int steps = (int)om.size();
+        T *cur = (steps % 2 == 0) ? z : zb, *oth = (steps % 2 == 0) ? zb : z;
+        const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_cheb_first<T>, dim3(g), dim3(256), 0, st, r, dinv, (T)inv_theta, cur, n, skip);
+        hipError_t e = hipGetLastError();
+        for (int k = 0; k < steps && e == hipSuccess; k++) {
+            ChebStep<T> cs;
+            cs.rhs = r;
+            cs.dinv = dinv;
+            cs.zc = (T)(1.0 + om[k]);
+            cs.om = k == 0 ? (T)0 : (T)om[k];  // z_0 = 0
+            cs.ga = (T)ga[k];
+            e = launch_cheb_step<T>(a, NC, map, cur, oth, cs, tmp, skip, st);
+            std::swap(cur, oth);
+        }
+        return e;
+    }
+};
+
 static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
     Work w(c);
     const size_t n = c->nint();
@@ -759,15 +840,30 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     sc = part2 + 2 * 1024;
     if (!Ap) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
     int rc;
-    if (prm->use_prec == 2) {
+    ChebPrec<double> cheb;
+    if (prm->use_prec == 2 || prm->use_prec == 4) {
         std::vector<double> d = operator_diagonal(c->op);
         for (double &x : d) x = 1.0 / x;
         dinv = w.get();
+        if (!dinv) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
         HIPCHECK(hipMemcpyAsync(dinv, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        if (prm->use_prec == 4) {
+            if ((rc = cheb_lambda(c, d))) return rc;
+            cheb.a = c->args();
+            cheb.NC = (int)c->op.NC;
+            cheb.map = c->map;
+            cheb.dinv = dinv;
+            cheb.n = n;
+            cheb.zb = w.get();
+            cheb.tmp = w.get();
+            if (!cheb.tmp) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+            cheb.setup(c->cheb_deg, c->cheb_lmax, c->cheb_ratio);
+        }
     } else if (prm->use_prec == 3) {
         if ((rc = build_multilevel(c, false, c->mg_coarsest, c->mg_agg))) return rc;
     } else if (prm->use_prec != 0) {
-        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none), 2 (Jacobi) or 3 (multilevel)");
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none), 2 (Jacobi), 3 (multilevel) "
+                                "or 4 (Chebyshev)");
     }
     if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
     CgSpace<double> S;
@@ -775,6 +871,11 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
         S.z = w.get();
         if (!S.z) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
         S.prec = [c](const double *rr, double *z, const double *skip) { return multilevel_apply64(c, rr, z, skip); };
+    } else if (prm->use_prec == 4) {
+        S.z = w.get();
+        if (!S.z) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+        const hipStream_t cs = c->stream;
+        S.prec = [&cheb, cs](const double *rr, double *z, const double *skip) { return cheb.apply(rr, z, skip, cs); };
     }
     S.a = c->args();
     S.NC = (int)c->op.NC;
@@ -863,17 +964,34 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     float *d32 = v32, *r32 = v32 + n, *p32 = v32 + 2 * n, *Ap32 = v32 + 3 * n, *b32 = v32 + 4 * n;
     float *dinv32 = nullptr;
     if (!Ap) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
-    if (prm->use_prec == 2) {
+    ChebPrec<float> cheb;
+    if (prm->use_prec == 2 || prm->use_prec == 4) {
         std::vector<double> d = operator_diagonal(c->op);
         std::vector<float> di(d.size());
         for (size_t i = 0; i < d.size(); i++) di[i] = (float)(1.0 / d[i]);
         dinv32 = v32 + 5 * n;
         HIPCHECK(hipMemcpyAsync(dinv32, di.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
+        if (prm->use_prec == 4) {
+            for (double &x : d) x = 1.0 / x;
+            if ((rc = cheb_lambda(c, d))) return rc;
+            float *ex = nullptr;
+            HIPCHECK(hipMalloc((void **)&ex, sizeof(float) * 2 * n + 8));
+            w.v.push_back((double *)ex);
+            cheb.a = c->args32();
+            cheb.NC = (int)c->op.NC;
+            cheb.map = c->map32;
+            cheb.dinv = dinv32;
+            cheb.n = n;
+            cheb.zb = ex;
+            cheb.tmp = ex + n;
+            cheb.setup(c->cheb_deg, c->cheb_lmax, c->cheb_ratio);
+        }
     } else if (prm->use_prec == 3) {
         if ((rc = build_multilevel(c, true, c->mg_coarsest, c->mg_agg))) return rc;
     } else if (prm->use_prec != 0) {
-        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none), 2 (Jacobi) or 3 (multilevel)");
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none), 2 (Jacobi), 3 (multilevel) or "
+                                "4 (Chebyshev)");
     }
     auto t0 = std::chrono::steady_clock::now();
     if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
@@ -882,6 +1000,10 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     if (prm->use_prec == 3) {
         S.z = v32 + 6 * n;
         S.prec = [c](const float *rr, float *z, const double *skip) { return multilevel_apply32(c, rr, z, skip); };
+    } else if (prm->use_prec == 4) {
+        S.z = v32 + 6 * n;
+        const hipStream_t cs = c->stream;
+        S.prec = [&cheb, cs](const float *rr, float *z, const double *skip) { return cheb.apply(rr, z, skip, cs); };
     }
     S.a = c->args32();
     S.NC = (int)c->op.NC;
@@ -1284,6 +1406,16 @@ extern "C" int tr_compute_lhs_prec(tr_ctx *c, int precision, const double *f, do
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(v);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("tr_compute_lhs_prec: ") + hipGetErrorString(e));
+    return TR_OK;
+}
+
+// Chebyshev polynomial preconditioner (use_prec 4): its degree (1 = a
+// scaled Jacobi step) and the ratio lambda_max / lambda_min of the
+// interval of D^-1 A's spectrum it targets.
+int tr_ctx_set_chebyshev(tr_ctx *c, int degree, double ratio) {
+    if (!c || degree < 1 || degree > 64 || !(ratio > 1.0)) return fail(TR_ERR_ARG, "tr_ctx_set_chebyshev: bad argument");
+    c->cheb_deg = degree;
+    c->cheb_ratio = ratio;
     return TR_OK;
 }
 

@@ -68,6 +68,11 @@ struct tr_ctx {
     tr::Multilevel *mg = nullptr;  // multilevel preconditioner (use_prec 3), built on first use
     uint32_t mg_coarsest = 3000, mg_agg = 16; // hierarchy shape: coarsest-level size, aggregate size
     int mg_nu = 2;                            // damped-Jacobi steps on the coarsest level
+    // Chebyshev polynomial preconditioner (use_prec 4): degree, the ratio
+    // lambda_max / lambda_min of the interval it targets, and lambda_max of
+    // D^-1 A (power iteration on first use, 0 before)
+    int cheb_deg = 4;
+    double cheb_ratio = 100.0, cheb_lmax = 0.0;
     std::vector<void *> owned;
 
     tr::OpArgs<double> args() const { return args(map); }
@@ -170,5 +175,7 @@ void destroy_multilevel(Multilevel *mg);
 hipError_t multilevel_apply64(tr_ctx *c, const double *r, double *z, const double *skip);
 hipError_t multilevel_apply32(tr_ctx *c, const float *r, float *z, const double *skip);
 int multilevel_levels(const tr_ctx *c);
+// largest eigenvalue of D^-1 A (FP64 power iteration; dinv, x, y device [nint])
+int lambda_max(tr_ctx *c, const double *dinv, double *x, double *y, double &lam);
 
 }  // namespace tr

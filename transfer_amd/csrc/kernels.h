@@ -25,6 +25,7 @@ struct LaneMap {
     int mfma_mode = 1;  // FP32 operator under variant 4: 1 = FP32 MFMA, 2 = BF16 MFMA
     uint32_t grid = 0;      // variant 4: workgroups of the persistent grid (0 = one per tile)
     uint32_t grid_dot = 0;  // the same for the launch with the fused dot product
+    uint32_t grid_cheb = 0; // the same for the launch with a fused Chebyshev step
 };
 
 // Device operator data (see stencil.h), precision T.
@@ -57,6 +58,15 @@ struct OpArgs {
                            // tiles, the order of the walk; null = natural order
 };
 
+// One step of the Chebyshev iteration on D^-1 A z = D^-1 rhs, written as
+// out = zc z - om out_prev + ga dinv (rhs - A z), out_prev the iterate
+// before z (held in out on entry; not read when om == 0).
+template <typename T>
+struct ChebStep {
+    const T *rhs, *dinv;
+    T zc, om, ga;
+};
+
 // coefficient record stride (elements): NC rounded up to a 16-byte multiple
 template <typename T>
 constexpr int rec_stride(int NC) {
@@ -74,6 +84,11 @@ constexpr int tab_rec_bytes() {
 template <typename T>
 hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                         double *part, const double *skip, hipStream_t s);
+// One Chebyshev step (ChebStep): fused into the operator's write on the
+// matrix-core schedule, else the operator into tmp and an elementwise pass.
+template <typename T>
+hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const T *z, T *out,
+                            const ChebStep<T> &cs, T *tmp, const double *skip, hipStream_t s);
 // r[mask[i]] = 0 for the entries that specular reflection removes (no-op
 // without reflecting planes)
 template <typename T>
@@ -119,11 +134,12 @@ bool mfma_supported(uint32_t slm, int mode);
 size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit);
 template <typename T, int MODE>
 hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, const T *f, T *r, double *part,
-                         const double *skip, hipStream_t s);
+                         const double *skip, hipStream_t s, const ChebStep<T> *cheb = nullptr);
 // workgroups of the persistent variant-4 grid for this operator and WPB,
-// for the launch without or with the fused dot product
+// for the launch with epilogue epi (0 plain, 1 fused dot product, 2
+// Chebyshev step)
 template <typename T, int MODE>
-uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, bool dot);
+uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, int epi);
 
 // moments U, T and flux W of a solution (postproc.hip)
 hipError_t launch_moments(const double *f, const uint32_t *perm, const double *mval, const uint32_t *mpos,

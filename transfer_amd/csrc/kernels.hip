@@ -632,6 +632,42 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
     return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
 }
 
+// out = zc z - om out + ga dinv (rhs - t), t = A z (the unfused Chebyshev
+// step's elementwise pass)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cheb_upd(const T *__restrict__ z, T *__restrict__ out,
+                                                  const T *__restrict__ t, ChebStep<T> cs, size_t n,
+                                                  const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        T nv = cs.zc * z[i] + cs.ga * cs.dinv[i] * (cs.rhs[i] - t[i]);
+        if (cs.om != (T)0) nv -= cs.om * out[i];
+        out[i] = nv;
+    }
+}
+
+template <typename T>
+hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const T *z, T *out,
+                            const ChebStep<T> &cs, T *tmp, const double *skip, hipStream_t s) {
+    if (a.nP == 0) return hipSuccess;
+    hipError_t e;
+    if (m.variant == 4) {
+        if constexpr (sizeof(T) == 8)
+            e = launch_apply4<T, 0>(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+        else if (m.mfma_mode == 2)
+            e = launch_apply4<T, 2>(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+        else
+            e = launch_apply4<T, 1>(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+        return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
+    }
+    if (!tmp) return hipErrorInvalidValue;
+    if ((e = launch_apply<T>(a, NC, m, z, tmp, nullptr, skip, s)) != hipSuccess) return e;
+    const size_t n = (size_t)a.nP * a.slm;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_cheb_upd<T>, dim3(g), dim3(256), 0, s, z, out, tmp, cs, n, skip);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                              double *part, const double *skip, hipStream_t s) {
@@ -653,6 +689,10 @@ hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const
     return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
 }
 
+template hipError_t launch_cheb_step<double>(const OpArgs<double> &, int, const LaneMap &, const double *, double *,
+                                             const ChebStep<double> &, double *, const double *, hipStream_t);
+template hipError_t launch_cheb_step<float>(const OpArgs<float> &, int, const LaneMap &, const float *, float *,
+                                            const ChebStep<float> &, float *, const double *, hipStream_t);
 template hipError_t launch_zero_mask<double>(const OpArgs<double> &, double *, const double *, hipStream_t);
 template hipError_t launch_zero_mask<float>(const OpArgs<float> &, float *, const double *, hipStream_t);
 template hipError_t launch_apply<double>(const OpArgs<double> &, int, const LaneMap &, const double *,

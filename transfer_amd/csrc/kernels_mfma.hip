@@ -276,10 +276,15 @@ __device__ __forceinline__ void to_mfma_layout(T (&acc)[NCH][J], int lane) {
 // that is a multiple of the 8 XCDs, the workgroups of one XCD (block ids
 // congruent mod 8) share a contiguous range of tiles, so neighbouring
 // vertices' data stay in that XCD's L2.
-template <typename T, int MODE, int J, int NC, int WT, bool DOT>
+// EPI: 0 = r = A f; 1 = the same and the fused dot product f . r (one
+// FP64 partial per workgroup in part); 2 = one Chebyshev step fused into the
+// write, r = zc f - om r + ga dinv (rhs - A f) (r on entry holds the
+// previous iterate, not read when om == 0)
+template <typename T, int MODE, int J, int NC, int WT, int EPI>
 __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict__ f,
                                                 T *__restrict__ r, double *__restrict__ part,
-                                                const double *__restrict__ skip) {
+                                                const double *__restrict__ skip, ChebStep<T> cs) {
+    constexpr bool DOT = EPI == 1;
     if (skip && *skip != 0.0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int RS = rec_stride<T>(NC);
@@ -420,8 +425,15 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
                 const uint32_t vr = v0 + (MODE == 0 ? (uint32_t)(q + 4 * i) : (uint32_t)(4 * q + i));
                 if (n < slm && vr < nP) {
                     const T val = (T)D[t][i];
-                    r[(size_t)vr * slm + n] = val;
-                    if (DOT) d = fma((double)f[(size_t)vr * slm + n], (double)val, d);
+                    const size_t o = (size_t)vr * slm + n;
+                    if constexpr (EPI == 2) {
+                        T nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (cs.rhs[o] - val);
+                        if (cs.om != (T)0) nv -= cs.om * r[o];
+                        r[o] = nv;
+                    } else {
+                        r[o] = val;
+                    }
+                    if (DOT) d = fma((double)f[o], (double)val, d);
                 }
             }
         }
@@ -463,8 +475,8 @@ size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit) {
     return (size_t)(NC + 3) * S * NT * 64 * BW * (mode == 2 && nsplit > 1 ? 2 : 1);
 }
 
-// The kernel instantiation for (J, NC, W) with DOT = D, or null.
-template <typename T, int MODE, bool D>
+// The kernel instantiation for (J, NC, W) with epilogue D, or null.
+template <typename T, int MODE, int D>
 static const void *apply4_kernel(int J, int NC, uint32_t W) {
 #define TR_K4(JJ, NCC, WW) reinterpret_cast<const void *>(&k_apply4<T, MODE, JJ, NCC, WW, D>)
 #define TR_K4W(JJ, NCC) (W == 15 ? TR_K4(JJ, NCC, 15) : TR_K4(JJ, NCC, 16))
@@ -485,10 +497,16 @@ static const void *apply4_kernel(int J, int NC, uint32_t W) {
 // (occupancy x CUs), rounded down to a multiple of the 8 XCDs, at most one
 // per tile.
 template <typename T, int MODE>
-uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, bool dot) {
+static const void *apply4_kernel_epi(int epi, uint32_t slm, int NC, uint32_t W) {
+    const int J = mfma_J(slm, (int)sizeof(T));
+    return epi == 1 ? apply4_kernel<T, MODE, 1>(J, NC, W)
+                    : (epi == 2 ? apply4_kernel<T, MODE, 2>(J, NC, W) : apply4_kernel<T, MODE, 0>(J, NC, W));
+}
+
+template <typename T, int MODE>
+uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, int epi) {
     const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
-    const void *k = dot ? apply4_kernel<T, MODE, true>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W)
-                        : apply4_kernel<T, MODE, false>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W);
+    const void *k = apply4_kernel_epi<T, MODE>(epi, a.slm, NC, a.W);
     if (!k || ntiles == 0) return ntiles;
     const size_t sh = apply4_lds_bytes(MODE, NC, a.slm, a.nsplit);
     int nb = 0, dev = 0, cus = 0;
@@ -505,7 +523,7 @@ uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, bool dot) {
 
 template <typename T, int MODE>
 hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, const T *f, T *r, double *part,
-                         const double *skip, hipStream_t s) {
+                         const double *skip, hipStream_t s, const ChebStep<T> *cheb) {
     if (a.nP == 0) return hipSuccess;
     const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
     // the unrolled slot loops read exactly W = 15 or 16 volume slots and at
@@ -516,22 +534,23 @@ hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, con
         return hipErrorInvalidValue;
     dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
     const size_t sh = apply4_lds_bytes(MODE, NC, a.slm, a.nsplit);
-    const void *k = part ? apply4_kernel<T, MODE, true>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W)
-                         : apply4_kernel<T, MODE, false>(mfma_J(a.slm, (int)sizeof(T)), NC, a.W);
+    if (cheb && (part || !cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
+    const void *k = apply4_kernel_epi<T, MODE>(cheb ? 2 : (part ? 1 : 0), a.slm, NC, a.W);
     if (!k) return hipErrorInvalidValue;
     OpArgs<T> aa = a;
-    void *args[] = {&aa, &f, &r, &part, &skip};
+    ChebStep<T> cs = cheb ? *cheb : ChebStep<T>{nullptr, nullptr, (T)0, (T)0, (T)0};
+    void *args[] = {&aa, &f, &r, &part, &skip, &cs};
     return hipLaunchKernel(k, g, b, args, sh, s);
 }
 
 template hipError_t launch_apply4<double, 0>(const OpArgs<double> &, int, int, uint32_t, const double *, double *,
-                                             double *, const double *, hipStream_t);
+                                             double *, const double *, hipStream_t, const ChebStep<double> *);
 template hipError_t launch_apply4<float, 1>(const OpArgs<float> &, int, int, uint32_t, const float *, float *,
-                                            double *, const double *, hipStream_t);
+                                            double *, const double *, hipStream_t, const ChebStep<float> *);
 template hipError_t launch_apply4<float, 2>(const OpArgs<float> &, int, int, uint32_t, const float *, float *,
-                                            double *, const double *, hipStream_t);
-template uint32_t apply4_grid<double, 0>(const OpArgs<double> &, int, int, bool);
-template uint32_t apply4_grid<float, 1>(const OpArgs<float> &, int, int, bool);
-template uint32_t apply4_grid<float, 2>(const OpArgs<float> &, int, int, bool);
+                                            double *, const double *, hipStream_t, const ChebStep<float> *);
+template uint32_t apply4_grid<double, 0>(const OpArgs<double> &, int, int, int);
+template uint32_t apply4_grid<float, 1>(const OpArgs<float> &, int, int, int);
+template uint32_t apply4_grid<float, 2>(const OpArgs<float> &, int, int, int);
 
 }  // namespace tr

@@ -132,7 +132,7 @@ static hipError_t vcycle(Multilevel &mg, size_t k, const T *r, T *z, const doubl
 }
 
 // Largest eigenvalue of D^-1 A by power iteration (FP64, 20 steps).
-static int lambda_max(tr_ctx *c, const double *dinv, double *x, double *y, double &lam) {
+int lambda_max(tr_ctx *c, const double *dinv, double *x, double *y, double &lam) {
     const size_t n = c->nint();
     std::vector<double> h(n);
     for (size_t i = 0; i < n; i++) h[i] = 0.5 + rng_uniform(11, 7, i);
