@@ -186,6 +186,8 @@ int tr_ctx_schedule(tr_ctx *ctx, int precision, int *out5);
  * staging with buffer loads and per-slot coefficient records, 4 = angular
  * contraction on the matrix cores, J ignored). */
 int tr_ctx_set_schedule(tr_ctx *ctx, int precision, int J, int WPB, int variant);
+/* (variant 5: the matrix-core schedule for 33..80 harmonics, FP32 operator
+ * with a BF16 angular precision only, J ignored.) */
 /* Multilevel preconditioner (use_prec 3 of the device solvers, extension):
  * Galerkin coarse operators of the same Kronecker form under aggregation of
  * `agg` consecutive vertices, coarsened until at most `coarsest` vertices
@@ -201,7 +203,9 @@ int tr_ctx_set_chebyshev(tr_ctx *ctx, int degree, double ratio);
 /* Angular precision of the FP32 operator used by TR_SOLVER_MIXED:
  * 0 = FP32 throughout, 1 = BF16 matrix-core projection with FP32
  * accumulation, 3 (default) = BF16 with the hi/lo split (three BF16
- * products per term, ~16 significant bits). */
+ * products per term, ~16 significant bits; for 33..80 harmonics, whose
+ * schedule keeps the couplings in BF16 only, the gathered values alone are
+ * split: two products per term). */
 int tr_ctx_set_angular_precision(tr_ctx *ctx, int prec);
 /* Returns the internal numbering: perm[new] = caller vertex id. */
 int tr_ctx_permutation(const tr_ctx *ctx, uint32_t *perm);

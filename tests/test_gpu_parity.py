@@ -154,6 +154,38 @@ def test_bf16_angular_projection(name):
         ctx.deviceFree(d_r)
 
 
+@pytest.mark.parametrize("name", ["ball16-K5", "ball10-K5-hg", "ball12-K4"])
+def test_bf16_blocked_projection(name):
+    """The FP32 operator for 33..80 harmonics with the BF16 matrix-core
+    projection in blocks of 16 harmonics (variant 5) against the FP64
+    oracle.  Couplings in BF16 (8 significant bits) and the gathered values
+    plain (bar 2e-2 relative to max|A f|) or split into BF16 hi + lo (the
+    couplings' rounding remains: bar 1e-2); every workgroup size."""
+    if name in cases():
+        P, ctx, op = setup(name)
+    else:
+        P = ta.problems.ball(10, maxk=5, kind="hg") if name == "ball10-K5-hg" else ta.problems.ball(12, maxk=4)
+        ctx = ta.HipContext(P.mesh, P.ang)
+        op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    f = rvec(op, 7)
+    d_f, d_r = ctx.vector_from(f), ctx.allocVector()
+    want = op.lhs(f)
+    try:
+        for prec, bar in ((1, 2e-2), (3, 1e-2)):
+            ctx.set_angular_precision(prec)
+            for wpb in (2, 4, 8):
+                ctx.set_schedule(1, J=0, WPB=wpb, variant=5)
+                ctx.computeLhsPrec(1, d_f, d_r)
+                e = rel(ctx.copyToHost(d_r), want)
+                print("[gpu-parity] %s bf16-blocked prec=%d WPB=%d rel err %.2e (%s)" % (
+                    name, prec, wpb, e, ctx.kernel_name(1)), flush=True)
+                assert e <= bar
+    finally:
+        ctx.set_angular_precision(3)
+        ctx.deviceFree(d_f)
+        ctx.deviceFree(d_r)
+
+
 def test_copies_blas_reductions():
     P, ctx, op = setup("slab12-K1-iso")
     x, y = rvec(op, 2), rvec(op, 3)

@@ -25,10 +25,12 @@ def build(name):
         return ta.problems.cube(8)
     if name == "ball12-hg":
         return ta.problems.ball(12, maxk=2, kind="hg")
+    if name == "ball10-K5-hg":
+        return ta.problems.ball(10, maxk=5, kind="hg")
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("name", ["slab8", "ball16", "cube8", "ball12-hg"])
+@pytest.mark.parametrize("name", ["slab8", "ball16", "cube8", "ball12-hg", "ball10-K5-hg"])
 @pytest.mark.parametrize("solver,prec", [(ta.TR_SOLVER_PCG_FP64, 0), (ta.TR_SOLVER_PCG_FP64, 2),
                                          (ta.TR_SOLVER_PCG_FP64, 3), (ta.TR_SOLVER_PCG_FP64, 4),
                                          (ta.TR_SOLVER_MIXED, 2), (ta.TR_SOLVER_MIXED, 3),

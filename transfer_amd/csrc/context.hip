@@ -89,6 +89,63 @@ static bool v4_ok(const HostOperator &op) {
            (op.NC == 7 || op.NC == 10);
 }
 
+// Whether the K-blocked matrix-core schedule (variant 5, FP32 operator)
+// covers this operator: 33..80 harmonics, 15 or 16 volume slots.
+static bool v5_ok(const HostOperator &op) {
+    const double lim = 4294967295.0;
+    if ((double)op.nP * op.slm * 4 >= lim || (double)op.W * op.nP * 4 >= lim ||
+        (double)op.W * op.nP * rec_stride<float>((int)op.NC) * 4 >= lim || (double)op.WB * op.nB * 4 * 4 >= lim)
+        return false;
+    return mfma5_supported(op.slm) && (op.W == 15 || op.W == 16) && op.WB <= TR_WBMAX &&
+           (op.NC == 7 || op.NC == 10);
+}
+
+static uint16_t to_bf16(float x);
+
+// B blocks of variant 5: [section][harmonic block][channel][tile][lane], 4
+// BF16 per lane (K index 4q + u of the block, output column 16t + l % 16)
+static std::vector<unsigned char> mfma_tables5(const HostOperator &op) {
+    const uint32_t slm = op.slm, NT = (slm + 15) / 16;
+    auto mat = [&](int sec, int c, uint32_t n, uint32_t m) -> double {
+        if (n >= slm || m >= slm) return 0.0;
+        if (sec == 1) return op.onc[((size_t)c * slm + n) * slm + m];
+        if (c == 0) return n == m ? op.massfac[n] : 0.0;
+        return op.omc[((size_t)(c - 1) * slm + n) * slm + m];
+    };
+    std::vector<unsigned char> out;
+    for (int sec = 0; sec < 2; sec++) {
+        const int nch = sec == 0 ? (int)op.NC : 3;
+        std::vector<unsigned char> t((size_t)NT * nch * NT * 64 * 8, 0);
+        for (uint32_t kb = 0; kb < NT; kb++)
+            for (int c = 0; c < nch; c++)
+                for (uint32_t tt = 0; tt < NT; tt++)
+                    for (int l = 0; l < 64; l++) {
+                        const uint32_t n = 16u * tt + (uint32_t)(l % 16), q = (uint32_t)(l / 16);
+                        uint16_t h[4];
+                        for (int u = 0; u < 4; u++) h[u] = to_bf16((float)mat(sec, c, n, 16u * kb + 4u * q + (uint32_t)u));
+                        std::memcpy(&t[((((size_t)kb * nch + c) * NT + tt) * 64 + l) * 8], h, 8);
+                    }
+        out.insert(out.end(), t.begin(), t.end());
+    }
+    return out;
+}
+
+// The K-blocked matrix-core schedule with its persistent grids.
+static LaneMap mfma5_map_ctx(const tr_ctx *c, int wpb) {
+    LaneMap m;
+    m.variant = 5;
+    m.WPB = wpb;
+    m.J = 4;
+    m.L = 4;
+    m.VPW = 16;
+    m.mfma_mode = 2;
+    const int NC = (int)c->op.NC;
+    m.grid = apply5_grid(c->args32(m), NC, wpb, 0);
+    m.grid_dot = apply5_grid(c->args32(m), NC, wpb, 1);
+    m.grid_cheb = apply5_grid(c->args32(m), NC, wpb, 2);
+    return m;
+}
+
 // The matrix-core schedule of this context with its persistent grid sized
 // from the kernel's occupancy.
 static LaneMap mfma_map_ctx(const tr_ctx *c, int wpb, int mode) {
@@ -108,6 +165,7 @@ static LaneMap mfma_map_ctx(const tr_ctx *c, int wpb, int mode) {
 }
 
 static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem, int nsplit = 3) {
+    if (m.variant == 5) return apply5_lds_bytes((int)op.NC, op.slm) + 64;
     if (m.variant == 4) return apply4_lds_bytes(elem == 8 ? 0 : m.mfma_mode, (int)op.NC, op.slm, nsplit) + 64;
     if (m.variant == 3) return apply3_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem);
     return m.variant == 2 ? apply2_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem)
@@ -922,6 +980,8 @@ int ensure_fp32(tr_ctx *c) {
     c->map32 = choose_map(op.slm, 8);
     if (c->ang_prec != 0 && v4_ok(op))
         c->map32 = mfma_map_ctx(c, 4, 2);
+    else if (c->ang_prec != 0 && v5_ok(op))
+        c->map32 = mfma5_map_ctx(c, 8);
     if (op.W > 16) c->map32.variant = 1;
     if ((double)op.W * op.nP * rec_stride<float>((int)op.NC) * 4 >= 2147483648.0) c->map32.variant = 2;
     int maxlds = 65536;
@@ -936,6 +996,7 @@ int ensure_fp32(tr_ctx *c) {
         (!op.mf0.empty() && (rc = upload(c, &c->f_mf0, to_f32(op.mf0)))) ||
         (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
         return rc;
+    if (v5_ok(op) && (rc = upload(c, (unsigned char **)&c->f_tab5, mfma_tables5(op)))) return rc;
     if (mfma_supported(op.slm, 1) &&
         ((rc = upload(c, (unsigned char **)&c->f_tab4, mfma_tables(op, 1, 1))) ||
          (rc = upload(c, (unsigned char **)&c->f_tab4b1, mfma_tables(op, 2, 1))) ||
@@ -1163,8 +1224,10 @@ extern "C" int tr_ctx_kernel_name(tr_ctx *c, int precision, char *buf, size_t ca
     const int NC = (int)c->op.NC, W = (int)c->op.W;
     std::string name;
     char tmp[160];
-    if (m.variant == 4)
-        snprintf(tmp, sizeof tmp, "k_apply4<%s, %d, %d, %d, %d, false>", T, precision ? m.mfma_mode : 0, m.J, NC, W);
+    if (m.variant == 5)
+        snprintf(tmp, sizeof tmp, "k_apply5<%d, %d, %d, 0>", NC, W, (int)((c->op.slm + 15) / 16));
+    else if (m.variant == 4)
+        snprintf(tmp, sizeof tmp, "k_apply4<%s, %d, %d, %d, %d, 0>", T, precision ? m.mfma_mode : 0, m.J, NC, W);
     else if (m.variant == 3 && (W == 15 || W == 16))
         snprintf(tmp, sizeof tmp, "k_apply3<%s, %d, %d, %d, false>", T, m.J, NC, W);
     else if (m.variant >= 2 && W <= TR_WMAX)
@@ -1248,7 +1311,7 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
     // exact schedules (FP64, or FP32 with ang_prec 0) race the VALU variants
     // against the FP64/FP32 MFMA one.
     std::vector<LaneMap> cands;
-    const bool bf16 = f32 && c->ang_prec != 0 && v4_ok(op);
+    const bool bf16 = f32 && c->ang_prec != 0 && (v4_ok(op) || v5_ok(op));
     if (!bf16)
         for (LaneMap m : map_candidates(op.slm, 11))
             for (int variant : {1, 2, 3})
@@ -1263,6 +1326,8 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
             LaneMap m = mfma_map_ctx(c, wpb, f32 ? (bf16 ? 2 : 1) : 0);
             cands.push_back(m);
         }
+    if (bf16 && v5_ok(op))
+        for (int wpb : {2, 4, 8}) cands.push_back(mfma5_map_ctx(c, wpb));
     {
         for (LaneMap m : cands) {
             {
@@ -1351,6 +1416,12 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
         if (rc) return rc;
     }
     const HostOperator &op = c->op;
+    if (variant == 5) {
+        if (precision != 1 || c->ang_prec == 0 || !v5_ok(op) || (WPB != 1 && WPB != 2 && WPB != 4 && WPB != 8))
+            return fail(TR_ERR_ARG, "tr_ctx_set_schedule: K-blocked matrix-core schedule unsupported here");
+        c->map32 = mfma5_map_ctx(c, WPB);
+        return TR_OK;
+    }
     if (variant == 4) {
         if (!v4_ok(op) || (WPB != 1 && WPB != 2 && WPB != 4 && WPB != 8))
             return fail(TR_ERR_ARG, "tr_ctx_set_schedule: matrix-core schedule unsupported here");
@@ -1446,6 +1517,8 @@ extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
         const HostOperator &op = c->op;
         if (prec != 0 && v4_ok(op))
             c->map32 = mfma_map_ctx(c, 4, 2);
+        else if (prec != 0 && v5_ok(op))
+            c->map32 = mfma5_map_ctx(c, 8);
         else {
             c->map32 = choose_map(op.slm, 8);
             if (op.W > 16) c->map32.variant = 1;

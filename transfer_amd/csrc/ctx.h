@@ -59,6 +59,7 @@ struct tr_ctx {
     float *f_crec = nullptr, *f_bcrec = nullptr;
     void *f_tab = nullptr;
     void *f_tab4 = nullptr, *f_tab4b1 = nullptr, *f_tab4b3 = nullptr;  // variant 4: FP32, BF16, BF16 hi/lo
+    void *f_tab5 = nullptr;  // variant 5: BF16 B blocks (33..80 harmonics)
     int ang_prec = 3;  // FP32 operator's angular contraction: 0 FP32, 1 BF16, 3 BF16 hi/lo split
     std::set<void *> vectors, raw;
     // batches: internal vertex range of problem p = voff[p] .. voff[p+1]
@@ -135,8 +136,11 @@ struct tr_ctx {
         a.bval = f_bval;
         a.bdiag = f_bdiag;
         a.crec = f_crec;
-        a.tab = map32.variant != 4 ? f_tab : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
-        a.nsplit = ang_prec == 1 ? 1 : 3;
+        a.tab = map32.variant == 5 ? f_tab5
+                : map32.variant != 4 ? f_tab
+                                     : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
+        // variant 5 keeps B in BF16 only: the A operand split at most
+        a.nsplit = ang_prec == 1 ? 1 : (map32.variant == 5 ? 2 : 3);
         a.bcrec = f_bcrec;
         a.mask = d.mask;
         a.nmask = d.nmask;

@@ -21,7 +21,8 @@ struct LaneMap {
     int J = 1, L = 1, VPW = 1, WPB = 4;
     int variant = 3;  // 1: stage every vertex of a wavefront at once; 2: batched (k_apply2);
                       // 3: batched with buffer loads and coefficient records (k_apply3);
-                      // 4: angular contraction on the matrix cores (k_apply4)
+                      // 4: angular contraction on the matrix cores (k_apply4);
+                      // 5: the same in blocks of 16 harmonics, FP32/BF16 (k_apply5)
     int mfma_mode = 1;  // FP32 operator under variant 4: 1 = FP32 MFMA, 2 = BF16 MFMA
     uint32_t grid = 0;      // variant 4: workgroups of the persistent grid (0 = one per tile)
     uint32_t grid_dot = 0;  // the same for the launch with the fused dot product
@@ -135,6 +136,14 @@ size_t apply4_lds_bytes(int mode, int NC, uint32_t slm, int nsplit);
 template <typename T, int MODE>
 hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, const T *f, T *r, double *part,
                          const double *skip, hipStream_t s, const ChebStep<T> *cheb = nullptr);
+// variant 5 (kernels_mfma.hip): FP32 operator for 33..80 harmonics, BF16
+// matrix-core projection in blocks of 16 harmonics; tab = B blocks
+// [block][channel][tile][lane] (volume, then boundary), nsplit 1 or 2
+bool mfma5_supported(uint32_t slm);
+size_t apply5_lds_bytes(int NC, uint32_t slm);
+uint32_t apply5_grid(const OpArgs<float> &a, int NC, int wpb, int epi);
+hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid, const float *f, float *r,
+                         double *part, const double *skip, hipStream_t s, const ChebStep<float> *cheb = nullptr);
 // workgroups of the persistent variant-4 grid for this operator and WPB,
 // for the launch with epilogue epi (0 plain, 1 fused dot product, 2
 // Chebyshev step)

@@ -560,7 +560,7 @@ uint32_t apply_tiles(uint32_t nP, const LaneMap &m) {
 
 uint32_t apply_blocks(uint32_t nP, const LaneMap &m) {
     const uint32_t tiles = apply_tiles(nP, m);
-    return (m.variant == 4 && m.grid_dot) ? std::min(m.grid_dot, tiles) : tiles;
+    return ((m.variant == 4 || m.variant == 5) && m.grid_dot) ? std::min(m.grid_dot, tiles) : tiles;
 }
 
 // Specular reflection: the listed entries of r vanish (see stencil.h).
@@ -585,6 +585,14 @@ template <typename T>
 hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                         double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
+    if (m.variant == 5) {
+        if constexpr (sizeof(T) == 8) {
+            return hipErrorInvalidValue;
+        } else {
+            hipError_t e = launch_apply5(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+            return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
+        }
+    }
     if (m.variant == 4) {
         hipError_t e;
         if constexpr (sizeof(T) == 8)
@@ -651,6 +659,14 @@ hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const 
                             const ChebStep<T> &cs, T *tmp, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
     hipError_t e;
+    if (m.variant == 5) {
+        if constexpr (sizeof(T) == 8) {
+            return hipErrorInvalidValue;
+        } else {
+            e = launch_apply5(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+            return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
+        }
+    }
     if (m.variant == 4) {
         if constexpr (sizeof(T) == 8)
             e = launch_apply4<T, 0>(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);

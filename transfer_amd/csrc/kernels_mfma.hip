@@ -91,7 +91,8 @@ typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 
 // One channel group of the angular contraction: D[t] += A_c B_c over the
 // NCH channels of acc (lane layout as above), B slices at B (hi) and B + lo
-// (lo part of the BF16 split).
+// (lo part of the BF16 split).  BF16 nsplit: 1 = A_hi B_hi, 2 = + A_lo B_hi
+// (the A operand split only), 3 = + A_hi B_lo.
 template <typename T, int MODE, int J, int NCH, typename ACC4, int NT>
 __device__ __forceinline__ void contract(const T (&acc)[NCH][J], const unsigned char *B, size_t lo, int lane,
                                          int nsplit, ACC4 (&D)[NT]) {
@@ -122,9 +123,11 @@ __device__ __forceinline__ void contract(const T (&acc)[NCH][J], const unsigned 
                     D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, D[t], 0, 0, 0);
                     if (nsplit > 1) {
                         const s4v al = __builtin_bit_cast(s4v, __builtin_convertvector(x - __builtin_convertvector(hb, f4v), bf4v));
-                        const s4v bl = *reinterpret_cast<const s4v *>(B + lo + bo);
                         D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, D[t], 0, 0, 0);
-                        D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
+                        if (nsplit > 2) {
+                            const s4v bl = *reinterpret_cast<const s4v *>(B + lo + bo);
+                            D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
+                        }
                     }
                 }
             }
@@ -187,10 +190,12 @@ __device__ inline double quad_bcast(double x, int src) {
 // first and sinks all FMAs below them, which holds NS slots of data live and
 // spills; empty volatile asm statements (which keep their relative order)
 // pin each slot's FMAs before the next load offsets are formed.
+// hoff / h0: byte and harmonic offset of the harmonic block gathered (the
+// K-blocked schedule of kernels with more than 32 harmonics).
 template <typename T, int J, int NCH, int RS, int NS, int DEPTH>
 __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
                                        __amdgpu_buffer_rsrc_t crs, uint32_t cstride, uint32_t coff, int nslot,
-                                       uint32_t slm, int q) {
+                                       uint32_t slm, int q, uint32_t hoff = 0, uint32_t h0 = 0) {
     constexpr int E = 16 / (int)sizeof(T), PE = (RS + 3) / 4;
     // opaque here, so that the per-slot products and tests are formed where
     // used instead of hoisted out of the tile loop (kept live, they spill)
@@ -198,7 +203,7 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
     slm = opaque_uniform(slm);
     T xs[DEPTH][J], cs[DEPTH][PE];
     auto load = [&](int s, T *x, T *cf) __attribute__((always_inline)) {
-        uint32_t base = nb[s] * slm * (uint32_t)sizeof(T) + 16u * (uint32_t)q;
+        uint32_t base = nb[s] * slm * (uint32_t)sizeof(T) + 16u * (uint32_t)q + hoff;
         uint32_t cv = coff + (uint32_t)(q * PE * (int)sizeof(T)) + (uint32_t)s * cstride;
         asm volatile("" : "+v"(base), "+v"(cv));
 #pragma unroll
@@ -233,7 +238,7 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
     // zero them
 #pragma unroll
     for (int j = 0; j < J; j++)
-        if (mfma_harm(E, q, j) >= slm)
+        if (mfma_harm(E, q, j) + h0 >= slm)
 #pragma unroll
             for (int c = 0; c < NCH; c++) acc[c][j] = 0;
 }
@@ -453,6 +458,219 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
             part[blockIdx.x] = t;
         }
     }
+}
+
+// Variant 5: the FP32 operator for 33 to 80 harmonics (K = 4, 5) with the
+// angular contraction on the matrix cores in BF16.  At these orders one
+// lane cannot hold a quarter of every channel's row (10 channels x 20
+// harmonics), so the harmonics go through the kernel in NT blocks of 16,
+// the K dimension of v_mfma_f32_16x16x16bf16: per block the ELL gather of
+// one 16-byte piece per lane (a vertex's four lanes read a 64-byte run of
+// each neighbour row), the move to the MFMA layout, and D[t] += A_c B_c[kb][t]
+// for every channel c and output tile t, the D accumulators covering all
+// ceil(slm/16) output tiles throughout.  B blocks are laid out
+// [block][channel][tile][lane] (8 bytes per lane); the volume blocks
+// (NC x NT x NT x 512 bytes, 128 KB at NC = 10, NT = 5) are staged in LDS
+// once per workgroup of the persistent grid, the boundary blocks are read
+// from global memory by the wavefronts that hold boundary vertices.  The
+// B values are BF16; nsplit 2 splits the A operand into BF16 hi + lo
+// (a.nsplit: 1 or 2).  EPI as k_apply4.
+template <int NC, int WT, int NT, int EPI>
+__global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__restrict__ f,
+                                                float *__restrict__ r, double *__restrict__ part,
+                                                const double *__restrict__ skip, ChebStep<float> cs) {
+    constexpr bool DOT = EPI == 1;
+    if (skip && *skip != 0.0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int RS = rec_stride<float>(NC);
+    constexpr int J = 4;
+    constexpr size_t BLK = (size_t)TR_WAVE * 8;               // one 16 x 16 BF16 B block
+    constexpr size_t VSEC = (size_t)NT * NC * NT * BLK;        // volume blocks, all harmonic blocks
+    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
+    const int wpb = blockDim.x / TR_WAVE;
+    const uint32_t slm = a.slm, nP = a.nP;
+    const int nsplit = a.nsplit;
+    {
+        const i32x4m *src = reinterpret_cast<const i32x4m *>(a.tab);
+        i32x4m *dst = reinterpret_cast<i32x4m *>(smem_raw);
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(VSEC / 16); i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const unsigned char *gB = reinterpret_cast<const unsigned char *>(a.tab) + VSEC;
+    const auto frs = rsrc4(f, (size_t)nP * slm * sizeof(float));
+    const auto brs = rsrc4(a.bof, (size_t)nP * 4);
+    const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
+    const auto bnrs = rsrc4(a.bnbr, (size_t)TR_WBMAX * a.nB * 4);
+    const auto crs = rsrc4(a.crec, (size_t)a.W * nP * RS * sizeof(float));
+    const auto bcrs = rsrc4(a.bcrec, (size_t)TR_WBMAX * a.nB * 4 * sizeof(float));
+    const int vloc = lane & 15, q = lane >> 4;
+    const int gv = lane >> 2, gq = lane & 3;
+    // 2 wavefronts per SIMD (one workgroup of up to 8 per CU: the LDS)
+    constexpr int BUDGET = 256;
+    constexpr int OVH = WT + 4 * NT + 30;
+    constexpr int PE = (RS + 3) / 4;
+    constexpr int ACCR = NC * J, SLOTR = J + PE;
+    constexpr int DEPTH = std::max(2, std::min(6, (BUDGET - ACCR - OVH) / SLOTR));
+    constexpr int DEPTHB = 4;
+
+    const uint32_t ntiles = (nP + 16u * wpb - 1) / (16u * wpb);
+    const uint32_t G = gridDim.x, X = 8;
+    uint32_t t0 = blockIdx.x, t1 = ntiles, tstep = G;
+    if (G % X == 0 && G <= ntiles) {
+        const uint32_t xcd = blockIdx.x % X, per = ntiles / X, rem = ntiles % X;
+        const uint32_t lo = xcd * per + (xcd < rem ? xcd : rem);
+        t0 = lo + blockIdx.x / X;
+        t1 = lo + per + (xcd < rem ? 1u : 0u);
+        tstep = G / X;
+    }
+    const uint32_t nwt = (nP + 15u) / 16u;
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
+    double d = 0;
+    for (uint32_t tile = t0; tile < t1; tile += tstep) {
+        const uint32_t e = tile * (uint32_t)wpb + wv;
+        const uint32_t v0 = 16u * ((a.tiles && e < nwt) ? a.tiles[e] : e);
+        uint32_t v = v0 + gv;
+        asm volatile("" : "+v"(v));
+        const bool valid = v < nP;
+        uint32_t nbc[WT];
+        int b = -1;
+        if (valid) {
+            load_idx<WT>(nbc, nrs, v * 4u, nP * 4u, WT);
+            b = (int)__builtin_amdgcn_raw_buffer_load_b32(brs, v * 4u, 0u, 0);
+        } else {
+#pragma unroll
+            for (int s2 = 0; s2 < WT; s2++) nbc[s2] = 0;
+        }
+        f4v D[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) D[t] = (f4v){0, 0, 0, 0};
+#pragma unroll 1
+        for (int kb = 0; kb < NT; kb++) {
+            float acc[NC][J];
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) acc[c][j] = 0;
+            if (valid)
+                gather<float, J, NC, RS, WT, DEPTH>(acc, nbc, frs, crs, nP * (uint32_t)(RS * sizeof(float)),
+                                                     v * (uint32_t)(RS * sizeof(float)), WT, slm, gq,
+                                                     64u * (uint32_t)kb, 16u * (uint32_t)kb);
+            if (a.mf0 && valid && gq == 0 && kb == 0) acc[0][0] *= a.mf0[v];
+            to_mfma_layout<float, 2, J, NC>(acc, lane);
+            uint32_t boff = (uint32_t)kb * (uint32_t)(NC * NT * BLK);
+            asm volatile("" : "+v"(boff));
+            contract<float, 2, J, NC>(acc, smem_raw + boff, 0, lane, nsplit, D);
+        }
+        if (__any(b >= 0)) {
+            uint32_t bn[TR_WBMAX];
+            if (b >= 0) load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, TR_WBMAX);
+#pragma unroll 1
+            for (int kb = 0; kb < NT; kb++) {
+                float bacc[3][J];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int j = 0; j < J; j++) bacc[c][j] = 0;
+                if (b >= 0)
+                    gather<float, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, bcrs, a.nB * (uint32_t)(4 * sizeof(float)),
+                                                            (uint32_t)b * (uint32_t)(4 * sizeof(float)), TR_WBMAX,
+                                                            slm, gq, 64u * (uint32_t)kb, 16u * (uint32_t)kb);
+                to_mfma_layout<float, 2, J, 3>(bacc, lane);
+                contract<float, 2, J, 3>(bacc, gB + (size_t)kb * 3 * NT * BLK, 0, lane, nsplit, D);
+            }
+        }
+        // ---- write: D rows 4(l/16) + i, column l % 16 of tile t
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const uint32_t n = 16u * t + (uint32_t)vloc;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t vr = v0 + (uint32_t)(4 * q + i);
+                if (n < slm && vr < nP) {
+                    const float val = D[t][i];
+                    const size_t o = (size_t)vr * slm + n;
+                    if constexpr (EPI == 2) {
+                        float nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (cs.rhs[o] - val);
+                        if (cs.om != 0.0f) nv -= cs.om * r[o];
+                        r[o] = nv;
+                    } else {
+                        r[o] = val;
+                    }
+                    if (DOT) d = fma((double)f[o], (double)val, d);
+                }
+            }
+        }
+    }
+    if (DOT) {
+        double *w = reinterpret_cast<double *>(smem_raw);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, TR_WAVE);
+        __syncthreads();
+        if (lane == 0) w[wave] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (int i = 0; i < wpb; i++) t += w[i];
+            part[blockIdx.x] = t;
+        }
+    }
+}
+
+bool mfma5_supported(uint32_t slm) { return slm > 32 && slm <= 80; }
+
+size_t apply5_lds_bytes(int NC, uint32_t slm) {
+    const size_t NT = (slm + 15) / 16;
+    return NT * (size_t)NC * NT * 512;
+}
+
+template <int EPI>
+static const void *apply5_kernel(int NC, uint32_t W, uint32_t slm) {
+    const int NT = (int)((slm + 15) / 16);
+#define TR_K5(NCC, WW, NTT) reinterpret_cast<const void *>(&k_apply5<NCC, WW, NTT, EPI>)
+#define TR_K5T(NCC, WW) (NT == 3 ? TR_K5(NCC, WW, 3) : (NT == 4 ? TR_K5(NCC, WW, 4) : TR_K5(NCC, WW, 5)))
+#define TR_K5W(NCC) (W == 15 ? TR_K5T(NCC, 15) : TR_K5T(NCC, 16))
+    if ((W != 15 && W != 16) || (NC != 7 && NC != 10) || NT < 3 || NT > 5) return nullptr;
+    return NC == 7 ? TR_K5W(7) : TR_K5W(10);
+#undef TR_K5W
+#undef TR_K5T
+#undef TR_K5
+}
+
+static const void *apply5_kernel_epi(int epi, int NC, uint32_t W, uint32_t slm) {
+    return epi == 1 ? apply5_kernel<1>(NC, W, slm) : (epi == 2 ? apply5_kernel<2>(NC, W, slm) : apply5_kernel<0>(NC, W, slm));
+}
+
+uint32_t apply5_grid(const OpArgs<float> &a, int NC, int wpb, int epi) {
+    const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
+    const void *k = apply5_kernel_epi(epi, NC, a.W, a.slm);
+    if (!k || ntiles == 0) return ntiles;
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, TR_WAVE * wpb, apply5_lds_bytes(NC, a.slm)) != hipSuccess ||
+        nb < 1 || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        return ntiles;
+    }
+    uint32_t g = (uint32_t)nb * (uint32_t)cus;
+    if (g >= 8) g -= g % 8;
+    return std::min(g, ntiles);
+}
+
+hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid, const float *f, float *r,
+                         double *part, const double *skip, hipStream_t s, const ChebStep<float> *cheb) {
+    if (a.nP == 0) return hipSuccess;
+    const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
+    if ((a.W != 15 && a.W != 16) || a.WB > TR_WBMAX || !a.crec || !a.tab || (a.nB && !a.bcrec) || wpb < 1 ||
+        wpb > 8 || !mfma5_supported(a.slm) || a.nsplit < 1 || a.nsplit > 2)
+        return hipErrorInvalidValue;
+    if (cheb && (part || !cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
+    const void *k = apply5_kernel_epi(cheb ? 2 : (part ? 1 : 0), NC, a.W, a.slm);
+    if (!k) return hipErrorInvalidValue;
+    dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
+    OpArgs<float> aa = a;
+    ChebStep<float> cs = cheb ? *cheb : ChebStep<float>{nullptr, nullptr, 0.0f, 0.0f, 0.0f};
+    void *args[] = {&aa, &f, &r, &part, &skip, &cs};
+    return hipLaunchKernel(k, g, b, args, apply5_lds_bytes(NC, a.slm), s);
 }
 
 int mfma_J(uint32_t slm, int elem_bytes) {
