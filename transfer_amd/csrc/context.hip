@@ -866,20 +866,36 @@ struct ChebPrec {
             rho = rn;
         }
     }
+    // the buffer that holds the first iterate when the result goes to z
+    T *first_buffer(T *z) const { return (om.size() % 2 == 0) ? z : zb; }
+    // r.z partials the last step writes (matrix-core schedules, at least
+    // one step, at most 1024 workgroups), else 0
+    int nparts() const {
+        const uint32_t p = cheb_parts(a.nP, map);
+        return (!om.empty() && p <= 1024) ? (int)p : 0;
+    }
     hipError_t apply(const T *r, T *z, const double *skip, hipStream_t st) const {
-        const int steps = (int)om.size();
-        T *cur = (steps % 2 == 0) ? z : zb, *oth = (steps % 2 == 0) ? zb : z;
         const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_cheb_first<T>, dim3(g), dim3(256), 0, st, r, dinv, (T)inv_theta, cur, n, skip);
+        hipLaunchKernelGGL(k_cheb_first<T>, dim3(g), dim3(256), 0, st, r, dinv, (T)inv_theta, first_buffer(z), n,
+                           skip);
         hipError_t e = hipGetLastError();
-        for (int k = 0; k < steps && e == hipSuccess; k++) {
+        return e != hipSuccess ? e : steps(r, z, skip, st, nullptr);
+    }
+    // the steps after the first iterate (already in first_buffer(z)); the
+    // last one also writes nparts() r.z partials to rzpart when given
+    hipError_t steps(const T *r, T *z, const double *skip, hipStream_t st, double *rzpart) const {
+        const int ns = (int)om.size();
+        T *cur = first_buffer(z), *oth = cur == z ? zb : z;
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < ns && e == hipSuccess; k++) {
             ChebStep<T> cs;
             cs.rhs = r;
             cs.dinv = dinv;
             cs.zc = (T)(1.0 + om[k]);
             cs.om = k == 0 ? (T)0 : (T)om[k];  // z_0 = 0
             cs.ga = (T)ga[k];
-            e = launch_cheb_step<T>(a, NC, map, cur, oth, cs, tmp, skip, st);
+            e = launch_cheb_step<T>(a, NC, map, cur, oth, cs, tmp, skip, st,
+                                    (k == ns - 1 && rzpart && nparts()) ? rzpart : nullptr);
             std::swap(cur, oth);
         }
         return e;
@@ -934,6 +950,13 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
         if (!S.z) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
         const hipStream_t cs = c->stream;
         S.prec = [&cheb, cs](const double *rr, double *z, const double *skip) { return cheb.apply(rr, z, skip, cs); };
+        S.prec_loop = [&cheb, cs](const double *rr, double *z, const double *skip, double *rzp) {
+            return cheb.steps(rr, z, skip, cs, rzp);
+        };
+        S.pre_out = cheb.first_buffer(S.z);
+        S.pre_dinv = dinv;
+        S.pre_scale = cheb.inv_theta;
+        S.prec_nparts = cheb.nparts();
     }
     S.a = c->args();
     S.NC = (int)c->op.NC;
@@ -1065,6 +1088,13 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         S.z = v32 + 6 * n;
         const hipStream_t cs = c->stream;
         S.prec = [&cheb, cs](const float *rr, float *z, const double *skip) { return cheb.apply(rr, z, skip, cs); };
+        S.prec_loop = [&cheb, cs](const float *rr, float *z, const double *skip, double *rzp) {
+            return cheb.steps(rr, z, skip, cs, rzp);
+        };
+        S.pre_out = cheb.first_buffer(S.z);
+        S.pre_dinv = dinv32;
+        S.pre_scale = (float)cheb.inv_theta;
+        S.prec_nparts = cheb.nparts();
     }
     S.a = c->args32();
     S.NC = (int)c->op.NC;

@@ -86,10 +86,15 @@ template <typename T>
 hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                         double *part, const double *skip, hipStream_t s);
 // One Chebyshev step (ChebStep): fused into the operator's write on the
-// matrix-core schedule, else the operator into tmp and an elementwise pass.
+// matrix-core schedules, else the operator into tmp and an elementwise pass.
+// With part != null (matrix-core schedules only) each workgroup of the
+// persistent grid (cheb_parts of them) also writes sum rhs . out (FP64).
 template <typename T>
 hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const T *z, T *out,
-                            const ChebStep<T> &cs, T *tmp, const double *skip, hipStream_t s);
+                            const ChebStep<T> &cs, T *tmp, const double *skip, hipStream_t s,
+                            double *part = nullptr);
+// partials written by launch_cheb_step with part, 0 where it cannot fuse them
+uint32_t cheb_parts(uint32_t nP, const LaneMap &m);
 // r[mask[i]] = 0 for the entries that specular reflection removes (no-op
 // without reflecting planes)
 template <typename T>

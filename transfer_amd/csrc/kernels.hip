@@ -654,29 +654,37 @@ __global__ __launch_bounds__(256) void k_cheb_upd(const T *__restrict__ z, T *__
     }
 }
 
+uint32_t cheb_parts(uint32_t nP, const LaneMap &m) {
+    if (m.variant != 4 && m.variant != 5) return 0;
+    const uint32_t per = 16u * (uint32_t)m.WPB, ntiles = (nP + per - 1) / per;
+    return m.grid_cheb ? std::min(m.grid_cheb, ntiles) : ntiles;
+}
+
 template <typename T>
 hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const T *z, T *out,
-                            const ChebStep<T> &cs, T *tmp, const double *skip, hipStream_t s) {
+                            const ChebStep<T> &cs, T *tmp, const double *skip, hipStream_t s, double *part) {
     if (a.nP == 0) return hipSuccess;
     hipError_t e;
+    // (a fused dot product also covers masked entries: rhs, a CG residual,
+    // is zero there)
     if (m.variant == 5) {
         if constexpr (sizeof(T) == 8) {
             return hipErrorInvalidValue;
         } else {
-            e = launch_apply5(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+            e = launch_apply5(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
             return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
         }
     }
     if (m.variant == 4) {
         if constexpr (sizeof(T) == 8)
-            e = launch_apply4<T, 0>(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+            e = launch_apply4<T, 0>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
         else if (m.mfma_mode == 2)
-            e = launch_apply4<T, 2>(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+            e = launch_apply4<T, 2>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
         else
-            e = launch_apply4<T, 1>(a, NC, m.WPB, m.grid_cheb, z, out, nullptr, skip, s, &cs);
+            e = launch_apply4<T, 1>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
         return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
     }
-    if (!tmp) return hipErrorInvalidValue;
+    if (!tmp || part) return hipErrorInvalidValue;
     if ((e = launch_apply<T>(a, NC, m, z, tmp, nullptr, skip, s)) != hipSuccess) return e;
     const size_t n = (size_t)a.nP * a.slm;
     const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
@@ -706,9 +714,10 @@ hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const
 }
 
 template hipError_t launch_cheb_step<double>(const OpArgs<double> &, int, const LaneMap &, const double *, double *,
-                                             const ChebStep<double> &, double *, const double *, hipStream_t);
+                                             const ChebStep<double> &, double *, const double *, hipStream_t,
+                                             double *);
 template hipError_t launch_cheb_step<float>(const OpArgs<float> &, int, const LaneMap &, const float *, float *,
-                                            const ChebStep<float> &, float *, const double *, hipStream_t);
+                                            const ChebStep<float> &, float *, const double *, hipStream_t, double *);
 template hipError_t launch_zero_mask<double>(const OpArgs<double> &, double *, const double *, hipStream_t);
 template hipError_t launch_zero_mask<float>(const OpArgs<float> &, float *, const double *, hipStream_t);
 template hipError_t launch_apply<double>(const OpArgs<double> &, int, const LaneMap &, const double *,

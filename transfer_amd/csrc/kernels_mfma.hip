@@ -284,12 +284,13 @@ __device__ __forceinline__ void to_mfma_layout(T (&acc)[NCH][J], int lane) {
 // EPI: 0 = r = A f; 1 = the same and the fused dot product f . r (one
 // FP64 partial per workgroup in part); 2 = one Chebyshev step fused into the
 // write, r = zc f - om r + ga dinv (rhs - A f) (r on entry holds the
-// previous iterate, not read when om == 0)
+// previous iterate, not read when om == 0); 3 = the Chebyshev step and the
+// dot product rhs . r of its result
 template <typename T, int MODE, int J, int NC, int WT, int EPI>
 __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict__ f,
                                                 T *__restrict__ r, double *__restrict__ part,
                                                 const double *__restrict__ skip, ChebStep<T> cs) {
-    constexpr bool DOT = EPI == 1;
+    constexpr bool DOT = EPI == 1 || EPI == 3;
     if (skip && *skip != 0.0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int RS = rec_stride<T>(NC);
@@ -431,14 +432,16 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
                 if (n < slm && vr < nP) {
                     const T val = (T)D[t][i];
                     const size_t o = (size_t)vr * slm + n;
-                    if constexpr (EPI == 2) {
-                        T nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (cs.rhs[o] - val);
+                    if constexpr (EPI >= 2) {
+                        const T rh = cs.rhs[o];
+                        T nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (rh - val);
                         if (cs.om != (T)0) nv -= cs.om * r[o];
                         r[o] = nv;
+                        if (EPI == 3) d = fma((double)rh, (double)nv, d);
                     } else {
                         r[o] = val;
+                        if (DOT) d = fma((double)f[o], (double)val, d);
                     }
-                    if (DOT) d = fma((double)f[o], (double)val, d);
                 }
             }
         }
@@ -479,7 +482,7 @@ template <int NC, int WT, int NT, int EPI>
 __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__restrict__ f,
                                                 float *__restrict__ r, double *__restrict__ part,
                                                 const double *__restrict__ skip, ChebStep<float> cs) {
-    constexpr bool DOT = EPI == 1;
+    constexpr bool DOT = EPI == 1 || EPI == 3;
     if (skip && *skip != 0.0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int RS = rec_stride<float>(NC);
@@ -589,14 +592,16 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
                 if (n < slm && vr < nP) {
                     const float val = D[t][i];
                     const size_t o = (size_t)vr * slm + n;
-                    if constexpr (EPI == 2) {
-                        float nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (cs.rhs[o] - val);
+                    if constexpr (EPI >= 2) {
+                        const float rh = cs.rhs[o];
+                        float nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (rh - val);
                         if (cs.om != 0.0f) nv -= cs.om * r[o];
                         r[o] = nv;
+                        if (EPI == 3) d = fma((double)rh, (double)nv, d);
                     } else {
                         r[o] = val;
+                        if (DOT) d = fma((double)f[o], (double)val, d);
                     }
-                    if (DOT) d = fma((double)f[o], (double)val, d);
                 }
             }
         }
@@ -637,7 +642,12 @@ static const void *apply5_kernel(int NC, uint32_t W, uint32_t slm) {
 }
 
 static const void *apply5_kernel_epi(int epi, int NC, uint32_t W, uint32_t slm) {
-    return epi == 1 ? apply5_kernel<1>(NC, W, slm) : (epi == 2 ? apply5_kernel<2>(NC, W, slm) : apply5_kernel<0>(NC, W, slm));
+    switch (epi) {
+        case 1: return apply5_kernel<1>(NC, W, slm);
+        case 2: return apply5_kernel<2>(NC, W, slm);
+        case 3: return apply5_kernel<3>(NC, W, slm);
+        default: return apply5_kernel<0>(NC, W, slm);
+    }
 }
 
 uint32_t apply5_grid(const OpArgs<float> &a, int NC, int wpb, int epi) {
@@ -663,8 +673,8 @@ hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid,
     if ((a.W != 15 && a.W != 16) || a.WB > TR_WBMAX || !a.crec || !a.tab || (a.nB && !a.bcrec) || wpb < 1 ||
         wpb > 8 || !mfma5_supported(a.slm) || a.nsplit < 1 || a.nsplit > 2)
         return hipErrorInvalidValue;
-    if (cheb && (part || !cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
-    const void *k = apply5_kernel_epi(cheb ? 2 : (part ? 1 : 0), NC, a.W, a.slm);
+    if (cheb && (!cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
+    const void *k = apply5_kernel_epi(cheb ? (part ? 3 : 2) : (part ? 1 : 0), NC, a.W, a.slm);
     if (!k) return hipErrorInvalidValue;
     dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
     OpArgs<float> aa = a;
@@ -717,8 +727,12 @@ static const void *apply4_kernel(int J, int NC, uint32_t W) {
 template <typename T, int MODE>
 static const void *apply4_kernel_epi(int epi, uint32_t slm, int NC, uint32_t W) {
     const int J = mfma_J(slm, (int)sizeof(T));
-    return epi == 1 ? apply4_kernel<T, MODE, 1>(J, NC, W)
-                    : (epi == 2 ? apply4_kernel<T, MODE, 2>(J, NC, W) : apply4_kernel<T, MODE, 0>(J, NC, W));
+    switch (epi) {
+        case 1: return apply4_kernel<T, MODE, 1>(J, NC, W);
+        case 2: return apply4_kernel<T, MODE, 2>(J, NC, W);
+        case 3: return apply4_kernel<T, MODE, 3>(J, NC, W);
+        default: return apply4_kernel<T, MODE, 0>(J, NC, W);
+    }
 }
 
 template <typename T, int MODE>
@@ -752,8 +766,8 @@ hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, con
         return hipErrorInvalidValue;
     dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
     const size_t sh = apply4_lds_bytes(MODE, NC, a.slm, a.nsplit);
-    if (cheb && (part || !cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
-    const void *k = apply4_kernel_epi<T, MODE>(cheb ? 2 : (part ? 1 : 0), a.slm, NC, a.W);
+    if (cheb && (!cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
+    const void *k = apply4_kernel_epi<T, MODE>(cheb ? (part ? 3 : 2) : (part ? 1 : 0), a.slm, NC, a.W);
     if (!k) return hipErrorInvalidValue;
     OpArgs<T> aa = a;
     ChebStep<T> cs = cheb ? *cheb : ChebStep<T>{nullptr, nullptr, (T)0, (T)0, (T)0};

@@ -43,6 +43,15 @@ struct CgSpace {
     // and z (n entries) holds M r
     std::function<hipError_t(const T *r, T *z, const double *skip)> prec;
     T *z = nullptr;
+    // optional split of a polynomial preconditioner inside the iteration
+    // loop: the update pass writes pre_out = pre_scale * pre_dinv * r (its
+    // first iterate), prec_loop(r, z, skip, rzpart) does the rest and, when
+    // prec_nparts > 0, writes that many r.z partials to rzpart itself
+    std::function<hipError_t(const T *r, T *z, const double *skip, double *rzpart)> prec_loop;
+    T *pre_out = nullptr;
+    const T *pre_dinv = nullptr;
+    T pre_scale = 0;
+    int prec_nparts = 0;
 };
 
 struct CgParams {

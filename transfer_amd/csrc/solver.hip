@@ -73,12 +73,15 @@ __device__ inline void sum_parts(const double *part, int n, int n2, double &a, d
 // x += alpha p, r -= alpha Ap, z = M r and the r.r, r.z partials, alpha =
 // rz / p.Ap from the nb p.Ap partials of the operator launch; workgroup 0
 // records p.Ap and alpha
+// With z1 != null also z1 = cscale cdinv r (the first Chebyshev iterate of
+// a polynomial preconditioner) from the new r.
 template <typename T>
 __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T *__restrict__ p,
                                                       T *__restrict__ r, const T *__restrict__ Ap,
                                                       const T *__restrict__ dinv, size_t n,
                                                       double *__restrict__ sc, const double *__restrict__ pap_part,
-                                                      int nb, int rzs, double *__restrict__ part) {
+                                                      int nb, int rzs, double *__restrict__ part,
+                                                      const T *__restrict__ cdinv, T cscale, T *__restrict__ z1) {
     if (sc[SC_CONV] != 0.0) return;
     __shared__ double al_sh;
     {
@@ -100,6 +103,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
         x[i] = fma(alpha, p[i], x[i]);
         const T ri = fma(-alpha, Ap[i], r[i]);
         r[i] = ri;
+        if (z1) z1[i] = cscale * cdinv[i] * ri;
         const T zi = dinv ? dinv[i] * ri : ri;
         rr = fma((double)ri, (double)ri, rr);
         rz = fma((double)ri, (double)zi, rz);
@@ -114,17 +118,25 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
 // beta = rz_new / rz from the G r.r / r.z partials (rz in slot rz_old, the
 // new value into slot rz_new by workgroup 0, with the iteration count and
 // the convergence flag); p = z + beta p, z = M r (dinv) or z given
+// (rr partials part[0 .. G), r.z partials part[TR_UPD_BLOCKS ..
+// TR_UPD_BLOCKS + Grz))
 template <typename T>
 __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const T *__restrict__ r,
                                                        const T *__restrict__ dinv, const T *__restrict__ z,
                                                        size_t n, double *__restrict__ sc,
-                                                       const double *__restrict__ part, int G, int rz_old,
+                                                       const double *__restrict__ part, int G, int Grz, int rz_old,
                                                        int rz_new, int maxit) {
     if (sc[SC_CONV] != 0.0) return;
     __shared__ double be_sh;
     {
-        double rr, rz;
-        sum_parts(part, G, TR_UPD_BLOCKS, rr, rz);
+        double rr, rz, dummy;
+        if (Grz == G) {
+            sum_parts(part, G, TR_UPD_BLOCKS, rr, rz);
+        } else {
+            sum_parts(part, G, -1, rr, dummy);
+            __syncthreads();
+            sum_parts(part + TR_UPD_BLOCKS, Grz, -1, rz, dummy);
+        }
         if (threadIdx.x == 0) {
             be_sh = rz / sc[rz_old];
             if (blockIdx.x == 0) {
@@ -256,15 +268,27 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
             launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
         else
             launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
+        // a polynomial preconditioner's first iterate comes out of the
+        // update pass, and its last step may write the r.z partials itself
+        const bool first = gen && S.prec_loop && S.pre_out;
         hipLaunchKernelGGL(k_update_fused<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap,
-                           gen ? nullptr : S.dinv, n, S.sc, S.part, (int)nb, rz_old, S.part2);
+                           gen ? nullptr : S.dinv, n, S.sc, S.part, (int)nb, rz_old, S.part2,
+                           first ? S.pre_dinv : (const T *)nullptr, S.pre_scale, first ? S.pre_out : (T *)nullptr);
+        int Grz = (int)G;
         if (gen) {
-            S.prec(S.r, S.z, S.sc + SC_CONV);
-            hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, (T *)nullptr, n,
-                               S.part2 + TR_UPD_BLOCKS, S.sc);
+            if (first) {
+                S.prec_loop(S.r, S.z, S.sc + SC_CONV, S.part2 + TR_UPD_BLOCKS);
+                if (S.prec_nparts) Grz = S.prec_nparts;
+            } else {
+                S.prec(S.r, S.z, S.sc + SC_CONV);
+            }
+            if (!(first && S.prec_nparts))
+                hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, (T *)nullptr, n,
+                                   S.part2 + TR_UPD_BLOCKS, S.sc);
         }
         hipLaunchKernelGGL(k_pupdate_fused<T>, dim3(G), dim3(256), 0, st, S.p, S.r, gen ? nullptr : S.dinv,
-                           gen ? (const T *)S.z : nullptr, n, S.sc, S.part2, (int)G, rz_old, rz_new, prm.maxit);
+                           gen ? (const T *)S.z : nullptr, n, S.sc, S.part2, (int)G, Grz, rz_old, rz_new,
+                           prm.maxit);
     }
     e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
