@@ -122,7 +122,7 @@ def bench_batch(count, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
-def bench_c3(steps, warmup):
+def bench_c3(steps, warmup, solve=True, copy_calibration=False):
     """The high-order scattering workload: the 96^3-grid staircase ball
     (2,780,400 tets, 485,385 vertices), K = 5 (66 harmonics, the 194-point
     Lebedev order), Henyey-Greenstein g = 0.6 truncated to P_7 with albedo
@@ -140,6 +140,9 @@ def bench_c3(steps, warmup):
     print("[bench-c3] %s nP=%d nT=%d slm=%d, setup %.1f s" % (P.name, nP, nT, slm, time.perf_counter() - t0),
           file=sys.stderr, flush=True)
     ctx.autotune(0)
+    if copy_calibration:
+        cms = ctx.bench_copy(256 << 20, 10)
+        print("[bench-c3] copy 256 MiB: %.3f ms" % cms, file=sys.stderr, flush=True)
     ctx.set_angular_precision(0)
     ctx.autotune(1)
     exact32 = ctx.schedule(1)
@@ -175,6 +178,9 @@ def bench_c3(steps, warmup):
         out[key]["rel_error_vs_fp64"] = float(np.linalg.norm(A32 - A64) / np.linalg.norm(A64))
     for v in (f, a64, a32):
         ctx.deviceFree(v)
+    if not solve:
+        print(json.dumps(out), flush=True)
+        return
     solves = {}
     # the mixed solver with the exact FP32 operator (VALU schedule) and with
     # the BF16 matrix-core projection; no Chebyshev polynomial here: it
@@ -254,7 +260,8 @@ def main():
         bench_batch(args.batch, args.steps, args.warmup)
         return
     if args.c3:
-        bench_c3(min(args.steps, 50), min(args.warmup, 5))
+        bench_c3(min(args.steps, 50), min(args.warmup, 5), solve=not args.no_solve,
+                 copy_calibration=args.copy_calibration)
         return
 
     import transfer_amd as ta

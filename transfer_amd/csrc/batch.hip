@@ -305,6 +305,8 @@ int build_blocks(const tr_ctx *c, DevMem &m, BlockTable &t, hipStream_t st) {
     HIPCHECK(hipMemcpyAsync(t.hi, hi.data(), 4 * hi.size(), hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(t.prob, prob.data(), 4 * prob.size(), hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(t.b0, b0.data(), 4 * b0.size(), hipMemcpyHostToDevice, st));
+    // the host tables are local: the copies must have read them
+    HIPCHECK(hipStreamSynchronize(st));
     return TR_OK;
 }
 
@@ -451,7 +453,9 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
             std::vector<float> d32(d.begin(), d.end());
             if (!(dinv32 = m.get<float>(n))) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
             HIPCHECK(hipMemcpyAsync(dinv32, d32.data(), sizeof(float) * n, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));  // d32 is a temporary
         }
+        HIPCHECK(hipStreamSynchronize(st));  // so is d
     }
     HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, b, st));
     HIPCHECK(hipMemsetAsync(x, 0, sizeof(double) * n, st));

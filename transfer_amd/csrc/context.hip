@@ -921,6 +921,8 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
         dinv = w.get();
         if (!dinv) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
         HIPCHECK(hipMemcpyAsync(dinv, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        // d is a temporary: the copy must have read it before it goes
+        HIPCHECK(hipStreamSynchronize(c->stream));
         if (prm->use_prec == 4) {
             if ((rc = cheb_lambda(c, d))) return rc;
             cheb.a = c->args();

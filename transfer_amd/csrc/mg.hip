@@ -152,6 +152,8 @@ int lambda_max(tr_ctx *c, const double *dinv, double *x, double *y, double &lam)
         for (double &v : h) v /= s;
         HIPCHECK(hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     }
+    // h is local: the last copy must have read it
+    HIPCHECK(hipStreamSynchronize(c->stream));
     return TR_OK;
 }
 
@@ -199,6 +201,7 @@ int build_multilevel(tr_ctx *c, bool f32, uint32_t min_coarse, uint32_t size) {
                   *z = dalloc<float>(*mg, n);
             if (!z) return fail(TR_ERR_HIP, "multilevel: out of device memory");
             HIPCHECK(hipMemcpyAsync(dv, df.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));  // df is a temporary
             mg->dinv32.push_back(dv);
             mg->t32.push_back(t);
             mg->r32.push_back(r);
@@ -231,6 +234,7 @@ static int build_levels(tr_ctx *c, Multilevel *mg, uint32_t min_coarse, uint32_t
             uint32_t *d = dalloc<uint32_t>(*mg, agg.size());
             if (!d) return fail(TR_ERR_HIP, "multilevel: out of device memory");
             HIPCHECK(hipMemcpyAsync(d, agg.data(), sizeof(uint32_t) * agg.size(), hipMemcpyHostToDevice, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));  // agg is a temporary
             mg->agg.push_back(d);
         }
         for (size_t k = 0; k < mg->lev.size(); k++) {
