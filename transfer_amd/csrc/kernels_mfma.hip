@@ -243,6 +243,49 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
             for (int c = 0; c < NCH; c++) acc[c][j] = 0;
 }
 
+// The gather with the slots' coefficient records already in registers
+// (cf[s][e]: element q*PE + e of slot s's record, loaded once per tile by
+// the K-blocked schedule); only the neighbour rows stream, DEPTH slots in
+// flight.
+template <typename T, int J, int NCH, int PE, int NS, int DEPTH>
+__device__ __forceinline__ void gather_pre(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
+                                           const T (&cf)[NS][PE], uint32_t slm, int q, uint32_t hoff, uint32_t h0) {
+    constexpr int E = 16 / (int)sizeof(T);
+    slm = opaque_uniform(slm);
+    T xs[DEPTH][J];
+    auto load = [&](int s, T *x) __attribute__((always_inline)) {
+        uint32_t base = nb[s] * slm * (uint32_t)sizeof(T) + 16u * (uint32_t)q + hoff;
+        asm volatile("" : "+v"(base));
+#pragma unroll
+        for (int j = 0; j < J; j++)
+            x[j] = bload1<T>(frs, base + (uint32_t)((64 * (j / E) + (j % E) * (int)sizeof(T))), 0u);
+    };
+    auto accum = [&](const T *x, const T *c0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const T w = quad_bcast(c0[c % PE], c / PE);
+#pragma unroll
+            for (int j = 0; j < J; j++) acc[c][j] = fma(w, x[j], acc[c][j]);
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+#pragma unroll
+            for (int j = 0; j < J; j++) asm volatile("" : "+v"(acc[c][j]));
+    };
+#pragma unroll
+    for (int d = 0; d < DEPTH && d < NS; d++) load(d, xs[d]);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        accum(xs[s % DEPTH], cf[s]);
+        if (s + DEPTH < NS) load(s + DEPTH, xs[s % DEPTH]);
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++)
+        if (mfma_harm(E, q, j) + h0 >= slm)
+#pragma unroll
+            for (int c = 0; c < NCH; c++) acc[c][j] = 0;
+}
+
 __device__ inline double bperm(int addr, double x) {
     i32x2m v = __builtin_bit_cast(i32x2m, x);
     v.x = __builtin_amdgcn_ds_bpermute(addr, v.x);
@@ -508,12 +551,14 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
     const auto bcrs = rsrc4(a.bcrec, (size_t)TR_WBMAX * a.nB * 4 * sizeof(float));
     const int vloc = lane & 15, q = lane >> 4;
     const int gv = lane >> 2, gq = lane & 3;
-    // 2 wavefronts per SIMD (one workgroup of up to 8 per CU: the LDS)
+    // 2 wavefronts per SIMD (one workgroup of up to 8 per CU: the LDS);
+    // the tile's coefficient records (WT x PE per lane) stay in registers
+    // across the harmonic blocks
     constexpr int BUDGET = 256;
-    constexpr int OVH = WT + 4 * NT + 30;
     constexpr int PE = (RS + 3) / 4;
-    constexpr int ACCR = NC * J, SLOTR = J + PE;
-    constexpr int DEPTH = std::max(2, std::min(6, (BUDGET - ACCR - OVH) / SLOTR));
+    constexpr int OVH = WT + 4 * NT + 30 + WT * PE;
+    constexpr int ACCR = NC * J, SLOTR = J;
+    constexpr int DEPTH = std::max(2, std::min(8, (BUDGET - ACCR - OVH) / SLOTR));
     constexpr int DEPTHB = 4;
 
     const uint32_t ntiles = (nP + 16u * wpb - 1) / (16u * wpb);
@@ -547,6 +592,20 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
         f4v D[NT];
 #pragma unroll
         for (int t = 0; t < NT; t++) D[t] = (f4v){0, 0, 0, 0};
+        // lane q's quarter of every slot's coefficient record (reads past
+        // the record's RS elements fall outside the descriptor: zeros)
+        float cf[WT][PE];
+        {
+            uint32_t cv = v * (uint32_t)(RS * sizeof(float)) + (uint32_t)(gq * PE * (int)sizeof(float));
+            const uint32_t cstride = opaque_uniform(nP * (uint32_t)(RS * sizeof(float)));
+            asm volatile("" : "+v"(cv));
+#pragma unroll
+            for (int s2 = 0; s2 < WT; s2++)
+#pragma unroll
+                for (int e2 = 0; e2 < PE; e2++)
+                    cf[s2][e2] = valid ? bload1<float>(crs, cv + (uint32_t)(e2 * (int)sizeof(float)), (uint32_t)s2 * cstride)
+                                       : 0.0f;
+        }
 #pragma unroll 1
         for (int kb = 0; kb < NT; kb++) {
             float acc[NC][J];
@@ -555,9 +614,8 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
 #pragma unroll
                 for (int j = 0; j < J; j++) acc[c][j] = 0;
             if (valid)
-                gather<float, J, NC, RS, WT, DEPTH>(acc, nbc, frs, crs, nP * (uint32_t)(RS * sizeof(float)),
-                                                     v * (uint32_t)(RS * sizeof(float)), WT, slm, gq,
-                                                     64u * (uint32_t)kb, 16u * (uint32_t)kb);
+                gather_pre<float, J, NC, PE, WT, DEPTH>(acc, nbc, frs, cf, slm, gq, 64u * (uint32_t)kb,
+                                                         16u * (uint32_t)kb);
             if (a.mf0 && valid && gq == 0 && kb == 0) acc[0][0] *= a.mf0[v];
             to_mfma_layout<float, 2, J, NC>(acc, lane);
             uint32_t boff = (uint32_t)kb * (uint32_t)(NC * NT * BLK);
