@@ -249,6 +249,7 @@ def main():
                     help="also time a plain 256 MiB device copy (counter calibration)")
     ap.add_argument("--c3", action="store_true", help="run the K = 5 scattering workload (ball96-hg) instead")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--only", default="", help="run only the named time_to_1e-8 solve (for short profiles)")
     ap.add_argument("--batch", type=int, default=0,
                     help="run the batching workload with this many problems instead (256 = the full one)")
     args = ap.parse_args()
@@ -312,6 +313,8 @@ def main():
                 ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(inner_tol=1e-5, maxit=30)),
                 ("mixed_fp32_ir_multilevel", ta.TR_SOLVER_MIXED, 3, dict(inner_tol=1e-4, maxit=30)),
                 ("mixed_fp32_ir_chebyshev", ta.TR_SOLVER_MIXED, 4, dict(inner_tol=1e-4, maxit=30))]
+        if args.only:
+            runs = [r for r in runs if r[0] == args.only]
         # Chebyshev preconditioner: degree 6 over [lambda_max / 300, lambda_max]
         ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO)
         for name, solver, prec, kw in runs:
@@ -334,6 +337,9 @@ def main():
                   % (lm["seconds"], lm["guess_seconds"], lm["solve_seconds"], lm["inner_iters"], lm["fallback"],
                      lm["true_rel"], lm["training_seconds"], lm["training_samples"]), file=sys.stderr, flush=True)
         out["time_to_1e-8"] = solves
+        if args.only:
+            print(json.dumps(out), flush=True)
+            return
         # the plain FP64 GPU solver: the reference's method (main.cpp's PCG
         # as configured) run on the device
         plain = solves["pcg_fp64_device"]["seconds"]
