@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-1e-8 residual & DOF·ordinate/s, learned+mixed vs plain FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-CHEB_DEGREE, CHEB_RATIO = 6, 300.0
+CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT = 6, 300.0, 2
 
 
 def operator_bytes(nT, nP, nF, slm, p):
@@ -315,8 +315,9 @@ def main():
                 ("mixed_fp32_ir_chebyshev", ta.TR_SOLVER_MIXED, 4, dict(inner_tol=1e-4, maxit=30))]
         if args.only:
             runs = [r for r in runs if r[0] == args.only]
-        # Chebyshev preconditioner: degree 6 over [lambda_max / 300, lambda_max]
-        ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO)
+        # Chebyshev preconditioner: degree 6 over [lambda_max / 300, lambda_max],
+        # steps with the gathered values split into BF16 hi + lo
+        ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
         for name, solver, prec, kw in runs:
             if solver == ta.TR_SOLVER_MIXED:
                 out["schedule_fp32"] = ctx.schedule(1)

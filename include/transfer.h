@@ -198,8 +198,11 @@ int tr_ctx_set_multilevel(tr_ctx *ctx, uint32_t coarsest, uint32_t agg, int nu);
  * extension): `degree` steps of the Chebyshev iteration on D^-1 A z = D^-1 r
  * over [lambda_max / ratio, lambda_max] (lambda_max of D^-1 A from a power
  * iteration), each step one operator launch with the update fused into its
- * write; defaults 4, 100. */
-int tr_ctx_set_chebyshev(tr_ctx *ctx, int degree, double ratio);
+ * write.  step_split (TR_SOLVER_MIXED with a BF16 matrix-core operator):
+ * the BF16 split of the steps, 0 = the operator's own, 2 = the gathered
+ * values split and the couplings BF16 only, 1 = plain BF16.  Defaults 4,
+ * 100, 2. */
+int tr_ctx_set_chebyshev(tr_ctx *ctx, int degree, double ratio, int step_split);
 /* Angular precision of the FP32 operator used by TR_SOLVER_MIXED:
  * 0 = FP32 throughout, 1 = BF16 matrix-core projection with FP32
  * accumulation, 3 (default) = BF16 with the hi/lo split (three BF16

@@ -116,7 +116,7 @@ _sig("tr_compute_rhs", C.c_int, vp, vp)
 _sig("tr_compute_lhs_prec", C.c_int, vp, C.c_int, vp, vp)
 _sig("tr_compute_moments", C.c_int, vp, vp, vp, vp)
 _sig("tr_ctx_set_multilevel", C.c_int, vp, C.c_uint32, C.c_uint32, C.c_int)
-_sig("tr_ctx_set_chebyshev", C.c_int, vp, C.c_int, C.c_double)
+_sig("tr_ctx_set_chebyshev", C.c_int, vp, C.c_int, C.c_double, C.c_int)
 _sig("tr_compute_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
 _sig("tr_expn", C.c_double, C.c_int, C.c_double)
 _sig("tr_mlp_features", C.c_int, C.POINTER(MeshView), C.c_double, f32p)
@@ -444,8 +444,8 @@ class HipContext:
         self.copyToDev(v, np.ascontiguousarray(host, np.float64).reshape(-1))
         return v
 
-    def set_chebyshev(self, degree=4, ratio=100.0):
-        check(lib.tr_ctx_set_chebyshev(self._h, int(degree), float(ratio)))
+    def set_chebyshev(self, degree=4, ratio=100.0, step_split=2):
+        check(lib.tr_ctx_set_chebyshev(self._h, int(degree), float(ratio), int(step_split)))
 
     def set_multilevel(self, coarsest=2048, agg=8, nu=2):
         check(lib.tr_ctx_set_multilevel(self._h, int(coarsest), int(agg), int(nu)))

@@ -1065,6 +1065,11 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
             HIPCHECK(hipMalloc((void **)&ex, sizeof(float) * 2 * n + 8));
             w.v.push_back((double *)ex);
             cheb.a = c->args32();
+            // a lighter BF16 split for the steps keeps the polynomial linear
+            // and symmetric (the couplings' rounding is a fixed perturbation)
+            if (c->cheb_split > 0 && c->cheb_split < cheb.a.nsplit &&
+                (c->map32.variant == 4 || c->map32.variant == 5) && c->map32.mfma_mode == 2)
+                cheb.a.nsplit = c->cheb_split;
             cheb.NC = (int)c->op.NC;
             cheb.map = c->map32;
             cheb.dinv = dinv32;
@@ -1515,10 +1520,12 @@ extern "C" int tr_compute_lhs_prec(tr_ctx *c, int precision, const double *f, do
 // Chebyshev polynomial preconditioner (use_prec 4): its degree (1 = a
 // scaled Jacobi step) and the ratio lambda_max / lambda_min of the
 // interval of D^-1 A's spectrum it targets.
-int tr_ctx_set_chebyshev(tr_ctx *c, int degree, double ratio) {
-    if (!c || degree < 1 || degree > 64 || !(ratio > 1.0)) return fail(TR_ERR_ARG, "tr_ctx_set_chebyshev: bad argument");
+int tr_ctx_set_chebyshev(tr_ctx *c, int degree, double ratio, int step_split) {
+    if (!c || degree < 1 || degree > 64 || !(ratio > 1.0) || step_split < 0 || step_split > 3)
+        return fail(TR_ERR_ARG, "tr_ctx_set_chebyshev: bad argument");
     c->cheb_deg = degree;
     c->cheb_ratio = ratio;
+    c->cheb_split = step_split;
     return TR_OK;
 }
 

@@ -74,6 +74,7 @@ struct tr_ctx {
     // D^-1 A (power iteration on first use, 0 before)
     int cheb_deg = 4;
     double cheb_ratio = 100.0, cheb_lmax = 0.0;
+    int cheb_split = 2;  // BF16 split of the polynomial's steps: 0 = the operator's, else 1..3
     std::vector<void *> owned;
 
     tr::OpArgs<double> args() const { return args(map); }
