@@ -130,6 +130,22 @@ int tr_mesh_set_fields(tr_mesh *m, const double *kappa, const double *Ip);
 int tr_mesh_set_face_kind_plane(tr_mesh *m, int axis, double value, double tol,
                                 int kind);
 
+/* Netgen neutral volume mesh (.vol), the reference's mesh input (read by
+ * meshProcessor in MeshData.cpp:10): `points` and `volumeelements` (1-based
+ * point numbers; material number = region).  With a config, kappa and I_p
+ * of every tet come from its region's [region N] section (MeshData.cpp:
+ * 62-63), else kappa = 1, I_p = 0.  Other sections are ignored; boundary
+ * faces are the faces with one tet. */
+int tr_mesh_load_vol(const char *path, const tr_config *cfg, tr_mesh **out);
+/* region[nT]: the material numbers of a loaded mesh (0 for generated ones). */
+int tr_mesh_get_regions(const tr_mesh *m, uint32_t *region);
+/* Legacy VTK unstructured grid of the fields main.cpp:209-216 saves (through
+ * meshProcessor's saveVtk there): W[3][nT] per tet (vector), U[nP],
+ * T[9][nP] (tensor, the layout of tr_compute_moments), aU, aW, aT [nP] per
+ * vertex.  Host arrays; any may be NULL. */
+int tr_write_vtk(const tr_mesh *m, const char *path, const double *U, const double *T,
+                 const double *W, const double *aU, const double *aW, const double *aT);
+
 /* Per-tet field generators (seeded; SplitMix64 counter-based RNG). */
 enum {
     TR_FIELD_CONST = 0,      /* p[0]                                          */
@@ -371,6 +387,23 @@ int tr_ctx_mlp_features(tr_ctx *ctx, double albedo, float *feat, int *sweeps);
  * when ip_scale <= 0, each vertex's own source (exp of feature 7). */
 int tr_ctx_learned_guess(tr_ctx *ctx, tr_mlp *m, double albedo, double ip_scale,
                          double *f);
+
+/* The dense system dump of main.cpp:73-114 (small meshes, N*N <= 2^28):
+ * one text line per unknown k outside the padding lanes with the nP*slm
+ * entries of A e_k and b[k]; then the symmetry check |Z_ij - Z_ji| >
+ * 100 eps (|Z_ij| + |Z_ji|), eps the unit roundoff (Util.cpp:4-14).
+ * *n_asym receives the number of pairs i < j that fail it, *max_ratio the
+ * largest |Z_ij - Z_ji| / (eps (|Z_ij| + |Z_ji|)); both may be NULL. */
+int tr_dump_slae(tr_ctx *ctx, const char *path, uint64_t *n_asym, double *max_ratio);
+
+/* Lanczos estimate of the operator's extreme eigenvalues (Eigenvalue.cpp:
+ * 5-88, eigest; disabled in main.cpp:117-123): up to `steps` Lanczos steps
+ * from the right-hand side, on A (jacobi 0) or on D^-1/2 A D^-1/2, the
+ * spectrum of D^-1 A (jacobi 1), stopping when the largest Ritz value moves
+ * by less than tol of itself (eigest: 1e-6; 0 = all steps).  out4 =
+ * {smallest Ritz value, largest, the largest extrapolated as eigest does,
+ * steps taken}. */
+int tr_eig_estimate(tr_ctx *ctx, int steps, int jacobi, double tol, double *out4);
 
 /* Seconds per operator application (median over reps), for benchmarking. */
 double tr_bench_lhs(tr_ctx *ctx, int reps);

@@ -91,3 +91,63 @@ def test_milne_slab_on_device(solver):
     assert rel(U, Uo) <= 1e-7
     assert rel(U, Ua) <= 0.04
     ctx.deviceFree(d_x)
+
+
+def test_dump_slae_matches_oracle_and_is_symmetric(tmp_path):
+    """main.cpp:73-114 through tr_dump_slae: every line is the unpadded column
+    A e_k and b[k] of the oracle's operator (1e-12 relative), one line per
+    unknown outside the padding, and no pair fails the reference's symmetry
+    check (100 eps (|Z_ij| + |Z_ji|))."""
+    from conftest import ang_dict
+    P = ta.problems.slab(n=2, albedo=0.3)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    n_asym, ratio = ctx.dump_slae(tmp_path / "slae.dat")
+    rows = np.loadtxt(tmp_path / "slae.dat")
+    nP, slm, aslm = op.nP, op.slm, op.aslm
+    assert rows.shape == (nP * slm, nP * slm + 1)
+    b = op.rhs()
+    ks = [k for k in range(nP * aslm) if k % aslm < slm]
+    worst = 0.0
+    for row, k in zip(rows, ks):
+        e = np.zeros(nP * aslm)
+        e[k] = 1.0
+        want = op.lhs(e).reshape(nP, aslm)[:, :slm].ravel()
+        worst = max(worst, np.abs(row[:-1] - want).max() / np.abs(want).max())
+        assert row[-1] == pytest.approx(b[k], rel=1e-13, abs=1e-300)
+    print("[gpu-postproc] slae dump: %d columns, max rel err %.2e, asymmetric pairs %d, max ratio %.1f" % (
+        len(ks), worst, n_asym, ratio), flush=True)
+    assert worst <= 1e-12 and n_asym == 0
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_lanczos_eigen_estimate(jacobi):
+    """tr_eig_estimate (Eigenvalue.cpp's eigest, with a correct tridiagonal)
+    against the dense spectrum of the oracle's operator on a small slab: with
+    eigest's stopping rule (1e-6) the largest Ritz value within 1e-5 of
+    lambda_max; run to 200 steps, the largest within 1e-9 and the smallest
+    above lambda_min and within 1% of it (Ritz values lie inside the
+    spectrum), for A and for D^-1 A."""
+    from conftest import ang_dict
+    P = ta.problems.slab(n=2, albedo=0.3)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    nP, slm, aslm = op.nP, op.slm, op.aslm
+    ks = [k for k in range(nP * aslm) if k % aslm < slm]
+    A = np.zeros((len(ks), len(ks)))
+    for c, k in enumerate(ks):
+        e = np.zeros(nP * aslm)
+        e[k] = 1.0
+        A[:, c] = op.lhs(e)[ks]
+    if jacobi:
+        s = 1.0 / np.sqrt(np.diag(A))
+        A = s[:, None] * A * s[None, :]
+    ev = np.linalg.eigvalsh(0.5 * (A + A.T))
+    _, hi0, extra, steps0 = ctx.eig_estimate(steps=30, jacobi=jacobi)
+    lo, hi, _, steps = ctx.eig_estimate(steps=200, jacobi=jacobi, tol=0.0)
+    print("[gpu-postproc] lanczos jacobi=%d: eigest rule %d steps lmax %.8e (extrapolated %.8e); %d steps Ritz "
+          "[%.8e, %.8e]; spectrum [%.8e, %.8e]" % (jacobi, steps0, hi0, extra, steps, lo, hi, ev[0], ev[-1]),
+          flush=True)
+    assert abs(hi0 - ev[-1]) <= 1e-5 * ev[-1]
+    assert abs(hi - ev[-1]) <= 1e-9 * ev[-1]
+    assert ev[0] * (1 - 1e-9) <= lo <= 1.01 * ev[0]

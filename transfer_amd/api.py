@@ -116,6 +116,11 @@ _sig("tr_compute_rhs", C.c_int, vp, vp)
 _sig("tr_compute_lhs_prec", C.c_int, vp, C.c_int, vp, vp)
 _sig("tr_compute_moments", C.c_int, vp, vp, vp, vp)
 _sig("tr_ctx_set_multilevel", C.c_int, vp, C.c_uint32, C.c_uint32, C.c_int)
+_sig("tr_mesh_load_vol", C.c_int, C.c_char_p, vp, C.POINTER(vp))
+_sig("tr_mesh_get_regions", C.c_int, vp, u32p)
+_sig("tr_write_vtk", C.c_int, vp, C.c_char_p, f64p, f64p, f64p, f64p, f64p, f64p)
+_sig("tr_dump_slae", C.c_int, vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double))
+_sig("tr_eig_estimate", C.c_int, vp, C.c_int, C.c_int, C.c_double, f64p)
 _sig("tr_ctx_set_chebyshev", C.c_int, vp, C.c_int, C.c_double, C.c_int)
 _sig("tr_compute_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
 _sig("tr_expn", C.c_double, C.c_int, C.c_double)
@@ -243,6 +248,25 @@ class Mesh:
         check(lib.tr_mesh_create(len(coords), ptr(coords), len(tets), ptr(tets, u32p),
                                  ptr(kappa), ptr(Ip), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load_vol(cls, path, config=None):
+        """A Netgen .vol mesh (tr_mesh_load_vol); kappa and I_p per tet from
+        the config's [region N] sections when a Config is given."""
+        h = vp()
+        check(lib.tr_mesh_load_vol(str(path).encode(), None if config is None else config._h, C.byref(h)))
+        return cls(h)
+
+    def regions(self):
+        out = np.empty(self.nT, np.uint32)
+        check(lib.tr_mesh_get_regions(self._h, ptr(out, u32p)))
+        return out
+
+    def write_vtk(self, path, U=None, T=None, W=None, aU=None, aW=None, aT=None):
+        """Legacy VTK of the solution fields (tr_write_vtk): W [3][nT], U [nP],
+        T [9][nP], aU/aW/aT [nP]; host arrays, any may be None."""
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (U, T, W, aU, aW, aT)]
+        check(lib.tr_write_vtk(self._h, str(path).encode(), *[ptr(a) for a in arrs]))
 
     def refresh_view(self):
         check(lib.tr_mesh_get_view(self._h, C.byref(self.view)))
@@ -492,6 +516,21 @@ class HipContext:
         finally:
             self.deviceFree(dT)
             self.deviceFree(dW)
+
+    def eig_estimate(self, steps=30, jacobi=False, tol=1e-6):
+        """tr_eig_estimate: (smallest Ritz value, largest, extrapolated
+        largest, steps) of A or, with jacobi, of D^-1 A."""
+        out = np.zeros(4)
+        check(lib.tr_eig_estimate(self._h, int(steps), int(bool(jacobi)), float(tol), ptr(out)))
+        return out[0], out[1], out[2], int(out[3])
+
+    def dump_slae(self, path):
+        """main.cpp:73-114 (tr_dump_slae): the dense system to `path`; returns
+        (pairs failing the symmetry check, largest asymmetry in units of
+        eps (|Z_ij| + |Z_ji|))."""
+        n, r = C.c_uint64(0), C.c_double(0)
+        check(lib.tr_dump_slae(self._h, str(path).encode(), C.byref(n), C.byref(r)))
+        return n.value, r.value
 
     def getHostSmall(self, src):
         out = np.empty(self.nP * self.slm)

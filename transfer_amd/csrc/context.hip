@@ -1517,6 +1517,116 @@ extern "C" int tr_compute_lhs_prec(tr_ctx *c, int precision, const double *f, do
     return TR_OK;
 }
 
+namespace tr {
+// x[i] *= s[i] over the internal [v][slm] entries
+__global__ __launch_bounds__(256) void k_mul_elem(double *__restrict__ x, const double *__restrict__ s, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] *= s[i];
+}
+
+// Extreme eigenvalues of the symmetric tridiagonal matrix (diagonal a[m],
+// off-diagonal b[m-1]) by Sturm-sequence bisection: the number of
+// eigenvalues below x is the number of negative pivots of T - x I.
+static void tridiag_extremes(const std::vector<double> &a, const std::vector<double> &b, double &lo, double &hi) {
+    const size_t m = a.size();
+    double gl = 1e300, gh = -1e300;
+    for (size_t i = 0; i < m; i++) {
+        const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+        gl = std::min(gl, a[i] - r);
+        gh = std::max(gh, a[i] + r);
+    }
+    auto below = [&](double x) {
+        int cnt = 0;
+        double d = 1.0;
+        for (size_t i = 0; i < m; i++) {
+            d = (a[i] - x) - (i > 0 ? b[i - 1] * b[i - 1] / d : 0.0);
+            if (d == 0.0) d = -1e-300;
+            if (d < 0) cnt++;
+        }
+        return cnt;
+    };
+    auto kth = [&](int k) {  // k-th smallest (0-based)
+        double l = gl, h = gh;
+        for (int it = 0; it < 200 && h - l > 1e-15 * std::max(std::fabs(l), std::fabs(h)); it++) {
+            const double x = 0.5 * (l + h);
+            (below(x) > k ? h : l) = x;
+        }
+        return 0.5 * (l + h);
+    };
+    lo = kth(0);
+    hi = kth((int)m - 1);
+}
+}  // namespace tr
+
+// Lanczos estimate of the extreme eigenvalues (Eigenvalue.cpp:53-88, eigest):
+// `steps` Lanczos steps on A (jacobi = 0) or on D^-1/2 A D^-1/2, whose
+// spectrum is that of D^-1 A (jacobi = 1), from the right-hand side b as
+// the start vector (main.cpp:117-123), stopping early when the largest Ritz
+// value changes by less than tol of itself (eigest: 1e-6; tol = 0 runs all
+// steps, for the smallest Ritz value, which converges more slowly).  out[0] = smallest Ritz value,
+// out[1] = largest, out[2] = the largest extrapolated as eigest does
+// (lmax + (lmax - lold) / (1 - q), q = (lmax - lold) / (lold - loold)), out[3]
+// = the number of steps taken.  The Ritz values are the tridiagonal's
+// extreme eigenvalues by Sturm bisection, with the off-diagonal beta_{j+1}
+// coupling rows j and j+1 (the reference's trieig is handed b with b[0] = 0
+// in front and so couples each pair of rows through the previous beta).
+int tr_eig_estimate(tr_ctx *c, int steps, int jacobi, double tol, double *out4) {
+    if (!c || !out4 || steps < 1 || steps > 1000 || !(tol >= 0)) return fail(TR_ERR_ARG, "tr_eig_estimate: bad argument");
+    const size_t n = c->nint();
+    Work w(c);
+    double *v = w.get(), *v0 = w.get(), *wv = w.get(), *tmp = w.get(), *sc = nullptr;
+    if (!tmp) return fail(TR_ERR_HIP, "tr_eig_estimate: out of device memory");
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    if (jacobi) {
+        std::vector<double> d = operator_diagonal(c->op);
+        for (double &x : d) x = 1.0 / std::sqrt(x);
+        sc = w.get();
+        if (!sc) return fail(TR_ERR_HIP, "tr_eig_estimate: out of device memory");
+        HIPCHECK(hipMemcpyAsync(sc, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    }
+    int rc = tr_compute_rhs(c, v);
+    if (rc) return rc;
+    const double nv = std::sqrt(dot(c, v, v));
+    if (!(nv > 0)) return fail(TR_ERR_ARG, "tr_eig_estimate: zero right-hand side");
+    if ((rc = axpby(c, v, 1.0 / nv, nullptr, 0.0)) || (rc = axpby(c, v0, 0.0, nullptr, 0.0))) return rc;
+    std::vector<double> a, b;
+    double beta = 0, lmax = 0, lold = 0, loold = 0, lmin = 0;
+    int j = 0;
+    for (; j < steps; j++) {
+        // wv = S A S v, S = D^-1/2 (jacobi) or I
+        if (sc) {
+            HIPCHECK(hipMemcpyAsync(tmp, v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+            hipLaunchKernelGGL(k_mul_elem, dim3(g), dim3(256), 0, c->stream, tmp, sc, n);
+            if ((rc = lhs(c, tmp, wv))) return rc;
+            hipLaunchKernelGGL(k_mul_elem, dim3(g), dim3(256), 0, c->stream, wv, sc, n);
+        } else if ((rc = lhs(c, v, wv))) {
+            return rc;
+        }
+        if ((rc = axpby(c, wv, 1.0, v0, -beta))) return rc;
+        const double alpha = dot(c, wv, v);
+        if ((rc = axpby(c, wv, 1.0, v, -alpha))) return rc;
+        a.push_back(alpha);
+        beta = std::sqrt(dot(c, wv, wv));
+        loold = lold;
+        lold = lmax;
+        tridiag_extremes(a, b, lmin, lmax);
+        if ((j > 0 && std::fabs(lmax - lold) < tol * std::fabs(lmax)) || !(beta > 0)) {
+            j++;
+            break;
+        }
+        b.push_back(beta);
+        std::swap(v0, v);  // v0 = v_j, v = (w - alpha v_j - beta v_{j-1}) / beta_{j+1}
+        if ((rc = axpby(c, v, 0.0, wv, 1.0 / beta))) return rc;
+    }
+    const double q = (lold != loold) ? (lmax - lold) / (lold - loold) : 0.0;
+    out4[0] = lmin;
+    out4[1] = lmax;
+    out4[2] = (q != 1.0) ? lmax + (lmax - lold) / (1.0 - q) : lmax;
+    out4[3] = j;
+    return TR_OK;
+}
+
 // Chebyshev polynomial preconditioner (use_prec 4): its degree (1 = a
 // scaled Jacobi step) and the ratio lambda_max / lambda_min of the
 // interval of D^-1 A's spectrum it targets.

@@ -51,6 +51,7 @@ struct Mesh {
     std::vector<tr_face> bnd;
     std::vector<uint8_t> face_kind;  // [nF]
     std::vector<uint32_t> face_tet;  // [nF] owning tet
+    std::vector<uint32_t> region;    // [nT] material number of a mesh file, empty for generated meshes
 };
 int build_mesh(uint32_t nP, const double *coords, uint32_t nT,
                const uint32_t *tets, const double *kappa, const double *Ip,
