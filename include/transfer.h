@@ -180,7 +180,15 @@ typedef struct tr_ctx tr_ctx;
 
 /* CudaContext(dev, MeshData, AngularData) (CudaContext.cpp:76-102):
  * uploads mesh and angular data (DeviceMeshData.cpp:8-52,
- * DeviceAngularData.cpp:7-67).  flags: reserved, pass 0. */
+ * DeviceAngularData.cpp:7-67).  flags: 0 (the reference's operator) or
+ * TR_CTX_GENERAL_NORMALS. */
+/* Vacuum faces whose normal is not along an axis get the exact boundary
+ * term <|Omega.n| Y Y'> of their own normal and their true area (extension;
+ * the reference's axis rule, kernels.cu:359-361 "WORKS ONLY IF NORMAL IS
+ * (+/-1,0,0)", Issue #15, stays the default). Supported by the operator,
+ * the Diag operator, the FP64 and mixed device solvers with use_prec 0, 2
+ * and 4; not by the multilevel preconditioner or batches. */
+#define TR_CTX_GENERAL_NORMALS 1
 int tr_ctx_create(int device, const tr_mesh_view *mesh,
                   const tr_angular_view *ang, int flags, tr_ctx **out);
 int tr_ctx_destroy(tr_ctx *ctx);               /* ~CudaContext :104-110 */

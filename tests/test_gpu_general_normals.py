@@ -1,0 +1,115 @@
+"""GPU: vacuum faces off the axes (TR_CTX_GENERAL_NORMALS; the reference's
+axis rule, kernels.cu:359-361, is exact only along the axes, Issue #15).
+
+The pin is rotation covariance.  A mesh B that is mesh A rotated by R poses
+the same problem in rotated directions, so the harmonic coefficients at
+every vertex transform by the rotation T of the even harmonics up to 2K,
+T[lm][l'm'] = <Y_lm(Omega) Y_l'm'(R^T Omega)> (exact: harmonics of degree
+<= 2K are closed under rotation, and the rule below integrates degree 4K
+exactly).  With general normals A_B (T f) = T (A_A f) at 1e-12 relative to
+max|A f|, for a generic rotation (every boundary face off the axes) and a
+rotation about z (the z faces stay on an axis): both boundary paths at once.
+Solutions then agree in U = 4 pi f_00 (rotation invariant) to the solver
+tolerance, and without the flag (the reference's rule) they do not."""
+import numpy as np
+import pytest
+
+import transfer_amd as ta
+from angular_np import harmonics, sphere_rule
+
+pytestmark = pytest.mark.gpu
+K = 2
+
+
+def rotation(kind):
+    if kind == "z30":
+        c, s = np.cos(np.pi / 6), np.sin(np.pi / 6)
+        return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
+    a, b, g = 0.4, 1.1, -0.7
+    Rz = lambda t: np.array([[np.cos(t), -np.sin(t), 0], [np.sin(t), np.cos(t), 0], [0, 0, 1]])
+    Ry = lambda t: np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+    return Rz(a) @ Ry(b) @ Rz(g)
+
+
+def sh_rotation(R):
+    mu, ph, w = sphere_rule(24, 48)
+    s = np.sqrt(1 - mu ** 2)
+    d = np.stack([s * np.cos(ph), s * np.sin(ph), mu])
+    dr = R.T @ d
+    ls = list(range(0, 2 * K + 1, 2))
+    Y = harmonics(ls, d[2], np.arctan2(d[1], d[0]))
+    Yr = harmonics(ls, np.clip(dr[2], -1, 1), np.arctan2(dr[1], dr[0]))
+    return (Y * w) @ Yr.T
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    A = ta.Mesh.box((4, 4, 4), (0, 0, 0), (1, 1, 1), jitter=0.2, seed=5)
+    coords, tets, cen, vol = A.geometry()
+    kappa = 1.0 + 2.0 * ta.rng_uniform(3, 1, len(tets))
+    Ip = np.ones(len(tets))
+    A.set_fields(kappa=kappa, Ip=Ip)
+    return A, coords, tets, kappa, Ip
+
+
+@pytest.mark.parametrize("kind", ["generic", "z30"])
+def test_operator_is_rotation_covariant(meshes, kind):
+    A, coords, tets, kappa, Ip = meshes
+    R = rotation(kind)
+    B = ta.Mesh.from_arrays(coords @ R.T, tets, kappa, Ip)
+    ang = ta.Angular(K)
+    ctxA, ctxB = ta.HipContext(A, ang), ta.HipContext(B, ang, general_normals=True)
+    T = sh_rotation(R)
+    assert np.abs(T @ T.T - np.eye(len(T))).max() < 1e-12
+    nP, slm, aslm = A.nP, ctxA.slm, ctxA.aslm
+    rng = np.random.default_rng(1)
+    fA = np.zeros((nP, aslm))
+    fA[:, :slm] = rng.standard_normal((nP, slm))
+    fB = np.zeros((nP, aslm))
+    fB[:, :slm] = fA[:, :slm] @ T.T
+    dA, dB, rA, rB = ctxA.vector_from(fA.ravel()), ctxB.vector_from(fB.ravel()), ctxA.allocVector(), ctxB.allocVector()
+    ctxA.computeLhs(dA, rA)
+    ctxB.computeLhs(dB, rB)
+    want = ctxA.copyToHost(rA).reshape(nP, aslm)[:, :slm] @ T.T
+    got = ctxB.copyToHost(rB).reshape(nP, aslm)[:, :slm]
+    err = np.abs(got - want).max() / np.abs(want).max()
+    ctxB.computeLhsPrec(1, dB, rB)
+    err32 = np.abs(ctxB.copyToHost(rB).reshape(nP, aslm)[:, :slm] - want).max() / np.abs(want).max()
+    ctxB.computeLhsDiag(dB, rB)  # the Diag operator runs with the general faces
+    assert np.isfinite(ctxB.copyToHost(rB)).all()
+    # the reference's axis rule on the rotated mesh
+    ctx0 = ta.HipContext(B, ang)
+    d0, r0 = ctx0.vector_from(fB.ravel()), ctx0.allocVector()
+    ctx0.computeLhs(d0, r0)
+    err0 = np.abs(ctx0.copyToHost(r0).reshape(nP, aslm)[:, :slm] - want).max() / np.abs(want).max()
+    print("[gpu-normals] %s: A_B T f vs T A_A f rel err %.2e (FP32 operator %.2e); axis rule %.2e" % (
+        kind, err, err32, err0), flush=True)
+    assert err <= 1e-12 and err32 <= 1e-5 and err0 > 1e-3
+
+
+def test_solutions_agree_in_scalar_radiation(meshes):
+    A, coords, tets, kappa, Ip = meshes
+    R = rotation("generic")
+    B = ta.Mesh.from_arrays(coords @ R.T, tets, kappa, Ip)
+    ang = ta.Angular(K)
+    out = {}
+    for name, mesh, gen, solver, prec in (("A", A, False, ta.TR_SOLVER_PCG_FP64, 2),
+                                         ("B", B, True, ta.TR_SOLVER_PCG_FP64, 2),
+                                         ("B-mixed-cheb", B, True, ta.TR_SOLVER_MIXED, 4),
+                                         ("B-axis-rule", B, False, ta.TR_SOLVER_PCG_FP64, 2)):
+        ctx = ta.HipContext(mesh, ang, general_normals=gen)
+        d = ctx.allocVector()
+        st, _ = ctx.solve(d, solver=solver, use_prec=prec, maxit=5000 if solver != ta.TR_SOLVER_MIXED else 30,
+                          tol_abs=0.0, tol_rel=1e-11 if solver != ta.TR_SOLVER_MIXED else 1e-9, inner_maxit=5000,
+                          inner_tol=1e-4)
+        assert st.status == 0
+        out[name] = ctx.moments(d)[0]
+        print("[gpu-normals] %s: %d iterations (%d inner), true rel %.2e" % (name, st.iters, st.preck, st.true_rel),
+              flush=True)
+    ref = np.abs(out["A"]).max()
+    e = {k: np.abs(out[k] - out["A"]).max() / ref for k in ("B", "B-mixed-cheb", "B-axis-rule")}
+    print("[gpu-normals] U against the unrotated solve: %s" % e, flush=True)
+    assert e["B"] <= 1e-8 and e["B-mixed-cheb"] <= 1e-6 and e["B-axis-rule"] > 1e-3
+    with pytest.raises(ta.TransferError, match="axes"):
+        ctx = ta.HipContext(B, ang, general_normals=True)
+        ctx.solve(ctx.allocVector(), solver=ta.TR_SOLVER_PCG_FP64, use_prec=3, maxit=10, tol_abs=0.0, tol_rel=1e-8)

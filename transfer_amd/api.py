@@ -22,6 +22,7 @@ u8p = C.POINTER(C.c_uint8)
 f32p = C.POINTER(C.c_float)
 vp = C.c_void_p
 
+TR_CTX_GENERAL_NORMALS = 1
 TR_SOLVER_REFERENCE = 0
 TR_SOLVER_PCG_FP64 = 1
 TR_SOLVER_MIXED = 2
@@ -398,11 +399,12 @@ class HipContext:
     Device vectors are opaque handles (int addresses).  Host arrays use the
     reference layout [nP][aslm] in the caller's vertex order."""
 
-    def __init__(self, mesh, ang, device=0, mesh_view=None, ang_view=None):
+    def __init__(self, mesh, ang, device=0, mesh_view=None, ang_view=None, general_normals=False):
         self._h = vp()
         mv = mesh_view if mesh_view is not None else mesh.view
         av = ang_view if ang_view is not None else ang.view
-        check(lib.tr_ctx_create(int(device), C.byref(mv), C.byref(av), 0, C.byref(self._h)))
+        flags = TR_CTX_GENERAL_NORMALS if general_normals else 0
+        check(lib.tr_ctx_create(int(device), C.byref(mv), C.byref(av), flags, C.byref(self._h)))
         self._keep = (mesh, ang, mv, av)
         self._sizes()
 

@@ -12,6 +12,7 @@
 #include <stdexcept>
 
 #include "internal.h"
+#include "stencil.h"
 
 namespace tr {
 
@@ -221,6 +222,46 @@ void build_angular(int maxk, double albedo, double g, int hg_lmax, Angular &a) {
                     for (int a2 = 0; a2 < slm; a2++)
                         O[(size_t)a1 * slm + a2] += w * mu * Ye[a1] * Ye[a2];
             }
+    }
+}
+
+}  // namespace tr
+
+namespace tr {
+
+void abs_normal_matrix(int maxk, const double nn[3], std::vector<double> &O) {
+    const int slm = (1 + maxk) * (1 + 2 * maxk), lmax = 2 * maxk;
+    const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    const double n[3] = {nn[0] / len, nn[1] / len, nn[2] / len};
+    // an orthonormal frame (t1, t2, n)
+    double t1[3];
+    if (std::fabs(n[0]) < 0.9) {
+        t1[0] = 0; t1[1] = -n[2]; t1[2] = n[1];
+    } else {
+        t1[0] = n[2]; t1[1] = 0; t1[2] = -n[0];
+    }
+    const double l1 = std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+    for (double &x : t1) x /= l1;
+    const double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+    std::vector<double> gx, gw;
+    const int nh = 2 * maxk + 2, M = 4 * maxk + 6;
+    gauss_legendre(nh, gx, gw);
+    std::vector<double> Yall((size_t)(lmax + 1) * (lmax + 1)), Y(slm);
+    O.assign((size_t)slm * slm, 0.0);
+    for (int i = 0; i < nh; i++) {
+        const double mu = 0.5 * (gx[i] + 1.0), wmu = 0.5 * gw[i], s = std::sqrt(std::fmax(0.0, 1 - mu * mu));
+        for (int j = 0; j < M; j++) {
+            const double ph = 2 * M_PI * (j + 0.5) / M, c = std::cos(ph), sn = std::sin(ph);
+            double d[3];
+            for (int a = 0; a < 3; a++) d[a] = mu * n[a] + s * (c * t1[a] + sn * t2[a]);
+            sh_eval_all(lmax, d[0], d[1], d[2], Yall.data());
+            for (int l = 0, lm = 0; l <= lmax; l += 2)
+                for (int m = -l; m <= l; m++, lm++) Y[lm] = Yall[l * l + l + m];
+            // (1/2pi) int over the hemisphere of mu Y Y' = sum wmu (1/M) mu Y Y'
+            const double w = wmu * mu / M;
+            for (int a1 = 0; a1 < slm; a1++)
+                for (int a2 = 0; a2 < slm; a2++) O[(size_t)a1 * slm + a2] += w * Y[a1] * Y[a2];
+        }
     }
 }
 

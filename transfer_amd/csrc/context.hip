@@ -364,11 +364,12 @@ extern "C" {
 
 int tr_ctx_create(int device, const tr_mesh_view *mesh, const tr_angular_view *ang, int flags,
                   tr_ctx **out) {
-    (void)flags;
     if (!mesh || !ang || !out) return fail(TR_ERR_ARG, "tr_ctx_create: null argument");
+    if (flags & ~TR_CTX_GENERAL_NORMALS) return fail(TR_ERR_ARG, "tr_ctx_create: unknown flags");
     *out = nullptr;
     tr_ctx *c = new tr_ctx();
     c->device = device;
+    c->op.general_normals = (flags & TR_CTX_GENERAL_NORMALS) != 0;
     int rc = assemble_operator(*mesh, *ang, c->op);
     if (rc) {
         delete c;
@@ -415,6 +416,11 @@ int tr::ctx_init(tr_ctx *c, int device) {
         (!op.mf0.empty() && (rc = upload(c, &c->d_mf0, op.mf0))) ||
         (!op.vsum.empty() && (rc = upload(c, &c->d_vsum, op.vsum))) ||
         (!op.xyz.empty() && (rc = upload(c, &c->d_xyz, op.xyz))) ||
+        (!op.gvert.empty() &&
+         ((rc = upload(c, &c->d_gvert, op.gvert)) || (rc = upload(c, &c->d_gstart, op.gstart)) ||
+          (rc = upload(c, &c->d_gface, op.gface)) || (rc = upload(c, &c->d_gpos, op.gpos)) ||
+          (rc = upload(c, &c->d_gfv, op.gfv)) || (rc = upload(c, &c->d_garea, op.garea)) ||
+          (rc = upload(c, &c->d_gO, op.gO)))) ||
         (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
         (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
         tr_ctx_destroy(c);
@@ -787,6 +793,15 @@ std::vector<double> operator_diagonal(const HostOperator &op) {
                     }
             d[(size_t)v * slm + lm] = s;
         }
+    // faces off the axes: O(n_F)[lm][lm] (S_F/12) 2 for the vertex's own term
+    for (size_t b = 0; b < op.gvert.size(); b++) {
+        const uint32_t v = op.gvert[b];
+        for (uint32_t j = op.gstart[b]; j < op.gstart[b + 1]; j++) {
+            const uint32_t F = op.gface[j];
+            for (uint32_t lm = 0; lm < slm; lm++)
+                d[(size_t)v * slm + lm] += 2.0 * op.garea[F] * op.gO[((size_t)F * slm + lm) * slm + lm];
+        }
+    }
     return d;
 }
 
@@ -871,6 +886,7 @@ struct ChebPrec {
     // r.z partials the last step writes (matrix-core schedules, at least
     // one step, at most 1024 workgroups), else 0
     int nparts() const {
+        if (a.ngv) return 0;  // the steps with faces off the axes are not fused
         const uint32_t p = cheb_parts(a.nP, map);
         return (!om.empty() && p <= 1024) ? (int)p : 0;
     }
@@ -1019,6 +1035,8 @@ int ensure_fp32(tr_ctx *c) {
         (rc = upload(c, &c->f_bdiag, to_f32(op.bdiag))) || (rc = upload(c, &c->f_crec, coef_records<float>(op))) ||
         (rc = upload(c, &c->f_bcrec, bcoef_records<float>(op))) ||
         (!op.mf0.empty() && (rc = upload(c, &c->f_mf0, to_f32(op.mf0)))) ||
+        (!op.gvert.empty() &&
+         ((rc = upload(c, &c->f_garea, to_f32(op.garea))) || (rc = upload(c, &c->f_gO, to_f32(op.gO))))) ||
         (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
         return rc;
     if (v5_ok(op) && (rc = upload(c, (unsigned char **)&c->f_tab5, mfma_tables5(op)))) return rc;

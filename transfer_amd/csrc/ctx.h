@@ -44,6 +44,10 @@ struct tr_ctx {
     double *d_mf0 = nullptr;       // [nP] lm = 0 mass factor per vertex (batches), or null
     float *f_mf0 = nullptr;
     double *d_vsum = nullptr, *d_xyz = nullptr;  // learned-guess feature inputs [3][nP] each, or null
+    // vacuum faces off the axes (general normals, see stencil.h)
+    uint32_t *d_gvert = nullptr, *d_gstart = nullptr, *d_gface = nullptr, *d_gpos = nullptr, *d_gfv = nullptr;
+    double *d_garea = nullptr, *d_gO = nullptr;
+    float *f_garea = nullptr, *f_gO = nullptr;
     double *d_feat_tmp = nullptr;  // [2][nP] kappa and optical depth + 8 flags (guess.hip), on first use
     float *d_guess_tmp = nullptr;  // [9][nP] features and prediction (guess.hip), on first use
     double *d_mom_val = nullptr;   // moments table [3][slm]
@@ -109,6 +113,14 @@ struct tr_ctx {
         a.nmask = (uint32_t)op.mask.size();
         a.mf0 = d_mf0;
         a.tiles = d_tiles;
+        a.ngv = (uint32_t)op.gvert.size();
+        a.gvert = d_gvert;
+        a.gstart = d_gstart;
+        a.gface = d_gface;
+        a.gpos = d_gpos;
+        a.gfv = d_gfv;
+        a.garea = d_garea;
+        a.gO = d_gO;
         return a;
     }
     tr::OpArgs<float> args32() const { return args32(map32); }
@@ -147,6 +159,14 @@ struct tr_ctx {
         a.nmask = d.nmask;
         a.mf0 = f_mf0;
         a.tiles = d.tiles;
+        a.ngv = d.ngv;
+        a.gvert = d.gvert;
+        a.gstart = d.gstart;
+        a.gface = d.gface;
+        a.gpos = d.gpos;
+        a.gfv = d.gfv;
+        a.garea = f_garea;
+        a.gO = f_gO;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }

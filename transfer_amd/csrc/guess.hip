@@ -87,6 +87,12 @@ __global__ __launch_bounds__(256) void k_feat_final(uint32_t nP, uint32_t W, con
     }
 }
 
+// tau = 0 on the vertices of vacuum faces off the axes (not in bof)
+__global__ __launch_bounds__(256) void k_feat_gzero(uint32_t ngv, const uint32_t *__restrict__ gvert,
+                                                    double *__restrict__ tau) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ngv; i += gridDim.x * blockDim.x) tau[gvert[i]] = 0.0;
+}
+
 // f[v][00] = y[v] * I_p / (1 - albedo), zero elsewhere; I_p = ip_scale, or
 // exp(feature 7) (the vertex's own source) when ip_scale <= 0
 __global__ __launch_bounds__(256) void k_learned_guess(const float *__restrict__ y, const float *__restrict__ feat,
@@ -131,6 +137,8 @@ extern "C" int tr_ctx_mlp_features(tr_ctx *c, double albedo, float *feat, int *s
     int *flags = (int *)(tau + nP);
     const unsigned g = tr::blocks_of(nP);
     hipLaunchKernelGGL(tr::k_feat_init, dim3(g), dim3(256), 0, c->stream, nP, c->d_vsum, c->d_bof, kap, tau);
+    const uint32_t ngv = (uint32_t)c->op.gvert.size();
+    if (ngv) hipLaunchKernelGGL(tr::k_feat_gzero, dim3(tr::blocks_of(ngv)), dim3(256), 0, c->stream, ngv, c->d_gvert, tau);
     HIPCHECK(hipGetLastError());
     // chunks of 8 sweeps; converged when the chunk's last sweep changed
     // nothing (at most nP sweeps: Bellman-Ford's bound)

@@ -57,6 +57,11 @@ struct OpArgs {
     const T *mf0;          // [nP] per-vertex factor of the lm = 0 mass term (batches), may be null
     const uint32_t *tiles; // variant 4: [ceil(nP/16)] a permutation of the 16-vertex wavefront
                            // tiles, the order of the walk; null = natural order
+    // vacuum faces off the axes with their exact angular matrices (see
+    // stencil.h), applied by k_bgen after the operator kernel; ngv = 0: none
+    uint32_t ngv = 0;
+    const uint32_t *gvert = nullptr, *gstart = nullptr, *gface = nullptr, *gpos = nullptr, *gfv = nullptr;
+    const T *garea = nullptr, *gO = nullptr;
 };
 
 // One step of the Chebyshev iteration on D^-1 A z = D^-1 rhs, written as

@@ -585,11 +585,14 @@ template <typename T>
 hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                         double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
+    // a fused dot product would miss the faces off the axes
+    if (part && a.ngv) return hipErrorInvalidValue;
     if (m.variant == 5) {
         if constexpr (sizeof(T) == 8) {
             return hipErrorInvalidValue;
         } else {
             hipError_t e = launch_apply5(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+            if (e == hipSuccess) e = launch_bgen(a, false, f, r, skip, s);
             return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
         }
     }
@@ -601,6 +604,7 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
             e = launch_apply4<T, 2>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
         else
             e = launch_apply4<T, 1>(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+        if (e == hipSuccess) e = launch_bgen(a, false, f, r, skip, s);
         return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
     }
     dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
@@ -637,7 +641,54 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
 #undef TR_CASE
 #undef TR_LAUNCH
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_bgen(a, false, f, r, skip, s);
     return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
+}
+
+// The vacuum faces off the axes (general normals): for the vertex v of
+// block b, r_v += sum over its faces F of O(n_F) g_F, g_F = sum_{k in F}
+// (S_F/12)(1 + delta_vk) f_k (DIAG: the diagonal of O(n_F) only, the Diag
+// operator's boundary term).  One workgroup per such vertex, a thread per
+// harmonic; g_F staged in LDS.
+template <typename T, bool DIAG>
+__global__ __launch_bounds__(128) void k_bgen(OpArgs<T> a, const T *__restrict__ f, T *__restrict__ r,
+                                              const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    __shared__ T g[128];
+    const uint32_t slm = a.slm, t = threadIdx.x, b = blockIdx.x;
+    const uint32_t v = a.gvert[b];
+    T acc = 0;
+    for (uint32_t j = a.gstart[b]; j < a.gstart[b + 1]; j++) {
+        const uint32_t F = a.gface[j], li = a.gpos[j];
+        const T w = a.garea[F];
+        if (t < slm) {
+            T s = 0;
+            for (uint32_t k = 0; k < 3; k++) s += (k == li ? 2 * w : w) * f[(size_t)a.gfv[3 * F + k] * slm + t];
+            g[t] = s;
+        }
+        __syncthreads();
+        if (t < slm) {
+            const T *O = a.gO + ((size_t)F * slm + t) * slm;
+            if (DIAG) {
+                acc += O[t] * g[t];
+            } else {
+                for (uint32_t m = 0; m < slm; m++) acc += O[m] * g[m];
+            }
+        }
+        __syncthreads();
+    }
+    if (t < slm) r[(size_t)v * slm + t] += acc;
+}
+
+template <typename T>
+static hipError_t launch_bgen(const OpArgs<T> &a, bool diag, const T *f, T *r, const double *skip, hipStream_t s) {
+    if (!a.ngv) return hipSuccess;
+    if (a.slm > 128) return hipErrorInvalidValue;
+    if (diag)
+        hipLaunchKernelGGL((k_bgen<T, true>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip);
+    else
+        hipLaunchKernelGGL((k_bgen<T, false>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip);
+    return hipGetLastError();
 }
 
 // out = zc z - om out + ga dinv (rhs - t), t = A z (the unfused Chebyshev
@@ -667,7 +718,7 @@ hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const 
     hipError_t e;
     // (a fused dot product also covers masked entries: rhs, a CG residual,
     // is zero there)
-    if (m.variant == 5) {
+    if (m.variant == 5 && !a.ngv) {
         if constexpr (sizeof(T) == 8) {
             return hipErrorInvalidValue;
         } else {
@@ -675,7 +726,7 @@ hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const 
             return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
         }
     }
-    if (m.variant == 4) {
+    if (m.variant == 4 && !a.ngv) {
         if constexpr (sizeof(T) == 8)
             e = launch_apply4<T, 0>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
         else if (m.mfma_mode == 2)
@@ -696,6 +747,7 @@ template <typename T>
 hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f, T *r,
                              double *part, const double *skip, hipStream_t s) {
     if (a.nP == 0) return hipSuccess;
+    if (part && a.ngv) return hipErrorInvalidValue;
     dim3 g(apply_tiles(a.nP, m)), b(TR_WAVE * m.WPB);
 #define TR_CASE(JJ)                                                                          \
     case JJ:                                                                                 \
@@ -710,6 +762,7 @@ hipError_t launch_apply_diag(const OpArgs<T> &a, int NC, const LaneMap &m, const
     }
 #undef TR_CASE
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_bgen(a, true, f, r, skip, s);
     return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
 }
 

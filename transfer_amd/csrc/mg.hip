@@ -178,6 +178,8 @@ static int build_levels(tr_ctx *c, Multilevel *mg, uint32_t min_coarse, uint32_t
 
 int build_multilevel(tr_ctx *c, bool f32, uint32_t min_coarse, uint32_t size) {
     int rc;
+    if (!c->op.gvert.empty())
+        return fail(TR_ERR_ARG, "multilevel preconditioner: not with faces off the axes (TR_CTX_GENERAL_NORMALS)");
     if (!c->mg) {
         // assembled aside and attached only when complete
         Multilevel *mg = new Multilevel();

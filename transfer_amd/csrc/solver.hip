@@ -233,8 +233,10 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
     const size_t n = S.n;
     const unsigned G = upd_grid(n);
     // dot partials written by the operator launch (the angular-diagonal
-    // kernel is never persistent)
-    const uint32_t nb = prm.diag_op ? apply_tiles(S.a.nP, S.map) : apply_blocks(S.a.nP, S.map);
+    // kernel is never persistent); with faces off the axes the boundary
+    // term comes after the operator kernel, so p.Ap is a pass of its own
+    const bool fdot = S.a.ngv == 0;
+    const uint32_t nb = !fdot ? G : (prm.diag_op ? apply_tiles(S.a.nP, S.map) : apply_blocks(S.a.nP, S.map));
     auto t0 = std::chrono::steady_clock::now();
     hipError_t e;
 #define HC(x)                                                                               \
@@ -265,9 +267,12 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
     for (int it = 0; it < chunk; it++) {
         const int rz_old = (it & 1) ? SC_RZ2 : SC_RZ, rz_new = (it & 1) ? SC_RZ : SC_RZ2;
         if (prm.diag_op)
-            launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
+            launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st);
         else
-            launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, S.part, S.sc + SC_CONV, st);
+            launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st);
+        if (!fdot)
+            hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, (const T *)S.p, (const T *)S.Ap, (T *)nullptr, n,
+                               S.part, S.sc);
         // a polynomial preconditioner's first iterate comes out of the
         // update pass, and its last step may write the r.z partials itself
         const bool first = gen && S.prec_loop && S.pre_out;

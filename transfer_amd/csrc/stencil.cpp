@@ -271,17 +271,40 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
 
     // ---- vacuum-boundary stencil
     op.bvert.clear();
+    op.gvert.clear();
+    op.gstart.assign(1, 0);
+    op.gface.clear();
+    op.gpos.clear();
+    std::vector<int64_t> gid(op.general_normals ? mv.nF : 0, -1);  // face -> general face index
+    std::vector<uint32_t> gfaces;                                   // general face index -> face
     std::vector<std::vector<uint32_t>> bsl;
     std::vector<std::vector<double>> bco;
     for (uint32_t i = 0; i < nP; i++) {
         uint32_t vo = op.perm[i];
         std::vector<uint32_t> sl;
         std::vector<double> co;
+        const size_t g0 = op.gface.size();
         for (uint32_t j = mv.facestart[vo]; j < mv.facestart[vo + 1]; j++) {
             uint32_t fi = mv.faceidx[j];
             if (mv.face_kind && mv.face_kind[fi] != 0) continue;
             const tr_face &F = mv.bnd[fi];
             const uint32_t lf = mv.facepos[j];
+            if (op.general_normals) {
+                // faces not along an axis take their exact angular matrix
+                // (the reference's axis rule, kernels.cu:359-361, is exact
+                // only along the axes)
+                const double S = std::sqrt(F.s[0] * F.s[0] + F.s[1] * F.s[1] + F.s[2] * F.s[2]);
+                const double mx = std::max({std::fabs(F.s[0]), std::fabs(F.s[1]), std::fabs(F.s[2])});
+                if (S > 0 && mx < (1.0 - 1e-12) * S) {
+                    if (gid[fi] < 0) {
+                        gid[fi] = (int64_t)gfaces.size();
+                        gfaces.push_back(fi);
+                    }
+                    op.gface.push_back((uint32_t)gid[fi]);
+                    op.gpos.push_back(lf);
+                    continue;
+                }
+            }
             double surf = std::fabs(F.s[0] + F.s[1] + F.s[2]);
             int axis = (2 * std::fabs(F.s[0]) > surf) ? 0 : ((2 * std::fabs(F.s[1]) > surf) ? 1 : 2);
             for (int k = 0; k < 3; k++) {
@@ -298,6 +321,27 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
             op.bvert.push_back(i);
             bsl.push_back(sl);
             bco.push_back(co);
+        }
+        if (op.gface.size() > g0) {
+            op.gvert.push_back(i);
+            op.gstart.push_back((uint32_t)op.gface.size());
+        }
+    }
+    // general faces: vertices (new numbering), S/12 and <|Omega.n| Y Y'>
+    op.gfv.resize(3 * gfaces.size());
+    op.garea.resize(gfaces.size());
+    op.gO.resize(gfaces.size() * (size_t)slm * slm);
+    if (!gfaces.empty()) {
+        int maxk = 0;
+        while ((uint32_t)((1 + maxk) * (1 + 2 * maxk)) < slm) maxk++;
+        if ((uint32_t)((1 + maxk) * (1 + 2 * maxk)) != slm) return fail(TR_ERR_ARG, "assemble_operator: slm is not (1+K)(1+2K)");
+        std::vector<double> O;
+        for (size_t g = 0; g < gfaces.size(); g++) {
+            const tr_face &F = mv.bnd[gfaces[g]];
+            for (int k = 0; k < 3; k++) op.gfv[3 * g + k] = op.iperm[F.p[k]];
+            op.garea[g] = std::sqrt(F.s[0] * F.s[0] + F.s[1] * F.s[1] + F.s[2] * F.s[2]) / 12.0;
+            abs_normal_matrix(maxk, F.s, O);
+            std::copy(O.begin(), O.end(), op.gO.begin() + g * (size_t)slm * slm);
         }
     }
     op.nB = (uint32_t)op.bvert.size();

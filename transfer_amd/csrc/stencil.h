@@ -72,7 +72,22 @@ struct HostOperator {
     // levels): vsum[0..2][nP] = sums over the incident tets of V, kappa V
     // and kappa V I_p; xyz[3][nP] = the vertex coordinates
     std::vector<double> vsum, xyz;
+    // vacuum faces whose normal is not along an axis, with their exact
+    // angular matrices (extension; only when general_normals is set before
+    // assembly, else such faces take the reference's axis rule):
+    //   r_i += O(n_F) sum_{k in F} (S_F / 12)(1 + delta_ik) f_k
+    // gvert[ngv] the vertices with such faces, gstart[ngv+1] into
+    // gface/gpos (face, the vertex's place in it), gfv[nGF][3] the faces'
+    // vertices, garea[nGF] = S_F / 12, gO[nGF][slm][slm] = <|Omega.n| Y Y'>
+    bool general_normals = false;
+    std::vector<uint32_t> gvert, gstart, gface, gpos, gfv;
+    std::vector<double> garea, gO;
 };
+
+// <|Omega.n| Y_out Y_in> for the unit normal n: [slm][slm] (symmetric),
+// exact (a Gauss-Legendre x trapezoid rule on the hemisphere around n; the
+// integrand is even, and a polynomial of degree 4K+1 there).
+void abs_normal_matrix(int maxk, const double n[3], std::vector<double> &O);
 
 // The block-diagonal union of independent operators with identical angular
 // data (the problems of a batch), vertices of problem p at
