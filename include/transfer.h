@@ -341,9 +341,9 @@ typedef struct tr_batch_stats {
 } tr_batch_stats;
 
 /* Solves every problem of a batch from f = 0 to its own stopping test
- * (TR_SOLVER_PCG_FP64 or TR_SOLVER_MIXED; use_prec 0 or 2 = Jacobi) with
- * per-problem CG scalars; one operator launch per iteration covers all
- * problems.  total: aggregate (iters = max, preck = sum of inner, true_rel =
+ * (TR_SOLVER_PCG_FP64 or TR_SOLVER_MIXED; use_prec 0, 2 = Jacobi or 4 =
+ * Chebyshev with one lambda_max for the whole union) with per-problem CG
+ * scalars; one operator launch per iteration covers all problems.  total: aggregate (iters = max, preck = sum of inner, true_rel =
  * worst problem); per: [nprob] or NULL. */
 int tr_batch_solve(tr_ctx *ctx, const tr_solve_params *prm, double *f,
                    tr_solve_stats *total, tr_batch_stats *per);

@@ -59,15 +59,19 @@ def test_batch_operator_matches_each_problem():
 
 
 @pytest.mark.parametrize("solver", [ta.TR_SOLVER_PCG_FP64, ta.TR_SOLVER_MIXED])
-def test_batch_solve_matches_individual_solves(solver):
+@pytest.mark.parametrize("prec", [2, 4])
+def test_batch_solve_matches_individual_solves(solver, prec):
+    """Every problem of a batched solve, with Jacobi (2) or the Chebyshev
+    polynomial over the union (4), reaches the FP64 residual and the
+    solution of the same problem solved alone."""
     B = small_batch()
-    print("[gpu-batch] solve solver=%d" % solver, flush=True)
+    print("[gpu-batch] solve solver=%d prec=%d" % (solver, prec), flush=True)
     ctx = ta.Batch(B.meshes, B.ang, B.albedo)
     ctx.autotune(0)
     if solver == ta.TR_SOLVER_MIXED:
         ctx.autotune(1)
     d_x = ctx.allocVector()
-    st, per = ctx.solve_batch(d_x, solver=solver, use_prec=2, tol_rel=1e-8)
+    st, per = ctx.solve_batch(d_x, solver=solver, use_prec=prec, tol_rel=1e-8)
     assert st.status == 0
     x = ctx.copyToHost(d_x).reshape(ctx.nP, ctx.aslm)
     for p, m in enumerate(B.meshes):

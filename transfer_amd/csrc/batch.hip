@@ -20,6 +20,7 @@
 #include <thread>
 #include <vector>
 
+#include "cheb.h"
 #include "ctx.h"
 #include "internal.h"
 #include "kernels.h"
@@ -103,7 +104,8 @@ __global__ void k_bfin_alpha(const double *__restrict__ part, const uint32_t *__
     }
 }
 
-// x += a p, r -= a Ap, z = D^-1 r; partials r.r and r.z
+// x += a p, r -= a Ap, z = D^-1 r; partials r.r and r.z; with z1 != null
+// also z1 = cscale cdinv r (a Chebyshev preconditioner's first iterate)
 template <typename T>
 __global__ __launch_bounds__(TR_BLK_THREADS) void k_bupdate(T *__restrict__ x, const T *__restrict__ p,
                                                             T *__restrict__ r, const T *__restrict__ Ap,
@@ -112,7 +114,8 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bupdate(T *__restrict__ x, c
                                                             const uint32_t *__restrict__ hi,
                                                             const uint32_t *__restrict__ prob,
                                                             const double *__restrict__ sc, double *__restrict__ part,
-                                                            uint32_t nblk, const double *__restrict__ skip) {
+                                                            uint32_t nblk, const double *__restrict__ skip,
+                                                            const T *__restrict__ cdinv, T cscale, T *__restrict__ z1) {
     if (skip && *skip != 0.0) return;
     const double a = sc[(size_t)prob[blockIdx.x] * BS_N + BS_ALPHA];
     double rr = 0, rz = 0, w = 0;
@@ -120,6 +123,7 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bupdate(T *__restrict__ x, c
         x[i] = (T)((double)x[i] + a * (double)p[i]);
         const T ri = (T)((double)r[i] - a * (double)Ap[i]);
         r[i] = ri;
+        if (z1) z1[i] = cscale * cdinv[i] * ri;
         const double zi = dinv ? (double)dinv[i] * (double)ri : (double)ri;
         rr = fma((double)ri, (double)ri, rr);
         rz = fma((double)ri, zi, rz);
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bfin_beta(const double *__re
     if (threadIdx.x == 0 && !running) all[0] = 1;
 }
 
-// p = D^-1 r + beta p
+// p = z + beta p, z = zb (a preconditioner's output) or D^-1 r
 template <typename T>
 __global__ __launch_bounds__(TR_BLK_THREADS) void k_bpupdate(T *__restrict__ p, const T *__restrict__ r,
                                                              const T *__restrict__ dinv,
@@ -173,11 +177,12 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bpupdate(T *__restrict__ p, 
                                                              const uint32_t *__restrict__ hi,
                                                              const uint32_t *__restrict__ prob,
                                                              const double *__restrict__ sc,
-                                                             const double *__restrict__ skip) {
+                                                             const double *__restrict__ skip,
+                                                             const T *__restrict__ zb) {
     if (skip && *skip != 0.0) return;
     const double be = sc[(size_t)prob[blockIdx.x] * BS_N + BS_BETA];
     for (uint32_t i = lo[blockIdx.x] + threadIdx.x; i < hi[blockIdx.x]; i += TR_BLK_THREADS) {
-        const double z = dinv ? (double)dinv[i] * (double)r[i] : (double)r[i];
+        const double z = zb ? (double)zb[i] : (dinv ? (double)dinv[i] * (double)r[i] : (double)r[i]);
         p[i] = (T)(z + be * (double)p[i]);
     }
 }
@@ -317,16 +322,27 @@ struct BatchCg {
 
 // Batched PCG from the x held on entry (T = double with the FP64 operator,
 // float with the FP32 one).  sc: nprob*BS_N doubles; part >= 3*nblk.
+// With cheb (z: its output buffer) the preconditioner is the Chebyshev
+// polynomial over the union: one interval [lambda_max / ratio, lambda_max]
+// with lambda_max of the whole union's D^-1 A bounds every problem's
+// spectrum, so it is a valid SPD preconditioner for each of them.
 template <typename T>
 int pcg_batch(tr_ctx *c, const BlockTable &t, const OpArgs<T> &a, const LaneMap &map, T *x, const T *b, T *r, T *p,
               T *Ap, const T *dinv, double *part, double *sc, double *all, const BatchCg &prm, hipStream_t st,
-              int &launches) {
+              int &launches, const ChebPrec<T> *cheb = nullptr, T *z = nullptr) {
     const int NC = (int)c->op.NC;
     const dim3 gb(t.nblk), tb(TR_BLK_THREADS);
+    if (cheb && !z) return fail(TR_ERR_ARG, "batch: Chebyshev preconditioner without its buffer");
     HIPCHECK(launch_apply<T>(a, NC, map, x, r, nullptr, nullptr, st));
     hipLaunchKernelGGL(k_binit<T>, gb, tb, 0, st, b, r, p, dinv, t.lo, t.hi, part, t.nblk);
+    if (cheb) {
+        HIPCHECK(cheb->apply(r, z, nullptr, st));
+        hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, r, z, t.lo, t.hi, part + t.nblk, nullptr);
+    }
     hipLaunchKernelGGL(k_bfin_init, dim3(1), tb, 0, st, part, t.b0, t.nprob, t.nblk, sc, prm.tol_abs, prm.tol_rel,
                        all);
+    if (cheb)  // p = z (beta = 0 after k_bfin_init)
+        hipLaunchKernelGGL(k_bpupdate<T>, gb, tb, 0, st, p, r, dinv, t.lo, t.hi, t.prob, sc, all, (const T *)z);
     HIPCHECK(hipGetLastError());
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -335,9 +351,16 @@ int pcg_batch(tr_ctx *c, const BlockTable &t, const OpArgs<T> &a, const LaneMap 
         launch_apply<T>(a, NC, map, p, Ap, nullptr, all, st);
         hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, p, Ap, t.lo, t.hi, part, all);
         hipLaunchKernelGGL(k_bfin_alpha, dim3((t.nprob + 255) / 256), dim3(256), 0, st, part, t.b0, t.nprob, sc, all);
-        hipLaunchKernelGGL(k_bupdate<T>, gb, tb, 0, st, x, p, r, Ap, dinv, t.lo, t.hi, t.prob, sc, part, t.nblk, all);
+        hipLaunchKernelGGL(k_bupdate<T>, gb, tb, 0, st, x, p, r, Ap, cheb ? (const T *)nullptr : dinv, t.lo, t.hi,
+                           t.prob, sc, part, t.nblk, all, cheb ? cheb->dinv : (const T *)nullptr,
+                           cheb ? (T)cheb->inv_theta : (T)0, cheb ? cheb->first_buffer(z) : (T *)nullptr);
+        if (cheb) {
+            cheb->steps(r, z, all, st, nullptr);
+            hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, r, z, t.lo, t.hi, part + t.nblk, all);
+        }
         hipLaunchKernelGGL(k_bfin_beta, dim3(1), tb, 0, st, part, t.b0, t.nprob, t.nblk, sc, prm.maxit, all);
-        hipLaunchKernelGGL(k_bpupdate<T>, gb, tb, 0, st, p, r, dinv, t.lo, t.hi, t.prob, sc, all);
+        hipLaunchKernelGGL(k_bpupdate<T>, gb, tb, 0, st, p, r, dinv, t.lo, t.hi, t.prob, sc, all,
+                           cheb ? (const T *)z : (const T *)nullptr);
     }
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
@@ -427,7 +450,8 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
     if (c->voff.empty()) c->voff = {0, c->op.nP};
     if (prm->solver != TR_SOLVER_PCG_FP64 && prm->solver != TR_SOLVER_MIXED)
         return fail(TR_ERR_ARG, "tr_batch_solve: TR_SOLVER_PCG_FP64 or TR_SOLVER_MIXED");
-    if (prm->use_prec != 0 && prm->use_prec != 2) return fail(TR_ERR_ARG, "tr_batch_solve: use_prec 0 or 2");
+    if (prm->use_prec != 0 && prm->use_prec != 2 && prm->use_prec != 4)
+        return fail(TR_ERR_ARG, "tr_batch_solve: use_prec 0 (none), 2 (Jacobi) or 4 (Chebyshev)");
     const bool mixed = prm->solver == TR_SOLVER_MIXED;
     int rc;
     if (mixed && (rc = ensure_fp32(c))) return rc;
@@ -444,7 +468,11 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
     float *dinv32 = nullptr;
     if (!b || !r || !p || !Ap || !part || !sc || !all || !fac) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
     auto t0 = std::chrono::steady_clock::now();
-    if (prm->use_prec == 2) {
+    ChebPrec<double> cheb;
+    ChebPrec<float> cheb32;
+    double *zc = nullptr;
+    float *zc32 = nullptr;
+    if (prm->use_prec == 2 || prm->use_prec == 4) {
         std::vector<double> d = operator_diagonal(c->op);
         for (double &v : d) v = 1.0 / v;
         if (!(dinv = m.get<double>(n))) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
@@ -456,6 +484,36 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
             HIPCHECK(hipStreamSynchronize(st));  // d32 is a temporary
         }
         HIPCHECK(hipStreamSynchronize(st));  // so is d
+        if (prm->use_prec == 4) {
+            // one lambda_max for the union bounds every problem's D^-1 A
+            if ((rc = cheb_lambda(c, d))) return rc;
+            if (!mixed) {
+                cheb.a = c->args();
+                cheb.NC = (int)c->op.NC;
+                cheb.map = c->map;
+                cheb.dinv = dinv;
+                cheb.n = n;
+                cheb.zb = m.get<double>(n);
+                cheb.tmp = m.get<double>(n);
+                zc = m.get<double>(n);
+                if (!zc) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
+                cheb.setup(c->cheb_deg, c->cheb_lmax, c->cheb_ratio);
+            } else {
+                cheb32.a = c->args32();
+                if (c->cheb_split > 0 && c->cheb_split < cheb32.a.nsplit &&
+                    (c->map32.variant == 4 || c->map32.variant == 5) && c->map32.mfma_mode == 2)
+                    cheb32.a.nsplit = c->cheb_split;
+                cheb32.NC = (int)c->op.NC;
+                cheb32.map = c->map32;
+                cheb32.dinv = dinv32;
+                cheb32.n = n;
+                cheb32.zb = m.get<float>(n);
+                cheb32.tmp = m.get<float>(n);
+                zc32 = m.get<float>(n);
+                if (!zc32) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
+                cheb32.setup(c->cheb_deg, c->cheb_lmax, c->cheb_ratio);
+            }
+        }
     }
     HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, b, st));
     HIPCHECK(hipMemsetAsync(x, 0, sizeof(double) * n, st));
@@ -468,7 +526,8 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
     cg.tol_abs = prm->tol_abs;
     cg.tol_rel = prm->tol_rel;
     if (!mixed) {
-        if ((rc = pcg_batch<double>(c, t, c->args(), c->map, x, b, r, p, Ap, dinv, part, sc, all, cg, st, launches)))
+        if ((rc = pcg_batch<double>(c, t, c->args(), c->map, x, b, r, p, Ap, dinv, part, sc, all, cg, st, launches,
+                                    zc ? &cheb : nullptr, zc)))
             return rc;
         HIPCHECK(hipMemcpyAsync(scv.data(), sc, sizeof(double) * scv.size(), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -521,7 +580,7 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
             HIPCHECK(hipMemsetAsync(x32, 0, sizeof(float) * n, st));
             int l = 0;
             if ((rc = pcg_batch<float>(c, t, a32, c->map32, x32, b32, r32, p32, Ap32, dinv32, part, sc, all, icg, st,
-                                       l)))
+                                       l, zc32 ? &cheb32 : nullptr, zc32)))
                 return rc;
             launches += l;
             HIPCHECK(hipMemcpyAsync(scv.data(), sc, sizeof(double) * scv.size(), hipMemcpyDeviceToHost, st));

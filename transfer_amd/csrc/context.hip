@@ -18,6 +18,7 @@
 #include "ctx.h"
 #include "internal.h"
 #include "kernels.h"
+#include "cheb.h"
 #include "solver.h"
 #include "stencil.h"
 
@@ -87,6 +88,24 @@ static bool v4_ok(const HostOperator &op) {
         return false;
     return mfma_supported(op.slm, 0) && (op.W == 15 || op.W == 16) && op.WB <= TR_WBMAX &&
            (op.NC == 7 || op.NC == 10);
+}
+
+// Whether the buffer-load VALU schedule (variant 3) covers this operator
+// with elem-byte values: at most 16 volume slots, and the vector, the
+// neighbour table and one slot's coefficient records each below 4 GiB
+// (32-bit buffer offsets, one descriptor per slot).
+static bool v3_ok(const HostOperator &op, size_t elem) {
+    const double lim = 4294967295.0;
+    const int rs = elem == 8 ? rec_stride<double>((int)op.NC) : rec_stride<float>((int)op.NC);
+    return op.W <= 16 && (double)op.nP * op.slm * elem < lim && (double)op.W * op.nP * 4 < lim &&
+           (double)op.nP * rs * elem < lim;
+}
+
+// the VALU variant of a default schedule: 1 (plain loads) for wide
+// stencils, 2 where the buffer-load schedule's offsets would not fit
+static int valu_variant(const HostOperator &op, size_t elem, int v) {
+    if (op.W > 16) return 1;
+    return v == 3 && !v3_ok(op, elem) ? 2 : v;
 }
 
 // Whether the K-blocked matrix-core schedule (variant 5, FP32 operator)
@@ -391,8 +410,7 @@ int tr::ctx_init(tr_ctx *c, int device) {
         return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
     }
     c->map = choose_map(c->op.slm, 4);
-    if (c->op.W > 16) c->map.variant = 1;
-    if ((double)c->op.W * c->op.nP * rec_stride<double>((int)c->op.NC) * 8 >= 2147483648.0) c->map.variant = 2;
+    c->map.variant = valu_variant(c->op, 8, c->map.variant);
     int maxlds = 65536;
     hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     while (c->map.WPB > 1 && lds_need(c->map, c->op, 8) > (size_t)maxlds) c->map.WPB /= 2;
@@ -830,7 +848,7 @@ static int start_point(tr_ctx *c, const tr_solve_params *prm, double *f, const d
 // preconditioner.  lambda_max comes from an FP64 power iteration times 1.1.
 // Each of the deg - 1 steps is one operator launch with the update fused
 // into its write (matrix-core schedule).
-static int cheb_lambda(tr_ctx *c, const std::vector<double> &dinv) {
+int cheb_lambda(tr_ctx *c, const std::vector<double> &dinv) {
     if (c->cheb_lmax > 0) return TR_OK;
     const size_t n = c->nint();
     double *d = nullptr, *x = nullptr, *y = nullptr;
@@ -848,75 +866,6 @@ static int cheb_lambda(tr_ctx *c, const std::vector<double> &dinv) {
     c->cheb_lmax = 1.1 * lam;
     return TR_OK;
 }
-
-// z = s dinv r, the first Chebyshev iterate from z = 0
-template <typename T>
-__global__ __launch_bounds__(256) void k_cheb_first(const T *__restrict__ r, const T *__restrict__ dinv, T s,
-                                                    T *__restrict__ z, size_t n, const double *__restrict__ skip) {
-    if (skip && *skip != 0.0) return;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        z[i] = s * dinv[i] * r[i];
-}
-
-template <typename T>
-struct ChebPrec {
-    OpArgs<T> a;
-    int NC = 7;
-    LaneMap map;
-    const T *dinv = nullptr;
-    T *zb = nullptr, *tmp = nullptr;
-    size_t n = 0;
-    double inv_theta = 0;
-    std::vector<double> om, ga;  // steps 1 .. deg-1
-    void setup(int deg, double lmax, double ratio) {
-        const double lmin = lmax / ratio, th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sig = th / de;
-        double rho = 1.0 / sig;
-        inv_theta = 1.0 / th;
-        om.clear();
-        ga.clear();
-        for (int k = 1; k < deg; k++) {
-            const double rn = 1.0 / (2.0 * sig - rho);
-            om.push_back(rn * rho);
-            ga.push_back(2.0 * rn / de);
-            rho = rn;
-        }
-    }
-    // the buffer that holds the first iterate when the result goes to z
-    T *first_buffer(T *z) const { return (om.size() % 2 == 0) ? z : zb; }
-    // r.z partials the last step writes (matrix-core schedules, at least
-    // one step, at most 1024 workgroups), else 0
-    int nparts() const {
-        if (a.ngv) return 0;  // the steps with faces off the axes are not fused
-        const uint32_t p = cheb_parts(a.nP, map);
-        return (!om.empty() && p <= 1024) ? (int)p : 0;
-    }
-    hipError_t apply(const T *r, T *z, const double *skip, hipStream_t st) const {
-        const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_cheb_first<T>, dim3(g), dim3(256), 0, st, r, dinv, (T)inv_theta, first_buffer(z), n,
-                           skip);
-        hipError_t e = hipGetLastError();
-        return e != hipSuccess ? e : steps(r, z, skip, st, nullptr);
-    }
-    // the steps after the first iterate (already in first_buffer(z)); the
-    // last one also writes nparts() r.z partials to rzpart when given
-    hipError_t steps(const T *r, T *z, const double *skip, hipStream_t st, double *rzpart) const {
-        const int ns = (int)om.size();
-        T *cur = first_buffer(z), *oth = cur == z ? zb : z;
-        hipError_t e = hipSuccess;
-        for (int k = 0; k < ns && e == hipSuccess; k++) {
-            ChebStep<T> cs;
-            cs.rhs = r;
-            cs.dinv = dinv;
-            cs.zc = (T)(1.0 + om[k]);
-            cs.om = k == 0 ? (T)0 : (T)om[k];  // z_0 = 0
-            cs.ga = (T)ga[k];
-            e = launch_cheb_step<T>(a, NC, map, cur, oth, cs, tmp, skip, st,
-                                    (k == ns - 1 && rzpart && nparts()) ? rzpart : nullptr);
-            std::swap(cur, oth);
-        }
-        return e;
-    }
-};
 
 static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
     Work w(c);
@@ -1023,8 +972,7 @@ int ensure_fp32(tr_ctx *c) {
         c->map32 = mfma_map_ctx(c, 4, 2);
     else if (c->ang_prec != 0 && v5_ok(op))
         c->map32 = mfma5_map_ctx(c, 8);
-    if (op.W > 16) c->map32.variant = 1;
-    if ((double)op.W * op.nP * rec_stride<float>((int)op.NC) * 4 >= 2147483648.0) c->map32.variant = 2;
+    if (c->map32.variant <= 3) c->map32.variant = valu_variant(op, 4, c->map32.variant);
     int maxlds = 65536;
     hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     while (c->map32.WPB > 1 && lds_need(c->map32, op, 4) > (size_t)maxlds) c->map32.WPB /= 2;
@@ -1372,6 +1320,7 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
             for (int variant : {1, 2, 3})
                 for (int wpb : {1, 2, 4}) {
                     if (variant >= 2 && op.W > 16) continue;
+                    if (variant == 3 && !v3_ok(op, elem)) continue;
                     m.variant = variant;
                     m.WPB = wpb;
                     cands.push_back(m);
@@ -1492,7 +1441,8 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
         return TR_OK;
     }
     if (J < 1 || op.slm % J || (int)(op.slm / J) > 64 || (WPB != 1 && WPB != 2 && WPB != 4) ||
-        (variant < 1 || variant > 3) || (variant >= 2 && op.W > 16))
+        (variant < 1 || variant > 3) || (variant >= 2 && op.W > 16) ||
+        (variant == 3 && !v3_ok(op, precision ? 4 : 8)))
         return fail(TR_ERR_ARG, "tr_ctx_set_schedule: unsupported schedule");
     bool ok = false;
     for (const LaneMap &m : map_candidates(op.slm, 11)) ok = ok || m.J == J;
@@ -1688,7 +1638,7 @@ extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
             c->map32 = mfma5_map_ctx(c, 8);
         else {
             c->map32 = choose_map(op.slm, 8);
-            if (op.W > 16) c->map32.variant = 1;
+            c->map32.variant = valu_variant(op, 4, c->map32.variant);
         }
     }
     return TR_OK;

@@ -202,5 +202,8 @@ hipError_t multilevel_apply32(tr_ctx *c, const float *r, float *z, const double 
 int multilevel_levels(const tr_ctx *c);
 // largest eigenvalue of D^-1 A (FP64 power iteration; dinv, x, y device [nint])
 int lambda_max(tr_ctx *c, const double *dinv, double *x, double *y, double &lam);
+// lambda_max of D^-1 A for the Chebyshev preconditioner (1.1 x the power
+// iteration's; dinv = 1 / diag(A) on the host), computed once per context
+int cheb_lambda(tr_ctx *c, const std::vector<double> &dinv);
 
 }  // namespace tr

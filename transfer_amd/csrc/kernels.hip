@@ -369,7 +369,6 @@ __global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict
     if (valid) {
         const auto frs = rsrc(f, (size_t)nP * slm * sizeof(T));
         const auto nrs = rsrc(a.nbr, (size_t)a.W * nP * 4);
-        const auto crs = rsrc(a.crec, (size_t)a.W * nP * RS * sizeof(T));
         uint32_t nb[WT];
 #pragma unroll
         for (int s = 0; s < WT; s++)
@@ -381,7 +380,10 @@ __global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict
             // J | slm, so a J whose slab is a multiple of 16 bytes keeps every
             // lane's offset 16-byte aligned; records are 16-byte multiples
             bload<T, J, (J * sizeof(T)) % 16 == 0>(frs, (nb[s] * slm + lm0) * (uint32_t)sizeof(T), 0u, x);
-            bload<T, RS, true>(crs, cvoff, (uint32_t)s * nP * (uint32_t)(RS * sizeof(T)), cf);
+            // one descriptor per slot: a slot's records stay below 4 GiB
+            // where all W of them need not (unions of many problems)
+            const auto crs = rsrc(a.crec + (size_t)s * nP * RS, (size_t)nP * RS * sizeof(T));
+            bload<T, RS, true>(crs, cvoff, 0u, cf);
 #pragma unroll
             for (int c = 0; c < NC; c++)
 #pragma unroll

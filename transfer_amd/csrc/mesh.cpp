@@ -118,20 +118,33 @@ int build_mesh(uint32_t nP, const double *coords, uint32_t nT,
         std::array<uint32_t, 3> key;
         uint32_t tet, loc;
     };
-    std::vector<FaceRec> fr((size_t)4 * nT);
+    auto face_key = [&](uint32_t t, int j) {
+        std::array<uint32_t, 3> key;
+        int q = 0;
+        for (int k = 0; k < 4; k++)
+            if (k != j) key[q++] = tets[(size_t)4 * t + k];
+        std::sort(key.begin(), key.end());
+        return key;
+    };
+    // faces in (key, tet) order: a counting sort by the smallest vertex,
+    // then a sort of each (short) group by the other two vertices and tet
+    std::vector<size_t> gstart((size_t)nP + 1, 0);
     for (uint32_t t = 0; t < nT; t++)
-        for (int j = 0; j < 4; j++) {
-            FaceRec &f = fr[(size_t)4 * t + j];
-            int q = 0;
-            for (int k = 0; k < 4; k++)
-                if (k != j) f.key[q++] = tets[(size_t)4 * t + k];
-            std::sort(f.key.begin(), f.key.end());
-            f.tet = t;
-            f.loc = j;
-        }
-    std::sort(fr.begin(), fr.end(), [](const FaceRec &a, const FaceRec &b) {
-        return a.key != b.key ? a.key < b.key : a.tet < b.tet;
-    });
+        for (int j = 0; j < 4; j++) gstart[face_key(t, j)[0] + 1]++;
+    for (uint32_t v = 0; v < nP; v++) gstart[v + 1] += gstart[v];
+    std::vector<FaceRec> fr((size_t)4 * nT);
+    {
+        std::vector<size_t> fill(gstart.begin(), gstart.end() - 1);
+        for (uint32_t t = 0; t < nT; t++)
+            for (int j = 0; j < 4; j++) {
+                const std::array<uint32_t, 3> key = face_key(t, j);
+                fr[fill[key[0]]++] = FaceRec{key, t, (uint32_t)j};
+            }
+    }
+    for (uint32_t v = 0; v < nP; v++)
+        std::sort(fr.begin() + gstart[v], fr.begin() + gstart[v + 1], [](const FaceRec &a, const FaceRec &b) {
+            return a.key != b.key ? a.key < b.key : a.tet < b.tet;
+        });
     m.bnd.clear();
     m.face_tet.clear();
     for (size_t i = 0; i < fr.size();) {
