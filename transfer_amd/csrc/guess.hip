@@ -7,10 +7,14 @@
 // Dijkstra, is the fixpoint of the Bellman equations
 //   tau_v = min(tau_v, min_w tau_w + |x_v - x_w| (kappa_v + kappa_w) / 2)
 // over the edge neighbours w (the operator's ELL slots), tau = 0 on vacuum
-// faces.  Sweeps update tau in place (every value read is the length of some
-// path, so races only speed convergence) until a whole sweep changes
-// nothing; the candidate values are the same path sums in the same
-// left-to-right order as Dijkstra's, so the fixpoint is the same numbers.
+// faces.  The edge weights |x_v - x_w| (kappa_v + kappa_w) / 2 are formed
+// once.  Sweeps update tau in place (every value read is the length of some
+// path, so races only speed convergence); each launch relaxes its block's
+// 256 consecutive vertices (a compact piece of the mesh in the operator's
+// Morton order) several times, so distances cross a block in one launch.
+// The iteration stops after a launch that changes nothing; the candidate
+// values are the same path sums in the same left-to-right order as
+// Dijkstra's, so the fixpoint is the same numbers.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -31,29 +35,52 @@ __global__ __launch_bounds__(256) void k_feat_init(uint32_t nP, const double *__
     }
 }
 
-__global__ __launch_bounds__(256) void k_feat_relax(uint32_t nP, uint32_t W, const uint32_t *__restrict__ nbr,
-                                                    const double *__restrict__ xyz, const double *__restrict__ kap,
-                                                    double *tau, int *__restrict__ changed) {
+// wgt[s-1][v] = |x_v - x_w| (kappa_w + kappa_v) / 2 for slot s >= 1 (the
+// operations of the host's Dijkstra, unfused)
+__global__ __launch_bounds__(256) void k_feat_wgt(uint32_t nP, uint32_t W, const uint32_t *__restrict__ nbr,
+                                                  const double *__restrict__ xyz, const double *__restrict__ kap,
+                                                  double *__restrict__ wgt) {
 #pragma clang fp contract(off)
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nP; v += gridDim.x * blockDim.x) {
-        const double t = tau[v];
         const double xv = xyz[v], yv = xyz[(size_t)nP + v], zv = xyz[(size_t)2 * nP + v], kv = kap[v];
-        double best = t;
         for (uint32_t s = 1; s < W; s++) {
             const uint32_t w = nbr[(size_t)s * nP + v];
-            if (w == v) continue;
             const double dx = xyz[w] - xv, dy = xyz[(size_t)nP + w] - yv, dz = xyz[(size_t)2 * nP + w] - zv;
             double len2 = dx * dx;
             len2 = len2 + dy * dy;
             len2 = len2 + dz * dz;
-            const double nd = tau[w] + sqrt(len2) * 0.5 * (kap[w] + kv);
-            best = nd < best ? nd : best;
-        }
-        if (best < t) {
-            tau[v] = best;
-            *changed = 1;
+            wgt[(size_t)(s - 1) * nP + v] = sqrt(len2) * 0.5 * (kap[w] + kv);
         }
     }
+}
+
+// PASSES relaxations of the block's 256 consecutive vertices (one thread
+// each, one block per 256 vertices)
+template <int PASSES>
+__global__ __launch_bounds__(256) void k_feat_relax(uint32_t nP, uint32_t W, const uint32_t *__restrict__ nbr,
+                                                    const double *__restrict__ wgt, double *tau,
+                                                    int *__restrict__ changed) {
+#pragma clang fp contract(off)
+    const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+    int any = 0;
+    for (int p = 0; p < PASSES; p++) {
+        if (v < nP) {
+            const double t = tau[v];
+            double best = t;
+            for (uint32_t s = 1; s < W; s++) {
+                const uint32_t w = nbr[(size_t)s * nP + v];
+                if (w == v) continue;
+                const double nd = tau[w] + wgt[(size_t)(s - 1) * nP + v];
+                best = nd < best ? nd : best;
+            }
+            if (best < t) {
+                tau[v] = best;
+                any = 1;
+            }
+        }
+        __syncthreads();
+    }
+    if (any) *changed = 1;
 }
 
 __global__ __launch_bounds__(256) void k_feat_final(uint32_t nP, uint32_t W, const uint32_t *__restrict__ nbr,
@@ -130,31 +157,35 @@ extern "C" int tr_ctx_mlp_features(tr_ctx *c, double albedo, float *feat, int *s
         return fail(TR_ERR_ARG, "tr_ctx_mlp_features: albedo in [0, 1)");
     const uint32_t nP = c->op.nP, W = c->op.W;
     if (!c->d_feat_tmp) {
-        HIPCHECK(hipMalloc((void **)&c->d_feat_tmp, sizeof(double) * 2 * nP + sizeof(int) * 8));
+        HIPCHECK(hipMalloc((void **)&c->d_feat_tmp, sizeof(double) * ((size_t)W + 1) * nP + sizeof(int) * 8));
         c->owned.push_back(c->d_feat_tmp);
     }
-    double *kap = c->d_feat_tmp, *tau = kap + nP;
-    int *flags = (int *)(tau + nP);
+    double *kap = c->d_feat_tmp, *tau = kap + nP, *wgt = tau + nP;
+    int *flags = (int *)(wgt + (size_t)(W - 1) * nP);
     const unsigned g = tr::blocks_of(nP);
     hipLaunchKernelGGL(tr::k_feat_init, dim3(g), dim3(256), 0, c->stream, nP, c->d_vsum, c->d_bof, kap, tau);
     const uint32_t ngv = (uint32_t)c->op.gvert.size();
     if (ngv) hipLaunchKernelGGL(tr::k_feat_gzero, dim3(tr::blocks_of(ngv)), dim3(256), 0, c->stream, ngv, c->d_gvert, tau);
+    hipLaunchKernelGGL(tr::k_feat_wgt, dim3(g), dim3(256), 0, c->stream, nP, W, c->d_nbr, c->d_xyz, kap, wgt);
     HIPCHECK(hipGetLastError());
-    // chunks of 8 sweeps; converged when the chunk's last sweep changed
-    // nothing (at most nP sweeps: Bellman-Ford's bound)
+    // chunks of 4 launches of 4 passes; converged when the chunk's last
+    // launch changed nothing (at most nP launches: Bellman-Ford's bound)
+    constexpr int PASSES = 4, CHUNK = 4;
+    const unsigned gb = (nP + 255u) / 256u;
     int done = 0, total = 0;
     while (!done) {
-        HIPCHECK(hipMemsetAsync(flags, 0, sizeof(int) * 8, c->stream));
-        for (int s = 0; s < 8; s++)
-            hipLaunchKernelGGL(tr::k_feat_relax, dim3(g), dim3(256), 0, c->stream, nP, W, c->d_nbr, c->d_xyz, kap,
+        HIPCHECK(hipMemsetAsync(flags, 0, sizeof(int) * CHUNK, c->stream));
+        for (int s = 0; s < CHUNK; s++)
+            hipLaunchKernelGGL(tr::k_feat_relax<PASSES>, dim3(gb), dim3(256), 0, c->stream, nP, W, c->d_nbr, wgt,
                                tau, flags + s);
         HIPCHECK(hipGetLastError());
         int last = 0;
-        HIPCHECK(hipMemcpyAsync(&last, flags + 7, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(&last, flags + CHUNK - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
-        total += 8;
-        done = last == 0 || (uint64_t)total > (uint64_t)nP + 8;
+        total += CHUNK;
+        done = last == 0 || (uint64_t)total > (uint64_t)nP + CHUNK;
     }
+    total *= PASSES;
     hipLaunchKernelGGL(tr::k_feat_final, dim3(g), dim3(256), 0, c->stream, nP, W, c->d_nbr, c->d_perm, c->d_vsum, kap,
                        tau, c->d_mf0, albedo, feat);
     HIPCHECK(hipGetLastError());

@@ -100,19 +100,22 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ feat, cons
     }
     __syncthreads();
     const unsigned short *W1 = W, *W2 = W + MH * MF, *w3 = W + MH * MF + MH * MH;
+    // h1 = SiLU(z1) once per unit, in registers (fully unrolled: static
+    // indices), for the 64 x 64 layer
+    float h1[MH];
+#pragma unroll
     for (int a = 0; a < MH; a++) {
         float z = p[O_B1 + a];
 #pragma unroll
         for (int f = 0; f < MF; f++) z = fmaf(bf2f(W1[a * MF + f]), xh[f], z);
         Z1[s][a] = z;
+        h1[a] = z * sigm(z);
     }
     float z3 = p[O_B3];
     for (int a = 0; a < MH; a++) {
         float z = p[O_B2 + a];
-        for (int b = 0; b < MH; b++) {
-            const float z1 = Z1[s][b];
-            z = fmaf(bf2f(W2[a * MH + b]), z1 * sigm(z1), z);
-        }
+#pragma unroll
+        for (int b = 0; b < MH; b++) z = fmaf(bf2f(W2[a * MH + b]), h1[b], z);
         Z2[s][a] = z;
         z3 = fmaf(bf2f(w3[a]), z * sigm(z), z3);
     }
