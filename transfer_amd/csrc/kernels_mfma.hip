@@ -432,6 +432,32 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
         // per problem in a batch: 1 - albedo, with massfac[0] = 1); harmonic
         // 0 is register 0 of chunk 0
         if (a.mf0 && valid && gq == 0) acc[0][0] *= a.mf0[v];
+        // the Chebyshev step's rhs and dinv at this lane's outputs load while
+        // the tile contracts (the gather's registers are free by now).  FP32
+        // EPI 2 only: there it fits the 168 registers of 3 wavefronts per
+        // SIMD; FP64 and the other epilogues would lose occupancy
+        constexpr int NPF = (sizeof(T) == 4 && EPI == 2) ? 2 : 0;
+        T pf[NPF > 0 ? NPF : 1][NT][4];
+        if constexpr (NPF > 0) {
+            // not above the gather, whose buffers would then share registers with them
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const uint32_t n = 16u * t + (uint32_t)vloc;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t vr = v0 + (MODE == 0 ? (uint32_t)(q + 4 * i) : (uint32_t)(4 * q + i));
+                    const bool ok = n < slm && vr < nP;
+                    const size_t o = (size_t)vr * slm + n;
+                    if constexpr (EPI >= 2) {
+                        pf[0][t][i] = ok ? cs.rhs[o] : (T)0;
+                        pf[1][t][i] = ok ? cs.dinv[o] : (T)0;
+                    } else {
+                        pf[0][t][i] = ok ? f[o] : (T)0;
+                    }
+                }
+            }
+        }
         to_mfma_layout<T, MODE, J, NC>(acc, lane);
         // the next tile's indices load while this tile contracts
         const int b = bcur;
@@ -476,14 +502,14 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
                     const T val = (T)D[t][i];
                     const size_t o = (size_t)vr * slm + n;
                     if constexpr (EPI >= 2) {
-                        const T rh = cs.rhs[o];
-                        T nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (rh - val);
+                        const T rh = NPF ? pf[0][t][i] : cs.rhs[o];
+                        T nv = cs.zc * f[o] + cs.ga * (NPF ? pf[1][t][i] : cs.dinv[o]) * (rh - val);
                         if (cs.om != (T)0) nv -= cs.om * r[o];
                         r[o] = nv;
                         if (EPI == 3) d = fma((double)rh, (double)nv, d);
                     } else {
                         r[o] = val;
-                        if (DOT) d = fma((double)f[o], (double)val, d);
+                        if (DOT) d = fma((double)(NPF ? pf[0][t][i] : f[o]), (double)val, d);
                     }
                 }
             }
