@@ -36,3 +36,22 @@ def test_errors_are_reported_not_thrown():
         ta.Angular(9)
     with pytest.raises(ta.TransferError):
         ta.Mesh.box((0, 1, 1), (0, 0, 0), (1, 1, 1))
+
+
+def test_batch_create_rejects_flags():
+    """Batches take the reference's boundary axis rule only: a flag (such as
+    TR_CTX_GENERAL_NORMALS) is an argument error, raised before any device
+    call."""
+    import ctypes as C
+    import numpy as np
+    import pytest
+    import transfer_amd as ta
+    from transfer_amd.api import lib, check, MeshView
+    m = ta.Mesh.box((2, 2, 2), (0, 0, 0), (1, 1, 1))
+    ang = ta.Angular(1)
+    views = (MeshView * 1)(m.view)
+    h = C.c_void_p()
+    alb = np.zeros(1)
+    with pytest.raises(ta.TransferError, match="flags"):
+        check(lib.tr_batch_create(0, 1, views, C.byref(ang.view), alb.ctypes.data_as(C.POINTER(C.c_double)),
+                                  ta.TR_CTX_GENERAL_NORMALS, C.byref(h)))

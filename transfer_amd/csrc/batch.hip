@@ -395,8 +395,9 @@ extern "C" {
 
 int tr_batch_create(int device, int nprob, const tr_mesh_view *meshes, const tr_angular_view *ang,
                     const double *albedo, int flags, tr_ctx **out) {
-    (void)flags;
     if (nprob < 1 || !meshes || !ang || !out) return fail(TR_ERR_ARG, "tr_batch_create: bad argument");
+    if (flags != 0)  // TR_CTX_GENERAL_NORMALS: the union's stencils take the reference's axis rule only
+        return fail(TR_ERR_ARG, "tr_batch_create: flags must be 0 (general normals are not supported in batches)");
     *out = nullptr;
     std::vector<HostOperator> ops((size_t)nprob);
     std::vector<int> rcs((size_t)nprob, 0);
