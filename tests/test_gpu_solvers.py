@@ -77,11 +77,12 @@ def test_multilevel_hierarchy_shapes():
         assert np.abs(x - ref).max() <= 1e-8 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("case", ["fused", "unfused", "reflective"])
+@pytest.mark.parametrize("case", ["fused", "fused-valu", "unfused", "reflective"])
 def test_chebyshev_preconditioner(case):
     """The Chebyshev polynomial preconditioner at several degrees, with its
-    steps fused into the matrix-core operator's write ("fused") or run as
-    operator + elementwise pass on a VALU schedule ("unfused"), and on a
+    steps fused into the matrix-core operator's write ("fused") or into the
+    buffer-load VALU schedule's ("fused-valu"), or run as operator +
+    elementwise pass on the plain VALU schedule ("unfused"), and on a
     problem with reflecting planes (vanishing entries): every solve reaches
     the FP64 residual, higher degrees take fewer outer iterations, and all
     return the same solution."""
@@ -90,9 +91,10 @@ def test_chebyshev_preconditioner(case):
     else:
         P = ta.problems.cube(10, seed=4)
     ctx = ta.HipContext(P.mesh, P.ang)
-    if case == "unfused":
-        ctx.set_schedule(0, J=4, WPB=4, variant=3)
-        ctx.set_schedule(1, J=4, WPB=4, variant=3)
+    if case in ("fused-valu", "unfused"):
+        variant = 3 if case == "fused-valu" else 1
+        ctx.set_schedule(0, J=4, WPB=4, variant=variant)
+        ctx.set_schedule(1, J=4, WPB=4, variant=variant)
     ref, its = None, []
     for solver in (ta.TR_SOLVER_PCG_FP64, ta.TR_SOLVER_MIXED):
         for deg in (1, 3, 6):

@@ -335,10 +335,14 @@ __device__ inline void bload(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t 
     }
 }
 
-template <typename T, int J, int NC, int WT, bool DOT>
+// EPI: 0 = r = A f; 1 = the same and the fused dot f . r; 2 = one Chebyshev
+// step fused into the write, r = zc f - om r + ga dinv (rhs - A f) (r on
+// entry holds the previous iterate, not read when om == 0)
+template <typename T, int J, int NC, int WT, int EPI>
 __global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict__ f,
                                                 T *__restrict__ r, double *__restrict__ part,
-                                                const double *__restrict__ skip) {
+                                                const double *__restrict__ skip, ChebStep<T> cs) {
+    constexpr bool DOT = EPI == 1;
     if (skip && *skip != 0.0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int RS = rec_stride<T>(NC), TB = tab_rec_bytes<T>(), CH = NC + 3;
@@ -456,8 +460,15 @@ __global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict
                 }
             }
             const T R = R0 + R1;
-            r[(size_t)vv * slm + lm] = R;
-            if (DOT) d = fma((double)f[(size_t)vv * slm + lm], (double)R, d);
+            const size_t e = (size_t)vv * slm + lm;
+            if constexpr (EPI == 2) {
+                T nv = cs.zc * f[e] + cs.ga * cs.dinv[e] * (cs.rhs[e] - R);
+                if (cs.om != (T)0) nv -= cs.om * r[e];
+                r[e] = nv;
+            } else {
+                r[e] = R;
+                if (DOT) d = fma((double)f[e], (double)R, d);
+            }
         }
     }
     if (DOT) {
@@ -620,9 +631,11 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
 #define TR_LAUNCH(JJ, NCC, D)                                                                   \
     do {                                                                                        \
         if (v3 && a.W == 15)                                                                    \
-            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 15, D>), g, b, sh, s, a, f, r, part, skip);  \
+            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 15, D ? 1 : 0>), g, b, sh, s, a, f, r, part, skip, \
+                               ChebStep<T>{});                                                  \
         else if (v3)                                                                            \
-            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 16, D>), g, b, sh, s, a, f, r, part, skip);  \
+            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 16, D ? 1 : 0>), g, b, sh, s, a, f, r, part, skip, \
+                               ChebStep<T>{});                                                  \
         else if (v2)                                                                            \
             hipLaunchKernelGGL((k_apply2<T, JJ, NCC, D>), g, b, sh, s, a, f, r, part, skip);      \
         else                                                                                    \
@@ -737,7 +750,36 @@ hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const 
             e = launch_apply4<T, 1>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
         return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
     }
-    if (!tmp || part) return hipErrorInvalidValue;
+    if (part) return hipErrorInvalidValue;
+    if (m.variant == 3 && !a.ngv && (a.W == 15 || a.W == 16) && a.crec && a.tab) {
+        // the buffer-load VALU schedule with the step fused into its write
+        dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
+        const size_t sh = apply3_lds_bytes(m, NC, a.slm, a.arow, a.brow, sizeof(T));
+#define TR_CHEB3(JJ, NCC)                                                                                     \
+    do {                                                                                                      \
+        if (a.W == 15)                                                                                        \
+            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 15, 2>), g, b, sh, s, a, z, out, nullptr, skip, cs);     \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_apply3<T, JJ, NCC, 16, 2>), g, b, sh, s, a, z, out, nullptr, skip, cs);     \
+    } while (0)
+#define TR_CASE(JJ)              \
+    case JJ:                     \
+        if (NC == 7)             \
+            TR_CHEB3(JJ, 7);     \
+        else                     \
+            TR_CHEB3(JJ, 10);    \
+        break;
+        switch (m.J) {
+            TR_J_CASES(TR_CASE)
+            default:
+                return hipErrorInvalidValue;
+        }
+#undef TR_CASE
+#undef TR_CHEB3
+        e = hipGetLastError();
+        return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
+    }
+    if (!tmp) return hipErrorInvalidValue;
     if ((e = launch_apply<T>(a, NC, m, z, tmp, nullptr, skip, s)) != hipSuccess) return e;
     const size_t n = (size_t)a.nP * a.slm;
     const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
