@@ -15,12 +15,22 @@ benchmark configurations, from the library's own generators.
     batch     256 slabs n x n x 4n (n ~ U{8..32} from the seed), per-tet
               ln kappa ~ N(0, 1), albedo ~ U[0, 0.95] per problem, I_p = 1,
               K = 2 (50-point order); `count` and `nrange` shrink it for tests
+    sphere    the two-region sphere of the reference's soltype-1 check
+              (main.cpp:190-197; radiation.cfg.example: kappa 11 and I_p 1
+              in the core r < 0.3, kappa 1 and I_p 0 outside), radius 1:
+              the n^3 Kuhn cubes of [-1,1]^3 mapped onto the ball by
+              x -> x |x|_inf / |x|_2 (nested cube surfaces onto nested
+              spheres), so every boundary face is a chord face of the unit
+              sphere with its normal off the axes; regions by tet centroid,
+              no scattering
     milne     the step problem of milne.geo as a 1-D slab: [-L,L] x [-h,h]^2,
               kappa = 1, I_p = 1 for x < 0 and 0 for x > 0, vacuum at
               x = +-L, specular reflection on the four side planes (so the
               solution is the exact 1-D one, U(x) = 2 pi (E2 ... ) terms)
 The Lebedev N-point orders map to the harmonic order K = (degree(N) - 2) // 4.
 """
+import numpy as np
+
 from .api import Angular, Mesh, rng_uniform
 
 
@@ -53,6 +63,18 @@ def cube(n=32, maxk=3, seed=5, albedo=None):
     if albedo is None:
         albedo = 0.99 * rng_uniform(seed, 6, 1)[0]
     return Problem("cube%d" % n, m, Angular(maxk, albedo, 0.0, 0), albedo)
+
+
+def sphere(n=32, maxk=3, core=0.3, kappa=(11.0, 1.0), Ip=(1.0, 0.0)):
+    box = Mesh.box((n, n, n), (-1, -1, -1), (1, 1, 1))
+    xyz, tets, _, _ = box.geometry()
+    r2 = np.linalg.norm(xyz, axis=1)
+    rinf = np.abs(xyz).max(axis=1)
+    xyz = xyz * np.where(r2 > 0, rinf / np.where(r2 > 0, r2, 1.0), 1.0)[:, None]
+    rc = np.linalg.norm(xyz[tets.astype(np.int64)].mean(axis=1), axis=1)
+    k = np.where(rc < core, kappa[0], kappa[1])
+    ip = np.where(rc < core, Ip[0], Ip[1])
+    return Problem("sphere%d" % n, Mesh.from_arrays(xyz, tets, k, ip), Angular(maxk))
 
 
 def milne(n=32, maxk=3, L=1.0, ncross=2):
