@@ -93,11 +93,18 @@ def test_solutions_agree_in_scalar_radiation(meshes):
     B = ta.Mesh.from_arrays(coords @ R.T, tets, kappa, Ip)
     ang = ta.Angular(K)
     out = {}
+    # Chebyshev steps fused into the matrix-core ("B-*-cheb") and the
+    # buffer-load VALU ("-v3") operators, the faces' share added after them
     for name, mesh, gen, solver, prec in (("A", A, False, ta.TR_SOLVER_PCG_FP64, 2),
                                          ("B", B, True, ta.TR_SOLVER_PCG_FP64, 2),
+                                         ("B-fp64-cheb", B, True, ta.TR_SOLVER_PCG_FP64, 4),
                                          ("B-mixed-cheb", B, True, ta.TR_SOLVER_MIXED, 4),
+                                         ("B-mixed-cheb-v3", B, True, ta.TR_SOLVER_MIXED, 4),
                                          ("B-axis-rule", B, False, ta.TR_SOLVER_PCG_FP64, 2)):
         ctx = ta.HipContext(mesh, ang, general_normals=gen)
+        if name.endswith("-v3"):
+            for p in (0, 1):
+                ctx.set_schedule(p, J=3, WPB=4, variant=3)  # slm 15 at K = 2
         d = ctx.allocVector()
         st, _ = ctx.solve(d, solver=solver, use_prec=prec, maxit=5000 if solver != ta.TR_SOLVER_MIXED else 30,
                           tol_abs=0.0, tol_rel=1e-11 if solver != ta.TR_SOLVER_MIXED else 1e-9, inner_maxit=5000,
@@ -107,9 +114,10 @@ def test_solutions_agree_in_scalar_radiation(meshes):
         print("[gpu-normals] %s: %d iterations (%d inner), true rel %.2e" % (name, st.iters, st.preck, st.true_rel),
               flush=True)
     ref = np.abs(out["A"]).max()
-    e = {k: np.abs(out[k] - out["A"]).max() / ref for k in ("B", "B-mixed-cheb", "B-axis-rule")}
+    e = {k: np.abs(out[k] - out["A"]).max() / ref for k in out if k != "A"}
     print("[gpu-normals] U against the unrotated solve: %s" % e, flush=True)
-    assert e["B"] <= 1e-8 and e["B-mixed-cheb"] <= 1e-6 and e["B-axis-rule"] > 1e-3
+    assert e["B"] <= 1e-8 and e["B-fp64-cheb"] <= 1e-8 and e["B-axis-rule"] > 1e-3
+    assert e["B-mixed-cheb"] <= 1e-6 and e["B-mixed-cheb-v3"] <= 1e-6
     with pytest.raises(ta.TransferError, match="axes"):
         ctx = ta.HipContext(B, ang, general_normals=True)
         ctx.solve(ctx.allocVector(), solver=ta.TR_SOLVER_PCG_FP64, use_prec=3, maxit=10, tol_abs=0.0, tol_rel=1e-8)

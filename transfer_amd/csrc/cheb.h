@@ -52,7 +52,7 @@ struct ChebPrec {
     // r.z partials the last step writes (matrix-core schedules, at least
     // one step, at most 1024 workgroups), else 0
     int nparts() const {
-        if (a.ngv) return 0;  // the steps with faces off the axes are not fused
+        if (a.ngv) return 0;  // the fused r.z dot would miss the share of faces off the axes
         const uint32_t p = cheb_parts(a.nP, map);
         return (!om.empty() && p <= 1024) ? (int)p : 0;
     }

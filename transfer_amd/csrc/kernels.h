@@ -92,8 +92,8 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
                         double *part, const double *skip, hipStream_t s);
 // One Chebyshev step (ChebStep): fused into the operator's write on the
 // matrix-core schedules and the buffer-load VALU schedule (variant 3), else
-// the operator into tmp and an elementwise pass (without faces off the axes
-// only: those take the elementwise pass everywhere).
+// the operator into tmp and an elementwise pass.  Faces off the axes add
+// their share afterwards (k_bgen), so with them part must be null.
 // With part != null (matrix-core schedules only) each workgroup of the
 // persistent grid (cheb_parts of them) also writes sum rhs . out (FP64).
 template <typename T>

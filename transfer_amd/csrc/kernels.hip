@@ -666,8 +666,11 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
 // operator's boundary term).  One workgroup per such vertex, a thread per
 // harmonic; g_F staged in LDS.
 template <typename T, bool DIAG>
+// (cheb != 0: the operator kernel wrote a fused Chebyshev step into r, and
+// this adds the faces' share of A f to it, r -= ga dinv (faces' A f), the
+// step being linear in A f)
 __global__ __launch_bounds__(128) void k_bgen(OpArgs<T> a, const T *__restrict__ f, T *__restrict__ r,
-                                              const double *__restrict__ skip) {
+                                              const double *__restrict__ skip, ChebStep<T> cs, int cheb) {
     if (skip && *skip != 0.0) return;
     __shared__ T g[128];
     const uint32_t slm = a.slm, t = threadIdx.x, b = blockIdx.x;
@@ -692,17 +695,25 @@ __global__ __launch_bounds__(128) void k_bgen(OpArgs<T> a, const T *__restrict__
         }
         __syncthreads();
     }
-    if (t < slm) r[(size_t)v * slm + t] += acc;
+    if (t < slm) {
+        const size_t o = (size_t)v * slm + t;
+        if (cheb)
+            r[o] -= cs.ga * cs.dinv[o] * acc;
+        else
+            r[o] += acc;
+    }
 }
 
 template <typename T>
-static hipError_t launch_bgen(const OpArgs<T> &a, bool diag, const T *f, T *r, const double *skip, hipStream_t s) {
+static hipError_t launch_bgen(const OpArgs<T> &a, bool diag, const T *f, T *r, const double *skip, hipStream_t s,
+                              const ChebStep<T> *cs = nullptr) {
     if (!a.ngv) return hipSuccess;
-    if (a.slm > 128) return hipErrorInvalidValue;
+    if (a.slm > 128 || (diag && cs)) return hipErrorInvalidValue;
+    const ChebStep<T> c = cs ? *cs : ChebStep<T>{};
     if (diag)
-        hipLaunchKernelGGL((k_bgen<T, true>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip);
+        hipLaunchKernelGGL((k_bgen<T, true>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip, c, 0);
     else
-        hipLaunchKernelGGL((k_bgen<T, false>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip);
+        hipLaunchKernelGGL((k_bgen<T, false>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip, c, cs ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -732,26 +743,30 @@ hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const 
     if (a.nP == 0) return hipSuccess;
     hipError_t e;
     // (a fused dot product also covers masked entries: rhs, a CG residual,
-    // is zero there)
-    if (m.variant == 5 && !a.ngv) {
+    // is zero there).  Faces off the axes: k_bgen adds their share to the
+    // fused step afterwards (the fused dot would miss it: not with part).
+    if (part && a.ngv) return hipErrorInvalidValue;
+    if (m.variant == 5) {
         if constexpr (sizeof(T) == 8) {
             return hipErrorInvalidValue;
         } else {
             e = launch_apply5(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
+            if (e == hipSuccess) e = launch_bgen(a, false, z, out, skip, s, &cs);
             return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
         }
     }
-    if (m.variant == 4 && !a.ngv) {
+    if (m.variant == 4) {
         if constexpr (sizeof(T) == 8)
             e = launch_apply4<T, 0>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
         else if (m.mfma_mode == 2)
             e = launch_apply4<T, 2>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
         else
             e = launch_apply4<T, 1>(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
+        if (e == hipSuccess) e = launch_bgen(a, false, z, out, skip, s, &cs);
         return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
     }
     if (part) return hipErrorInvalidValue;
-    if (m.variant == 3 && !a.ngv && (a.W == 15 || a.W == 16) && a.crec && a.tab) {
+    if (m.variant == 3 && (a.W == 15 || a.W == 16) && a.crec && a.tab) {
         // the buffer-load VALU schedule with the step fused into its write
         dim3 g(apply_blocks(a.nP, m)), b(TR_WAVE * m.WPB);
         const size_t sh = apply3_lds_bytes(m, NC, a.slm, a.arow, a.brow, sizeof(T));
@@ -777,6 +792,7 @@ hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const 
 #undef TR_CASE
 #undef TR_CHEB3
         e = hipGetLastError();
+        if (e == hipSuccess) e = launch_bgen(a, false, z, out, skip, s, &cs);
         return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
     }
     if (!tmp) return hipErrorInvalidValue;
