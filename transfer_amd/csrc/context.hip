@@ -1103,7 +1103,7 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         // true reduction) when that is less
         {
             const double goal = std::max(prm->tol_rel * bn, prm->tol_abs);
-            cp.tol_rel = std::max(prm->inner_tol, std::min(0.1, 0.3 * goal / rn));
+            cp.tol_rel = std::max(prm->inner_tol, std::min(0.1, 0.6 * goal / rn));
         }
         HIPCHECK(launch_to_f32_scaled(r, b32, 1.0 / rn, n, c->stream));
         HIPCHECK(hipMemsetAsync(d32, 0, sizeof(float) * n, c->stream));
