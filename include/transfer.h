@@ -326,8 +326,8 @@ int tr_solve(tr_ctx *ctx, const tr_solve_params *prm, double *f,
  * scattering albedo[p] (NULL = none; the angular view must carry no
  * scattering of its own).  Every tr_ctx call works on the union: vectors
  * are [union vertex][aslm] with problem p's vertices at voff[p] ..
- * voff[p+1] (tr_batch_info) in the order of `meshes`.  flags must be 0
- * (TR_CTX_GENERAL_NORMALS is not supported for batches). */
+ * voff[p+1] (tr_batch_info) in the order of `meshes`.  flags: 0 or
+ * TR_CTX_GENERAL_NORMALS (as tr_ctx_create, for every problem). */
 int tr_batch_create(int device, int nprob, const tr_mesh_view *meshes,
                     const tr_angular_view *ang, const double *albedo, int flags,
                     tr_ctx **out);

@@ -121,3 +121,42 @@ def test_solutions_agree_in_scalar_radiation(meshes):
     with pytest.raises(ta.TransferError, match="axes"):
         ctx = ta.HipContext(B, ang, general_normals=True)
         ctx.solve(ctx.allocVector(), solver=ta.TR_SOLVER_PCG_FP64, use_prec=3, maxit=10, tol_abs=0.0, tol_rel=1e-8)
+
+
+def test_batch_with_general_normals(meshes):
+    """A batch of the mesh, its generic rotation and its z rotation with
+    general normals: the union operator restricted to each problem equals
+    that problem's own context (1e-12), and the batched Chebyshev solve gives
+    each problem's U of the solve alone (1e-8), which is also the unrotated
+    one's."""
+    A, coords, tets, kappa, Ip = meshes
+    ms = [A] + [ta.Mesh.from_arrays(coords @ rotation(k).T, tets, kappa, Ip) for k in ("generic", "z30")]
+    ang = ta.Angular(K)
+    bat = ta.Batch(ms, ang, [0.0] * len(ms), general_normals=True)
+    rng = np.random.default_rng(9)
+    f = np.zeros((bat.nP, bat.aslm))
+    f[:, :bat.slm] = rng.standard_normal((bat.nP, bat.slm))
+    d_f, d_r = bat.vector_from(f.ravel()), bat.allocVector()
+    bat.computeLhs(d_f, d_r)
+    r = bat.copyToHost(d_r).reshape(bat.nP, bat.aslm)
+    d_x = bat.allocVector()
+    st, per = bat.solve_batch(d_x, solver=ta.TR_SOLVER_PCG_FP64, use_prec=4, maxit=5000, tol_rel=1e-11)
+    assert st.status == 0
+    x = bat.copyToHost(d_x)
+    Ub = bat.moments(d_x)[0]
+    for p, m in enumerate(ms):
+        lo, hi = int(bat.voff[p]), int(bat.voff[p + 1])
+        one = ta.HipContext(m, ang, general_normals=True)
+        o_f, o_r = one.vector_from(f[lo:hi].ravel()), one.allocVector()
+        one.computeLhs(o_f, o_r)
+        want = one.copyToHost(o_r).reshape(hi - lo, one.aslm)
+        e_op = np.abs(r[lo:hi] - want).max() / np.abs(want).max()
+        d = one.allocVector()
+        s1, _ = one.solve(d, solver=ta.TR_SOLVER_PCG_FP64, use_prec=2, maxit=5000, tol_abs=0.0, tol_rel=1e-11)
+        U1 = one.moments(d)[0]
+        e_u = np.abs(Ub[lo:hi] - U1).max() / np.abs(U1).max()
+        print("[gpu-normals] batch problem %d: operator %.2e, U vs alone %.2e (%d iterations)" % (
+            p, e_op, e_u, per[p]["iters"]), flush=True)
+        assert e_op <= 1e-12 and e_u <= 1e-8
+        one.close()
+    assert np.isfinite(x).all()

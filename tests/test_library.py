@@ -38,10 +38,9 @@ def test_errors_are_reported_not_thrown():
         ta.Mesh.box((0, 1, 1), (0, 0, 0), (1, 1, 1))
 
 
-def test_batch_create_rejects_flags():
-    """Batches take the reference's boundary axis rule only: a flag (such as
-    TR_CTX_GENERAL_NORMALS) is an argument error, raised before any device
-    call."""
+def test_batch_create_rejects_unknown_flags():
+    """tr_batch_create takes 0 or TR_CTX_GENERAL_NORMALS; any other flag is
+    an argument error, raised before any device call."""
     import ctypes as C
     import numpy as np
     import pytest
@@ -54,4 +53,4 @@ def test_batch_create_rejects_flags():
     alb = np.zeros(1)
     with pytest.raises(ta.TransferError, match="flags"):
         check(lib.tr_batch_create(0, 1, views, C.byref(ang.view), alb.ctypes.data_as(C.POINTER(C.c_double)),
-                                  ta.TR_CTX_GENERAL_NORMALS, C.byref(h)))
+                                  ta.TR_CTX_GENERAL_NORMALS << 1, C.byref(h)))

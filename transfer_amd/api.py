@@ -639,11 +639,12 @@ class Batch(HipContext):
     (tr_batch_create / tr_batch_solve).  Problem p owns vertices
     voff[p] .. voff[p+1] of the union vectors."""
 
-    def __init__(self, meshes, ang, albedo=None, device=0):
+    def __init__(self, meshes, ang, albedo=None, device=0, general_normals=False):
         self._h = vp()
         views = (MeshView * len(meshes))(*[m.view for m in meshes])
         alb = None if albedo is None else np.ascontiguousarray(albedo, np.float64)
-        check(lib.tr_batch_create(int(device), len(meshes), views, C.byref(ang.view), ptr(alb), 0,
+        flags = TR_CTX_GENERAL_NORMALS if general_normals else 0
+        check(lib.tr_batch_create(int(device), len(meshes), views, C.byref(ang.view), ptr(alb), flags,
                                   C.byref(self._h)))
         self._keep = (meshes, ang, views, alb)
         self._sizes()

@@ -396,8 +396,7 @@ extern "C" {
 int tr_batch_create(int device, int nprob, const tr_mesh_view *meshes, const tr_angular_view *ang,
                     const double *albedo, int flags, tr_ctx **out) {
     if (nprob < 1 || !meshes || !ang || !out) return fail(TR_ERR_ARG, "tr_batch_create: bad argument");
-    if (flags != 0)  // TR_CTX_GENERAL_NORMALS: the union's stencils take the reference's axis rule only
-        return fail(TR_ERR_ARG, "tr_batch_create: flags must be 0 (general normals are not supported in batches)");
+    if (flags & ~TR_CTX_GENERAL_NORMALS) return fail(TR_ERR_ARG, "tr_batch_create: unknown flags");
     *out = nullptr;
     std::vector<HostOperator> ops((size_t)nprob);
     std::vector<int> rcs((size_t)nprob, 0);
@@ -406,7 +405,10 @@ int tr_batch_create(int device, int nprob, const tr_mesh_view *meshes, const tr_
     std::vector<std::thread> th;
     for (unsigned k = 0; k < nt; k++)
         th.emplace_back([&, k]() {
-            for (int p = (int)k; p < nprob; p += (int)nt) rcs[p] = assemble_operator(meshes[p], *ang, ops[p]);
+            for (int p = (int)k; p < nprob; p += (int)nt) {
+                ops[p].general_normals = (flags & TR_CTX_GENERAL_NORMALS) != 0;
+                rcs[p] = assemble_operator(meshes[p], *ang, ops[p]);
+            }
         });
     for (auto &t : th) t.join();
     for (int p = 0; p < nprob; p++)

@@ -501,10 +501,26 @@ int concat_operators(const std::vector<HostOperator> &ops, const std::vector<dou
         u.xyz.resize((size_t)3 * nP);
     }
     voff.assign(ops.size() + 1, 0);
+    u.gstart.assign(1, 0);
     uint32_t v0 = 0, b0 = 0;
     for (size_t p = 0; p < ops.size(); p++) {
         const HostOperator &o = ops[p];
         voff[p] = v0;
+        // vacuum faces off the axes: vertices and faces shifted by the
+        // problems before this one
+        u.general_normals = u.general_normals || o.general_normals;
+        const uint32_t f0 = (uint32_t)u.garea.size(), j0 = (uint32_t)u.gface.size();
+        for (size_t b = 0; b < o.gvert.size(); b++) {
+            u.gvert.push_back(v0 + o.gvert[b]);
+            u.gstart.push_back(j0 + o.gstart[b + 1]);
+        }
+        for (size_t j = 0; j < o.gface.size(); j++) {
+            u.gface.push_back(f0 + o.gface[j]);
+            u.gpos.push_back(o.gpos[j]);
+        }
+        for (uint32_t w : o.gfv) u.gfv.push_back(v0 + w);
+        u.garea.insert(u.garea.end(), o.garea.begin(), o.garea.end());
+        u.gO.insert(u.gO.end(), o.gO.begin(), o.gO.end());
         for (uint32_t i = 0; i < o.nP; i++) {
             const uint32_t g = v0 + i;
             u.perm[g] = v0 + o.perm[i];
