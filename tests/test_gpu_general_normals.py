@@ -100,8 +100,11 @@ def test_solutions_agree_in_scalar_radiation(meshes):
                                          ("B-fp64-cheb", B, True, ta.TR_SOLVER_PCG_FP64, 4),
                                          ("B-mixed-cheb", B, True, ta.TR_SOLVER_MIXED, 4),
                                          ("B-mixed-cheb-v3", B, True, ta.TR_SOLVER_MIXED, 4),
+                                         ("B-fp64-multilevel", B, True, ta.TR_SOLVER_PCG_FP64, 3),
                                          ("B-axis-rule", B, False, ta.TR_SOLVER_PCG_FP64, 2)):
         ctx = ta.HipContext(mesh, ang, general_normals=gen)
+        if name.endswith("multilevel"):
+            ctx.set_multilevel(16, 8, 2)  # coarse levels even on this small mesh
         if name.endswith("-v3"):
             for p in (0, 1):
                 ctx.set_schedule(p, J=3, WPB=4, variant=3)  # slm 15 at K = 2
@@ -117,10 +120,7 @@ def test_solutions_agree_in_scalar_radiation(meshes):
     e = {k: np.abs(out[k] - out["A"]).max() / ref for k in out if k != "A"}
     print("[gpu-normals] U against the unrotated solve: %s" % e, flush=True)
     assert e["B"] <= 1e-8 and e["B-fp64-cheb"] <= 1e-8 and e["B-axis-rule"] > 1e-3
-    assert e["B-mixed-cheb"] <= 1e-6 and e["B-mixed-cheb-v3"] <= 1e-6
-    with pytest.raises(ta.TransferError, match="axes"):
-        ctx = ta.HipContext(B, ang, general_normals=True)
-        ctx.solve(ctx.allocVector(), solver=ta.TR_SOLVER_PCG_FP64, use_prec=3, maxit=10, tol_abs=0.0, tol_rel=1e-8)
+    assert e["B-mixed-cheb"] <= 1e-6 and e["B-mixed-cheb-v3"] <= 1e-6 and e["B-fp64-multilevel"] <= 1e-8
 
 
 def test_batch_with_general_normals(meshes):

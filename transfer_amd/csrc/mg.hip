@@ -178,8 +178,9 @@ static int build_levels(tr_ctx *c, Multilevel *mg, uint32_t min_coarse, uint32_t
 
 int build_multilevel(tr_ctx *c, bool f32, uint32_t min_coarse, uint32_t size) {
     int rc;
-    if (!c->op.gvert.empty())
-        return fail(TR_ERR_ARG, "multilevel preconditioner: not with faces off the axes (TR_CTX_GENERAL_NORMALS)");
+    // faces off the axes (TR_CTX_GENERAL_NORMALS) stay on the fine level: the
+    // coarse operators are the Galerkin products of the stencil without
+    // them (still SPD), the smoother and residual use the full operator
     if (!c->mg) {
         // assembled aside and attached only when complete
         Multilevel *mg = new Multilevel();
