@@ -229,20 +229,26 @@ void build_angular(int maxk, double albedo, double g, int hg_lmax, Angular &a) {
 
 namespace tr {
 
-void abs_normal_matrix(int maxk, const double nn[3], std::vector<double> &O) {
-    const int slm = (1 + maxk) * (1 + 2 * maxk), lmax = 2 * maxk;
+// an orthonormal frame (t1, t2, n) around the unit vector of nn
+static void normal_frame(const double nn[3], double t1[3], double t2[3], double n[3]) {
     const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
-    const double n[3] = {nn[0] / len, nn[1] / len, nn[2] / len};
-    // an orthonormal frame (t1, t2, n)
-    double t1[3];
+    for (int a = 0; a < 3; a++) n[a] = nn[a] / len;
     if (std::fabs(n[0]) < 0.9) {
         t1[0] = 0; t1[1] = -n[2]; t1[2] = n[1];
     } else {
         t1[0] = n[2]; t1[1] = 0; t1[2] = -n[0];
     }
     const double l1 = std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
-    for (double &x : t1) x /= l1;
-    const double t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+    for (int a = 0; a < 3; a++) t1[a] /= l1;
+    t2[0] = n[1] * t1[2] - n[2] * t1[1];
+    t2[1] = n[2] * t1[0] - n[0] * t1[2];
+    t2[2] = n[0] * t1[1] - n[1] * t1[0];
+}
+
+void abs_normal_matrix(int maxk, const double nn[3], std::vector<double> &O) {
+    const int slm = (1 + maxk) * (1 + 2 * maxk), lmax = 2 * maxk;
+    double t1[3], t2[3], n[3];
+    normal_frame(nn, t1, t2, n);
     std::vector<double> gx, gw;
     const int nh = 2 * maxk + 2, M = 4 * maxk + 6;
     gauss_legendre(nh, gx, gw);
@@ -261,6 +267,44 @@ void abs_normal_matrix(int maxk, const double nn[3], std::vector<double> &O) {
             const double w = wmu * mu / M;
             for (int a1 = 0; a1 < slm; a1++)
                 for (int a2 = 0; a2 < slm; a2++) O[(size_t)a1 * slm + a2] += w * Y[a1] * Y[a2];
+        }
+    }
+}
+
+uint32_t rotation_size(int maxk) {
+    uint32_t s = 0;
+    for (int l = 0; l <= 2 * maxk; l += 2) s += (uint32_t)((2 * l + 1) * (2 * l + 1));
+    return s;
+}
+
+void normal_rotation(int maxk, const double nn[3], std::vector<double> &R) {
+    const int lmax = 2 * maxk;
+    double t1[3], t2[3], n[3];
+    normal_frame(nn, t1, t2, n);
+    // Gauss-Legendre in mu x trapezoid in phi over the whole sphere, exact
+    // for the products of two harmonics of degree <= 2K
+    std::vector<double> gx, gw;
+    const int nq = 2 * maxk + 1, M = 4 * maxk + 2;
+    gauss_legendre(nq, gx, gw);
+    std::vector<double> Yl((size_t)(lmax + 1) * (lmax + 1)), Yg((size_t)(lmax + 1) * (lmax + 1));
+    R.assign(rotation_size(maxk), 0.0);
+    for (int i = 0; i < nq; i++) {
+        const double mu = gx[i], s = std::sqrt(std::fmax(0.0, 1 - mu * mu));
+        for (int j = 0; j < M; j++) {
+            const double ph = 2 * M_PI * (j + 0.5) / M, c = std::cos(ph), sn = std::sin(ph);
+            const double w = 0.5 * gw[i] / M;  // the average over the sphere
+            const double lx = s * c, ly = s * sn, lz = mu;
+            double d[3];
+            for (int a = 0; a < 3; a++) d[a] = lx * t1[a] + ly * t2[a] + lz * n[a];
+            sh_eval_all(lmax, lx, ly, lz, Yl.data());
+            sh_eval_all(lmax, d[0], d[1], d[2], Yg.data());
+            size_t off = 0;
+            for (int l = 0; l <= lmax; l += 2) {
+                const int nb = 2 * l + 1;
+                for (int a = 0; a < nb; a++)
+                    for (int b = 0; b < nb; b++) R[off + (size_t)a * nb + b] += w * Yg[l * l + a] * Yl[l * l + b];
+                off += (size_t)nb * nb;
+            }
         }
     }
 }

@@ -438,7 +438,8 @@ int tr::ctx_init(tr_ctx *c, int device) {
          ((rc = upload(c, &c->d_gvert, op.gvert)) || (rc = upload(c, &c->d_gstart, op.gstart)) ||
           (rc = upload(c, &c->d_gface, op.gface)) || (rc = upload(c, &c->d_gpos, op.gpos)) ||
           (rc = upload(c, &c->d_gfv, op.gfv)) || (rc = upload(c, &c->d_garea, op.garea)) ||
-          (rc = upload(c, &c->d_gO, op.gO)))) ||
+          (rc = upload(c, &c->d_gR, op.gR)) || (rc = upload(c, &c->d_gOd, op.gOd)) ||
+          (rc = upload(c, &c->d_gOz, op.gOz)))) ||
         (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
         (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
         tr_ctx_destroy(c);
@@ -817,7 +818,7 @@ std::vector<double> operator_diagonal(const HostOperator &op) {
         for (uint32_t j = op.gstart[b]; j < op.gstart[b + 1]; j++) {
             const uint32_t F = op.gface[j];
             for (uint32_t lm = 0; lm < slm; lm++)
-                d[(size_t)v * slm + lm] += 2.0 * op.garea[F] * op.gO[((size_t)F * slm + lm) * slm + lm];
+                d[(size_t)v * slm + lm] += 2.0 * op.garea[F] * op.gOd[(size_t)F * slm + lm];
         }
     }
     return d;
@@ -984,7 +985,8 @@ int ensure_fp32(tr_ctx *c) {
         (rc = upload(c, &c->f_bcrec, bcoef_records<float>(op))) ||
         (!op.mf0.empty() && (rc = upload(c, &c->f_mf0, to_f32(op.mf0)))) ||
         (!op.gvert.empty() &&
-         ((rc = upload(c, &c->f_garea, to_f32(op.garea))) || (rc = upload(c, &c->f_gO, to_f32(op.gO))))) ||
+         ((rc = upload(c, &c->f_garea, to_f32(op.garea))) || (rc = upload(c, &c->f_gR, to_f32(op.gR))) ||
+          (rc = upload(c, &c->f_gOd, to_f32(op.gOd))) || (rc = upload(c, &c->f_gOz, to_f32(op.gOz))))) ||
         (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
         return rc;
     if (v5_ok(op) && (rc = upload(c, (unsigned char **)&c->f_tab5, mfma_tables5(op)))) return rc;

@@ -46,8 +46,8 @@ struct tr_ctx {
     double *d_vsum = nullptr, *d_xyz = nullptr;  // learned-guess feature inputs [3][nP] each, or null
     // vacuum faces off the axes (general normals, see stencil.h)
     uint32_t *d_gvert = nullptr, *d_gstart = nullptr, *d_gface = nullptr, *d_gpos = nullptr, *d_gfv = nullptr;
-    double *d_garea = nullptr, *d_gO = nullptr;
-    float *f_garea = nullptr, *f_gO = nullptr;
+    double *d_garea = nullptr, *d_gR = nullptr, *d_gOd = nullptr, *d_gOz = nullptr;
+    float *f_garea = nullptr, *f_gR = nullptr, *f_gOd = nullptr, *f_gOz = nullptr;
     double *d_feat_tmp = nullptr;  // [2][nP] kappa and optical depth + 8 flags (guess.hip), on first use
     float *d_guess_tmp = nullptr;  // [9][nP] features and prediction (guess.hip), on first use
     double *d_mom_val = nullptr;   // moments table [3][slm]
@@ -120,7 +120,9 @@ struct tr_ctx {
         a.gpos = d_gpos;
         a.gfv = d_gfv;
         a.garea = d_garea;
-        a.gO = d_gO;
+        a.gR = d_gR;
+        a.gOd = d_gOd;
+        a.gOz = d_gOz;
         return a;
     }
     tr::OpArgs<float> args32() const { return args32(map32); }
@@ -166,7 +168,9 @@ struct tr_ctx {
         a.gpos = d.gpos;
         a.gfv = d.gfv;
         a.garea = f_garea;
-        a.gO = f_gO;
+        a.gR = f_gR;
+        a.gOd = f_gOd;
+        a.gOz = f_gOz;
         return a;
     }
     size_t nint() const { return (size_t)op.nP * op.slm; }

@@ -61,7 +61,7 @@ struct OpArgs {
     // stencil.h), applied by k_bgen after the operator kernel; ngv = 0: none
     uint32_t ngv = 0;
     const uint32_t *gvert = nullptr, *gstart = nullptr, *gface = nullptr, *gpos = nullptr, *gfv = nullptr;
-    const T *garea = nullptr, *gO = nullptr;
+    const T *garea = nullptr, *gR = nullptr, *gOd = nullptr, *gOz = nullptr;
 };
 
 // One step of the Chebyshev iteration on D^-1 A z = D^-1 rhs, written as

@@ -662,35 +662,80 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
 
 // The vacuum faces off the axes (general normals): for the vertex v of
 // block b, r_v += sum over its faces F of O(n_F) g_F, g_F = sum_{k in F}
-// (S_F/12)(1 + delta_vk) f_k (DIAG: the diagonal of O(n_F) only, the Diag
-// operator's boundary term).  One workgroup per such vertex, a thread per
-// harmonic; g_F staged in LDS.
+// (S_F/12)(1 + delta_vk) f_k.  One workgroup per such vertex, a thread per
+// harmonic, g_F staged in LDS.  O(n_F) g = R (O_z (R^T g)) with R the face's
+// rotation (one block per even l, a block's harmonics contiguous) and O_z
+// shared by all faces, so a face brings rotation_size values, not slm^2.
+// DIAG: diag(O(n_F)) g, the Diag operator's boundary term.  cheb != 0: the
+// operator kernel wrote a fused Chebyshev step into r, and this adds the
+// faces' share of A f to it, r -= ga dinv (faces' A f), the step being
+// linear in A f.
 template <typename T, bool DIAG>
-// (cheb != 0: the operator kernel wrote a fused Chebyshev step into r, and
-// this adds the faces' share of A f to it, r -= ga dinv (faces' A f), the
-// step being linear in A f)
 __global__ __launch_bounds__(128) void k_bgen(OpArgs<T> a, const T *__restrict__ f, T *__restrict__ r,
                                               const double *__restrict__ skip, ChebStep<T> cs, int cheb) {
     if (skip && *skip != 0.0) return;
-    __shared__ T g[128];
-    const uint32_t slm = a.slm, t = threadIdx.x, b = blockIdx.x;
+    // dynamic LDS: O_z [slm][slm] when it takes at most 8 KB (else it is
+    // read from global memory: staged per vertex it would cost occupancy),
+    // the face's rotation [nr], g, y
+    extern __shared__ __align__(16) unsigned char bgen_smem[];
+    const uint32_t slm = a.slm, t = threadIdx.x, b = blockIdx.x, nt = blockDim.x;
     const uint32_t v = a.gvert[b];
+    // this thread's harmonic: block start s0 and size nb, block offset ro
+    // in a face's packed rotation, rotation_size nr
+    uint32_t s0 = 0, nb = 1, ro = 0, nr = 0;
+    for (uint32_t l = 0, s = 0; s < slm; l += 2) {
+        const uint32_t n = 2 * l + 1;
+        if (t >= s && t < s + n) {
+            s0 = s;
+            nb = n;
+            ro = nr;
+        }
+        s += n;
+        nr += n * n;
+    }
+    const uint32_t p = t - s0;
+    const bool ozl = !DIAG && (size_t)slm * slm * sizeof(T) <= 8192;
+    T *Ozs = reinterpret_cast<T *>(bgen_smem);
+    T *Rs = Ozs + (ozl ? (size_t)slm * slm : 0);
+    T *g = Rs + (DIAG ? 0 : nr), *y = g + 128;
+    if (ozl)
+        for (uint32_t i = t; i < slm * slm; i += nt) Ozs[i] = a.gOz[i];
+    const T *Ozp = ozl ? Ozs : a.gOz;
     T acc = 0;
     for (uint32_t j = a.gstart[b]; j < a.gstart[b + 1]; j++) {
         const uint32_t F = a.gface[j], li = a.gpos[j];
         const T w = a.garea[F];
+        if (!DIAG) {
+            const T *R = a.gR + (size_t)F * nr;
+            for (uint32_t i = t; i < nr; i += nt) Rs[i] = R[i];
+        }
         if (t < slm) {
-            T s = 0;
-            for (uint32_t k = 0; k < 3; k++) s += (k == li ? 2 * w : w) * f[(size_t)a.gfv[3 * F + k] * slm + t];
-            g[t] = s;
+            T sum = 0;
+            for (uint32_t k = 0; k < 3; k++) sum += (k == li ? 2 * w : w) * f[(size_t)a.gfv[3 * F + k] * slm + t];
+            g[t] = sum;
         }
         __syncthreads();
-        if (t < slm) {
-            const T *O = a.gO + ((size_t)F * slm + t) * slm;
-            if (DIAG) {
-                acc += O[t] * g[t];
-            } else {
-                for (uint32_t m = 0; m < slm; m++) acc += O[m] * g[m];
+        if (DIAG) {
+            if (t < slm) acc += a.gOd[(size_t)F * slm + t] * g[t];
+        } else {
+            const T *Rb = Rs + ro;  // this thread's block, [nb][nb]
+            if (t < slm) {  // y = R^T g
+                T sum = 0;
+                for (uint32_t c = 0; c < nb; c++) sum += Rb[c * nb + p] * g[s0 + c];
+                y[t] = sum;
+            }
+            __syncthreads();
+            if (t < slm) {  // g = O_z y (g is free again)
+                T sum = 0;
+                const T *Oz = Ozp + (size_t)t * slm;
+                for (uint32_t m = 0; m < slm; m++) sum += Oz[m] * y[m];
+                g[t] = sum;
+            }
+            __syncthreads();
+            if (t < slm) {  // += R g
+                T sum = 0;
+                for (uint32_t c = 0; c < nb; c++) sum += Rb[p * nb + c] * g[s0 + c];
+                acc += sum;
             }
         }
         __syncthreads();
@@ -709,11 +754,22 @@ static hipError_t launch_bgen(const OpArgs<T> &a, bool diag, const T *f, T *r, c
                               const ChebStep<T> *cs = nullptr) {
     if (!a.ngv) return hipSuccess;
     if (a.slm > 128 || (diag && cs)) return hipErrorInvalidValue;
+    uint32_t nr = 0;
+    for (uint32_t l = 0, q = 0; q < a.slm; l += 2) {
+        q += 2 * l + 1;
+        nr += (2 * l + 1) * (2 * l + 1);
+    }
+    // one wavefront per vertex up to 64 harmonics; O_z, the face's rotation
+    // and two vectors in LDS
+    const unsigned nt = a.slm <= 64 ? 64 : 128;
+    const bool ozl = !diag && (size_t)a.slm * a.slm * sizeof(T) <= 8192;  // as the kernel
+    const size_t sh = sizeof(T) * ((ozl ? (size_t)a.slm * a.slm : 0) + (diag ? 0 : nr) + 256);
+    if (sh > 65536) return hipErrorInvalidValue;
     const ChebStep<T> c = cs ? *cs : ChebStep<T>{};
     if (diag)
-        hipLaunchKernelGGL((k_bgen<T, true>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip, c, 0);
+        hipLaunchKernelGGL((k_bgen<T, true>), dim3(a.ngv), dim3(nt), sh, s, a, f, r, skip, c, 0);
     else
-        hipLaunchKernelGGL((k_bgen<T, false>), dim3(a.ngv), dim3(128), 0, s, a, f, r, skip, c, cs ? 1 : 0);
+        hipLaunchKernelGGL((k_bgen<T, false>), dim3(a.ngv), dim3(nt), sh, s, a, f, r, skip, c, cs ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -330,18 +330,37 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
     // general faces: vertices (new numbering), S/12 and <|Omega.n| Y Y'>
     op.gfv.resize(3 * gfaces.size());
     op.garea.resize(gfaces.size());
-    op.gO.resize(gfaces.size() * (size_t)slm * slm);
+    op.gR.clear();
+    op.gOd.assign(gfaces.size() * (size_t)slm, 0.0);
+    op.gOz.clear();
     if (!gfaces.empty()) {
         int maxk = 0;
         while ((uint32_t)((1 + maxk) * (1 + 2 * maxk)) < slm) maxk++;
         if ((uint32_t)((1 + maxk) * (1 + 2 * maxk)) != slm) return fail(TR_ERR_ARG, "assemble_operator: slm is not (1+K)(1+2K)");
-        std::vector<double> O;
+        const double zaxis[3] = {0.0, 0.0, 1.0};
+        abs_normal_matrix(maxk, zaxis, op.gOz);
+        const uint32_t nr = rotation_size(maxk);
+        op.gR.resize(gfaces.size() * (size_t)nr);
+        std::vector<double> Rf;
         for (size_t g = 0; g < gfaces.size(); g++) {
             const tr_face &F = mv.bnd[gfaces[g]];
             for (int k = 0; k < 3; k++) op.gfv[3 * g + k] = op.iperm[F.p[k]];
             op.garea[g] = std::sqrt(F.s[0] * F.s[0] + F.s[1] * F.s[1] + F.s[2] * F.s[2]) / 12.0;
-            abs_normal_matrix(maxk, F.s, O);
-            std::copy(O.begin(), O.end(), op.gO.begin() + g * (size_t)slm * slm);
+            normal_rotation(maxk, F.s, Rf);
+            std::copy(Rf.begin(), Rf.end(), op.gR.begin() + g * (size_t)nr);
+            // diagonal of R O_z R^T: harmonic a of block l (start s, size nb)
+            for (int l = 0, s = 0, off = 0; l <= 2 * maxk; l += 2) {
+                const int nb = 2 * l + 1;
+                for (int a = 0; a < nb; a++) {
+                    double v = 0;
+                    for (int c = 0; c < nb; c++)
+                        for (int d = 0; d < nb; d++)
+                            v += Rf[off + a * nb + c] * op.gOz[(size_t)(s + c) * slm + s + d] * Rf[off + a * nb + d];
+                    op.gOd[g * (size_t)slm + s + a] = v;
+                }
+                s += nb;
+                off += nb * nb;
+            }
         }
     }
     op.nB = (uint32_t)op.bvert.size();
@@ -520,7 +539,9 @@ int concat_operators(const std::vector<HostOperator> &ops, const std::vector<dou
         }
         for (uint32_t w : o.gfv) u.gfv.push_back(v0 + w);
         u.garea.insert(u.garea.end(), o.garea.begin(), o.garea.end());
-        u.gO.insert(u.gO.end(), o.gO.begin(), o.gO.end());
+        u.gR.insert(u.gR.end(), o.gR.begin(), o.gR.end());
+        u.gOd.insert(u.gOd.end(), o.gOd.begin(), o.gOd.end());
+        if (u.gOz.empty()) u.gOz = o.gOz;
         for (uint32_t i = 0; i < o.nP; i++) {
             const uint32_t g = v0 + i;
             u.perm[g] = v0 + o.perm[i];

@@ -78,16 +78,25 @@ struct HostOperator {
     //   r_i += O(n_F) sum_{k in F} (S_F / 12)(1 + delta_ik) f_k
     // gvert[ngv] the vertices with such faces, gstart[ngv+1] into
     // gface/gpos (face, the vertex's place in it), gfv[nGF][3] the faces'
-    // vertices, garea[nGF] = S_F / 12, gO[nGF][slm][slm] = <|Omega.n| Y Y'>
+    // vertices, garea[nGF] = S_F / 12.  <|Omega.n| Y Y'> = R O_z R^T, with
+    // gR[nGF][rotation_size] the faces' rotations (normal_rotation), gOz
+    // [slm][slm] = O_z and gOd[nGF][slm] the diagonals of R O_z R^T
     bool general_normals = false;
     std::vector<uint32_t> gvert, gstart, gface, gpos, gfv;
-    std::vector<double> garea, gO;
+    std::vector<double> garea, gR, gOd, gOz;
 };
 
 // <|Omega.n| Y_out Y_in> for the unit normal n: [slm][slm] (symmetric),
 // exact (a Gauss-Legendre x trapezoid rule on the hemisphere around n; the
 // integrand is even, and a polynomial of degree 4K+1 there).
 void abs_normal_matrix(int maxk, const double n[3], std::vector<double> &O);
+
+// The rotation of the even harmonics up to 2K that takes the z axis to n
+// (the frame of abs_normal_matrix): R_l[a][b] = <Y_la(Q w) Y_lb(w)>, Q the
+// frame, one (2l+1)^2 block per even l, packed (rotation_size values).
+// abs_normal_matrix(n) = R O_z R^T with O_z = abs_normal_matrix(z axis).
+uint32_t rotation_size(int maxk);
+void normal_rotation(int maxk, const double n[3], std::vector<double> &R);
 
 // The block-diagonal union of independent operators with identical angular
 // data (the problems of a batch), vertices of problem p at
