@@ -9,7 +9,7 @@ With general normals (TR_CTX_GENERAL_NORMALS) the vacuum boundary term is
 exact for every face, and the error left is the P_N truncation and the mesh
 (the core's staircase edge).  In the optically thin shell the radiation
 streams out of a small source, so P_N converges slowly there: measured on
-this mesh, 0.71, 0.26 and 0.18 at K = 1, 3, 5, with the core at 1%.  The reference's axis rule applies a wrong boundary term on such a
+this mesh, 0.71, 0.26, 0.19 and 0.18 at K = 1, 3, 4, 5, with the core at 1%.  The reference's axis rule applies a wrong boundary term on such a
 mesh: its shell error is 1.00 at K = 1 and 0.47 at K = 3.  Errors are
 relative RMS over the vertices of each region."""
 import numpy as np
@@ -40,12 +40,12 @@ def test_sphere_against_exact_transport():
     rs = np.linspace(0.0, 1.0, 401)
     table = se.U_table(rs)
     err = {}
-    for K in (1, 3, 5):
+    for K in (1, 3, 4, 5):
         P = ta.problems.sphere(40, maxk=K)
         r = np.linalg.norm(P.mesh.geometry()[0], axis=1)
         Ue = np.interp(r, rs, table)
         core, shell = r < 0.2, r > 0.45
-        for general in ((True, False) if K < 5 else (True,)):
+        for general in ((True, False) if K < 4 else (True,)):
             print("[gpu-sphere] solving K=%d general=%s (%d vertices)" % (K, general, len(r)), flush=True)
             U = solve_U(P, general)
             assert np.all(np.isfinite(U))
@@ -58,4 +58,5 @@ def test_sphere_against_exact_transport():
     for K in (1, 3):
         assert err[(K, True)][1] < 0.8 * err[(K, False)][1]
     assert err[(5, True)][1] < err[(3, True)][1] < err[(1, True)][1]
-    assert max(err[(K, True)][0] for K in (1, 3, 5)) <= 0.03
+    assert err[(4, True)][1] < err[(1, True)][1]
+    assert max(err[(K, True)][0] for K in (1, 3, 4, 5)) <= 0.03

@@ -676,9 +676,12 @@ __global__ __launch_bounds__(128) void k_bgen(OpArgs<T> a, const T *__restrict__
     if (skip && *skip != 0.0) return;
     // dynamic LDS: O_z [slm][slm] when it takes at most 8 KB (else it is
     // read from global memory: staged per vertex it would cost occupancy),
-    // the face's rotation [nr], g, y
+    // then per face slot the face's rotation [nr], g [32|128] and y [32|128].
+    // With slm <= 32 the 64 lanes take two faces at once (slot = lane / 32).
     extern __shared__ __align__(16) unsigned char bgen_smem[];
-    const uint32_t slm = a.slm, t = threadIdx.x, b = blockIdx.x, nt = blockDim.x;
+    const uint32_t slm = a.slm, b = blockIdx.x, nt = blockDim.x;
+    const uint32_t ns = slm <= 32 ? 2u : 1u, hw = slm <= 32 ? 32u : 128u;  // slots, harmonics per slot
+    const uint32_t slot = threadIdx.x / hw, t = threadIdx.x % hw, tps = nt / ns;  // tps: threads per slot
     const uint32_t v = a.gvert[b];
     // this thread's harmonic: block start s0 and size nb, block offset ro
     // in a face's packed rotation, rotation_size nr
@@ -696,43 +699,51 @@ __global__ __launch_bounds__(128) void k_bgen(OpArgs<T> a, const T *__restrict__
     const uint32_t p = t - s0;
     const bool ozl = !DIAG && (size_t)slm * slm * sizeof(T) <= 8192;
     T *Ozs = reinterpret_cast<T *>(bgen_smem);
-    T *Rs = Ozs + (ozl ? (size_t)slm * slm : 0);
-    T *g = Rs + (DIAG ? 0 : nr), *y = g + 128;
+    T *Rall = Ozs + (ozl ? (size_t)slm * slm : 0);
+    T *gall = Rall + (DIAG ? 0 : (size_t)ns * nr);
+    T *Rs = Rall + (DIAG ? 0 : (size_t)slot * nr), *g = gall + (size_t)slot * 2 * hw, *y = g + hw;
     if (ozl)
-        for (uint32_t i = t; i < slm * slm; i += nt) Ozs[i] = a.gOz[i];
+        for (uint32_t i = threadIdx.x; i < slm * slm; i += nt) Ozs[i] = a.gOz[i];
     const T *Ozp = ozl ? Ozs : a.gOz;
+    const uint32_t j0 = a.gstart[b], j1 = a.gstart[b + 1];
     T acc = 0;
-    for (uint32_t j = a.gstart[b]; j < a.gstart[b + 1]; j++) {
-        const uint32_t F = a.gface[j], li = a.gpos[j];
-        const T w = a.garea[F];
-        if (!DIAG) {
-            const T *R = a.gR + (size_t)F * nr;
-            for (uint32_t i = t; i < nr; i += nt) Rs[i] = R[i];
+    for (uint32_t jb = j0; jb < j1; jb += ns) {
+        const uint32_t j = jb + slot;
+        const bool live = j < j1 && t < slm;
+        uint32_t F = 0;
+        if (j < j1) {
+            F = a.gface[j];
+            if (!DIAG) {
+                const T *R = a.gR + (size_t)F * nr;
+                for (uint32_t i = t; i < nr; i += tps) Rs[i] = R[i];
+            }
         }
-        if (t < slm) {
+        if (live) {
+            const uint32_t li = a.gpos[j];
+            const T w = a.garea[F];
             T sum = 0;
             for (uint32_t k = 0; k < 3; k++) sum += (k == li ? 2 * w : w) * f[(size_t)a.gfv[3 * F + k] * slm + t];
             g[t] = sum;
         }
         __syncthreads();
         if (DIAG) {
-            if (t < slm) acc += a.gOd[(size_t)F * slm + t] * g[t];
+            if (live) acc += a.gOd[(size_t)F * slm + t] * g[t];
         } else {
             const T *Rb = Rs + ro;  // this thread's block, [nb][nb]
-            if (t < slm) {  // y = R^T g
+            if (live) {  // y = R^T g
                 T sum = 0;
                 for (uint32_t c = 0; c < nb; c++) sum += Rb[c * nb + p] * g[s0 + c];
                 y[t] = sum;
             }
             __syncthreads();
-            if (t < slm) {  // g = O_z y (g is free again)
+            if (live) {  // g = O_z y (g is free again)
                 T sum = 0;
                 const T *Oz = Ozp + (size_t)t * slm;
                 for (uint32_t m = 0; m < slm; m++) sum += Oz[m] * y[m];
                 g[t] = sum;
             }
             __syncthreads();
-            if (t < slm) {  // += R g
+            if (live) {  // += R g
                 T sum = 0;
                 for (uint32_t c = 0; c < nb; c++) sum += Rb[p * nb + c] * g[s0 + c];
                 acc += sum;
@@ -740,7 +751,13 @@ __global__ __launch_bounds__(128) void k_bgen(OpArgs<T> a, const T *__restrict__
         }
         __syncthreads();
     }
-    if (t < slm) {
+    // the slots' sums (slot 1 through the first g of slot 0's area)
+    if (ns == 2) {
+        if (slot == 1 && t < slm) gall[t] = acc;
+        __syncthreads();
+        if (slot == 0 && t < slm) acc += gall[t];
+    }
+    if (slot == 0 && t < slm) {
         const size_t o = (size_t)v * slm + t;
         if (cheb)
             r[o] -= cs.ga * cs.dinv[o] * acc;
@@ -759,11 +776,12 @@ static hipError_t launch_bgen(const OpArgs<T> &a, bool diag, const T *f, T *r, c
         q += 2 * l + 1;
         nr += (2 * l + 1) * (2 * l + 1);
     }
-    // one wavefront per vertex up to 64 harmonics; O_z, the face's rotation
-    // and two vectors in LDS
+    // one wavefront per vertex up to 64 harmonics (two faces at once up to
+    // 32); O_z, each slot's rotation and two vectors in LDS
     const unsigned nt = a.slm <= 64 ? 64 : 128;
-    const bool ozl = !diag && (size_t)a.slm * a.slm * sizeof(T) <= 8192;  // as the kernel
-    const size_t sh = sizeof(T) * ((ozl ? (size_t)a.slm * a.slm : 0) + (diag ? 0 : nr) + 256);
+    const size_t ns = a.slm <= 32 ? 2 : 1, hw = a.slm <= 32 ? 32 : 128;  // as the kernel
+    const bool ozl = !diag && (size_t)a.slm * a.slm * sizeof(T) <= 8192;
+    const size_t sh = sizeof(T) * ((ozl ? (size_t)a.slm * a.slm : 0) + (diag ? 0 : ns * nr) + ns * 2 * hw);
     if (sh > 65536) return hipErrorInvalidValue;
     const ChebStep<T> c = cs ? *cs : ChebStep<T>{};
     if (diag)
