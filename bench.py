@@ -127,6 +127,30 @@ def bench_batch(count, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
+def bench_sphere(n, steps, warmup):
+    """The curved-boundary workload: the two-region sphere of the soltype-1
+    check (problems.sphere, every boundary face off the axes), the FP64
+    operator with the reference's axis rule and with general normals, at
+    K = 3 and 5."""
+    import transfer_amd as ta
+    out = dict(metric="FP64 operator time on a curved sphere mesh, axis rule vs general normals", unit="us",
+               n_gpus=1, steps=steps, warmup=warmup, higher_is_better=False, data="synthetic", dtype="f64",
+               config=dict(workload="sphere%d: %d^3 Kuhn cubes mapped onto the unit ball, core r < 0.3" % (n, n)))
+    for K in (3, 5):
+        P = ta.problems.sphere(n, maxk=K)
+        for gen in (False, True):
+            ctx = ta.HipContext(P.mesh, P.ang, general_normals=gen)
+            ctx.autotune(0)
+            w, k, _, _ = ctx.bench_operator(warmup, steps, precision=0)
+            key = "K%d_%s" % (K, "general_normals" if gen else "axis_rule")
+            out[key] = dict(operator_us_per_step=w / steps * 1e6, kernel=ctx.kernel_name(0), nP=ctx.nP,
+                            nF=P.mesh.nF)
+            print("[bench-sphere] %s: %.1f us per application (%s)" % (key, w / steps * 1e6, ctx.kernel_name(0)),
+                  file=sys.stderr, flush=True)
+            ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_c3(steps, warmup, solve=True, copy_calibration=False):
     """The high-order scattering workload: the 96^3-grid staircase ball
     (2,780,400 tets, 485,385 vertices), K = 5 (66 harmonics, the 194-point
@@ -260,6 +284,8 @@ def main():
     ap.add_argument("--c3", action="store_true", help="run the K = 5 scattering workload (ball96-hg) instead")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--only", default="", help="run only the named time_to_1e-8 solve (for short profiles)")
+    ap.add_argument("--sphere", type=int, default=0,
+                    help="run the curved sphere workload with this many cubes per axis instead (48: the measured one)")
     ap.add_argument("--batch", type=int, default=0,
                     help="run the batching workload with this many problems instead (256 = the full one)")
     args = ap.parse_args()
@@ -269,6 +295,9 @@ def main():
         sys.exit(2)
     if args.batch:
         bench_batch(args.batch, args.steps, args.warmup)
+        return
+    if args.sphere:
+        bench_sphere(args.sphere, min(args.steps, 50), min(args.warmup, 5))
         return
     if args.c3:
         bench_c3(min(args.steps, 50), min(args.warmup, 5), solve=not args.no_solve,
