@@ -178,7 +178,7 @@ int tr_config_strings(const tr_config *c, char *meshfile, char *outfile,
 
 typedef struct tr_ctx tr_ctx;
 
-/* CudaContext(dev, MeshData, AngularData) (CudaContext.cpp:76-102):
+/* CudaContext(dev, MeshData, AngularData) (CudaContext.cpp:24-50):
  * uploads mesh and angular data (DeviceMeshData.cpp:8-52,
  * DeviceAngularData.cpp:7-67).  flags: 0 (the reference's operator) or
  * TR_CTX_GENERAL_NORMALS. */
@@ -191,8 +191,8 @@ typedef struct tr_ctx tr_ctx;
 #define TR_CTX_GENERAL_NORMALS 1
 int tr_ctx_create(int device, const tr_mesh_view *mesh,
                   const tr_angular_view *ang, int flags, tr_ctx **out);
-int tr_ctx_destroy(tr_ctx *ctx);               /* ~CudaContext :104-110 */
-uint32_t tr_ctx_N(const tr_ctx *ctx);          /* N() = nP*aslm :119-121 */
+int tr_ctx_destroy(tr_ctx *ctx);               /* ~CudaContext CudaContext.cpp:52-58 */
+uint32_t tr_ctx_N(const tr_ctx *ctx);          /* N() = nP*aslm CudaContext.cpp:67-69 */
 uint32_t tr_ctx_slm(const tr_ctx *ctx);
 uint32_t tr_ctx_aslm(const tr_ctx *ctx);
 uint32_t tr_ctx_nP(const tr_ctx *ctx);
@@ -201,7 +201,7 @@ uint32_t tr_ctx_nP(const tr_ctx *ctx);
  * this context's mesh and keeps the fastest.  precision 0 = FP64 operator,
  * 1 = FP32 operator (mixed solves).  report (may be NULL) gets one line per
  * candidate.  No reference counterpart (the reference launches fixed
- * geometry, CudaContext.cpp:160-172). */
+ * geometry, CudaContext.cpp:108-120). */
 int tr_ctx_autotune(tr_ctx *ctx, int precision, char *report, size_t cap);
 /* The schedule in use: out5 = {J, L, VPW, WPB, variant} */
 int tr_ctx_schedule(tr_ctx *ctx, int precision, int *out5);
@@ -234,20 +234,30 @@ int tr_ctx_set_chebyshev(tr_ctx *ctx, int degree, double ratio, int step_split);
  * schedule keeps the couplings in BF16 only, the gathered values alone are
  * split: two products per term). */
 int tr_ctx_set_angular_precision(tr_ctx *ctx, int prec);
+/* Robustness of TR_SOLVER_MIXED (extension; the reference has only its
+ * iteration caps, main.cpp:142, 232): when a refinement step cuts the true
+ * FP64 residual by less than `stagnation` (it must reach stagnation times
+ * the previous one), or makes it larger or non-finite (that update is
+ * taken back), or the inner CG breaks down, the solve continues with
+ * TR_SOLVER_PCG_FP64 (same use_prec) from the last good iterate and
+ * reports stats.fallback = 2.  The inner CG stops early when its residual
+ * norm has not dropped by 10% over `inner_stall_window` iterations (0 =
+ * never).  Defaults 0.5 and 100. */
+int tr_ctx_set_refinement(tr_ctx *ctx, double stagnation, int inner_stall_window);
 /* Returns the internal numbering: perm[new] = caller vertex id. */
 int tr_ctx_permutation(const tr_ctx *ctx, uint32_t *perm);
 
-void *tr_device_alloc(tr_ctx *ctx, size_t bytes);   /* zero-filled, :112-117; NULL on failure */
-double *tr_alloc_vector(tr_ctx *ctx);               /* :123-125; NULL on failure */
-int tr_device_free(tr_ctx *ctx, void *mem);         /* :127-129 */
-int tr_copy_to_dev(tr_ctx *ctx, void *dst, const void *src, size_t bytes);   /* :131-133 */
-int tr_copy_to_host(tr_ctx *ctx, void *dst, const void *src, size_t bytes);  /* :135-137 */
-/* getHostSmall (:139-150): dst[nP][slm] (caller-allocated, not malloc'd) */
+void *tr_device_alloc(tr_ctx *ctx, size_t bytes);   /* zero-filled, CudaContext.cpp:60-65; NULL on failure */
+double *tr_alloc_vector(tr_ctx *ctx);               /* CudaContext.cpp:71-73; NULL on failure */
+int tr_device_free(tr_ctx *ctx, void *mem);         /* CudaContext.cpp:75-77 */
+int tr_copy_to_dev(tr_ctx *ctx, void *dst, const void *src, size_t bytes);   /* CudaContext.cpp:79-81 */
+int tr_copy_to_host(tr_ctx *ctx, void *dst, const void *src, size_t bytes);  /* CudaContext.cpp:83-85 */
+/* getHostSmall (CudaContext.cpp:87-98): dst[nP][slm] (caller-allocated, not malloc'd) */
 int tr_get_host_small(tr_ctx *ctx, const double *src, double *dst);
 
-int tr_compute_lhs(tr_ctx *ctx, const double *f, double *Af);      /* :160-172 */
-int tr_compute_lhs_diag(tr_ctx *ctx, const double *f, double *Af); /* :174-186 */
-int tr_compute_rhs(tr_ctx *ctx, double *b);                        /* :151-158 */
+int tr_compute_lhs(tr_ctx *ctx, const double *f, double *Af);      /* CudaContext.cpp:108-120 */
+int tr_compute_lhs_diag(tr_ctx *ctx, const double *f, double *Af); /* CudaContext.cpp:122-134 */
+int tr_compute_rhs(tr_ctx *ctx, double *b);                        /* CudaContext.cpp:99-106 */
 /* computeLhs with the operator in FP32 (precision 1; the inner operator of
  * the mixed solver) or FP64 (precision 0); f and Af stay FP64 vectors. */
 int tr_compute_lhs_prec(tr_ctx *ctx, int precision, const double *f, double *Af);
@@ -258,11 +268,21 @@ int tr_compute_moments(tr_ctx *ctx, const double *f, double *U, double *T);
  * device T of tr_compute_moments and the mesh view the context was built
  * from. */
 int tr_compute_flux(tr_ctx *ctx, const tr_mesh_view *mesh, const double *T, double *W);
-int tr_scale(tr_ctx *ctx, double *x, double wx);                   /* :188-196 */
-int tr_add_prod(tr_ctx *ctx, double *x, const double *y, double wy);          /* :198-206 */
-int tr_mul_add(tr_ctx *ctx, double *x, double wx, const double *y);           /* :208-216 */
-int tr_mul_add_prod(tr_ctx *ctx, double *x, double wx, const double *y, double wy); /* :218-225 */
-/* norm/dot (:227-250) over the slm real lanes; NaN on failure */
+/* Boundary flux (the "boundary flux" output of the reference workflow, which
+ * no reference function computes; derived from the vacuum boundary term of
+ * kernels.cu:414-437): device F[nF] in the view's face order, the outward
+ * flux density through each boundary face averaged over the face,
+ * 4 pi sum_lm O_n[lm][00] f_lm with the face's O_n as the operator takes it
+ * (axis rule, kernels.cu:419-425, or its own normal under
+ * TR_CTX_GENERAL_NORMALS); 0 on reflecting faces.  Times the face area
+ * (|s_x + s_y + s_z| under the axis rule) it sums to 4 pi times the lm = 0
+ * rows of the boundary term.  Single-problem contexts only. */
+int tr_compute_boundary_flux(tr_ctx *ctx, const tr_mesh_view *mesh, const double *f, double *F);
+int tr_scale(tr_ctx *ctx, double *x, double wx);                   /* CudaContext.cpp:137-144 */
+int tr_add_prod(tr_ctx *ctx, double *x, const double *y, double wy);          /* CudaContext.cpp:147-154 */
+int tr_mul_add(tr_ctx *ctx, double *x, double wx, const double *y);           /* CudaContext.cpp:157-164 */
+int tr_mul_add_prod(tr_ctx *ctx, double *x, double wx, const double *y, double wy); /* CudaContext.cpp:166-173 */
+/* norm/dot (CudaContext.cpp:175-198) over the slm real lanes; NaN on failure */
 double tr_norm(tr_ctx *ctx, const double *x);
 double tr_dot(tr_ctx *ctx, const double *x, const double *y);
 int tr_synchronize(tr_ctx *ctx);
@@ -309,7 +329,9 @@ typedef struct tr_solve_stats {
     double true_rel;     /* ||b - A f|| / ||b|| with the FP64 operator */
     double seconds;      /* wall time of the solve, device-synchronised */
     int status;          /* 0 converged, TR_ERR_NOCONV otherwise */
-    int fallback;        /* 1: the x0 guess was rejected and the solve started from 0 */
+    int fallback;        /* 1: the x0 guess was rejected and the solve started
+                            from 0; 2: TR_SOLVER_MIXED stagnated and finished
+                            with TR_SOLVER_PCG_FP64 (tr_ctx_set_refinement) */
 } tr_solve_stats;
 
 /* Solves A f = b, b = computeRhs, from f = 0.  f is a device vector.
@@ -457,6 +479,16 @@ int tr_stencil_export(const tr_stencil *s, uint32_t *perm, uint32_t *nbr,
                       double *coef, uint32_t *bvert, uint32_t *bnbr,
                       double *bcoef, double *omc, double *onc, double *massfac,
                       double *rhs);
+
+/* The exact boundary matrix of a vacuum face with normal n (general
+ * normals): O(n) = <|Omega.n| Y Y'> [slm][slm] by a Gauss-Legendre x
+ * trapezoid rule on the hemisphere around n; its column lm' = 00 by the
+ * Funk-Hecke formula; and the block-diagonal rotation R of the even
+ * harmonics (one (2l+1)^2 block per even l, *size values) with
+ * O(n) = R O(z) R^T, the form the face kernel applies.  Host only. */
+int tr_abs_normal_matrix(int maxk, const double *n, double *O);
+int tr_abs_normal_column(int maxk, const double *n, double *col);
+int tr_normal_rotation(int maxk, const double *n, double *R, uint32_t *size);
 
 #ifdef __cplusplus
 }

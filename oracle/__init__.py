@@ -202,6 +202,12 @@ class OracleProblem:
         lib().or_compute_flux(C.byref(self.pr), _p(np.ascontiguousarray(T)), _p(W))
         return W.reshape(3, self.nT)
 
+    def boundary_flux(self, f):
+        """Outward flux density per boundary face, face-averaged."""
+        F = np.zeros(self.nF)
+        lib().or_boundary_flux(C.byref(self.pr), _p(np.ascontiguousarray(f)), _p(F))
+        return F
+
 
 def expn(n, x):
     return lib().or_expn(int(n), float(x))

@@ -22,21 +22,25 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 
-#define OR_PI 3.14159265358979323846264338327950288L
 
+/* The angular integrals expand products of harmonics of degree up to 2K + 2
+ * into monomials whose coefficients alternate in sign; at K = 7 that sum
+ * cancels about nine digits, so it is carried in binary128. */
+typedef __float128 qreal;
 /* ===================================================================== */
 /*  Spherical harmonics as polynomials in (x, y, z)                       */
 /* ===================================================================== */
 
 typedef struct {
     int a, b, c;
-    long double v;
+    qreal v;
 } term;
 
 typedef struct {
@@ -45,14 +49,14 @@ typedef struct {
 } poly;
 
 #define GMAX 48
-static long double g_tab[GMAX][GMAX][GMAX];
+static qreal g_tab[GMAX][GMAX][GMAX];
 static int g_ready = 0;
 
 /* (1/4pi) * integral over the unit sphere of |x|^a |y|^b |z|^c */
-static long double avg_abs(int a, int b, int c) {
-    return tgammal((a + 1) / 2.0L) * tgammal((b + 1) / 2.0L) *
-           tgammal((c + 1) / 2.0L) /
-           (2.0L * OR_PI * tgammal((a + b + c + 3) / 2.0L));
+static qreal avg_abs(int a, int b, int c) {
+    return tgammaq((a + 1) / 2.0L) * tgammaq((b + 1) / 2.0L) *
+           tgammaq((c + 1) / 2.0L) /
+           (2.0Q * M_PIq * tgammaq((a + b + c + 3) / 2.0L));
 }
 
 static void g_init(void) {
@@ -65,35 +69,35 @@ static void g_init(void) {
 }
 
 /* <x^a y^b z^c>: zero unless every exponent is even */
-static long double mono_avg(int a, int b, int c) {
+static qreal mono_avg(int a, int b, int c) {
     if ((a | b | c) & 1) return 0;
     return g_tab[a][b][c];
 }
 
 /* <|x_axis| x^a y^b z^c> (axis 0,1,2) */
-static long double mono_avg_absaxis(int axis, int a, int b, int c) {
+static qreal mono_avg_absaxis(int axis, int a, int b, int c) {
     if ((a | b | c) & 1) return 0;
     if (axis == 0) return g_tab[a + 1][b][c];
     if (axis == 1) return g_tab[a][b + 1][c];
     return g_tab[a][b][c + 1];
 }
 
-static long double fact(int n) {
-    long double r = 1;
+static qreal fact(int n) {
+    qreal r = 1;
     for (int i = 2; i <= n; i++) r *= i;
     return r;
 }
 
-static long double binom(int n, int k) {
+static qreal binom(int n, int k) {
     return fact(n) / (fact(k) * fact(n - k));
 }
 
 /* Legendre polynomial coefficients P_l(z) = sum_k pc[k] z^k */
-static void legendre_coeffs(int l, long double *pc) {
-    long double p0[GMAX] = {0}, p1[GMAX] = {0}, p2[GMAX];
+static void legendre_coeffs(int l, qreal *pc) {
+    qreal p0[GMAX] = {0}, p1[GMAX] = {0}, p2[GMAX];
     p0[0] = 1;
     if (l == 0) {
-        memcpy(pc, p0, sizeof(long double) * (l + 1));
+        memcpy(pc, p0, sizeof(qreal) * (l + 1));
         return;
     }
     p1[1] = 1;
@@ -106,7 +110,7 @@ static void legendre_coeffs(int l, long double *pc) {
         memcpy(p0, p1, sizeof(p0));
         memcpy(p1, p2, sizeof(p1));
     }
-    memcpy(pc, p1, sizeof(long double) * (l + 1));
+    memcpy(pc, p1, sizeof(qreal) * (l + 1));
 }
 
 /* Y_lm = c_lm * (d^|m| P_l/dz^|m|)(z) * {Re,Im}(x+iy)^|m|.
@@ -114,15 +118,15 @@ static void legendre_coeffs(int l, long double *pc) {
  * <Y^2> = 1, cos(m phi) for m >= 0, sin(|m| phi) for m < 0. */
 static void sh_poly(int l, int m, poly *Y) {
     int am = m < 0 ? -m : m;
-    long double pc[GMAX], dc[GMAX];
+    qreal pc[GMAX], dc[GMAX];
     legendre_coeffs(l, pc);
     for (int k = 0; k <= l - am; k++)
         dc[k] = pc[k + am] * fact(k + am) / fact(k);
-    long double c = sqrtl((2.0L * l + 1) * (am == 0 ? 1.0L : 2.0L) *
+    qreal c = sqrtq((2.0L * l + 1) * (am == 0 ? 1.0L : 2.0L) *
                           fact(l - am) / fact(l + am));
     Y->n = 0;
     for (int j = 0; j <= am; j++) {
-        long double rv;
+        qreal rv;
         if (m >= 0) {
             if (j & 1) continue;
             rv = binom(am, j) * ((j / 2) & 1 ? -1.0L : 1.0L);
@@ -142,9 +146,9 @@ static void sh_poly(int l, int m, poly *Y) {
 }
 
 /* <P Q x^ea y^eb z^ec> */
-static long double pair_avg(const poly *P, const poly *Q, int ea, int eb,
+static qreal pair_avg(const poly *P, const poly *Q, int ea, int eb,
                             int ec) {
-    long double s = 0;
+    qreal s = 0;
     for (int i = 0; i < P->n; i++)
         for (int j = 0; j < Q->n; j++)
             s += P->t[i].v * Q->t[j].v *
@@ -153,8 +157,8 @@ static long double pair_avg(const poly *P, const poly *Q, int ea, int eb,
     return s;
 }
 
-static long double pair_avg_abs(const poly *P, const poly *Q, int axis) {
-    long double s = 0;
+static qreal pair_avg_abs(const poly *P, const poly *Q, int axis) {
+    qreal s = 0;
     for (int i = 0; i < P->n; i++)
         for (int j = 0; j < Q->n; j++)
             s += P->t[i].v * Q->t[j].v *
@@ -184,10 +188,10 @@ int or_sh_eval(int lmax, double x, double y, double z, double *out) {
         for (int m = -l; m <= l; m++) {
             poly Y;
             sh_poly(l, m, &Y);
-            long double s = 0;
+            qreal s = 0;
             for (int i = 0; i < Y.n; i++)
-                s += Y.t[i].v * powl(x, Y.t[i].a) * powl(y, Y.t[i].b) *
-                     powl(z, Y.t[i].c);
+                s += Y.t[i].v * powq(x, Y.t[i].a) * powq(y, Y.t[i].b) *
+                     powq(z, Y.t[i].c);
             out[l * l + l + m] = (double)s;
         }
     return n;
@@ -212,14 +216,14 @@ int or_angular(int maxk, double albedo, double g, int hg_lmax, double *omega,
             lof[lm] = l;
         }
     /* scattering moments g_l (HG: g^l, truncated at hg_lmax) */
-    long double gl[GMAX];
-    for (int l = 0; l < GMAX; l++) gl[l] = (l <= hg_lmax) ? powl(g, l) : 0;
+    qreal gl[GMAX];
+    for (int l = 0; l < GMAX; l++) gl[l] = (l <= hg_lmax) ? powq(g, l) : 0;
     gl[0] = 1;
     if (massfac)
         for (int lm = 0; lm < slm; lm++)
             massfac[lm] = (double)(1.0L - albedo * gl[lof[lm]]);
 
-    long double *val = (long double *)malloc(sizeof(long double) * 9 * slm * slm);
+    qreal *val = (qreal *)malloc(sizeof(qreal) * 9 * slm * slm);
     if (albedo == 0) {
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
@@ -235,13 +239,13 @@ int or_angular(int maxk, double albedo, double g, int hg_lmax, double *omega,
         int lo = 2 * maxk + 1, no = 0;
         for (int l = 1; l <= lo; l += 2) no += 2 * l + 1;
         poly *Yo = (poly *)malloc(sizeof(poly) * no);
-        long double *w = (long double *)malloc(sizeof(long double) * no);
+        qreal *w = (qreal *)malloc(sizeof(qreal) * no);
         for (int l = 1, o = 0; l <= lo; l += 2)
             for (int m = -l; m <= l; m++, o++) {
                 sh_poly(l, m, &Yo[o]);
                 w[o] = 1.0L / (1.0L - albedo * gl[l]);
             }
-        long double *A = (long double *)malloc(sizeof(long double) * 3 * slm * no);
+        qreal *A = (qreal *)malloc(sizeof(qreal) * 3 * slm * no);
         for (int i = 0; i < 3; i++) {
             int e[3] = {0, 0, 0};
             e[i]++;
@@ -258,7 +262,7 @@ int or_angular(int maxk, double albedo, double g, int hg_lmax, double *omega,
             for (int j = 0; j < 3; j++)
                 for (int a = 0; a < slm; a++)
                     for (int b = 0; b < slm; b++) {
-                        long double s = 0;
+                        qreal s = 0;
                         for (int o = 0; o < no; o++)
                             s += A[(i * slm + b) * no + o] * w[o] *
                                  A[(j * slm + a) * no + o];
@@ -672,6 +676,38 @@ void or_compute_flux(const or_problem *pr, const double *T, double *W) {
                     w += t->s[k][a] * T[(size_t)(3 * a + j) * pr->nP + t->p[k]];
             W[(size_t)j * pr->nT + i] = (1.0 / 3.0) * w / t->kappa_volume;
         }
+    }
+}
+
+/* Outward flux density through each vacuum boundary face (no reference
+ * function outputs it; derived from the vacuum boundary term of
+ * kernels.cu:414-437).  At a vacuum face the even-parity form has
+ * psi- = sign(Omega.n) psi+, so the outward flux is the integral of
+ * |Omega.n| psi+ over the sphere, 4 pi sum_lm O_n[lm][00] f_lm, with O_n
+ * chosen by the axis rule of kernels.cu:419-425.  F[j] is its mean over
+ * face j (the P1 field's face average: the mean of the three vertices);
+ * faces of kind 1 (reflecting) carry none.  Weighted by surf =
+ * |s_x + s_y + s_z| (kernels.cu:419) these sum to 4 pi times the lm = 0
+ * rows of the boundary term summed over all vertices. */
+void or_boundary_flux(const or_problem *pr, const double *f, double *F) {
+    const double fourpi = 12.5663706143591729538505735331;
+    const or_idx slm = pr->slm, aslm = pr->aslm;
+    for (size_t j = 0; j < pr->nF; j++) {
+        const or_face *tr = &pr->bnd[j];
+        if (pr->face_kind && pr->face_kind[j] != 0) {
+            F[j] = 0;
+            continue;
+        }
+        double surf = fabs(tr->s[0] + tr->s[1] + tr->s[2]);
+        const double *On = (2 * fabs(tr->s[0]) > surf) ? pr->Ox
+                          : (2 * fabs(tr->s[1]) > surf) ? pr->Oy : pr->Oz;
+        double acc = 0;
+        for (or_idx lm = 0; lm < slm; lm++) {
+            double fs = 0;
+            for (int k = 0; k < 3; k++) fs += f[(size_t)tr->p[k] * aslm + lm];
+            acc += On[(size_t)aslm * lm + 0] * fs;
+        }
+        F[j] = fourpi * acc / 3.0;
     }
 }
 

@@ -90,7 +90,7 @@ void or_add_prod(const or_problem *pr, double *x, const double *y, double wy);
 void or_mul_add(const or_problem *pr, double *x, double wx, const double *y);
 void or_mul_add_prod(const or_problem *pr, double *x, double wx,
                      const double *y, double wy);
-/* reductions (kernels.cu:55-115, CudaContext.cpp:227-250): padding excluded */
+/* reductions (kernels.cu:55-115, CudaContext.cpp:175-198): padding excluded */
 double or_norm(const or_problem *pr, const double *x);
 double or_dot(const or_problem *pr, const double *x, const double *y);
 
@@ -120,6 +120,9 @@ void or_compute_moments(const or_problem *pr, const double *omega_host,
                         const or_idx *omega_pos_host, const double *f,
                         double *U, double *T);
 void or_compute_flux(const or_problem *pr, const double *T, double *W);
+/* Outward flux density F[nF] per boundary face (mean over the face), the
+ * axis rule's O_n (kernels.cu:414-425); 0 on reflecting faces. */
+void or_boundary_flux(const or_problem *pr, const double *f, double *F);
 
 /* ---- analytic step problem (main.cpp:498-515, E1.cpp:136-159) ---- */
 double or_expn(int n, double x);

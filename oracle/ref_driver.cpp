@@ -1,7 +1,10 @@
 // ref_driver.cpp -- exposes the reference's host-only setup code through a
 // C ABI so tests can compare against it.  TEST INFRASTRUCTURE ONLY: it is
 // compiled against the reference's own sources (oracle/Makefile target
-// `ref`), never shipped, and never linked into the product.
+// `ref`) and never linked into the product.  Its outputs are written once
+// to the committed fixture tests/golden/ref_angular.npz
+// (tests/golden/make_ref_fixtures.py); the GPU tests read that fixture, so
+// nothing built from the reference is loaded there.
 #include <stdint.h>
 #include <string.h>
 

@@ -29,3 +29,21 @@ def random_vector(nP, slm, aslm, rng, smooth=False):
     f = np.zeros((nP, aslm))
     f[:, :slm] = rng.standard_normal((nP, slm))
     return f.reshape(-1)
+
+
+_REF_FIX = os.path.join(ROOT, "tests", "golden", "ref_angular.npz")
+
+
+def ref_angular_fixture(K):
+    """The reference's own AngularData arrays for K (1..7) from the committed
+    fixture (tests/golden/make_ref_fixtures.py wrote it from the reference's
+    sources), as the oracle takes them."""
+    z = np.load(_REF_FIX)
+    s = (1 + K) * (1 + 2 * K)
+    return dict(omega=z["K%d_omega" % K], omega_pos=z["K%d_omega_pos" % K], Ox=z["K%d_Ox" % K],
+                Oy=z["K%d_Oy" % K], Oz=z["K%d_Oz" % K], slm=s)
+
+
+def ref_xE1_fixture():
+    z = np.load(_REF_FIX)
+    return z["xE1_x"], z["xE1"]

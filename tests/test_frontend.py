@@ -12,8 +12,8 @@ import transfer_amd as ta
 from conftest import ROOT, ang_dict
 
 
-@pytest.mark.parametrize("K,albedo,g,L", [(1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0), (5, 0, 0, 0),
-                                          (1, 0.5, 0, 0), (3, 0.9, 0.6, 7), (5, 0.9, 0.6, 7)])
+@pytest.mark.parametrize("K,albedo,g,L", [(1, 0, 0, 0), (2, 0, 0, 0), (3, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0), (7, 0, 0, 0),
+                                          (1, 0.5, 0, 0), (3, 0.9, 0.6, 7), (5, 0.9, 0.6, 7), (7, 0.9, 0.6, 7)])
 def test_product_angular_matches_oracle(K, albedo, g, L):
     a = ta.Angular(K, albedo, g, L)
     o = oracle.angular(K, albedo, g, L)
@@ -175,3 +175,18 @@ def test_stencil_assembly_matches_oracle(case, rng):
     b = op.rhs().reshape(op.nP, op.aslm)[:, 0]
     np.testing.assert_allclose(st.rhs, b[st.perm], rtol=1e-14, atol=1e-300)
     assert sorted(st.perm) == list(range(op.nP))
+
+
+def test_zero_opacity_rejected_by_assembly():
+    """kappa = 0 in one tet has no operator (kernels.cu:252 divides by
+    9 kappa V): the host assembly fails with TR_ERR_ARG, before any device
+    work."""
+    m = ta.Mesh.box((3, 3, 3), (0, 0, 0), (1, 1, 1))
+    ar = m.arrays()
+    ar["mesh"]["kappa_volume"][7] = 0.0           # tr_mesh_set_fields refuses kappa <= 0 itself
+    a = ta.Angular(1)
+    with pytest.raises(ta.TransferError) as e:
+        ta.Stencil(ta.mesh_view_of(ar), a.view)
+    assert e.value.code == ta.TR_ERR_ARG and "kappa" in str(e.value)
+    with pytest.raises(ta.TransferError):
+        m.set_fields(kappa=np.where(np.arange(m.nT) == 7, 0.0, 1.0), Ip=m.field("const", [1.0]))

@@ -31,8 +31,8 @@ def rotation(kind):
     return Rz(a) @ Ry(b) @ Rz(g)
 
 
-def sh_rotation(R):
-    mu, ph, w = sphere_rule(24, 48)
+def sh_rotation(R, K=K):
+    mu, ph, w = sphere_rule(2 * K + 8, 4 * K + 16)
     s = np.sqrt(1 - mu ** 2)
     d = np.stack([s * np.cos(ph), s * np.sin(ph), mu])
     dr = R.T @ d
@@ -52,14 +52,19 @@ def meshes():
     return A, coords, tets, kappa, Ip
 
 
-@pytest.mark.parametrize("kind", ["generic", "z30"])
-def test_operator_is_rotation_covariant(meshes, kind):
+# K = 2 (slm 15): two faces per wavefront, O_z in LDS; K = 4, 5 (slm 45,
+# 66): one face per wavefront; K = 7 (slm 120): a workgroup of 128 lanes per
+# face with O_z read from global memory
+@pytest.mark.parametrize("kind,K", [("generic", 2), ("z30", 2), ("generic", 4), ("generic", 5), ("z30", 5),
+                                    ("generic", 7)])
+def test_operator_is_rotation_covariant(meshes, kind, K):
     A, coords, tets, kappa, Ip = meshes
     R = rotation(kind)
     B = ta.Mesh.from_arrays(coords @ R.T, tets, kappa, Ip)
     ang = ta.Angular(K)
+    print("[gpu-normals] covariance %s K=%d" % (kind, K), flush=True)
     ctxA, ctxB = ta.HipContext(A, ang), ta.HipContext(B, ang, general_normals=True)
-    T = sh_rotation(R)
+    T = sh_rotation(R, K)
     assert np.abs(T @ T.T - np.eye(len(T))).max() < 1e-12
     nP, slm, aslm = A.nP, ctxA.slm, ctxA.aslm
     rng = np.random.default_rng(1)
@@ -75,8 +80,20 @@ def test_operator_is_rotation_covariant(meshes, kind):
     err = np.abs(got - want).max() / np.abs(want).max()
     ctxB.computeLhsPrec(1, dB, rB)
     err32 = np.abs(ctxB.copyToHost(rB).reshape(nP, aslm)[:, :slm] - want).max() / np.abs(want).max()
-    ctxB.computeLhsDiag(dB, rB)  # the Diag operator runs with the general faces
-    assert np.isfinite(ctxB.copyToHost(rB)).all()
+    # the Diag operator with the general faces (gOd = diag(R O_z R^T)): its
+    # diagonal is A's, probed with unit vectors at boundary vertices
+    bv = np.unique(B.arrays()["bnd"]["p"][:, :3])[:3]
+    for v in bv:
+        for lm in sorted({0, slm // 2, slm - 1}):
+            e = np.zeros((nP, aslm))
+            e[v, lm] = 1.0
+            de = ctxB.vector_from(e.ravel())
+            ctxB.computeLhs(de, rB)
+            a_ii = ctxB.copyToHost(rB).reshape(nP, aslm)[v, lm]
+            ctxB.computeLhsDiag(de, rB)
+            d_ii = ctxB.copyToHost(rB).reshape(nP, aslm)[v, lm]
+            assert abs(d_ii - a_ii) <= 1e-12 * abs(a_ii), (v, lm, d_ii, a_ii)
+            ctxB.deviceFree(de)
     # the reference's axis rule on the rotated mesh
     ctx0 = ta.HipContext(B, ang)
     d0, r0 = ctx0.vector_from(fB.ravel()), ctx0.allocVector()

@@ -90,7 +90,60 @@ def test_milne_slab_on_device(solver):
     print("[gpu-postproc] milne U err vs exact %.4f, vs oracle %.2e" % (rel(U, Ua), rel(U, Uo)), flush=True)
     assert rel(U, Uo) <= 1e-7
     assert rel(U, Ua) <= 0.04
+    # emergent flux at x = +-1 against the exact slab flux 2 pi (E3(d1) - E3(d2))
+    # (main.cpp:505-508 is the half-space form 2 pi E3); the oracle's CPU
+    # solution on this mesh is off by 0.02% at +1 and 0.7% at -1 (P_N truncation)
+    F = ctx.boundary_flux(P.mesh, d_x)
+    xyz, bnd, kind = ar["coords"], ar["bnd"], ar["face_kind"]
+    fx = np.array([xyz[b["p"][:3], 0].mean() for b in bnd])
+    right, left = (kind == 0) & (fx > 1 - 1e-9), (kind == 0) & (fx < -1 + 1e-9)
+    er = abs(F[right].mean() / (2 * np.pi * (ta.expn(3, 1.0) - ta.expn(3, 2.0))) - 1)
+    el = abs(F[left].mean() / (2 * np.pi * (0.5 - ta.expn(3, 1.0))) - 1)
+    print("[gpu-postproc] milne emergent flux err +1: %.2e, -1: %.2e" % (er, el), flush=True)
+    assert er < 0.002 and el < 0.012
+    assert np.all(F[kind == 1] == 0)
     ctx.deviceFree(d_x)
+
+
+@pytest.mark.parametrize("name", ["milne", "ball-hg", "sphere-general"])
+def test_boundary_flux_matches_oracle_and_balance(name):
+    """tr_compute_boundary_flux against the oracle (axis rule, 1e-13
+    relative) and, for every face rule, against the lm = 0 balance: the
+    flux times the face area summed over the boundary is 4 pi times the
+    lm = 0 rows of A f minus those of the mass term (the stiffness rows sum
+    to zero for the constant test function)."""
+    if name == "milne":
+        P = ta.problems.milne(6, 2)
+    elif name == "ball-hg":
+        P = ta.problems.ball(10, maxk=3, kind="hg")
+    else:
+        P = ta.problems.sphere(8, maxk=4)
+    gen = name == "sphere-general"
+    print("[gpu-postproc] boundary flux", name, flush=True)
+    ctx = ta.HipContext(P.mesh, P.ang, general_normals=gen)
+    ar = P.mesh.arrays()
+    op = oracle.OracleProblem(ar, ang_dict(P.ang))
+    rng = np.random.default_rng(5)
+    f = np.zeros((op.nP, op.aslm))
+    f[:, :op.slm] = rng.standard_normal((op.nP, op.slm))
+    d_f, d_r = ctx.vector_from(f.ravel()), ctx.allocVector()
+    F = ctx.boundary_flux(P.mesh, d_f)
+    if not gen:
+        Fo = op.boundary_flux(f.ravel())
+        assert rel(F, Fo) <= 1e-13
+    ctx.computeLhs(d_f, d_r)
+    Af = ctx.copyToHost(d_r).reshape(op.nP, op.aslm)
+    s = ar["bnd"]["s"]
+    area = np.linalg.norm(s, axis=1) if gen else np.abs(s.sum(axis=1))
+    tets = np.array([t["p"] for t in ar["mesh"]])
+    kv = np.array([t["kappa_volume"] for t in ar["mesh"]])
+    mass00 = np.sum(kv * P.ang.massfac[0] / 4 * f[tets, 0].sum(axis=1))
+    want = 4 * np.pi * (Af[:, 0].sum() - mass00)
+    got = np.sum(area * F)
+    print("[gpu-postproc] %s balance %.6e vs %.6e" % (name, got, want), flush=True)
+    assert abs(got - want) <= 1e-11 * np.sum(np.abs(area * F))
+    ctx.deviceFree(d_f)
+    ctx.deviceFree(d_r)
 
 
 def test_dump_slae_matches_oracle_and_is_symmetric(tmp_path):

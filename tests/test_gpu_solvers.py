@@ -114,3 +114,89 @@ def test_chebyshev_preconditioner(case):
                 ref = x
             assert np.abs(x - ref).max() <= 1e-7 * np.abs(ref).max()
     assert its[0] > its[1] > its[2]
+
+
+# ---------------------------------------------------------------------------
+# Failure handling: every condition is forced from the host (bad starting
+# vectors, iteration caps, a refinement threshold no step can meet), so each
+# kernel still runs to its normal end.
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("solver", [ta.TR_SOLVER_PCG_FP64, ta.TR_SOLVER_MIXED])
+@pytest.mark.parametrize("guess", ["nan", "inf", "huge"])
+def test_bad_guess_falls_back_to_zero_start(solver, guess):
+    """A NaN, Inf or adversarial (1e12 times a random vector) x0 is rejected
+    by the residual test (||b - A x0|| >= ||b||, NaN fails it too): the solve
+    starts from 0 (fallback 1) and converges to the FP64 tolerance."""
+    P = build("ball12-hg")
+    ctx = ta.HipContext(P.mesh, P.ang)
+    op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    rng = np.random.default_rng(3)
+    x0 = np.zeros((op.nP, op.aslm))
+    x0[:, :op.slm] = {"nan": np.nan, "inf": np.inf}.get(guess, 0.0)
+    if guess == "huge":
+        x0[:, :op.slm] = 1e12 * rng.standard_normal((op.nP, op.slm))
+    d_f = ctx.vector_from(x0.ravel())
+    st, _ = ctx.solve(d_f, solver=solver, use_prec=2, x0=True, maxit=20000 if solver != ta.TR_SOLVER_MIXED else 30,
+                      tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000, inner_tol=1e-4)
+    f = ctx.copyToHost(d_f)
+    b = op.rhs()
+    res = op.norm(b - op.lhs(f)) / op.norm(b)
+    print("[gpu-solvers] guess=%s solver=%d fallback=%d status=%d true_rel=%.2e" % (guess, solver, st.fallback,
+                                                                                     st.status, res), flush=True)
+    assert st.fallback == 1 and st.status == 0 and np.isfinite(f).all() and res <= 1.5e-8
+    ctx.close()
+
+
+@pytest.mark.parametrize("solver", [ta.TR_SOLVER_REFERENCE, ta.TR_SOLVER_PCG_FP64, ta.TR_SOLVER_MIXED])
+def test_iteration_cap_reports_noconv(solver):
+    """maxit = 1 (main.cpp:142's cap, tightened) ends without convergence:
+    status TR_ERR_NOCONV, a finite iterate."""
+    P = build("ball16")
+    ctx = ta.HipContext(P.mesh, P.ang)
+    d_f = ctx.allocVector()
+    st, _ = ctx.solve(d_f, solver=solver, use_prec=2 if solver != ta.TR_SOLVER_REFERENCE else 0, maxit=1,
+                      tol_abs=0.0, tol_rel=1e-10, inner_maxit=5, inner_tol=1e-8)
+    f = ctx.copyToHost(d_f)
+    print("[gpu-solvers] maxit=1 solver=%d status=%d iters=%d" % (solver, st.status, st.iters), flush=True)
+    assert st.status == ta.TR_ERR_NOCONV and st.iters <= 1 and np.isfinite(f).all()
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [2, 4])
+def test_mixed_stagnation_falls_back_to_fp64(prec):
+    """A refinement threshold no step can meet (each step must cut the true
+    residual 1e30-fold) sends the mixed solve to the plain FP64 solver after
+    its first step: fallback 2, converged, the oracle's residual re-checked.
+    An inner stagnation window of 2 iterations (CG must cut ||r|| by 10%
+    every 2 iterations) stops inner solves early, and refinement still
+    converges."""
+    P = build("ball12-hg")
+    ctx = ta.HipContext(P.mesh, P.ang)
+    op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    b = op.rhs()
+    for stag, window, want_fb in ((1e-30, 100, 2), (0.5, 2, 0)):
+        ctx.set_refinement(stag, window)
+        d_f = ctx.allocVector()
+        st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_MIXED, use_prec=prec, maxit=60, tol_abs=0.0, tol_rel=1e-8,
+                          inner_maxit=20000, inner_tol=1e-4)
+        f = ctx.copyToHost(d_f)
+        res = op.norm(b - op.lhs(f)) / op.norm(b)
+        print("[gpu-solvers] stagnation=%g window=%d prec=%d: fallback=%d status=%d iters=%d inner=%d "
+              "true_rel=%.2e" % (stag, window, prec, st.fallback, st.status, st.iters, st.preck, res), flush=True)
+        assert st.status == 0 and res <= 1.5e-8
+        if want_fb == 2:
+            assert st.fallback == 2
+        ctx.deviceFree(d_f)
+    ctx.close()
+
+
+def test_zero_opacity_is_rejected():
+    """kappa = 0 in one tet: the stiffness term's 1/(9 kappa V)
+    (kernels.cu:252) has no value; context creation fails with TR_ERR_ARG."""
+    m = ta.Mesh.box((3, 3, 3), (0, 0, 0), (1, 1, 1))
+    ar = m.arrays()
+    ar["mesh"]["kappa_volume"][5] = 0.0
+    with pytest.raises(ta.TransferError) as e:
+        ta.HipContext(m, ta.Angular(1), mesh_view=ta.mesh_view_of(ar))
+    assert e.value.code == ta.TR_ERR_ARG

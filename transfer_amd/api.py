@@ -105,6 +105,10 @@ _sig("tr_ctx_autotune", C.c_int, vp, C.c_int, C.c_char_p, C.c_size_t)
 _sig("tr_ctx_schedule", C.c_int, vp, C.c_int, C.POINTER(C.c_int))
 _sig("tr_ctx_set_schedule", C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int)
 _sig("tr_ctx_set_angular_precision", C.c_int, vp, C.c_int)
+_sig("tr_ctx_set_refinement", C.c_int, vp, C.c_double, C.c_int)
+_sig("tr_abs_normal_matrix", C.c_int, C.c_int, f64p, f64p)
+_sig("tr_abs_normal_column", C.c_int, C.c_int, f64p, f64p)
+_sig("tr_normal_rotation", C.c_int, C.c_int, f64p, f64p, u32p)
 _sig("tr_device_alloc", vp, vp, C.c_size_t)
 _sig("tr_alloc_vector", vp, vp)
 _sig("tr_device_free", C.c_int, vp, vp)
@@ -124,6 +128,7 @@ _sig("tr_dump_slae", C.c_int, vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C
 _sig("tr_eig_estimate", C.c_int, vp, C.c_int, C.c_int, C.c_double, f64p)
 _sig("tr_ctx_set_chebyshev", C.c_int, vp, C.c_int, C.c_double, C.c_int)
 _sig("tr_compute_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
+_sig("tr_compute_boundary_flux", C.c_int, vp, C.POINTER(MeshView), vp, vp)
 _sig("tr_expn", C.c_double, C.c_int, C.c_double)
 _sig("tr_mlp_features", C.c_int, C.POINTER(MeshView), C.c_double, f32p)
 _sig("tr_mlp_create", C.c_int, C.c_int, f32p, C.c_uint64, C.POINTER(vp))
@@ -164,13 +169,20 @@ _sig("tr_stencil_coarsen", C.c_int, vp, C.c_uint32, C.POINTER(vp), u32p)
 _sig("tr_stencil_export", C.c_int, vp, u32p, u32p, f64p, u32p, u32p, f64p, f64p, f64p, f64p, f64p)
 
 
+TR_OK, TR_ERR_ARG, TR_ERR_HIP, TR_ERR_STATE, TR_ERR_NOCONV = 0, -1, -2, -3, -4
+
+
 class TransferError(RuntimeError):
-    pass
+    """A failed C-ABI call: `code` is its TR_ERR_* status."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc):
     if rc != 0:
-        raise TransferError("transfer_amd error %d: %s" % (rc, lib.tr_last_error().decode()))
+        raise TransferError("transfer_amd error %d: %s" % (rc, lib.tr_last_error().decode()), rc)
     return rc
 
 
@@ -317,6 +329,36 @@ class Mesh:
             self._h = None
 
 
+def mesh_view_of(ar):
+    """A MeshView over host arrays in the layout of Mesh.arrays() (the
+    caller keeps `ar` alive while the view is used)."""
+    v = MeshView()
+    v.nP, v.nT, v.nF = len(ar["tetstart"]) - 1, len(ar["mesh"]), len(ar["bnd"])
+    for k in ("tetstart", "tetidx", "tetpos", "facestart", "faceidx", "facepos"):
+        setattr(v, k, ptr(ar[k], u32p))
+    v.mesh = ar["mesh"].ctypes.data
+    v.bnd = ar["bnd"].ctypes.data
+    v.coords = ptr(ar["coords"])
+    v.face_kind = ptr(ar["face_kind"], u8p) if len(ar["face_kind"]) else None
+    v._keep = ar
+    return v
+
+
+def abs_normal(maxk, n):
+    """(O(n) [slm][slm], its column 00, the packed rotation blocks R) of a
+    face normal n (host only)."""
+    s = (1 + maxk) * (1 + 2 * maxk)
+    n = np.ascontiguousarray(n, np.float64)
+    O, col = np.zeros(s * s), np.zeros(s)
+    check(lib.tr_abs_normal_matrix(int(maxk), ptr(n), ptr(O)))
+    check(lib.tr_abs_normal_column(int(maxk), ptr(n), ptr(col)))
+    size = C.c_uint32()
+    check(lib.tr_normal_rotation(int(maxk), ptr(n), None, C.byref(size)))
+    R = np.zeros(size.value)
+    check(lib.tr_normal_rotation(int(maxk), ptr(n), ptr(R), None))
+    return O.reshape(s, s), col, R
+
+
 def rng_uniform(seed, stream, n):
     out = np.empty(n)
     check(lib.tr_rng_uniform(int(seed), int(stream), int(n), ptr(out)))
@@ -435,6 +477,11 @@ class HipContext:
         """0 FP32, 1 BF16 matrix-core projection, 3 BF16 hi/lo split"""
         check(lib.tr_ctx_set_angular_precision(self._h, int(prec)))
 
+    def set_refinement(self, stagnation=0.5, inner_stall_window=100):
+        """Mixed solver: FP64 fallback when a refinement step cuts the true
+        residual by less than `stagnation`; inner CG stagnation window."""
+        check(lib.tr_ctx_set_refinement(self._h, float(stagnation), int(inner_stall_window)))
+
     def permutation(self):
         p = np.empty(self.nP, np.uint32)
         check(lib.tr_ctx_permutation(self._h, ptr(p, u32p)))
@@ -518,6 +565,17 @@ class HipContext:
         finally:
             self.deviceFree(dT)
             self.deviceFree(dW)
+
+    def boundary_flux(self, mesh, f):
+        """tr_compute_boundary_flux of the device vector f: host F[nF], the
+        outward flux density through each boundary face, face-averaged."""
+        nF = mesh.nF
+        dF = self.deviceAlloc(8 * max(nF, 1))
+        try:
+            check(lib.tr_compute_boundary_flux(self._h, C.byref(mesh.view), f, dF))
+            return self.copyToHost(dF, 8 * nF)
+        finally:
+            self.deviceFree(dF)
 
     def eig_estimate(self, steps=30, jacobi=False, tol=1e-6):
         """tr_eig_estimate: (smallest Ritz value, largest, extrapolated

@@ -27,22 +27,32 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compiles every source (incremental: only those whose object is older
+    than the source or than any header) and links the library."""
     os.makedirs(OBJ, exist_ok=True)
-    jobs = []
+    headers = [os.path.join(CSRC, n) for n in os.listdir(CSRC) if n.endswith(".h")]
+    headers.append(os.path.join(ROOT, "include", "transfer.h"))
+    newest_header = max(os.path.getmtime(h) for h in headers)
+    jobs, objs = [], []
     for name in sorted(os.listdir(CSRC)):
         src = os.path.join(CSRC, name)
         base, ext = os.path.splitext(name)
         obj = os.path.join(OBJ, base + ".o")
+        if ext not in (".cpp", ".hip"):
+            continue
+        objs.append(obj)
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header):
+            continue
         if ext == ".cpp":
             jobs.append(["g++"] + CXX_FLAGS + ["-c", src, "-o", obj])
-        elif ext == ".hip":
+        else:
             jobs.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
-        for r in ex.map(_run, jobs):
-            if verbose and (r.stdout or r.stderr):
-                sys.stderr.write(r.stdout + r.stderr)
-    objs = [j[-1] for j in jobs]
+    if jobs:
+        with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+            for r in ex.map(_run, jobs):
+                if verbose and (r.stdout or r.stderr):
+                    sys.stderr.write(r.stdout + r.stderr)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
@@ -50,4 +60,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -271,6 +271,34 @@ void abs_normal_matrix(int maxk, const double nn[3], std::vector<double> &O) {
     }
 }
 
+// Column lm' = 00 of abs_normal_matrix(n): <|Omega.n| Y_lm>.  |Omega.n|
+// is zonal about n, so by the Funk-Hecke formula <g(Omega.n) Y_lm> =
+// lambda_l Y_lm(n) with lambda_l = (1/2) int_{-1}^{1} g(t) P_l(t) dt, which
+// for g = |t| and even l is int_0^1 t P_l(t) dt (Gauss-Legendre on [0, 1],
+// exact for its degree l + 1).
+void abs_normal_column(int maxk, const double nn[3], double *col) {
+    const int lmax = 2 * maxk;
+    const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    std::vector<double> Yall((size_t)(lmax + 1) * (lmax + 1));
+    sh_eval_all(lmax, nn[0] / len, nn[1] / len, nn[2] / len, Yall.data());
+    std::vector<double> gx, gw;
+    gauss_legendre(maxk + 2, gx, gw);
+    for (int l = 0, lm = 0; l <= lmax; l += 2) {
+        double lam = 0;
+        for (size_t i = 0; i < gx.size(); i++) {
+            const double t = 0.5 * (gx[i] + 1.0);
+            double p0 = 1, p1 = t;
+            for (int k = 2; k <= l; k++) {
+                const double p2 = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            lam += 0.5 * gw[i] * t * (l == 0 ? 1.0 : p1);
+        }
+        for (int m = -l; m <= l; m++, lm++) col[lm] = lam * Yall[l * l + l + m];
+    }
+}
+
 uint32_t rotation_size(int maxk) {
     uint32_t s = 0;
     for (int l = 0; l <= 2 * maxk; l += 2) s += (uint32_t)((2 * l + 1) * (2 * l + 1));

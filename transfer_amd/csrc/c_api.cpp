@@ -366,3 +366,34 @@ int tr_stencil_export(const tr_stencil *s, uint32_t *perm, uint32_t *nbr, double
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Angular matrices of a face normal off the axes (host only; test hooks).
+extern "C" {
+
+int tr_abs_normal_matrix(int maxk, const double *n, double *O) {
+    if (maxk < 0 || maxk > 7 || !n || !O) return fail(TR_ERR_ARG, "tr_abs_normal_matrix: bad argument");
+    std::vector<double> M;
+    abs_normal_matrix(maxk, n, M);
+    std::copy(M.begin(), M.end(), O);
+    return TR_OK;
+}
+
+int tr_abs_normal_column(int maxk, const double *n, double *col) {
+    if (maxk < 0 || maxk > 7 || !n || !col) return fail(TR_ERR_ARG, "tr_abs_normal_column: bad argument");
+    abs_normal_column(maxk, n, col);
+    return TR_OK;
+}
+
+int tr_normal_rotation(int maxk, const double *n, double *R, uint32_t *size) {
+    if (maxk < 0 || maxk > 7 || !n) return fail(TR_ERR_ARG, "tr_normal_rotation: bad argument");
+    if (size) *size = rotation_size(maxk);
+    if (R) {
+        std::vector<double> M;
+        normal_rotation(maxk, n, M);
+        std::copy(M.begin(), M.end(), R);
+    }
+    return TR_OK;
+}
+
+}  // extern "C"

@@ -1,5 +1,5 @@
 // context.hip -- tr_ctx: the device context behind the C ABI, replacing
-// CudaContext (CudaContext.h:9-50, CudaContext.cpp:76-250).
+// CudaContext (CudaContext.h:9-50, CudaContext.cpp:24-198).
 //
 // Differences from the reference that a caller can observe are none by
 // design: vectors keep the reference's [vertex][aslm] layout at the copy
@@ -635,6 +635,61 @@ int tr_compute_flux(tr_ctx *c, const tr_mesh_view *mesh, const double *T, double
     return TR_OK;
 }
 
+// Outward flux density through each boundary face of the mesh view the
+// context was built from, F[j] (device, the view's face order): the mean
+// over the face of 4 pi sum_lm O_n[lm][00] f_lm.  At a vacuum face the
+// even-parity form has psi- = sign(Omega.n) psi+, so the net outward flux
+// is the integral of |Omega.n| psi+ over the sphere.  O_n is the one the
+// operator's boundary term uses for the face (kernels.cu:414-425: the axis
+// rule; with TR_CTX_GENERAL_NORMALS, faces off the axes take their own
+// normal); reflecting faces carry none.  No reference function outputs it.
+int tr_compute_boundary_flux(tr_ctx *c, const tr_mesh_view *mesh, const double *f, double *F) {
+    if (!c || !mesh || !f || !F || (mesh->nF && !mesh->bnd))
+        return fail(TR_ERR_ARG, "tr_compute_boundary_flux: null argument");
+    if (mesh->nP != c->op.nP || !c->voff.empty())
+        return fail(TR_ERR_ARG, "tr_compute_boundary_flux: mesh does not match the context (batches: per problem)");
+    const uint32_t slm = c->op.slm, nF = mesh->nF;
+    if (!nF) return TR_OK;
+    int maxk = 0;
+    while ((uint32_t)((1 + maxk) * (1 + 2 * maxk)) < slm) maxk++;
+    std::vector<double> col((size_t)3 * slm);
+    for (int n = 0; n < 3; n++)
+        for (uint32_t lm = 0; lm < slm; lm++) col[(size_t)n * slm + lm] = c->op.onc[((size_t)n * slm + lm) * slm];
+    std::vector<uint32_t> rec((size_t)4 * nF);
+    for (uint32_t j = 0; j < nF; j++) {
+        const tr_face &fc = mesh->bnd[j];
+        for (int k = 0; k < 3; k++) {
+            if (fc.p[k] >= mesh->nP) return fail(TR_ERR_ARG, "tr_compute_boundary_flux: face vertex out of range");
+            rec[4 * j + k] = c->op.iperm[fc.p[k]];
+        }
+        uint32_t row;
+        const double S = std::sqrt(fc.s[0] * fc.s[0] + fc.s[1] * fc.s[1] + fc.s[2] * fc.s[2]);
+        const double mx = std::max({std::fabs(fc.s[0]), std::fabs(fc.s[1]), std::fabs(fc.s[2])});
+        if (mesh->face_kind && mesh->face_kind[j] != 0) {
+            row = ~0u;
+        } else if (c->op.general_normals && S > 0 && mx < (1.0 - 1e-12) * S) {
+            row = (uint32_t)(col.size() / slm);
+            col.resize(col.size() + slm);
+            abs_normal_column(maxk, fc.s, col.data() + (size_t)row * slm);
+        } else {
+            const double surf = std::fabs(fc.s[0] + fc.s[1] + fc.s[2]);
+            row = (2 * std::fabs(fc.s[0]) > surf) ? 0 : ((2 * std::fabs(fc.s[1]) > surf) ? 1 : 2);
+        }
+        rec[4 * j + 3] = row;
+    }
+    void *d = nullptr;
+    const size_t rb = sizeof(uint32_t) * rec.size(), cb = sizeof(double) * col.size();
+    HIPCHECK(hipMalloc(&d, rb + cb));
+    // synchronous copies: the host tables are temporaries
+    hipError_t e = hipMemcpy(d, rec.data(), rb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy((char *)d + rb, col.data(), cb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_boundary_flux(f, d, (const double *)((char *)d + rb), nF, slm, F, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("tr_compute_boundary_flux: ") + hipGetErrorString(e));
+    return TR_OK;
+}
+
 int tr_scale(tr_ctx *c, double *x, double wx) {
     if (!c || !x) return fail(TR_ERR_ARG, "tr_scale: null argument");
     return axpby(c, x, wx, nullptr, 0.0);
@@ -1002,6 +1057,16 @@ int ensure_fp32(tr_ctx *c) {
 
 // FP64 iterative refinement around an FP32 device CG:
 //   r = b - A64 x;  d = (A32)^-1 (r / |r|) to inner_tol;  x += |r| d
+// The plain preconditioned FP64 solver (TR_SOLVER_PCG_FP64, same use_prec)
+// takes over from the last good iterate when the FP64 residual fails to
+// drop: a refinement step that does not cut it by kStagnation, an update
+// that makes it larger or non-finite (taken back), or an inner breakdown.
+// The inner CG stops by itself when its residual stalls
+// (kInnerStallWindow).
+// (Defaults; tr_ctx_set_refinement changes them per context.)
+static constexpr double kStagnation = 0.5;
+static constexpr int kInnerStallWindow = 100;
+
 static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
     int rc = ensure_fp32(c);
     if (rc) return rc;
@@ -1089,17 +1154,36 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     cp.tol_abs = 0;
     cp.tol_rel = prm->inner_tol;
     cp.chunk = 16;
+    cp.stall_window = c->ir_stall_window;
+    double *xb = w.get();  // the iterate before the last update
+    if (!xb) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
     int outer = 0, inner = 0, flag = TR_ERR_NOCONV;
-    double rn = bn;
+    bool stalled = false, have_backup = false;
+    double rn = bn, rn_prev = INFINITY;
     for (; outer <= prm->maxit; outer++) {
         if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
         rn = std::sqrt(dot(c, r, r));
+        // the last update made the true FP64 residual non-finite or larger:
+        // take it back and finish with the FP64 solver
+        if (have_backup && (!std::isfinite(rn) || rn > rn_prev)) {
+            HIPCHECK(hipMemcpyAsync(f, xb, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+            rn = rn_prev;
+            stalled = true;
+            break;
+        }
         if (!std::isfinite(rn)) break;
         if (rn <= prm->tol_rel * bn || rn < prm->tol_abs) {
             flag = TR_OK;
             break;
         }
         if (outer == prm->maxit) break;
+        // a refinement step that did not cut the true residual by
+        // kStagnation: the FP32 inner solve cannot make progress here
+        if (have_backup && rn > c->ir_stagnation * rn_prev) {
+            stalled = true;
+            break;
+        }
+        rn_prev = rn;
         // the inner solve reduces its residual by inner_tol, or by what is
         // still needed (with a margin: the FP32 recursion overstates the
         // true reduction) when that is less
@@ -1112,18 +1196,41 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         CgResult res;
         if ((rc = pcg_device<float>(S, cp, c->stream, res))) return rc;
         inner += res.iters;
+        // a breakdown (NaN, or p.Ap <= 0 from an operator that is not SPD in
+        // FP32) leaves no usable correction: the update is not applied
+        if (res.flag == 3) {
+            stalled = true;
+            break;
+        }
+        HIPCHECK(hipMemcpyAsync(xb, f, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        have_backup = true;
         HIPCHECK(launch_add_f32_scaled(f, d32, rn, n, c->stream));
     }
     HIPCHECK(hipStreamSynchronize(c->stream));
+    int fb_iters = 0;
+    if (stalled) {
+        // the plain preconditioned FP64 solver from the last good iterate
+        tr_solve_params q = *prm;
+        q.solver = TR_SOLVER_PCG_FP64;
+        q.x0 = 1;
+        q.maxit = std::max(prm->inner_maxit, 1000) * 4;
+        tr_solve_stats s2{};
+        if ((rc = solve_device(c, &q, f, &s2))) return rc;
+        fb_iters = s2.iters;
+        rn = s2.true_rel * bn;
+        flag = s2.status;
+    }
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
-        st->iters = outer;
+        const int fb0 = st->fallback;
+        st->iters = outer + fb_iters;
         st->preck = inner;
         st->rnorm = rn;
         st->bnorm = bn;
         st->true_rel = bn > 0 ? rn / bn : rn;
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
         st->status = flag;
+        st->fallback = stalled ? 2 : fb0;
     }
     return TR_OK;
 }
@@ -1629,6 +1736,14 @@ extern "C" int tr_ctx_set_multilevel(tr_ctx *c, uint32_t coarsest, uint32_t agg,
 // Angular precision of the FP32 operator: 0 = FP32 arithmetic throughout,
 // 1 = BF16 matrix-core projection with FP32 accumulation, 3 = the same with
 // the hi/lo split (three BF16 products, ~16 significant bits).
+extern "C" int tr_ctx_set_refinement(tr_ctx *c, double stagnation, int inner_stall_window) {
+    if (!c || !(stagnation > 0) || inner_stall_window < 0)
+        return fail(TR_ERR_ARG, "tr_ctx_set_refinement: stagnation must be > 0 and the window >= 0");
+    c->ir_stagnation = stagnation;
+    c->ir_stall_window = inner_stall_window;
+    return TR_OK;
+}
+
 extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
     if (!c || (prec != 0 && prec != 1 && prec != 3)) return fail(TR_ERR_ARG, "tr_ctx_set_angular_precision: 0, 1 or 3");
     c->ang_prec = prec;

@@ -79,6 +79,11 @@ struct tr_ctx {
     int cheb_deg = 4;
     double cheb_ratio = 100.0, cheb_lmax = 0.0;
     int cheb_split = 2;  // BF16 split of the polynomial's steps: 0 = the operator's, else 1..3
+    // mixed solver: fall back to the FP64 solver when a refinement step
+    // cuts the true residual by less than this factor; stagnation window of
+    // the inner CG (0 = none)
+    double ir_stagnation = 0.5;
+    int ir_stall_window = 100;
     std::vector<void *> owned;
 
     tr::OpArgs<double> args() const { return args(map); }

@@ -165,6 +165,9 @@ uint32_t apply4_grid(const OpArgs<T> &a, int NC, int wpb, int epi);
 // moments U, T and flux W of a solution (postproc.hip)
 hipError_t launch_moments(const double *f, const uint32_t *perm, const double *mval, const uint32_t *mpos,
                           uint32_t nP, uint32_t slm, double *U, double *T, hipStream_t s);
+// outward flux density per boundary face (rec: uint4 per face, col: rows of slm)
+hipError_t launch_boundary_flux(const double *f, const void *rec, const double *col, uint32_t nF, uint32_t slm,
+                                double *F, hipStream_t s);
 hipError_t launch_flux(const tr_tetrahedron *mesh, uint32_t nT, uint32_t nP, const double *T, double *W,
                        hipStream_t s);
 

@@ -62,6 +62,34 @@ __global__ __launch_bounds__(256) void k_flux(const tr_tetrahedron *__restrict__
     }
 }
 
+// Outward flux density of each boundary face, face-averaged:
+//   F[j] = 4 pi sum_lm col[row_j][lm] (f_a + f_b + f_c)[lm] / 3
+// rec[j] = {a, b, c (internal vertex ids), row}; row ~0u: no flux
+// (reflecting face).  One lane per face; col rows are slm values.
+__global__ __launch_bounds__(256) void k_boundary_flux(const double *__restrict__ f, const uint4 *__restrict__ rec,
+                                                       const double *__restrict__ col, uint32_t nF, uint32_t slm,
+                                                       double *__restrict__ F) {
+    const double fourpi = 12.5663706143591729538505735331;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nF; j += gridDim.x * blockDim.x) {
+        const uint4 r = rec[j];
+        double acc = 0;
+        if (r.w != ~0u) {
+            const double *c = col + (size_t)r.w * slm;
+            const double *fa = f + (size_t)r.x * slm, *fb = f + (size_t)r.y * slm, *fc = f + (size_t)r.z * slm;
+            for (uint32_t lm = 0; lm < slm; lm++) acc += c[lm] * (fa[lm] + fb[lm] + fc[lm]);
+        }
+        F[j] = fourpi * acc * (1.0 / 3.0);
+    }
+}
+
+hipError_t launch_boundary_flux(const double *f, const void *rec, const double *col, uint32_t nF, uint32_t slm,
+                                double *F, hipStream_t s) {
+    if (!nF) return hipSuccess;
+    const uint32_t g = (nF + 255) / 256;
+    hipLaunchKernelGGL(k_boundary_flux, dim3(g), dim3(256), 0, s, f, (const uint4 *)rec, col, nF, slm, F);
+    return hipGetLastError();
+}
+
 hipError_t launch_moments(const double *f, const uint32_t *perm, const double *mval, const uint32_t *mpos,
                           uint32_t nP, uint32_t slm, double *U, double *T, hipStream_t s) {
     if (!nP) return hipSuccess;

@@ -19,8 +19,11 @@ enum {
     SC_BB = 5,
     SC_THR = 6,
     SC_CONV = 7,  // 0 running, 1 converged, 2 iteration cap, 3 breakdown
+                  // (NaN, or p.Ap <= 0), 4 stagnation (see CgParams)
     SC_ITER = 8,
     SC_RZ2 = 9,   // r.z of odd iterations (SC_RZ holds it for even ones)
+    SC_RRMIN = 10, // smallest r.r so far
+    SC_RRWIN = 11, // SC_RRMIN at the last stagnation check
     SC_N = 16
 };
 
@@ -59,6 +62,10 @@ struct CgParams {
     double tol_abs = 0, tol_rel = 1e-8;
     int chunk = 16;
     int diag_op = 0;   // 1: iterate with the angular-diagonal operator
+    // stagnation test (0 = off): every stall_window iterations the smallest
+    // r.r so far must have dropped below 0.81 of its value at the previous
+    // check (||r|| by 10%), else the solve stops with flag 4
+    int stall_window = 0;
 };
 
 struct CgResult {

@@ -2,7 +2,7 @@
 //
 // The reference drives CG from the host (main.cpp:125-162): every dot
 // product and norm ends in a blocking device-to-host copy of one REAL
-// (CudaContext.cpp:227-250).  Here alpha, beta and the residual norm stay
+// (CudaContext.cpp:175-198).  Here alpha, beta and the residual norm stay
 // in a small device array; one iteration is three launches
 //   apply A p (+ per-workgroup p.Ap)
 //   alpha = rz / p.Ap; x += alpha p, r -= alpha Ap, z = M r (+ r.r, r.z partials)
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
                                                        const T *__restrict__ dinv, const T *__restrict__ z,
                                                        size_t n, double *__restrict__ sc,
                                                        const double *__restrict__ part, int G, int Grz, int rz_old,
-                                                       int rz_new, int maxit) {
+                                                       int rz_new, int maxit, int stall) {
     if (sc[SC_CONV] != 0.0) return;
     __shared__ double be_sh;
     {
@@ -144,12 +144,18 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
                 sc[SC_BETA] = be_sh;
                 sc[rz_new] = rz;
                 sc[SC_ITER] += 1.0;
-                if (!(rr == rr) || !(rz == rz))
-                    sc[SC_CONV] = 3.0;  // breakdown: NaN
+                const double it = sc[SC_ITER];
+                if (rr < sc[SC_RRMIN]) sc[SC_RRMIN] = rr;
+                if (!(rr == rr) || !(rz == rz) || !(sc[SC_PAP] > 0.0))
+                    sc[SC_CONV] = 3.0;  // breakdown: NaN, or p.Ap <= 0
                 else if (rr <= sc[SC_THR])
                     sc[SC_CONV] = 1.0;  // converged
-                else if (sc[SC_ITER] >= (double)maxit)
+                else if (it >= (double)maxit)
                     sc[SC_CONV] = 2.0;  // iteration cap
+                else if (stall > 0 && fmod(it, (double)stall) == 0.0) {
+                    if (sc[SC_RRMIN] > 0.81 * sc[SC_RRWIN]) sc[SC_CONV] = 4.0;  // stagnation
+                    sc[SC_RRWIN] = sc[SC_RRMIN];
+                }
             }
         }
         __syncthreads();
@@ -220,6 +226,8 @@ __global__ __launch_bounds__(256) void k_fin_init(const double *__restrict__ par
         if (tol_rel > 0) thr = fmax(thr, tol_rel * tol_rel * bb);
         sc[SC_THR] = thr;
         sc[SC_ITER] = 0;
+        sc[SC_RRMIN] = rr;
+        sc[SC_RRWIN] = rr;
         sc[SC_CONV] = (rr <= thr) ? 1.0 : 0.0;
     }
 }
@@ -264,12 +272,18 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
     // slot k % 2 and writes slot (k + 1) % 2, and every launch starts at 0
     const int chunk = std::max(2, prm.chunk + (prm.chunk & 1));
     HC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    // the first error of a launch made during the capture (the graph would
+    // silently lack that node otherwise)
+    hipError_t ce = hipSuccess;
+    auto keep = [&ce](hipError_t x) {
+        if (ce == hipSuccess && x != hipSuccess) ce = x;
+    };
     for (int it = 0; it < chunk; it++) {
         const int rz_old = (it & 1) ? SC_RZ2 : SC_RZ, rz_new = (it & 1) ? SC_RZ : SC_RZ2;
         if (prm.diag_op)
-            launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st);
+            keep(launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st));
         else
-            launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st);
+            keep(launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st));
         if (!fdot)
             hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, (const T *)S.p, (const T *)S.Ap, (T *)nullptr, n,
                                S.part, S.sc);
@@ -279,13 +293,14 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
         hipLaunchKernelGGL(k_update_fused<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap,
                            gen ? nullptr : S.dinv, n, S.sc, S.part, (int)nb, rz_old, S.part2,
                            first ? S.pre_dinv : (const T *)nullptr, S.pre_scale, first ? S.pre_out : (T *)nullptr);
+        keep(hipGetLastError());
         int Grz = (int)G;
         if (gen) {
             if (first) {
-                S.prec_loop(S.r, S.z, S.sc + SC_CONV, S.part2 + TR_UPD_BLOCKS);
+                keep(S.prec_loop(S.r, S.z, S.sc + SC_CONV, S.part2 + TR_UPD_BLOCKS));
                 if (S.prec_nparts) Grz = S.prec_nparts;
             } else {
-                S.prec(S.r, S.z, S.sc + SC_CONV);
+                keep(S.prec(S.r, S.z, S.sc + SC_CONV));
             }
             if (!(first && S.prec_nparts))
                 hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, (T *)nullptr, n,
@@ -293,10 +308,15 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
         }
         hipLaunchKernelGGL(k_pupdate_fused<T>, dim3(G), dim3(256), 0, st, S.p, S.r, gen ? nullptr : S.dinv,
                            gen ? (const T *)S.z : nullptr, n, S.sc, S.part2, (int)G, Grz, rz_old, rz_new,
-                           prm.maxit);
+                           prm.maxit, prm.stall_window);
+        keep(hipGetLastError());
     }
     e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    if (ce != hipSuccess) {
+        hipGraphDestroy(graph);
+        return fail(TR_ERR_HIP, std::string("graph capture: a captured launch failed: ") + hipGetErrorString(ce));
+    }
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         hipGraphDestroy(graph);

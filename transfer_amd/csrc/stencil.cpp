@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <numeric>
 
 #include "internal.h"
@@ -95,6 +96,18 @@ int assemble_operator(const tr_mesh_view &mv, const tr_angular_view &av,
     op.nP = nP;
     op.slm = slm;
     op.aslm = (slm + 7) & ~7u;
+    // the stiffness couplings divide by 9 kappa V (kernels.cu:252): an
+    // opacity of zero (or a degenerate tet) has no operator
+    for (uint32_t t = 0; t < mv.nT; t++) {
+        const double kv = mv.mesh[t].kappa_volume;
+        if (!(kv > 0) || !std::isfinite(kv)) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "assemble_operator: tet %u has kappa*V = %g (must be positive and finite)", t, kv);
+            return fail(TR_ERR_ARG, msg);
+        }
+        for (int k = 0; k < 4; k++)
+            if (mv.mesh[t].p[k] >= nP) return fail(TR_ERR_ARG, "assemble_operator: tet vertex out of range");
+    }
 
     // ---- renumbering
     std::vector<uint32_t> gstart, gadj;

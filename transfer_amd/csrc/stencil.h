@@ -90,6 +90,8 @@ struct HostOperator {
 // exact (a Gauss-Legendre x trapezoid rule on the hemisphere around n; the
 // integrand is even, and a polynomial of degree 4K+1 there).
 void abs_normal_matrix(int maxk, const double n[3], std::vector<double> &O);
+// column 00 of abs_normal_matrix(n) (slm values): <|Omega.n| Y_lm>
+void abs_normal_column(int maxk, const double n[3], double *col);
 
 // The rotation of the even harmonics up to 2K that takes the z axis to n
 // (the frame of abs_normal_matrix): R_l[a][b] = <Y_la(Q w) Y_lb(w)>, Q the
