@@ -1,21 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark of the radiative-transfer operator and its PCG on one MI355X.
+"""Benchmark of the radiative-transfer operator and its preconditioned solve
+on one MI355X.
 
-Workload (the ball64 configuration): cubes of a 64^3 grid on [-1,1]^3 with
-centroid r <= 1 (824,256 tets, 147,253 vertices), kappa = 50 exp(-r/0.2),
-I_p = (1 - 0.8 r)^4, no scattering, harmonic order K = 3 (28 harmonics,
-the 110-point Lebedev order), FP64.
+Default workload (the high-order scattering configuration, the largest that
+fits one GPU): the 96^3-grid staircase ball (cubes of a 96^3 grid on
+[-1,1]^3 with centroid r <= 1: 2,780,400 tets, 485,385 vertices), K = 5
+(66 harmonics, the 194-point Lebedev order), Henyey-Greenstein scattering
+g = 0.6 truncated after P_7 with albedo 0.9, kappa = 1 + 9 U[0,1) per tet,
+I_p = 1.
 
-A step is one application of the operator A (computeLhs: volume + vacuum
-boundary kernels) to a device-resident vector.  value = DOF*ordinate/s =
-nP * slm * steps / wall seconds of the timed loop (device-synchronised on
-both sides).  Also reported: the roofline of the operator kernel (HIP
-events on its stream), the FP32 operator of the mixed solver, the time to
-a 1e-8 relative FP64 residual of every solver (the plain FP64 PCG is the
-reference algorithm, main.cpp:125-162, run on the device; the learned
-initial guess feeds the mixed solver) and the oracle timed on the host
-cores.  --batch N runs the batching workload, --c3 the K = 5 scattering
-workload, each as its own JSON line.
+A step is one application of the FP32 operator of the mixed solver (FP32
+gather, BF16 matrix-core angular projection with FP32 accumulation) to a
+device-resident vector.  value = DOF*ordinate/s = nP * slm * steps / wall
+seconds of the timed loop (device-synchronised on both sides).  Also
+reported: the roofline of that kernel (HIP events on its stream), the FP64
+operator and the exact FP32 operator beside it, each operator's error
+against FP64, the time to a 1e-8 relative FP64 residual of the plain FP64
+GPU solver (the reference's PCG, main.cpp:125-162, on the device), of the
+reference's invprec-preconditioned PCG (main.cpp:223-248), of the mixed
+solvers and of the learned guess in front of the mixed solver; the 64^3
+Planck ball as a secondary workload; the reference's CPU configuration
+(the 12x12x48 slab) solved on the GPU; and the oracle timed on the host
+cores.  --c2 runs the 64^3 Planck ball as the main workload, --batch N the
+batching workload, --sphere N the curved sphere, each as its own JSON line.
 """
 import argparse
 import json
@@ -39,41 +46,90 @@ def operator_bytes(nT, nP, nF, slm, p):
     return nT * (16 + 10 * p) + 2 * nP * slm * p + nF * (12 + p)
 
 
+def source_digest():
+    """sha256 (first 16 hex digits) of the operator kernels' sources: a
+    counter summary measured on other sources does not describe the kernel
+    that runs."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("kernels_mfma.hip", "kernels.hip", "kernels.h"):
+        with open(os.path.join(ROOT, "transfer_amd", "csrc", name), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the
-    factor the same counter reads on a plain copy of known size); None when
-    no summary covers this kernel."""
+    factor the same counter reads on a plain copy of known size).  Rows
+    carry the source digest they were measured on; (None, reason) when no
+    row covers this kernel on these sources."""
     fn = os.path.join(ROOT, "profiles", "traffic_%s.txt" % workload)
     if not os.path.exists(fn):
-        return None
+        return None, "no counter summary for %s" % workload
+    dig = source_digest()
+    stale = False
     for line in open(fn):
-        parts = line.split("|")
-        if len(parts) >= 3 and parts[0].strip() == kernel:
+        parts = [x.strip() for x in line.split("|")]
+        if len(parts) >= 4 and parts[0] == kernel:
+            if parts[1] != dig:
+                stale = True
+                continue
             try:
-                return float(parts[-1])
+                return float(parts[-1]), "profiles/traffic_%s.txt" % workload
             except ValueError:
-                return None
-    return None
+                return None, "unreadable row"
+    return None, ("summary measured on other kernel sources" if stale else "kernel not in the summary")
 
 
-def cpu_baseline(P, seconds=12.0):
-    import numpy as np
+def _oracle_problem(P, nthreads=0):
     import oracle
     a = P.ang
-    op = oracle.OracleProblem(P.mesh.arrays(), dict(omega=a.omega, omega_pos=a.omega_pos, Ox=a.Ox, Oy=a.Oy,
-                                                    Oz=a.Oz, massfac=a.massfac, slm=a.slm))
+    return oracle.OracleProblem(P.mesh.arrays(), dict(omega=a.omega, omega_pos=a.omega_pos, Ox=a.Ox, Oy=a.Oy,
+                                                      Oz=a.Oz, massfac=a.massfac, slm=a.slm), nthreads=nthreads)
+
+
+def _time_lhs(op, seconds, min_apps=1):
     f = op.rhs()
-    op.lhs(f)
     n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
+    while n < min_apps or time.perf_counter() - t0 < seconds:
         op.lhs(f)
         n += 1
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
-    return dict(value=op.nP * op.slm * n / dt, unit="DOF*ordinate/s", cores=cores, kind="port",
-                sample="%d applications of the FP64 oracle operator (kernels.cu restated, OpenMP over "
-                       "vertices) on the full ball64 mesh, %.1f s" % (n, dt))
+    return n, time.perf_counter() - t0
+
+
+def cpu_baseline(P, small, c1, seconds=8.0):
+    """The oracle (kernels.cu and main.cpp's PCG restated in C, OpenMP over
+    vertices) on the host cores, bounded samples of the workloads: the
+    operator on the full mesh with every thread and on a smaller mesh of the
+    same construction with one thread; the PCG of main.cpp:125-162 with and
+    without invprec (main.cpp:223-248) to a 1e-8 relative residual on the
+    reference's own CPU configuration (the 12x12x48 slab)."""
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    op = _oracle_problem(P)
+    op.lhs(op.rhs())
+    n, dt = _time_lhs(op, seconds, 2)
+    value = op.nP * op.slm * n / dt
+    del op
+    op1 = _oracle_problem(small, nthreads=1)
+    n1, dt1 = _time_lhs(op1, seconds / 2, 1)
+    one = op1.nP * op1.slm * n1 / dt1
+    pcg = {}
+    for threads in (nthreads, 1):
+        opc = _oracle_problem(c1, nthreads=0 if threads == nthreads else 1)
+        for up in ((False, True) if threads == nthreads else (False,)):
+            t0 = time.perf_counter()
+            _, st, _ = opc.pcg(use_prec=up, maxit=5000, tol_abs=0.0, tol_rel=1e-8, inner_maxit=1000, inner_tol=1e-8)
+            pcg["%s_threads%d" % ("invprec" if up else "plain", threads)] = dict(
+                seconds=time.perf_counter() - t0, iters=st.iters, inner_iters=st.preck)
+    out = dict(value=value, unit="DOF*ordinate/s", cores=nthreads, kind="port", nproc=os.cpu_count(),
+               one_thread=dict(value=one, sample="%d applications on %s (%d vertices), 1 thread, %.1f s"
+                               % (n1, small.name, op1.nP, dt1)),
+               sample="%d applications of the FP64 oracle operator on the full %s mesh with %d threads, %.1f s"
+                      % (n, P.name, nthreads, dt))
+    out["pcg_to_1e-8"] = dict(workload="%s (the reference's CPU configuration: %d vertices, K = %d)"
+                              % (c1.name, c1.mesh.nP, c1.ang.maxk), **pcg)
+    return out
 
 
 def bench_batch(count, steps, warmup):
@@ -151,138 +207,301 @@ def bench_sphere(n, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
-def bench_c3(steps, warmup, solve=True, copy_calibration=False):
-    """The high-order scattering workload: the 96^3-grid staircase ball
-    (2,780,400 tets, 485,385 vertices), K = 5 (66 harmonics, the 194-point
-    Lebedev order), Henyey-Greenstein g = 0.6 truncated to P_7 with albedo
-    0.9, kappa = 1 + 9 U[0,1) per tet, I_p = 1.  Reports the FP64 operator,
-    the FP32 operator with the BF16 matrix-core projection (the autotuned
-    default) and the exact FP32 operator, each FP32 operator's error against
-    the FP64 one on a random vector, and the solves to a 1e-8 relative FP64
-    residual."""
-    import numpy as np
-    import transfer_amd as ta
-    t0 = time.perf_counter()
-    P = ta.problems.ball(96, maxk=5, kind="hg")
-    ctx = ta.HipContext(P.mesh, P.ang)
+def log(tag, msg):
+    print("[%s] %s" % (tag, msg), file=sys.stderr, flush=True)
+
+
+def time_operator(ctx, P, prec, steps, warmup, p):
+    """(wall s, dict) of `steps` timed applications of the operator of
+    precision prec (0 FP64, 1 the FP32 operator of the mixed solver)."""
     nP, nT, slm = P.mesh.nP, P.mesh.nT, P.ang.slm
-    print("[bench-c3] %s nP=%d nT=%d slm=%d, setup %.1f s" % (P.name, nP, nT, slm, time.perf_counter() - t0),
-          file=sys.stderr, flush=True)
-    ctx.autotune(0)
-    if copy_calibration:
-        cms = ctx.bench_copy(256 << 20, 10)
-        print("[bench-c3] copy 256 MiB: %.3f ms" % cms, file=sys.stderr, flush=True)
-    ctx.set_angular_precision(0)
-    ctx.autotune(1)
-    exact32 = ctx.schedule(1)
-    ctx.set_angular_precision(3)
-    ctx.autotune(1)
-    out = dict(metric="C3 operator DOF*ordinate/s and time-to-1e-8", unit="DOF*ordinate/s", n_gpus=1, steps=steps,
-               warmup=warmup, higher_is_better=True, data="synthetic", dtype="f64",
-               config=dict(workload="ball96-hg: ball of 96^3-grid cubes, K=5 (slm 66), HG g=0.6 P7, albedo 0.9, "
-                                    "kappa=1+9U", nP=nP, nT=nT, nF=P.mesh.nF, slm=slm))
-    for prec, key, p in ((0, "operator_fp64", 8), (1, "operator_fp32", 4), (1, "operator_fp32_exact", 4)):
-        if key == "operator_fp32_exact":
-            ctx.set_angular_precision(0)
-            ctx.set_schedule(1, J=exact32["J"], WPB=exact32["WPB"], variant=exact32["variant"])
-        w, k, _, _ = ctx.bench_operator(warmup, steps, precision=prec)
-        b = operator_bytes(nT, nP, P.mesh.nF, slm, p)
-        out[key] = dict(kernel=ctx.kernel_name(prec), kernel_ms=k, dof_ordinate_per_s=nP * slm * steps / w,
-                        algorithmic_bytes=b, achieved_gbs=b / (k * 1e-3) / 1e9, schedule=ctx.schedule(prec))
-        print("[bench-c3] %s %s %.1f us" % (key, ctx.kernel_name(prec), k * 1e3), file=sys.stderr, flush=True)
-    out["value"] = out["operator_fp64"]["dof_ordinate_per_s"]
-    out["ms_per_step"] = out["operator_fp64"]["kernel_ms"]
-    f, a64, a32 = ctx.allocVector(), ctx.allocVector(), ctx.allocVector()
-    ctx.copyToDev(f, np.random.default_rng(1).standard_normal(ctx.N()))
-    ctx.computeLhs(f, a64)
-    A64 = ctx.copyToHost(a64)
-    for key, aprec in (("operator_fp32_exact", 0), ("operator_fp32", 3)):
-        ctx.set_angular_precision(aprec)
-        if aprec == 0:
-            ctx.set_schedule(1, J=exact32["J"], WPB=exact32["WPB"], variant=exact32["variant"])
-        else:
-            ctx.autotune(1)
-        ctx.computeLhsPrec(1, f, a32)
-        A32 = ctx.copyToHost(a32)
-        out[key]["rel_error_vs_fp64"] = float(np.linalg.norm(A32 - A64) / np.linalg.norm(A64))
-    for v in (f, a64, a32):
-        ctx.deviceFree(v)
-    if not solve:
-        print(json.dumps(out), flush=True)
-        return
-    solves = {}
-    # the mixed solver with the exact FP32 operator (VALU schedule) and with
-    # the BF16 matrix-core projection; no Chebyshev polynomial here: it
-    # amplifies the BF16 operator's rounding past what CG tolerates
-    for name, solver, prec, kw in (("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
-                                   ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
-                                   ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(inner_tol=1e-5, maxit=30)),
-                                   ("mixed_bf16mfma_ir_jacobi", ta.TR_SOLVER_MIXED, 2,
-                                    dict(inner_tol=1e-5, maxit=30))):
-        if name == "mixed_fp32_ir_jacobi":
-            ctx.set_angular_precision(0)
-            ctx.set_schedule(1, J=exact32["J"], WPB=exact32["WPB"], variant=exact32["variant"])
-        elif name == "mixed_bf16mfma_ir_jacobi":
-            ctx.set_angular_precision(3)
-            ctx.autotune(1)
+    w, k, kmin, kmax = ctx.bench_operator(warmup, steps, precision=prec)
+    b = operator_bytes(nT, nP, P.mesh.nF, slm, p)
+    return w, dict(kernel=ctx.kernel_name(prec), kernel_ms=k, kernel_ms_min=kmin, kernel_ms_max=kmax,
+                   dof_ordinate_per_s=nP * slm * steps / w, algorithmic_bytes=b,
+                   achieved_gbs=b / (k * 1e-3) / 1e9, schedule=ctx.schedule(prec))
+
+
+def run_solves(ctx, runs, tag, only=""):
+    """Each run from f = 0 to a 1e-8 relative FP64 residual."""
+    import transfer_amd as ta
+    out = {}
+    for name, solver, prec, kw in runs:
+        if only and name != only:
+            continue
+        setup = kw.pop("setup", None)
+        if setup:
+            setup()
         d = ctx.allocVector()
-        p = dict(maxit=5000, tol_abs=0.0, tol_rel=1e-8, inner_maxit=5000)
+        p = dict(maxit=20000 if solver != ta.TR_SOLVER_MIXED else 30, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000,
+                 inner_tol=1e-5)
         p.update(kw)
         st, _ = ctx.solve(d, solver=solver, use_prec=prec, **p)
-        solves[name] = dict(seconds=st.seconds, iters=st.iters, inner_iters=st.preck, true_rel=st.true_rel,
-                            converged=st.status == 0)
-        print("[bench-c3] %s: %.3f s, %d iters (%d inner), true rel %.2e" % (name, st.seconds, st.iters, st.preck,
-                                                                           st.true_rel), file=sys.stderr, flush=True)
         ctx.deviceFree(d)
-    out["time_to_1e-8"] = solves
-    best = min(v["seconds"] for k, v in solves.items() if "mixed" in k and v["converged"])
-    out["speedup_mixed_vs_plain_fp64"] = solves["pcg_fp64_device"]["seconds"] / best
-    print(json.dumps(out), flush=True)
+        out[name] = dict(seconds=st.seconds, iters=st.iters, inner_iters=st.preck, true_rel=st.true_rel,
+                         converged=st.status == 0, fallback=st.fallback)
+        log(tag, "%s: %.4f s, %d iters (%d inner), true rel %.2e%s" % (
+            name, st.seconds, st.iters, st.preck, st.true_rel, ", fallback %d" % st.fallback if st.fallback else ""))
+    return out
 
 
-def learned_mixed(ctx, P, seeds=48, n=12, use_prec=4, inner_tol=1e-4):
-    """The learned initial guess in front of the mixed solver on the bench
-    mesh.  The network (8 -> 64 -> 64 -> 1, BF16 weights) is trained first on
-    solver-generated cube problems (that time is reported, not counted);
-    the timed solve includes building the guess (features, network and
-    guess vector on the device), after one untimed warm-up call.  The solver falls back to a zero start when
-    the guess's residual is not below ||b||."""
-    import transfer_amd as ta
+def train_guess(train, tag, steps=800):
+    """The learned guess's network (8 -> 64 -> 64 -> 1, BF16 weights)
+    trained on solver-generated solutions of the problems `train`."""
     from transfer_amd import learned
     t0 = time.perf_counter()
-    feat, targ, _ = learned.generate(list(range(seeds)), n=n, maxk=P.ang.maxk, chunk=24)
+    feat, targ, _ = learned.generate_single(train)
     g = learned.LearnedGuess(seed=3)
-    hist = g.fit(feat, targ, steps=800, batch=8192, lr=3e-3)
-    train_s = time.perf_counter() - t0
-    # one untimed call (first-launch code loading and the context's feature
-    # scratch, as the operator's warm-up steps); the timed call below
-    # recomputes features, prediction and guess from scratch
+    hist = g.fit(feat, targ, steps=steps, batch=8192, lr=3e-3)
+    info = dict(training_seconds=time.perf_counter() - t0, training_problems=len(train),
+                training_samples=int(len(targ)), training_loss=float(hist[-10:].mean()))
+    log(tag, "learned guess trained on %d problems (%d samples) in %.1f s, loss %.2e" % (
+        len(train), len(targ), info["training_seconds"], info["training_loss"]))
+    return g, info
+
+
+def learned_mixed(ctx, P, g, solver_kw, tag):
+    """The learned guess in front of the mixed solver.  The timed part
+    builds the guess (features, network and guess vector on the device)
+    and solves, after one untimed call (first-launch code loading and the
+    context's feature scratch, as the operator's warm-up steps); the solver
+    scales the guess to its energy-optimal multiple and falls back to a
+    zero start when that is no better than 0."""
+    import transfer_amd as ta
     ctx.deviceFree(g.write_guess(ctx, P.mesh, P.albedo, Ip=None))
     ctx.synchronize()
     t1 = time.perf_counter()
     d_f = g.write_guess(ctx, P.mesh, P.albedo, Ip=None)
     ctx.synchronize()
     guess_s = time.perf_counter() - t1
-    st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_MIXED, use_prec=use_prec, x0=True, maxit=30, tol_abs=0.0,
-                      tol_rel=1e-8, inner_maxit=20000, inner_tol=inner_tol)
+    kw = dict(maxit=30, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000, inner_tol=1e-5)
+    kw.update(solver_kw)
+    prec = kw.pop("use_prec")
+    st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_MIXED, use_prec=prec, x0=True, **kw)
     ctx.deviceFree(d_f)
-    return dict(seconds=guess_s + st.seconds, guess_seconds=guess_s, solve_seconds=st.seconds, iters=st.iters,
-                inner_iters=st.preck, true_rel=st.true_rel, converged=st.status == 0, fallback=st.fallback,
-                training_seconds=train_s, training_samples=int(len(targ)), training_loss=float(hist[-10:].mean()))
+    out = dict(seconds=guess_s + st.seconds, guess_seconds=guess_s, solve_seconds=st.seconds, iters=st.iters,
+               inner_iters=st.preck, true_rel=st.true_rel, converged=st.status == 0, fallback=st.fallback)
+    log(tag, "learned guess + mixed: %.4f s (guess %.4f s, solve %.4f s), %d inner, fallback %d, true rel %.2e" % (
+        out["seconds"], guess_s, st.seconds, st.preck, st.fallback, st.true_rel))
+    return out
+
+
+def speedups(out, solves):
+    plain = solves["pcg_fp64_device"]["seconds"]
+    best_plain = min(v["seconds"] for k, v in solves.items() if k.startswith(("pcg_fp64", "reference"))
+                     and v["converged"])
+    mixed = min((v["seconds"], k) for k, v in solves.items() if "mixed" in k and v["converged"])
+    out["speedup_vs_plain_fp64"] = plain / mixed[0]
+    out["speedup_vs_best_plain_fp64"] = best_plain / mixed[0]
+    out["best_mixed"] = mixed[1]
+    if "learned_mixed" in solves and solves["learned_mixed"]["converged"]:
+        out["speedup_learned_mixed_vs_plain_fp64"] = plain / solves["learned_mixed"]["seconds"]
+
+
+def bench_c3(args):
+    """The default line: the 96^3 HG ball, K = 5."""
+    import numpy as np
+    import transfer_amd as ta
+    tag = "bench"
+    t0 = time.perf_counter()
+    P = ta.problems.ball(96, maxk=5, kind="hg")
+    ctx = ta.HipContext(P.mesh, P.ang)
+    nP, nT, slm = P.mesh.nP, P.mesh.nT, P.ang.slm
+    log(tag, "%s nP=%d nT=%d nF=%d slm=%d, setup %.1f s" % (P.name, nP, nT, P.mesh.nF, slm,
+                                                              time.perf_counter() - t0))
+    ctx.autotune(0)
+    ctx.set_angular_precision(0)
+    ctx.autotune(1)
+    exact32 = ctx.schedule(1)
+    ctx.set_angular_precision(3)
+    ctx.autotune(1)
+    mfma32 = ctx.schedule(1)
+    if args.copy_calibration:
+        cms = ctx.bench_copy(256 << 20, 10)
+        log(tag, "copy 256 MiB: %.3f ms, %.0f GB/s (read+write)" % (cms, 2 * (256 << 20) / cms / 1e6))
+    steps, warmup = args.steps, args.warmup
+
+    def use_exact():
+        ctx.set_angular_precision(0)
+        ctx.set_schedule(1, J=exact32["J"], WPB=exact32["WPB"], variant=exact32["variant"])
+
+    def use_mfma():
+        ctx.set_angular_precision(3)
+        ctx.set_schedule(1, J=mfma32["J"], WPB=mfma32["WPB"], variant=mfma32["variant"])
+
+    wall, op32 = time_operator(ctx, P, 1, steps, warmup, 4)
+    _, op64 = time_operator(ctx, P, 0, steps, warmup, 8)
+    use_exact()
+    _, op32x = time_operator(ctx, P, 1, steps, warmup, 4)
+    use_mfma()
+    for k, v in (("FP32/BF16", op32), ("FP64", op64), ("exact FP32", op32x)):
+        log(tag, "%s operator %s %.1f us (%.3g DOF*ordinate/s)" % (k, v["kernel"], v["kernel_ms"] * 1e3,
+                                                                   v["dof_ordinate_per_s"]))
+    # each FP32 operator's error against the FP64 one on a random vector
+    f, a64, a32 = ctx.allocVector(), ctx.allocVector(), ctx.allocVector()
+    ctx.copyToDev(f, np.random.default_rng(1).standard_normal(ctx.N()))
+    ctx.computeLhs(f, a64)
+    A64 = ctx.copyToHost(a64)
+    for d, setup in ((op32x, use_exact), (op32, use_mfma)):
+        setup()
+        ctx.computeLhsPrec(1, f, a32)
+        d["rel_error_vs_fp64"] = float(np.linalg.norm(ctx.copyToHost(a32) - A64) / np.linalg.norm(A64))
+    for v in (f, a64, a32):
+        ctx.deviceFree(v)
+    use_mfma()
+    traffic, tsrc = measured_traffic(op32["kernel"], P.name)
+    out = dict(metric=METRIC, value=op32["dof_ordinate_per_s"], unit="DOF*ordinate/s", n_gpus=1, steps=steps,
+               warmup=warmup, ms_per_step=wall / steps * 1e3, higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f32+bf16", data="synthetic",
+               config=dict(workload="%s: ball of 96^3-grid cubes, K=5 (slm 66), HG g=0.6 P7, albedo 0.9, "
+                                    "kappa=1+9U, I_p=1; step = one application of the mixed solver's FP32 "
+                                    "operator (BF16 matrix-core projection)" % P.name,
+                           nP=nP, nT=nT, nF=P.mesh.nF, slm=slm))
+    out["roofline"] = dict(bound="hbm", achieved=op32["achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                           frac=op32["achieved_gbs"] / HBM_PEAK_GBS, traffic=traffic, traffic_source=tsrc,
+                           kernel=op32["kernel"], kernel_ms=op32["kernel_ms"],
+                           algorithmic_bytes=op32["algorithmic_bytes"])
+    out["operator_fp32_bf16"] = op32
+    out["operator_fp64"] = op64
+    out["operator_fp32_exact"] = op32x
+    out["kernel_source_digest"] = source_digest()
+    if not args.no_solve:
+        runs = [("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
+                ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
+                ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(setup=use_exact)),
+                ("mixed_bf16mfma_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(setup=use_mfma))]
+        if args.invprec:
+            runs.insert(2, ("reference_pcg_fp64_invprec_host", ta.TR_SOLVER_REFERENCE, 1,
+                            dict(inner_tol=1e-8, inner_maxit=1000)))
+        solves = run_solves(ctx, runs, tag, args.only)
+        if not args.no_learned and not args.only:
+            best = min((v["seconds"], k) for k, v in solves.items() if "mixed" in k and v["converged"])[1]
+            (use_exact if best == "mixed_fp32_ir_jacobi" else use_mfma)()
+            train = [ta.problems.ball(n, maxk=5, kind="hg", seed=s) for n, s in
+                     ((16, 11), (20, 12), (24, 13), (20, 14), (16, 15), (24, 16), (28, 17), (28, 18))]
+            g, info = train_guess(train, tag)
+            lm = learned_mixed(ctx, P, g, dict(use_prec=2), tag)
+            lm.update(info, inner_solver=best)
+            solves["learned_mixed"] = lm
+            use_mfma()
+        out["time_to_1e-8"] = solves
+        if not args.only:
+            speedups(out, solves)
+    if not args.only and not args.no_secondary:
+        out["secondary_ball64"] = bench_c2_brief(min(steps, 50), min(warmup, 5))
+        out["c1_reference_config_on_gpu"] = c1_on_gpu()
+    if not args.no_cpu and not args.only:
+        out["cpu_baseline"] = cpu_baseline(P, ta.problems.ball(32, maxk=5, kind="hg"), ta.problems.slab())
+        log(tag, "cpu baseline %.3g DOF*ordinate/s (%d threads), 1 thread %.3g" % (
+            out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"], out["cpu_baseline"]["one_thread"]["value"]))
+    print(json.dumps(out), flush=True)
+
+
+def bench_c2_brief(steps, warmup):
+    """The 64^3 Planck ball (K = 3): both operators and the plain FP64 and
+    best mixed solves."""
+    import transfer_amd as ta
+    P = ta.problems.ball(64)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    ctx.autotune(0)
+    ctx.autotune(1)
+    _, op64 = time_operator(ctx, P, 0, steps, warmup, 8)
+    _, op32 = time_operator(ctx, P, 1, steps, warmup, 4)
+    ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
+    solves = run_solves(ctx, [("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
+                              ("mixed_fp32_ir_chebyshev", ta.TR_SOLVER_MIXED, 4, dict(inner_tol=1e-4))], "bench-c2")
+    ctx.close()
+    return dict(workload="%s: ball of 64^3-grid cubes, K=3, kappa=50exp(-r/0.2), I_p=(1-0.8r)^4" % P.name,
+                nP=P.mesh.nP, operator_fp64=op64, operator_fp32_bf16=op32, time_to_1e_8=solves,
+                speedup_mixed_vs_plain_fp64=solves["pcg_fp64_device"]["seconds"] /
+                solves["mixed_fp32_ir_chebyshev"]["seconds"])
+
+
+def c1_on_gpu():
+    """The reference's CPU configuration (12x12x48 jittered slab, K = 1,
+    isotropic albedo 0.5) solved on the GPU with the reference's algorithm
+    (host-driven, with and without invprec) and the device PCG."""
+    import transfer_amd as ta
+    P = ta.problems.slab()
+    ctx = ta.HipContext(P.mesh, P.ang)
+    solves = run_solves(ctx, [("reference_pcg_fp64_host", ta.TR_SOLVER_REFERENCE, 0, {}),
+                              ("reference_pcg_fp64_invprec_host", ta.TR_SOLVER_REFERENCE, 1,
+                               dict(inner_tol=1e-8, inner_maxit=1000)),
+                              ("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {})], "bench-c1")
+    ctx.close()
+    return dict(workload="%s: slab 12x12x48 cubes, jitter 0.15, K=1, albedo 0.5" % P.name, nP=P.mesh.nP,
+                time_to_1e_8=solves)
+
+
+def bench_c2(args):
+    """The 64^3 Planck ball (K = 3) as the main workload: the FP64 operator
+    as the step, every solver."""
+    import transfer_amd as ta
+    tag = "bench-c2"
+    P = ta.problems.ball(args.n)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    nP, nT, slm = P.mesh.nP, P.mesh.nT, P.ang.slm
+    log(tag, "%s nP=%d nT=%d nF=%d slm=%d" % (P.name, nP, nT, P.mesh.nF, slm))
+    ctx.autotune(0)
+    if args.copy_calibration:
+        cms = ctx.bench_copy(256 << 20, 10)
+        log(tag, "copy 256 MiB: %.3f ms" % cms)
+    wall, op64 = time_operator(ctx, P, 0, args.steps, args.warmup, 8)
+    ctx.autotune(1)
+    _, op32 = time_operator(ctx, P, 1, args.steps, args.warmup, 4)
+    traffic, tsrc = measured_traffic(op64["kernel"], P.name)
+    out = dict(metric=METRIC, value=op64["dof_ordinate_per_s"], unit="DOF*ordinate/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=wall / args.steps * 1e3, higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="%s: ball of 64^3-grid cubes, K=3 (slm 28), kappa=50exp(-r/0.2), "
+                                    "I_p=(1-0.8r)^4, FP64 operator apply" % P.name, nP=nP, nT=nT, nF=P.mesh.nF,
+                           slm=slm))
+    out["roofline"] = dict(bound="hbm", achieved=op64["achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                           frac=op64["achieved_gbs"] / HBM_PEAK_GBS, traffic=traffic, traffic_source=tsrc,
+                           kernel=op64["kernel"], kernel_ms=op64["kernel_ms"],
+                           algorithmic_bytes=op64["algorithmic_bytes"])
+    out["operator_fp32_bf16"] = op32
+    if not args.no_solve:
+        ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
+        runs = [("reference_pcg_fp64_host", ta.TR_SOLVER_REFERENCE, 0, {}),
+                ("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
+                ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
+                ("pcg_fp64_device_multilevel", ta.TR_SOLVER_PCG_FP64, 3, {}),
+                ("pcg_fp64_device_chebyshev", ta.TR_SOLVER_PCG_FP64, 4, {}),
+                ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, {}),
+                ("mixed_fp32_ir_multilevel", ta.TR_SOLVER_MIXED, 3, dict(inner_tol=1e-4)),
+                ("mixed_fp32_ir_chebyshev", ta.TR_SOLVER_MIXED, 4, dict(inner_tol=1e-4))]
+        solves = run_solves(ctx, runs, tag, args.only)
+        if not args.no_learned and not args.only:
+            train = [ta.problems.ball(n, maxk=3, seed=s) for n, s in ((16, 1), (20, 2), (24, 3), (28, 4))]
+            g, info = train_guess(train, tag)
+            lm = learned_mixed(ctx, P, g, dict(use_prec=4, inner_tol=1e-4), tag)
+            lm.update(info)
+            solves["learned_mixed"] = lm
+        out["time_to_1e-8"] = solves
+        if not args.only:
+            speedups(out, solves)
+    if not args.no_cpu and not args.only:
+        out["cpu_baseline"] = cpu_baseline(P, ta.problems.ball(32), ta.problems.slab())
+    print(json.dumps(out), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the ball64 and slab lines of the default run")
+    ap.add_argument("--invprec", action="store_true",
+                    help="also time the reference's invprec-preconditioned PCG on the main workload")
+    ap.add_argument("--n", type=int, default=64, help="--c2: cubes per axis of the Planck ball")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time a plain 256 MiB device copy (counter calibration)")
-    ap.add_argument("--c3", action="store_true", help="run the K = 5 scattering workload (ball96-hg) instead")
-    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--c2", action="store_true", help="run the 64^3 Planck ball (K = 3) as the main workload")
+    ap.add_argument("--c3", action="store_true", help="the default workload (accepted for older command lines)")
     ap.add_argument("--only", default="", help="run only the named time_to_1e-8 solve (for short profiles)")
     ap.add_argument("--sphere", type=int, default=0,
                     help="run the curved sphere workload with this many cubes per axis instead (48: the measured one)")
@@ -295,106 +514,12 @@ def main():
         sys.exit(2)
     if args.batch:
         bench_batch(args.batch, args.steps, args.warmup)
-        return
-    if args.sphere:
+    elif args.sphere:
         bench_sphere(args.sphere, min(args.steps, 50), min(args.warmup, 5))
-        return
-    if args.c3:
-        bench_c3(min(args.steps, 50), min(args.warmup, 5), solve=not args.no_solve,
-                 copy_calibration=args.copy_calibration)
-        return
-
-    import transfer_amd as ta
-    P = ta.problems.ball(args.n)
-    ctx = ta.HipContext(P.mesh, P.ang)
-    nP, nT, slm = P.mesh.nP, P.mesh.nT, P.ang.slm
-    print("[bench] %s nP=%d nT=%d nF=%d slm=%d" % (P.name, nP, nT, P.mesh.nF, slm), file=sys.stderr, flush=True)
-    tune = ctx.autotune(0)
-    sched = ctx.schedule(0)
-    print("[bench] autotuned FP64 schedule %s" % sched, file=sys.stderr, flush=True)
-    if args.copy_calibration:
-        cms = ctx.bench_copy(256 << 20, 10)
-        print("[bench] copy 256 MiB: %.3f ms, %.0f GB/s (read+write)" % (cms, 2 * (256 << 20) / cms / 1e6),
-              file=sys.stderr, flush=True)
-    wall, k_ms, k_min, k_max = ctx.bench_operator(args.warmup, args.steps, precision=0)
-    value = nP * slm * args.steps / wall
-    ms_per_step = wall / args.steps * 1e3
-    vbytes = operator_bytes(nT, nP, P.mesh.nF, slm, 8)
-    achieved = vbytes / (k_ms * 1e-3) / 1e9
-    kname = ctx.kernel_name(0)
-    print("[bench] FP64 operator %.1f us/step wall, kernel %s %.1f us (min %.1f, max %.1f)"
-          % (ms_per_step * 1e3, kname, k_ms * 1e3, k_min * 1e3, k_max * 1e3), file=sys.stderr, flush=True)
-    out = dict(metric=METRIC, value=value, unit="DOF*ordinate/s", n_gpus=1, steps=args.steps,
-               warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True, scaling="weak",
-               vs_baseline=None, dtype="f64", data="synthetic",
-               config=dict(workload="%s: ball of 64^3-grid cubes, K=3 (slm 28), kappa=50exp(-r/0.2), "
-                                    "I_p=(1-0.8r)^4, FP64 operator apply" % P.name,
-                           nP=nP, nT=nT, nF=P.mesh.nF, slm=slm))
-    out["roofline"] = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                           frac=achieved / HBM_PEAK_GBS, traffic=measured_traffic(kname, P.name),
-                           kernel=kname, kernel_ms=k_ms, algorithmic_bytes=vbytes)
-    out["schedule_fp64"] = sched
-    # the FP32 operator of the mixed solver (BF16 matrix-core projection)
-    tune32 = ctx.autotune(1)
-    w32, k32, _, _ = ctx.bench_operator(args.warmup, args.steps, precision=1)
-    b32 = operator_bytes(nT, nP, P.mesh.nF, slm, 4)
-    out["operator_fp32"] = dict(kernel=ctx.kernel_name(1), kernel_ms=k32, dof_ordinate_per_s=nP * slm * args.steps / w32,
-                                achieved_gbs=b32 / (k32 * 1e-3) / 1e9, algorithmic_bytes=b32,
-                                schedule=ctx.schedule(1))
-    print("[bench] FP32 operator kernel %s %.1f us" % (ctx.kernel_name(1), k32 * 1e3), file=sys.stderr, flush=True)
-    if not args.no_solve:
-        solves = {}
-        runs = [("reference_pcg_fp64_host", ta.TR_SOLVER_REFERENCE, 0, {}),
-                ("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
-                ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
-                ("pcg_fp64_device_multilevel", ta.TR_SOLVER_PCG_FP64, 3, {}),
-                ("pcg_fp64_device_chebyshev", ta.TR_SOLVER_PCG_FP64, 4, {}),
-                ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(inner_tol=1e-5, maxit=30)),
-                ("mixed_fp32_ir_multilevel", ta.TR_SOLVER_MIXED, 3, dict(inner_tol=1e-4, maxit=30)),
-                ("mixed_fp32_ir_chebyshev", ta.TR_SOLVER_MIXED, 4, dict(inner_tol=1e-4, maxit=30))]
-        if args.only:
-            runs = [r for r in runs if r[0] == args.only]
-        # Chebyshev preconditioner: degree 6 over [lambda_max / 300, lambda_max],
-        # steps with the gathered values split into BF16 hi + lo
-        ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
-        for name, solver, prec, kw in runs:
-            if solver == ta.TR_SOLVER_MIXED:
-                out["schedule_fp32"] = ctx.schedule(1)
-            d_f = ctx.allocVector()
-            p = dict(maxit=20000, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000)
-            p.update(kw)
-            st, _ = ctx.solve(d_f, solver=solver, use_prec=prec, **p)
-            solves[name] = dict(seconds=st.seconds, iters=st.iters, inner_iters=st.preck, true_rel=st.true_rel,
-                                converged=st.status == 0)
-            ctx.deviceFree(d_f)
-            print("[bench] %s: %.4f s, %d iters (%d inner), true rel %.2e" % (name, st.seconds, st.iters, st.preck,
-                                                                            st.true_rel), file=sys.stderr, flush=True)
-        if not args.no_learned:
-            lm = learned_mixed(ctx, P)
-            solves["learned_mixed_fp32_ir_chebyshev"] = lm
-            print("[bench] learned_mixed_fp32_ir_chebyshev: %.4f s (guess %.4f s, solve %.4f s), %d inner, "
-                  "fallback %d, true rel %.2e; training %.1f s on %d samples"
-                  % (lm["seconds"], lm["guess_seconds"], lm["solve_seconds"], lm["inner_iters"], lm["fallback"],
-                     lm["true_rel"], lm["training_seconds"], lm["training_samples"]), file=sys.stderr, flush=True)
-        out["time_to_1e-8"] = solves
-        if args.only:
-            print(json.dumps(out), flush=True)
-            return
-        # the plain FP64 GPU solver: the reference's method (main.cpp's PCG
-        # as configured) run on the device
-        plain = solves["pcg_fp64_device"]["seconds"]
-        best_plain = min(v["seconds"] for k, v in solves.items() if k.startswith(("pcg_fp64", "reference"))
-                         and v["converged"])
-        mixed = min((v["seconds"], k) for k, v in solves.items() if "mixed" in k and v["converged"])
-        out["speedup_vs_plain_fp64"] = plain / mixed[0]
-        out["speedup_vs_best_plain_fp64"] = best_plain / mixed[0]
-        out["best_mixed"] = mixed[1]
-        if "learned_mixed_fp32_ir_chebyshev" in solves:
-            out["speedup_learned_mixed_vs_plain_fp64"] = plain / solves["learned_mixed_fp32_ir_chebyshev"]["seconds"]
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(P)
-        print("[bench] cpu baseline %.3g DOF*ordinate/s" % out["cpu_baseline"]["value"], file=sys.stderr, flush=True)
-    print(json.dumps(out), flush=True)
+    elif args.c2:
+        bench_c2(args)
+    else:
+        bench_c3(args)
 
 
 if __name__ == "__main__":

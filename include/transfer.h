@@ -317,8 +317,11 @@ typedef struct tr_solve_params {
     double inner_tol;    /* main.cpp:239: 1e-8 absolute */
     int record;          /* leading iterations whose (alpha, beta, ||r||) to record */
     int x0;              /* TR_SOLVER_PCG_FP64 / MIXED: start from the contents of f
-                            (e.g. the learned guess) instead of 0; the solve
-                            falls back to 0 when ||b - A f|| >= ||b|| */
+                            (e.g. the learned guess), scaled by (f.b)/(f.Af)
+                            (its energy-norm best multiple, never farther from
+                            the solution than 0), instead of 0; the solve falls
+                            back to 0 when that scaling is undefined (NaN, Inf,
+                            f.Af <= 0) or f.b <= 0 */
 } tr_solve_params;
 
 typedef struct tr_solve_stats {

@@ -40,6 +40,10 @@ LaneMap mfma_map(uint32_t slm, int wpb, int mode) {
 std::vector<LaneMap> map_candidates(uint32_t slm, int maxJ) {
     static const int Js[] = {1, 2, 3, 4, 5, 6, 7, 11};
     std::vector<LaneMap> out;
+    bool any = false;
+    for (int J : Js)
+        if (J <= maxJ && slm % J == 0 && slm / J <= 64) any = true;
+    if (!any) maxJ = 11;
     for (int J : Js) {
         if (J > maxJ || slm % J) continue;
         int L = (int)(slm / J);
@@ -60,6 +64,12 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
     static const int Js[] = {1, 2, 3, 4, 5, 6, 7, 11};
     LaneMap best;
     double bu = -1;
+    // past maxJ (registers) only when no smaller J splits slm into <= 64
+    // lanes (91 harmonics: J = 7)
+    bool any = false;
+    for (int J : Js)
+        if (J <= maxJ && slm % J == 0 && slm / J <= 64) any = true;
+    if (!any) maxJ = 11;
     for (int J : Js) {
         if (J > maxJ || slm % J) continue;
         int L = (int)(slm / J);
@@ -189,6 +199,22 @@ static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem, in
     if (m.variant == 3) return apply3_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem);
     return m.variant == 2 ? apply2_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem)
                           : apply_lds_bytes(m, (int)op.NC, op.slm, elem);
+}
+
+// Fits a VALU schedule into the device's LDS: fewer wavefronts per
+// workgroup first, then the plain-load schedule (variant 1), which stages
+// no angular tables (at 91 or 120 harmonics the tables of variants 2 and 3
+// alone exceed the LDS).  False when nothing fits.
+static bool fit_lds(LaneMap &m, const HostOperator &op, size_t elem, int device) {
+    int maxlds = 65536;
+    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    while (m.WPB > 1 && lds_need(m, op, elem) > (size_t)maxlds) m.WPB /= 2;
+    if (lds_need(m, op, elem) > (size_t)maxlds && m.variant >= 2 && m.variant <= 3) {
+        m.variant = 1;
+        m.WPB = 4;
+        while (m.WPB > 1 && lds_need(m, op, elem) > (size_t)maxlds) m.WPB /= 2;
+    }
+    return lds_need(m, op, elem) <= (size_t)maxlds;
 }
 
 // [W][nP][RS] coefficient records from the [W][NC][nP] layout
@@ -375,6 +401,12 @@ static int axpby(tr_ctx *c, double *x, double wx, const double *y, double wy) {
     return TR_OK;
 }
 
+// r = b - Ap without reading r first (0 * r keeps a NaN that r held)
+static int residual(tr_ctx *c, double *r, const double *b, const double *Ap) {
+    HIPCHECK(hipMemcpyAsync(r, b, sizeof(double) * c->nint(), hipMemcpyDeviceToDevice, c->stream));
+    return axpby(c, r, 1, Ap, -1);
+}
+
 }  // namespace tr
 
 using namespace tr;
@@ -411,10 +443,7 @@ int tr::ctx_init(tr_ctx *c, int device) {
     }
     c->map = choose_map(c->op.slm, 4);
     c->map.variant = valu_variant(c->op, 8, c->map.variant);
-    int maxlds = 65536;
-    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
-    while (c->map.WPB > 1 && lds_need(c->map, c->op, 8) > (size_t)maxlds) c->map.WPB /= 2;
-    if (lds_need(c->map, c->op, 8) > (size_t)maxlds) {
+    if (!fit_lds(c->map, c->op, 8, device)) {
         tr_ctx_destroy(c);
         return fail(TR_ERR_ARG, "tr_ctx_create: angular order too large for LDS staging");
     }
@@ -788,7 +817,7 @@ static int solve_reference(tr_ctx *c, const tr_solve_params *prm, double *f, tr_
     auto t0 = std::chrono::steady_clock::now();
     if ((rc = axpby(c, f, 0, nullptr, 0)) || (rc = tr_compute_rhs(c, b))) return rc;
     double bnorm = std::sqrt(dot(c, b, b));
-    if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+    if ((rc = lhs(c, f, Ap)) || (rc = residual(c, r, b, Ap))) return rc;
     if (prm->use_prec) {
         if ((rc = invprec(c, prm, z, r, pAp, pp, pr, &ki))) return rc;
         preck += ki;
@@ -827,7 +856,7 @@ static int solve_reference(tr_ctx *c, const tr_solve_params *prm, double *f, tr_
     HIPCHECK(hipStreamSynchronize(c->stream));
     auto t1 = std::chrono::steady_clock::now();
     // true residual with the FP64 operator
-    if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+    if ((rc = lhs(c, f, Ap)) || (rc = residual(c, r, b, Ap))) return rc;
     double tr_ = std::sqrt(dot(c, r, r));
     if (st) {
         st->iters = k;
@@ -880,18 +909,27 @@ std::vector<double> operator_diagonal(const HostOperator &op) {
 }
 
 // Starting point of the device solvers: 0, or the caller's f (prm->x0)
-// unless its residual is no smaller than b's (then 0, reported as a
-// fallback).  r is scratch.
+// scaled by alpha = (f.b)/(f.Af), the multiple of f closest to the solution
+// in the energy norm: its squared A-norm error is that of 0 less
+// (f.b)^2/(f.Af), so it is accepted whenever that scaling is defined
+// (finite, f.Af > 0) and positive (f.b > 0; a guess anti-correlated with
+// the source is rejected).  A rejected guess gives 0, reported as a
+// fallback.  (A residual-norm test would reject good guesses: a scalar
+// guess leaves the higher moments' rows of b - A x0 larger than b.)
 static int start_point(tr_ctx *c, const tr_solve_params *prm, double *f, const double *b, double *r,
                        tr_solve_stats *st) {
     if (st) st->fallback = 0;
     if (prm->x0) {
         int rc = lhs(c, f, r);
-        if (rc || (rc = axpby(c, r, -1.0, b, 1.0))) return rc;
-        if (std::sqrt(dot(c, r, r)) < std::sqrt(dot(c, b, b))) return TR_OK;
+        if (rc) return rc;
+        const double fAf = dot(c, f, r), fb = dot(c, f, b);
+        if (std::isfinite(fAf) && std::isfinite(fb) && fAf > 0 && fb > 0)
+            return axpby(c, f, fb / fAf, nullptr, 0);
         if (st) st->fallback = 1;
     }
-    return axpby(c, f, 0, nullptr, 0);
+    // a fill, not 0 * f: a rejected guess may hold NaN or Inf
+    HIPCHECK(hipMemsetAsync(f, 0, sizeof(double) * c->nint(), c->stream));
+    return TR_OK;
 }
 
 // Chebyshev polynomial preconditioner (use_prec 4): z = the deg-th iterate
@@ -1001,7 +1039,7 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     cp.chunk = 16;
     CgResult res;
     if ((rc = pcg_device<double>(S, cp, c->stream, res))) return rc;
-    if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+    if ((rc = lhs(c, f, Ap)) || (rc = residual(c, r, b, Ap))) return rc;
     double trn = std::sqrt(dot(c, r, r)), bn = std::sqrt(res.bb);
     if (st) {
         st->iters = res.iters;
@@ -1028,10 +1066,11 @@ int ensure_fp32(tr_ctx *c) {
         c->map32 = mfma_map_ctx(c, 4, 2);
     else if (c->ang_prec != 0 && v5_ok(op))
         c->map32 = mfma5_map_ctx(c, 8);
-    if (c->map32.variant <= 3) c->map32.variant = valu_variant(op, 4, c->map32.variant);
-    int maxlds = 65536;
-    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-    while (c->map32.WPB > 1 && lds_need(c->map32, op, 4) > (size_t)maxlds) c->map32.WPB /= 2;
+    if (c->map32.variant <= 3) {
+        c->map32.variant = valu_variant(op, 4, c->map32.variant);
+        if (!fit_lds(c->map32, op, 4, c->device))
+            return fail(TR_ERR_ARG, "tr_solve: angular order too large for LDS staging (FP32 operator)");
+    }
     int rc;
     if ((rc = upload(c, &c->f_coef, to_f32(op.coef))) || (rc = upload(c, &c->f_bcoef, to_f32(op.bcoef))) ||
         (rc = upload(c, &c->f_aval, to_f32(op.aval))) || (rc = upload(c, &c->f_bval, to_f32(op.bval))) ||
@@ -1161,7 +1200,7 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     bool stalled = false, have_backup = false;
     double rn = bn, rn_prev = INFINITY;
     for (; outer <= prm->maxit; outer++) {
-        if ((rc = lhs(c, f, Ap)) || (rc = axpby(c, r, 0, b, 1)) || (rc = axpby(c, r, 1, Ap, -1))) return rc;
+        if ((rc = lhs(c, f, Ap)) || (rc = residual(c, r, b, Ap))) return rc;
         rn = std::sqrt(dot(c, r, r));
         // the last update made the true FP64 residual non-finite or larger:
         // take it back and finish with the FP64 solver
@@ -1213,7 +1252,8 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         tr_solve_params q = *prm;
         q.solver = TR_SOLVER_PCG_FP64;
         q.x0 = 1;
-        q.maxit = std::max(prm->inner_maxit, 1000) * 4;
+        // the caller's budget: inner_maxit per refinement step left
+        q.maxit = std::max(prm->inner_maxit, 1) * std::max(1, prm->maxit - outer + 1);
         tr_solve_stats s2{};
         if ((rc = solve_device(c, &q, f, &s2))) return rc;
         fb_iters = s2.iters;
@@ -1756,6 +1796,7 @@ extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
         else {
             c->map32 = choose_map(op.slm, 8);
             c->map32.variant = valu_variant(op, 4, c->map32.variant);
+            fit_lds(c->map32, op, 4, c->device);
         }
     }
     return TR_OK;

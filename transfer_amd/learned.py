@@ -45,6 +45,26 @@ def generate(seeds, n=32, maxk=3, chunk=64, inner_tol=1e-4, progress=None):
     return np.concatenate(feats), np.concatenate(targs), counts
 
 
+def generate_single(probs, inner_tol=1e-4, progress=None):
+    """generate() for problems solved one at a time: scattering a batch
+    cannot carry (Henyey-Greenstein), each problem with its own angular
+    data.  I_p = 1 is assumed, as in the workloads."""
+    feats, targs, counts = [], [], []
+    for k, p in enumerate(probs):
+        ctx = HipContext(p.mesh, p.ang)
+        d_x = ctx.allocVector()
+        st, _ = ctx.solve(d_x, solver=TR_SOLVER_MIXED, use_prec=2, tol_abs=0.0, tol_rel=1e-8, inner_tol=inner_tol,
+                          maxit=30, inner_maxit=20000)
+        U, _ = ctx.moments(d_x)
+        ctx.close()
+        feats.append(mlp_features(p.mesh, p.albedo))
+        targs.append(np.clip(U * (1.0 - p.albedo) / FOUR_PI, 0.0, 1.0).astype(np.float32))
+        counts.append(len(U))
+        if progress:
+            progress(k + 1)
+    return np.concatenate(feats), np.concatenate(targs), counts
+
+
 class LearnedGuess:
     def __init__(self, seed=1, params=None):
         self.net = MLP(params=params, seed=seed)
