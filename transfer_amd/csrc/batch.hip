@@ -578,7 +578,9 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
             if (all_done || it == prm->maxit) break;
             for (uint32_t q = 0; q < P; q++) outer[q] += !done[q];
             // inner: A32 d = r / |r| per problem (zero for converged ones)
+            // f is rewritten below: the copy must have read it (synchronous)
             HIPCHECK(hipMemcpyAsync(fac, f.data(), sizeof(double) * P, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));
             hipLaunchKernelGGL(k_bscale_f32, dim3(t.nblk), dim3(TR_BLK_THREADS), 0, st, r, b32, fac, t.lo, t.hi, t.prob);
             HIPCHECK(hipMemsetAsync(x32, 0, sizeof(float) * n, st));
             int l = 0;
@@ -593,6 +595,7 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
                 f[q] = done[q] ? 0.0 : rnorm[q];
             }
             HIPCHECK(hipMemcpyAsync(fac, f.data(), sizeof(double) * P, hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));
             hipLaunchKernelGGL(k_badd_f32, dim3(t.nblk), dim3(TR_BLK_THREADS), 0, st, x, x32, fac, t.lo, t.hi, t.prob);
         }
     }

@@ -130,33 +130,153 @@ static bool v5_ok(const HostOperator &op) {
 }
 
 static uint16_t to_bf16(float x);
+static float from_bf16(uint16_t h);
 
 // B blocks of variant 5: [section][harmonic block][channel][tile][lane], 4
 // BF16 per lane (K index 4q + u of the block, output column 16t + l % 16)
-static std::vector<unsigned char> mfma_tables5(const HostOperator &op) {
-    const uint32_t slm = op.slm, NT = (slm + 15) / 16;
+// Harmonics of the internal (lm) order grouped into the four reflection-
+// parity classes of the real even harmonics, cos(m phi) with m even, cos
+// with m odd, sin(|m| phi) with |m| even, sin with |m| odd (classes by the
+// mirror parities in x and y; z follows for even l), each in lm order.
+// Omega_a Omega_b has definite mirror parities, so every coupling matrix
+// maps a class onto one class: the diagonal channels (mass, xx, yy, zz,
+// and the boundary's |Omega_n|) each onto itself, xy onto (0 <-> 2, 1 <->
+// 3), xz onto (0 <-> 1, 2 <-> 3), yz onto (0 <-> 3, 1 <-> 2).
+static std::vector<uint16_t> parity_class_order(uint32_t slm, std::vector<uint32_t> &sizes) {
+    std::vector<uint16_t> order;
+    sizes.assign(4, 0);
+    for (int cls = 0; cls < 4; cls++)
+        for (int l = 0, lm = 0; (uint32_t)lm < slm; l += 2)
+            for (int m = -l; m <= l; m++, lm++) {
+                const int am = m < 0 ? -m : m;
+                const int k = (m >= 0 ? 0 : 2) + (am & 1);
+                if (k == cls) {
+                    order.push_back((uint16_t)lm);
+                    sizes[cls]++;
+                }
+            }
+    return order;
+}
+
+// B blocks of variant 5 and their metadata for the FP32 vectors' order hp
+// (hp[k] = internal harmonic at position k) split into the given blocks:
+// per input block kb, channel c and output block t the 16 x 16 block
+// (lane l holds K rows 4(l/16) .. +3 of column l % 16, BF16), stored only
+// when some coupling in it is nonzero.  Layout: volume hi blocks (kb, c, t
+// order), then volume lo blocks when lo, then the boundary's likewise.
+static std::vector<unsigned char> mfma_tables5(const HostOperator &op, const std::vector<uint16_t> &hp,
+                                               const std::vector<uint32_t> &h0, const std::vector<uint32_t> &w,
+                                               bool lo, Blk5 &b5) {
+    const uint32_t slm = op.slm, NT = (uint32_t)h0.size();
     auto mat = [&](int sec, int c, uint32_t n, uint32_t m) -> double {
-        if (n >= slm || m >= slm) return 0.0;
         if (sec == 1) return op.onc[((size_t)c * slm + n) * slm + m];
         if (c == 0) return n == m ? op.massfac[n] : 0.0;
         return op.omc[((size_t)(c - 1) * slm + n) * slm + m];
     };
+    b5 = Blk5();
+    b5.nb = NT;
+    b5.nsplitB = lo ? 2 : 1;
+    for (uint32_t k = 0; k < NT; k++) {
+        b5.h0[k] = h0[k];
+        b5.w[k] = w[k];
+    }
     std::vector<unsigned char> out;
     for (int sec = 0; sec < 2; sec++) {
         const int nch = sec == 0 ? (int)op.NC : 3;
-        std::vector<unsigned char> t((size_t)NT * nch * NT * 64 * 8, 0);
-        for (uint32_t kb = 0; kb < NT; kb++)
+        // couplings below 1e-12 of the section's largest are the rounding
+        // residues of the quadrature (the exact values vanish by parity):
+        // zero, so that their blocks are not stored
+        double big = 0;
+        for (int c = 0; c < nch; c++)
+            for (uint32_t n = 0; n < slm; n++)
+                for (uint32_t m = 0; m < slm; m++) big = std::max(big, std::fabs(mat(sec, c, n, m)));
+        const double tiny = 1e-12 * big;
+        std::vector<unsigned char> hi_part, lo_part;
+        uint32_t count = 0;
+        for (uint32_t kb = 0; kb < NT; kb++) {
+            uint64_t mask = 0;
+            const uint32_t first = count;
             for (int c = 0; c < nch; c++)
-                for (uint32_t tt = 0; tt < NT; tt++)
+                for (uint32_t tt = 0; tt < NT; tt++) {
+                    unsigned char bh[64 * 8], bl[64 * 8];
+                    bool nz = false;
                     for (int l = 0; l < 64; l++) {
-                        const uint32_t n = 16u * tt + (uint32_t)(l % 16), q = (uint32_t)(l / 16);
-                        uint16_t h[4];
-                        for (int u = 0; u < 4; u++) h[u] = to_bf16((float)mat(sec, c, n, 16u * kb + 4u * q + (uint32_t)u));
-                        std::memcpy(&t[((((size_t)kb * nch + c) * NT + tt) * 64 + l) * 8], h, 8);
+                        const uint32_t col = (uint32_t)(l % 16), q = (uint32_t)(l / 16);
+                        uint16_t h[4], g[4];
+                        for (int u = 0; u < 4; u++) {
+                            const uint32_t kk = 4u * q + (uint32_t)u;
+                            double x = 0.0;
+                            if (col < w[tt] && kk < w[kb]) x = mat(sec, c, hp[h0[tt] + col], hp[h0[kb] + kk]);
+                            if (std::fabs(x) <= tiny) x = 0.0;
+                            nz = nz || x != 0.0;
+                            h[u] = to_bf16((float)x);
+                            g[u] = to_bf16((float)x - from_bf16(h[u]));
+                        }
+                        std::memcpy(bh + l * 8, h, 8);
+                        std::memcpy(bl + l * 8, g, 8);
                     }
-        out.insert(out.end(), t.begin(), t.end());
+                    if (!nz) continue;
+                    mask |= 1ull << (c * NT + tt);
+                    hi_part.insert(hi_part.end(), bh, bh + sizeof bh);
+                    lo_part.insert(lo_part.end(), bl, bl + sizeof bl);
+                    count++;
+                }
+            if (sec == 0) {
+                b5.vmask[kb] = mask;
+                b5.voff[kb] = first;
+            } else {
+                b5.bmask[kb] = mask;
+                b5.boff[kb] = first;
+            }
+        }
+        (sec == 0 ? b5.nvol : b5.nbnd) = count;
+        out.insert(out.end(), hi_part.begin(), hi_part.end());
+        if (lo) out.insert(out.end(), lo_part.begin(), lo_part.end());
     }
     return out;
+}
+
+// Variant 5's FP32 order and blocks for this context: the parity classes
+// (blocks of at most 16 inside each class) for a single problem without
+// reflecting planes or faces off the axes (their kernels index harmonics in
+// the internal order), when that gives 3 to 5 blocks; else the internal
+// order in blocks of 16.  The lo parts of B are kept when the stored
+// blocks take at most 80 KB with them (two workgroups per CU).
+static int build_tables5(tr_ctx *c) {
+    const HostOperator &op = c->op;
+    const uint32_t slm = op.slm;
+    std::vector<uint32_t> sizes, h0, w;
+    std::vector<uint16_t> hp = parity_class_order(slm, sizes);
+    for (int cls = 0, base = 0; cls < 4; base += (int)sizes[cls], cls++)
+        for (uint32_t k = 0; k < sizes[cls]; k += 16) {
+            h0.push_back((uint32_t)base + k);
+            w.push_back(std::min<uint32_t>(16, sizes[cls] - k));
+        }
+    const bool classes = c->voff.empty() && op.gvert.empty() && op.mask.empty() && h0.size() >= 3 && h0.size() <= 5;
+    if (!classes) {
+        hp.resize(slm);
+        for (uint32_t k = 0; k < slm; k++) hp[k] = (uint16_t)k;
+        h0.clear();
+        w.clear();
+        for (uint32_t k = 0; k < slm; k += 16) {
+            h0.push_back(k);
+            w.push_back(std::min<uint32_t>(16, slm - k));
+        }
+    }
+    Blk5 b5;
+    std::vector<unsigned char> tab = mfma_tables5(op, hp, h0, w, true, b5);
+    if ((size_t)b5.nvol * 1024 > 80 * 1024) tab = mfma_tables5(op, hp, h0, w, false, b5);
+    int rc = upload(c, (unsigned char **)&c->f_tab5, tab);
+    if (rc) return rc;
+    c->b5 = b5;
+    if (classes) {
+        c->hp32 = hp;
+        if ((rc = upload(c, &c->d_hp32, hp))) return rc;
+    } else {
+        c->hp32.clear();
+        c->d_hp32 = nullptr;
+    }
+    return TR_OK;
 }
 
 // The K-blocked matrix-core schedule with its persistent grids.
@@ -194,7 +314,9 @@ static LaneMap mfma_map_ctx(const tr_ctx *c, int wpb, int mode) {
 }
 
 static size_t lds_need(const LaneMap &m, const HostOperator &op, size_t elem, int nsplit = 3) {
-    if (m.variant == 5) return apply5_lds_bytes((int)op.NC, op.slm) + 64;
+    // variant 5 stores at most 250 B blocks (128 KB) hi only, or at most
+    // 80 KB with lo parts (build_tables5): always within the LDS
+    if (m.variant == 5) return 64;
     if (m.variant == 4) return apply4_lds_bytes(elem == 8 ? 0 : m.mfma_mode, (int)op.NC, op.slm, nsplit) + 64;
     if (m.variant == 3) return apply3_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem);
     return m.variant == 2 ? apply2_lds_bytes(m, (int)op.NC, op.slm, op.arow, op.brow, elem)
@@ -1083,7 +1205,9 @@ int ensure_fp32(tr_ctx *c) {
           (rc = upload(c, &c->f_gOd, to_f32(op.gOd))) || (rc = upload(c, &c->f_gOz, to_f32(op.gOz))))) ||
         (rc = upload(c, (unsigned char **)&c->f_tab, table_records<float>(op))))
         return rc;
-    if (v5_ok(op) && (rc = upload(c, (unsigned char **)&c->f_tab5, mfma_tables5(op)))) return rc;
+    if (v5_ok(op) && (rc = build_tables5(c))) return rc;
+    // the persistent grids of variant 5 depend on its tables' LDS
+    if (c->map32.variant == 5) c->map32 = mfma5_map_ctx(c, c->map32.WPB);
     if (mfma_supported(op.slm, 1) &&
         ((rc = upload(c, (unsigned char **)&c->f_tab4, mfma_tables(op, 1, 1))) ||
          (rc = upload(c, (unsigned char **)&c->f_tab4b1, mfma_tables(op, 2, 1))) ||
@@ -1109,6 +1233,25 @@ static constexpr int kInnerStallWindow = 100;
 static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
     int rc = ensure_fp32(c);
     if (rc) return rc;
+    // the multilevel hierarchy's coarse operators take the internal
+    // harmonic order: under a reordered FP32 schedule this solve runs the
+    // FP32 VALU schedule instead (restored on return)
+    struct MapSwap {
+        tr_ctx *c;
+        LaneMap saved;
+        bool on;
+        ~MapSwap() {
+            if (on) c->map32 = saved;
+        }
+    } swap{c, c->map32, false};
+    if (prm->use_prec == 3 && c->perm32()) {
+        LaneMap m = choose_map(c->op.slm, 8);
+        m.variant = valu_variant(c->op, 4, m.variant);
+        if (!fit_lds(m, c->op, 4, c->device))
+            return fail(TR_ERR_ARG, "tr_solve: no FP32 schedule for the multilevel preconditioner");
+        c->map32 = m;
+        swap.on = true;
+    }
     Work w(c);
     const size_t n = c->nint();
     double *b = w.get(), *r = w.get(), *Ap = w.get();
@@ -1127,6 +1270,11 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         std::vector<double> d = operator_diagonal(c->op);
         std::vector<float> di(d.size());
         for (size_t i = 0; i < d.size(); i++) di[i] = (float)(1.0 / d[i]);
+        if (c->perm32()) {  // into the FP32 vectors' harmonic order
+            const uint32_t slm = c->op.slm;
+            for (size_t v = 0; v < c->op.nP; v++)
+                for (uint32_t k = 0; k < slm; k++) di[v * slm + k] = (float)(1.0 / d[v * slm + c->hp32[k]]);
+        }
         dinv32 = v32 + 5 * n;
         HIPCHECK(hipMemcpyAsync(dinv32, di.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1137,10 +1285,16 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
             HIPCHECK(hipMalloc((void **)&ex, sizeof(float) * 2 * n + 8));
             w.v.push_back((double *)ex);
             cheb.a = c->args32();
-            // a lighter BF16 split for the steps keeps the polynomial linear
-            // and symmetric (the couplings' rounding is a fixed perturbation)
-            if (c->cheb_split > 0 && c->cheb_split < cheb.a.nsplit &&
-                (c->map32.variant == 4 || c->map32.variant == 5) && c->map32.mfma_mode == 2)
+            // a lighter BF16 split for the steps of the 16-harmonic-tile
+            // schedule: its couplings in BF16 alone are a fixed symmetric
+            // perturbation, but the split of the gathered values rounds z
+            // itself (about 2^-17 relative), so the polynomial is linear only
+            // to that level.  Variant 5 keeps the operator's split: at 33..80
+            // harmonics and ten channels the couplings' BF16 rounding
+            // (relative to lambda_max) swamps the slow modes the polynomial
+            // targets.
+            if (c->cheb_split > 0 && c->cheb_split < cheb.a.nsplit && c->map32.variant == 4 &&
+                c->map32.mfma_mode == 2)
                 cheb.a.nsplit = c->cheb_split;
             cheb.NC = (int)c->op.NC;
             cheb.map = c->map32;
@@ -1230,7 +1384,7 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
             const double goal = std::max(prm->tol_rel * bn, prm->tol_abs);
             cp.tol_rel = std::max(prm->inner_tol, std::min(0.1, 0.6 * goal / rn));
         }
-        HIPCHECK(launch_to_f32_scaled(r, b32, 1.0 / rn, n, c->stream));
+        HIPCHECK(launch_to_f32_scaled(r, b32, 1.0 / rn, n, c->stream, c->perm32(), c->op.slm));
         HIPCHECK(hipMemsetAsync(d32, 0, sizeof(float) * n, c->stream));
         CgResult res;
         if ((rc = pcg_device<float>(S, cp, c->stream, res))) return rc;
@@ -1243,7 +1397,7 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         }
         HIPCHECK(hipMemcpyAsync(xb, f, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
         have_backup = true;
-        HIPCHECK(launch_add_f32_scaled(f, d32, rn, n, c->stream));
+        HIPCHECK(launch_add_f32_scaled(f, d32, rn, n, c->stream, c->perm32(), c->op.slm));
     }
     HIPCHECK(hipStreamSynchronize(c->stream));
     int fb_iters = 0;
@@ -1330,7 +1484,7 @@ extern "C" int tr_bench_operator(tr_ctx *c, int precision, int warmup, int steps
     // the first half of the FP64 buffer f; the result goes to r)
     HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, f, c->stream));
     float *f32 = reinterpret_cast<float *>(r), *r32 = f32 + c->nint();
-    if (precision == 1) HIPCHECK(launch_to_f32_scaled(f, f32, 1.0, c->nint(), c->stream));
+    if (precision == 1) HIPCHECK(launch_to_f32_scaled(f, f32, 1.0, c->nint(), c->stream, c->perm32(), c->op.slm));
     auto apply = [&]() -> hipError_t {
         if (precision == 1)
             return launch_apply<float>(c->args32(), (int)c->op.NC, c->map32, f32, r32, nullptr, nullptr, c->stream);
@@ -1623,11 +1777,11 @@ extern "C" int tr_compute_lhs_prec(tr_ctx *c, int precision, const double *f, do
     const size_t n = c->nint();
     float *v = nullptr;
     HIPCHECK(hipMalloc((void **)&v, sizeof(float) * 2 * n + 16));
-    hipError_t e = launch_to_f32_scaled(f, v, 1.0, n, c->stream);
+    hipError_t e = launch_to_f32_scaled(f, v, 1.0, n, c->stream, c->perm32(), c->op.slm);
     if (e == hipSuccess)
         e = launch_apply<float>(c->args32(), (int)c->op.NC, c->map32, v, v + n, nullptr, nullptr, c->stream);
-    if (e == hipSuccess) e = launch_axpby(Af, 0.0, nullptr, 0.0, n, c->stream);
-    if (e == hipSuccess) e = launch_add_f32_scaled(Af, v + n, 1.0, n, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(Af, 0, sizeof(double) * n, c->stream);
+    if (e == hipSuccess) e = launch_add_f32_scaled(Af, v + n, 1.0, n, c->stream, c->perm32(), c->op.slm);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(v);
     if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("tr_compute_lhs_prec: ") + hipGetErrorString(e));

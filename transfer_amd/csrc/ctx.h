@@ -64,6 +64,14 @@ struct tr_ctx {
     void *f_tab = nullptr;
     void *f_tab4 = nullptr, *f_tab4b1 = nullptr, *f_tab4b3 = nullptr;  // variant 4: FP32, BF16, BF16 hi/lo
     void *f_tab5 = nullptr;  // variant 5: BF16 B blocks (33..80 harmonics)
+    // variant 5: the harmonic blocks and stored B blocks (kernels.h, Blk5).
+    // Under variant 5 the FP32 vectors hold harmonic hp32[k] of the
+    // internal order at position k (reflection-parity classes, so that
+    // every block lies in one class; empty = the internal order itself);
+    // d_hp32 is its device copy.  FP64 vectors never change order.
+    tr::Blk5 b5;
+    std::vector<uint16_t> hp32;
+    uint16_t *d_hp32 = nullptr;
     int ang_prec = 3;  // FP32 operator's angular contraction: 0 FP32, 1 BF16, 3 BF16 hi/lo split
     std::set<void *> vectors, raw;
     // batches: internal vertex range of problem p = voff[p] .. voff[p+1]
@@ -159,8 +167,10 @@ struct tr_ctx {
         a.tab = map32.variant == 5 ? f_tab5
                 : map32.variant != 4 ? f_tab
                                      : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
-        // variant 5 keeps B in BF16 only: the A operand split at most
-        a.nsplit = ang_prec == 1 ? 1 : (map32.variant == 5 ? 2 : 3);
+        // variant 5 has B lo parts only when its stored blocks fit the LDS
+        // twice: else the A operand split at most
+        a.nsplit = ang_prec == 1 ? 1 : (map32.variant == 5 ? (b5.nsplitB == 2 ? 3 : 2) : 3);
+        a.b5 = b5;
         a.bcrec = f_bcrec;
         a.mask = d.mask;
         a.nmask = d.nmask;
@@ -178,6 +188,9 @@ struct tr_ctx {
         a.gOz = f_gOz;
         return a;
     }
+    // the FP32 vectors' harmonic order under the current FP32 schedule
+    // (null: the internal order)
+    const uint16_t *perm32() const { return map32.variant == 5 && !hp32.empty() ? d_hp32 : nullptr; }
     size_t nint() const { return (size_t)op.nP * op.slm; }
     size_t next() const { return (size_t)op.nP * op.aslm; }
 };
@@ -185,13 +198,17 @@ struct tr_ctx {
 namespace tr {
 
 // Uploads a host vector into a new device allocation owned by the context.
+// The copy is synchronous: callers pass temporaries, which end with the
+// call's full expression.
 template <typename T>
 int upload(tr_ctx *c, T **dst, const std::vector<T> &src) {
     size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
     HIPCHECK(hipMalloc((void **)dst, bytes));
     c->owned.push_back(*dst);
-    if (!src.empty())
+    if (!src.empty()) {
         HIPCHECK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    }
     return TR_OK;
 }
 

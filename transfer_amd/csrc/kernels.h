@@ -29,6 +29,20 @@ struct LaneMap {
     uint32_t grid_cheb = 0; // the same for the launch with a fused Chebyshev step
 };
 
+// Harmonic blocks of the K-blocked matrix-core schedule (variant 5): block
+// kb covers the w[kb] harmonics from h0[kb] of the FP32 vectors' order,
+// and B block (kb, c, t) (input block kb, channel c, output block t) is
+// stored only when bit c*nb + t of vmask[kb] is set, at position voff[kb]
+// + (the set bits below it); the boundary channels likewise with bmask and
+// boff.  nvol / nbnd: stored volume / boundary blocks; nsplitB 2 when each
+// has a BF16 lo part (after all hi parts of its section).
+struct Blk5 {
+    uint32_t nb = 0, nvol = 0, nbnd = 0, nsplitB = 1;
+    uint32_t h0[8] = {}, w[8] = {};
+    uint64_t vmask[8] = {}, bmask[8] = {};
+    uint32_t voff[8] = {}, boff[8] = {};
+};
+
 // Device operator data (see stencil.h), precision T.
 template <typename T>
 struct OpArgs {
@@ -62,6 +76,7 @@ struct OpArgs {
     uint32_t ngv = 0;
     const uint32_t *gvert = nullptr, *gstart = nullptr, *gface = nullptr, *gpos = nullptr, *gfv = nullptr;
     const T *garea = nullptr, *gR = nullptr, *gOd = nullptr, *gOz = nullptr;
+    Blk5 b5;               // variant 5 only
 };
 
 // One step of the Chebyshev iteration on D^-1 A z = D^-1 rhs, written as
@@ -152,7 +167,7 @@ hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, con
 // matrix-core projection in blocks of 16 harmonics; tab = B blocks
 // [block][channel][tile][lane] (volume, then boundary), nsplit 1 or 2
 bool mfma5_supported(uint32_t slm);
-size_t apply5_lds_bytes(int NC, uint32_t slm);
+size_t apply5_lds_bytes(const Blk5 &b5);
 uint32_t apply5_grid(const OpArgs<float> &a, int NC, int wpb, int epi);
 hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid, const float *f, float *r,
                          double *part, const double *skip, hipStream_t s, const ChebStep<float> *cheb = nullptr);

@@ -195,7 +195,8 @@ __device__ inline double quad_bcast(double x, int src) {
 template <typename T, int J, int NCH, int RS, int NS, int DEPTH>
 __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
                                        __amdgpu_buffer_rsrc_t crs, uint32_t cstride, uint32_t coff, int nslot,
-                                       uint32_t slm, int q, uint32_t hoff = 0, uint32_t h0 = 0) {
+                                       uint32_t slm, int q, uint32_t hoff = 0, uint32_t h0 = 0,
+                                       uint32_t hw = 0xffffffffu) {
     constexpr int E = 16 / (int)sizeof(T), PE = (RS + 3) / 4;
     // opaque here, so that the per-slot products and tests are formed where
     // used instead of hoisted out of the tile loop (kept live, they spill)
@@ -234,11 +235,11 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
             if (s + DEPTH < NS && s + DEPTH < nslot) load(s + DEPTH, xs[s % DEPTH], cs[s % DEPTH]);
         }
     }
-    // harmonics past slm (padding of the last pieces) read neighbouring data:
-    // zero them
+    // harmonics past slm (padding of the last pieces) or past the block's
+    // hw read neighbouring data: zero them
 #pragma unroll
     for (int j = 0; j < J; j++)
-        if (mfma_harm(E, q, j) + h0 >= slm)
+        if (mfma_harm(E, q, j) + h0 >= slm || (uint32_t)mfma_harm(E, q, j) >= hw)
 #pragma unroll
             for (int c = 0; c < NCH; c++) acc[c][j] = 0;
 }
@@ -249,7 +250,7 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
 // flight.
 template <typename T, int J, int NCH, int PE, int NS, int DEPTH>
 __device__ __forceinline__ void gather_pre(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
-                                           const T (&cf)[NS][PE], uint32_t slm, int q, uint32_t hoff, uint32_t h0) {
+                                           const T (&cf)[NS][PE], uint32_t slm, int q, uint32_t hoff, uint32_t hw) {
     constexpr int E = 16 / (int)sizeof(T);
     slm = opaque_uniform(slm);
     T xs[DEPTH][J];
@@ -279,9 +280,11 @@ __device__ __forceinline__ void gather_pre(T (&acc)[NCH][J], const uint32_t (&nb
         accum(xs[s % DEPTH], cf[s]);
         if (s + DEPTH < NS) load(s + DEPTH, xs[s % DEPTH]);
     }
+    // harmonics past the block's hw read the next block's (or the next
+    // row's) values: zero them
 #pragma unroll
     for (int j = 0; j < J; j++)
-        if (mfma_harm(E, q, j) + h0 >= slm)
+        if ((uint32_t)mfma_harm(E, q, j) >= hw)
 #pragma unroll
             for (int c = 0; c < NCH; c++) acc[c][j] = 0;
 }
@@ -535,18 +538,58 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
 // Variant 5: the FP32 operator for 33 to 80 harmonics (K = 4, 5) with the
 // angular contraction on the matrix cores in BF16.  At these orders one
 // lane cannot hold a quarter of every channel's row (10 channels x 20
-// harmonics), so the harmonics go through the kernel in NT blocks of 16,
-// the K dimension of v_mfma_f32_16x16x16bf16: per block the ELL gather of
-// one 16-byte piece per lane (a vertex's four lanes read a 64-byte run of
-// each neighbour row), the move to the MFMA layout, and D[t] += A_c B_c[kb][t]
-// for every channel c and output tile t, the D accumulators covering all
-// ceil(slm/16) output tiles throughout.  B blocks are laid out
-// [block][channel][tile][lane] (8 bytes per lane); the volume blocks
-// (NC x NT x NT x 512 bytes, 128 KB at NC = 10, NT = 5) are staged in LDS
-// once per workgroup of the persistent grid, the boundary blocks are read
-// from global memory by the wavefronts that hold boundary vertices.  The
-// B values are BF16; nsplit 2 splits the A operand into BF16 hi + lo
-// (a.nsplit: 1 or 2).  EPI as k_apply4.
+// harmonics), so the harmonics go through the kernel in NT blocks of at
+// most 16 (a.b5: block kb is harmonics h0 .. h0 + w - 1 of the vectors'
+// order), the K dimension of v_mfma_f32_16x16x16bf16: per block the ELL
+// gather of one 16-byte piece per lane (a vertex's four lanes read a
+// 64-byte run of each neighbour row), the move to the MFMA layout, and
+// D[t] += A_c B_c[kb][t] for every channel c and output block t whose B
+// block is not all zero, the D accumulators covering all output blocks
+// throughout.
+//
+// When the vectors' harmonics are in reflection-parity classes (cos or sin
+// in m, times the parity of m: 21/15/15/15 harmonics at K = 5), every block
+// lies in one class and each channel couples a class to one class only
+// (the mirror parities of Omega_a Omega_b), so only 62 of the 250 B blocks
+// at K = 5 with ten channels are nonzero: they are all that is stored and
+// multiplied.  The stored volume blocks are staged in LDS once per
+// workgroup of the persistent grid, BF16 hi parts then (nsplitB 2) lo
+// parts; the boundary blocks are read from global memory by the wavefronts
+// that hold boundary vertices.  a.nsplit: 1 = A_hi B_hi, 2 = + A_lo B_hi,
+// 3 = + A_hi B_lo (~16 significant bits).  EPI as k_apply4.
+template <int NC, int NT>
+__device__ __forceinline__ void contract5(const float (&acc)[NC][4], const unsigned char *B, uint32_t lo,
+                                          uint64_t mask, uint32_t off, int lane, int nsplit, f4v (&D)[NT]) {
+    constexpr size_t BLK = (size_t)TR_WAVE * 8;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (!((mask >> (c * NT)) & ((1ull << NT) - 1))) continue;
+        f4v x;
+#pragma unroll
+        for (int u = 0; u < 4; u++) x[u] = acc[c][u];
+        const bf4v hb = __builtin_convertvector(x, bf4v);
+        const s4v ah = __builtin_bit_cast(s4v, hb);
+        const s4v al = __builtin_bit_cast(s4v, __builtin_convertvector(x - __builtin_convertvector(hb, f4v), bf4v));
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const uint32_t bit = (uint32_t)(c * NT + t);
+            if (!((mask >> bit) & 1ull)) continue;
+            const uint32_t idx = off + (uint32_t)__builtin_popcountll(mask & ((1ull << bit) - 1ull));
+            const unsigned char *bp = B + (size_t)idx * BLK + (size_t)lane * 8;
+            const s4v bh = *reinterpret_cast<const s4v *>(bp);
+            D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, D[t], 0, 0, 0);
+            if (nsplit > 1) {
+                D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, D[t], 0, 0, 0);
+                if (nsplit > 2) {
+                    const s4v bl = *reinterpret_cast<const s4v *>(bp + lo);
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 template <int NC, int WT, int NT, int EPI>
 __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__restrict__ f,
                                                 float *__restrict__ r, double *__restrict__ part,
@@ -557,18 +600,20 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
     constexpr int RS = rec_stride<float>(NC);
     constexpr int J = 4;
     constexpr size_t BLK = (size_t)TR_WAVE * 8;               // one 16 x 16 BF16 B block
-    constexpr size_t VSEC = (size_t)NT * NC * NT * BLK;        // volume blocks, all harmonic blocks
     const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
     const int wpb = blockDim.x / TR_WAVE;
     const uint32_t slm = a.slm, nP = a.nP;
     const int nsplit = a.nsplit;
+    const uint32_t vlo = a.b5.nvol * (uint32_t)BLK;            // volume lo parts after the hi parts
+    const uint32_t vsec = vlo * a.b5.nsplitB;                  // bytes of the volume section
+    const uint32_t blo = a.b5.nbnd * (uint32_t)BLK;
     {
         const i32x4m *src = reinterpret_cast<const i32x4m *>(a.tab);
         i32x4m *dst = reinterpret_cast<i32x4m *>(smem_raw);
-        for (uint32_t i = threadIdx.x; i < (uint32_t)(VSEC / 16); i += blockDim.x) dst[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < vsec / 16u; i += blockDim.x) dst[i] = src[i];
     }
     __syncthreads();
-    const unsigned char *gB = reinterpret_cast<const unsigned char *>(a.tab) + VSEC;
+    const unsigned char *gB = reinterpret_cast<const unsigned char *>(a.tab) + vsec;
     const auto frs = rsrc4(f, (size_t)nP * slm * sizeof(float));
     const auto brs = rsrc4(a.bof, (size_t)nP * 4);
     const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
@@ -577,7 +622,6 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
     const auto bcrs = rsrc4(a.bcrec, (size_t)TR_WBMAX * a.nB * 4 * sizeof(float));
     const int vloc = lane & 15, q = lane >> 4;
     const int gv = lane >> 2, gq = lane & 3;
-    // 2 wavefronts per SIMD (one workgroup of up to 8 per CU: the LDS);
     // the tile's coefficient records (WT x PE per lane) stay in registers
     // across the harmonic blocks
     constexpr int BUDGET = 256;
@@ -639,20 +683,18 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
             for (int c = 0; c < NC; c++)
 #pragma unroll
                 for (int j = 0; j < J; j++) acc[c][j] = 0;
-            if (valid)
-                gather_pre<float, J, NC, PE, WT, DEPTH>(acc, nbc, frs, cf, slm, gq, 64u * (uint32_t)kb,
-                                                         16u * (uint32_t)kb);
+            const uint32_t h0 = a.b5.h0[kb], hw = a.b5.w[kb];
+            if (valid) gather_pre<float, J, NC, PE, WT, DEPTH>(acc, nbc, frs, cf, slm, gq, 4u * h0, hw);
             if (a.mf0 && valid && gq == 0 && kb == 0) acc[0][0] *= a.mf0[v];
             to_mfma_layout<float, 2, J, NC>(acc, lane);
-            uint32_t boff = (uint32_t)kb * (uint32_t)(NC * NT * BLK);
-            asm volatile("" : "+v"(boff));
-            contract<float, 2, J, NC>(acc, smem_raw + boff, 0, lane, nsplit, D);
+            contract5<NC, NT>(acc, smem_raw, vlo, a.b5.vmask[kb], a.b5.voff[kb], lane, nsplit, D);
         }
         if (__any(b >= 0)) {
             uint32_t bn[TR_WBMAX];
             if (b >= 0) load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, TR_WBMAX);
 #pragma unroll 1
             for (int kb = 0; kb < NT; kb++) {
+                if (!a.b5.bmask[kb]) continue;
                 float bacc[3][J];
 #pragma unroll
                 for (int c = 0; c < 3; c++)
@@ -661,19 +703,20 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
                 if (b >= 0)
                     gather<float, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, bcrs, a.nB * (uint32_t)(4 * sizeof(float)),
                                                             (uint32_t)b * (uint32_t)(4 * sizeof(float)), TR_WBMAX,
-                                                            slm, gq, 64u * (uint32_t)kb, 16u * (uint32_t)kb);
+                                                            slm, gq, 4u * a.b5.h0[kb], 0u, a.b5.w[kb]);
                 to_mfma_layout<float, 2, J, 3>(bacc, lane);
-                contract<float, 2, J, 3>(bacc, gB + (size_t)kb * 3 * NT * BLK, 0, lane, nsplit, D);
+                contract5<3, NT>(bacc, gB, blo, a.b5.bmask[kb], a.b5.boff[kb], lane, nsplit, D);
             }
         }
-        // ---- write: D rows 4(l/16) + i, column l % 16 of tile t
+        // ---- write: D rows 4(l/16) + i, column l % 16 of output block t
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            const uint32_t n = 16u * t + (uint32_t)vloc;
+            const uint32_t n = a.b5.h0[t] + (uint32_t)vloc;
+            const bool col_ok = (uint32_t)vloc < a.b5.w[t];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t vr = v0 + (uint32_t)(4 * q + i);
-                if (n < slm && vr < nP) {
+                if (col_ok && vr < nP) {
                     const float val = D[t][i];
                     const size_t o = (size_t)vr * slm + n;
                     if constexpr (EPI >= 2) {
@@ -707,14 +750,11 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
 
 bool mfma5_supported(uint32_t slm) { return slm > 32 && slm <= 80; }
 
-size_t apply5_lds_bytes(int NC, uint32_t slm) {
-    const size_t NT = (slm + 15) / 16;
-    return NT * (size_t)NC * NT * 512;
-}
+// LDS of variant 5: the stored volume B blocks (hi, and lo with nsplitB 2)
+size_t apply5_lds_bytes(const Blk5 &b5) { return (size_t)b5.nvol * 512 * b5.nsplitB; }
 
 template <int EPI>
-static const void *apply5_kernel(int NC, uint32_t W, uint32_t slm) {
-    const int NT = (int)((slm + 15) / 16);
+static const void *apply5_kernel(int NC, uint32_t W, uint32_t NT) {
 #define TR_K5(NCC, WW, NTT) reinterpret_cast<const void *>(&k_apply5<NCC, WW, NTT, EPI>)
 #define TR_K5T(NCC, WW) (NT == 3 ? TR_K5(NCC, WW, 3) : (NT == 4 ? TR_K5(NCC, WW, 4) : TR_K5(NCC, WW, 5)))
 #define TR_K5W(NCC) (W == 15 ? TR_K5T(NCC, 15) : TR_K5T(NCC, 16))
@@ -725,21 +765,21 @@ static const void *apply5_kernel(int NC, uint32_t W, uint32_t slm) {
 #undef TR_K5
 }
 
-static const void *apply5_kernel_epi(int epi, int NC, uint32_t W, uint32_t slm) {
+static const void *apply5_kernel_epi(int epi, int NC, uint32_t W, uint32_t NT) {
     switch (epi) {
-        case 1: return apply5_kernel<1>(NC, W, slm);
-        case 2: return apply5_kernel<2>(NC, W, slm);
-        case 3: return apply5_kernel<3>(NC, W, slm);
-        default: return apply5_kernel<0>(NC, W, slm);
+        case 1: return apply5_kernel<1>(NC, W, NT);
+        case 2: return apply5_kernel<2>(NC, W, NT);
+        case 3: return apply5_kernel<3>(NC, W, NT);
+        default: return apply5_kernel<0>(NC, W, NT);
     }
 }
 
 uint32_t apply5_grid(const OpArgs<float> &a, int NC, int wpb, int epi) {
     const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
-    const void *k = apply5_kernel_epi(epi, NC, a.W, a.slm);
+    const void *k = apply5_kernel_epi(epi, NC, a.W, a.b5.nb);
     if (!k || ntiles == 0) return ntiles;
     int nb = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, TR_WAVE * wpb, apply5_lds_bytes(NC, a.slm)) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, TR_WAVE * wpb, apply5_lds_bytes(a.b5)) != hipSuccess ||
         nb < 1 || hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
         (void)hipGetLastError();
@@ -754,17 +794,22 @@ hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid,
                          double *part, const double *skip, hipStream_t s, const ChebStep<float> *cheb) {
     if (a.nP == 0) return hipSuccess;
     const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
+    // blocks: 3..5, widths <= 16 inside slm; the lo parts of B for nsplit 3
+    bool blocks_ok = a.b5.nb >= 3 && a.b5.nb <= 5;
+    for (uint32_t k = 0; blocks_ok && k < a.b5.nb; k++)
+        blocks_ok = a.b5.w[k] >= 1 && a.b5.w[k] <= 16 && a.b5.h0[k] + a.b5.w[k] <= a.slm;
     if ((a.W != 15 && a.W != 16) || a.WB > TR_WBMAX || !a.crec || !a.tab || (a.nB && !a.bcrec) || wpb < 1 ||
-        wpb > 8 || !mfma5_supported(a.slm) || a.nsplit < 1 || a.nsplit > 2)
+        wpb > 8 || !mfma5_supported(a.slm) || a.nsplit < 1 || a.nsplit > 3 || !blocks_ok ||
+        (a.nsplit == 3 && a.b5.nsplitB != 2))
         return hipErrorInvalidValue;
     if (cheb && (!cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
-    const void *k = apply5_kernel_epi(cheb ? (part ? 3 : 2) : (part ? 1 : 0), NC, a.W, a.slm);
+    const void *k = apply5_kernel_epi(cheb ? (part ? 3 : 2) : (part ? 1 : 0), NC, a.W, a.b5.nb);
     if (!k) return hipErrorInvalidValue;
     dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
     OpArgs<float> aa = a;
     ChebStep<float> cs = cheb ? *cheb : ChebStep<float>{nullptr, nullptr, 0.0f, 0.0f, 0.0f};
     void *args[] = {&aa, &f, &r, &part, &skip, &cs};
-    return hipLaunchKernel(k, g, b, args, apply5_lds_bytes(NC, a.slm), s);
+    return hipLaunchKernel(k, g, b, args, apply5_lds_bytes(a.b5), s);
 }
 
 int mfma_J(uint32_t slm, int elem_bytes) {

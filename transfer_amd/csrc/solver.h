@@ -77,8 +77,12 @@ struct CgResult {
 template <typename T>
 int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResult &res);
 
-// dst = (float)(src * s);  x += s * (double)d
-hipError_t launch_to_f32_scaled(const double *src, float *dst, double s, size_t n, hipStream_t st);
-hipError_t launch_add_f32_scaled(double *x, const float *d, double s, size_t n, hipStream_t st);
+// dst = (float)(src * s);  x += s * (double)d; hp (with slm): the FP32
+// vectors' harmonic order, position k of a vertex's FP32 row holding
+// harmonic hp[k] of its FP64 row (null: the same order)
+hipError_t launch_to_f32_scaled(const double *src, float *dst, double s, size_t n, hipStream_t st,
+                                const uint16_t *hp = nullptr, uint32_t slm = 0);
+hipError_t launch_add_f32_scaled(double *x, const float *d, double s, size_t n, hipStream_t st,
+                                 const uint16_t *hp = nullptr, uint32_t slm = 0);
 
 }  // namespace tr

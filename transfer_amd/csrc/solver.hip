@@ -351,29 +351,48 @@ template int pcg_device<float>(const CgSpace<float> &, const CgParams &, hipStre
 
 namespace tr {
 
-__global__ void k_to_f32_scaled(const double *__restrict__ src, float *__restrict__ dst,
-                                double s, size_t n) {
+// FP64 internal vectors <-> FP32 vectors whose harmonics at each vertex
+// may be in another order: position k of a vertex's FP32 row holds
+// harmonic hp[k] of its FP64 row (hp null: the same order).
+__global__ void k_to_f32_scaled(const double *__restrict__ src, float *__restrict__ dst, double s, size_t n,
+                                const uint16_t *__restrict__ hp, uint32_t slm) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = (float)(src[i] * s);
+         i += (size_t)gridDim.x * blockDim.x) {
+        size_t j = i;
+        if (hp) {
+            const size_t v = i / slm, k = i - v * slm;
+            j = v * slm + hp[k];
+        }
+        dst[i] = (float)(src[j] * s);
+    }
 }
 
-__global__ void k_add_f32_scaled(double *__restrict__ x, const float *__restrict__ d, double s,
-                                 size_t n) {
+__global__ void k_add_f32_scaled(double *__restrict__ x, const float *__restrict__ d, double s, size_t n,
+                                 const uint16_t *__restrict__ hp, uint32_t slm) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        x[i] = fma(s, (double)d[i], x[i]);
+         i += (size_t)gridDim.x * blockDim.x) {
+        size_t j = i;
+        if (hp) {
+            const size_t v = i / slm, k = i - v * slm;
+            j = v * slm + hp[k];
+        }
+        x[j] = fma(s, (double)d[i], x[j]);
+    }
 }
 
-hipError_t launch_to_f32_scaled(const double *src, float *dst, double s, size_t n, hipStream_t st) {
+hipError_t launch_to_f32_scaled(const double *src, float *dst, double s, size_t n, hipStream_t st,
+                                const uint16_t *hp, uint32_t slm) {
+    if (hp && !slm) return hipErrorInvalidValue;
     unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_to_f32_scaled, dim3(g), dim3(256), 0, st, src, dst, s, n);
+    hipLaunchKernelGGL(k_to_f32_scaled, dim3(g), dim3(256), 0, st, src, dst, s, n, hp, slm);
     return hipGetLastError();
 }
 
-hipError_t launch_add_f32_scaled(double *x, const float *d, double s, size_t n, hipStream_t st) {
+hipError_t launch_add_f32_scaled(double *x, const float *d, double s, size_t n, hipStream_t st,
+                                 const uint16_t *hp, uint32_t slm) {
+    if (hp && !slm) return hipErrorInvalidValue;
     unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_add_f32_scaled, dim3(g), dim3(256), 0, st, x, d, s, n);
+    hipLaunchKernelGGL(k_add_f32_scaled, dim3(g), dim3(256), 0, st, x, d, s, n, hp, slm);
     return hipGetLastError();
 }
 
