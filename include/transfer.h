@@ -286,6 +286,15 @@ int tr_mul_add_prod(tr_ctx *ctx, double *x, double wx, const double *y, double w
 double tr_norm(tr_ctx *ctx, const double *x);
 double tr_dot(tr_ctx *ctx, const double *x, const double *y);
 int tr_synchronize(tr_ctx *ctx);
+/* No counterpart in the reference: tr_dot for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  *result
+ * (a device double) = x . y over the slm real lanes, computed on the
+ * context's stream with no allocation, no host copy and no
+ * synchronisation; the partials go to a static per-GPU scratch array of
+ * 1024 doubles (8 KB, held until the process ends), so vectors of any
+ * length fit.  Bit-identical to tr_dot.  Calls that overlap on one GPU
+ * share the scratch and race. */
+int tr_dot_noalloc(tr_ctx *ctx, const double *x, const double *y, double *result);
 
 const char *tr_last_error(void);
 

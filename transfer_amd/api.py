@@ -106,6 +106,7 @@ _sig("tr_ctx_schedule", C.c_int, vp, C.c_int, C.POINTER(C.c_int))
 _sig("tr_ctx_set_schedule", C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int)
 _sig("tr_ctx_set_angular_precision", C.c_int, vp, C.c_int)
 _sig("tr_ctx_set_refinement", C.c_int, vp, C.c_double, C.c_int)
+_sig("tr_dot_noalloc", C.c_int, vp, vp, vp, vp)
 _sig("tr_abs_normal_matrix", C.c_int, C.c_int, f64p, f64p)
 _sig("tr_abs_normal_column", C.c_int, C.c_int, f64p, f64p)
 _sig("tr_normal_rotation", C.c_int, C.c_int, f64p, f64p, u32p)
@@ -632,6 +633,11 @@ class HipContext:
         if v != v:
             raise TransferError(last_error())
         return v
+
+    def dot_noalloc(self, x, y, result):
+        """tr_dot_noalloc: x . y into the device double `result` (no
+        allocation, no synchronisation)."""
+        check(lib.tr_dot_noalloc(self._h, x, y, result))
 
     def synchronize(self):
         check(lib.tr_synchronize(self._h))

@@ -563,6 +563,10 @@ int tr::ctx_init(tr_ctx *c, int device) {
         delete c;
         return fail(TR_ERR_HIP, std::string("tr_ctx_create: ") + hipGetErrorString(e));
     }
+    if ((rc = noalloc_init(device))) {  // this GPU's address of the allocation-free dot's scratch
+        tr_ctx_destroy(c);
+        return rc;
+    }
     c->map = choose_map(c->op.slm, 4);
     c->map.variant = valu_variant(c->op, 8, c->map.variant);
     if (!fit_lds(c->map, c->op, 8, device)) {

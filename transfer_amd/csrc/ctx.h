@@ -212,6 +212,9 @@ int upload(tr_ctx *c, T **dst, const std::vector<T> &src) {
     return TR_OK;
 }
 
+// Sets up the allocation-free dot's scratch on `device` (current): once per
+// device (noalloc.hip).
+int noalloc_init(int device);
 // Second half of context creation: schedules and device data for the
 // operator already in c->op (the context is destroyed on failure).
 int ctx_init(tr_ctx *c, int device);
