@@ -366,21 +366,30 @@ def bench_c3(args):
     out["operator_fp32_exact"] = op32x
     out["kernel_source_digest"] = source_digest()
     if not args.no_solve:
+        # Chebyshev degree 6 over [lambda_max / 300, lambda_max], each step one
+        # fused operator launch (the BF16 matrix-core operator's hi/lo split)
+        ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
+        mixed_kw = {"mixed_fp32_ir_jacobi": (use_exact, dict(use_prec=2)),
+                    "mixed_bf16mfma_ir_jacobi": (use_mfma, dict(use_prec=2)),
+                    "mixed_bf16mfma_ir_chebyshev": (use_mfma, dict(use_prec=4, inner_tol=1e-4))}
         runs = [("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
                 ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
-                ("mixed_fp32_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(setup=use_exact)),
-                ("mixed_bf16mfma_ir_jacobi", ta.TR_SOLVER_MIXED, 2, dict(setup=use_mfma))]
+                ("pcg_fp64_device_chebyshev", ta.TR_SOLVER_PCG_FP64, 4, {})]
+        for name, (setup, kw) in mixed_kw.items():
+            kw = dict(kw)
+            runs.append((name, ta.TR_SOLVER_MIXED, kw.pop("use_prec"), dict(setup=setup, **kw)))
         if args.invprec:
-            runs.insert(2, ("reference_pcg_fp64_invprec_host", ta.TR_SOLVER_REFERENCE, 1,
+            runs.insert(3, ("reference_pcg_fp64_invprec_host", ta.TR_SOLVER_REFERENCE, 1,
                             dict(inner_tol=1e-8, inner_maxit=1000)))
         solves = run_solves(ctx, runs, tag, args.only)
         if not args.no_learned and not args.only:
             best = min((v["seconds"], k) for k, v in solves.items() if "mixed" in k and v["converged"])[1]
-            (use_exact if best == "mixed_fp32_ir_jacobi" else use_mfma)()
+            setup, kw = mixed_kw[best]
+            setup()
             train = [ta.problems.ball(n, maxk=5, kind="hg", seed=s) for n, s in
                      ((16, 11), (20, 12), (24, 13), (20, 14), (16, 15), (24, 16), (28, 17), (28, 18))]
             g, info = train_guess(train, tag)
-            lm = learned_mixed(ctx, P, g, dict(use_prec=2), tag)
+            lm = learned_mixed(ctx, P, g, kw, tag)
             lm.update(info, inner_solver=best)
             solves["learned_mixed"] = lm
             use_mfma()

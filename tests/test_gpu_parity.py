@@ -152,8 +152,8 @@ def test_bf16_angular_projection(name):
             ctx.set_angular_precision(prec)
             if not try_schedule(ctx, 1, J=0, WPB=4, variant=4):
                 # no 16x16 matrix-core tile of this order: 33..80 harmonics take
-                # the K-blocked schedule (variant 5, couplings BF16 only: bar
-                # 2e-2), more take the exact FP32 VALU schedule (bar 1e-5)
+                # the K-blocked schedule (variant 5, the same bars), more take
+                # the exact FP32 VALU schedule (bar 1e-5)
                 blocked = try_schedule(ctx, 1, J=0, WPB=4, variant=5)
                 if not blocked:
                     ctx.autotune(1)
@@ -161,7 +161,7 @@ def test_bf16_angular_projection(name):
                 e = rel(ctx.copyToHost(d_r), want)
                 print("[gpu-parity] %s angular prec=%d via %s rel err %.2e" % (name, prec, ctx.kernel_name(1), e),
                       flush=True)
-                assert e <= (2e-2 if blocked else 1e-5)
+                assert e <= (bar if blocked else 1e-5)
                 continue
             for wpb in (1, 4, 8):
                 ctx.set_schedule(1, J=0, WPB=wpb, variant=4)
@@ -178,10 +178,10 @@ def test_bf16_angular_projection(name):
 @pytest.mark.parametrize("name", ["ball16-K5", "ball10-K5-hg", "ball12-K4"])
 def test_bf16_blocked_projection(name):
     """The FP32 operator for 33..80 harmonics with the BF16 matrix-core
-    projection in blocks of 16 harmonics (variant 5) against the FP64
-    oracle.  Couplings in BF16 (8 significant bits) and the gathered values
-    plain (bar 2e-2 relative to max|A f|) or split into BF16 hi + lo (the
-    couplings' rounding remains: bar 1e-2); every workgroup size."""
+    projection in blocks of at most 16 harmonics (variant 5) against the
+    FP64 oracle.  Plain BF16 (gathered values and couplings rounded to 8
+    significant bits: bar 2e-2 relative to max|A f|, observed 3e-3) or the
+    hi/lo split of both (bar 2e-4, observed 6e-6); every workgroup size."""
     if name in cases():
         P, ctx, op = setup(name)
     else:
@@ -192,7 +192,7 @@ def test_bf16_blocked_projection(name):
     d_f, d_r = ctx.vector_from(f), ctx.allocVector()
     want = op.lhs(f)
     try:
-        for prec, bar in ((1, 2e-2), (3, 1e-2)):
+        for prec, bar in ((1, 2e-2), (3, 2e-4)):
             ctx.set_angular_precision(prec)
             for wpb in (2, 4, 8):
                 ctx.set_schedule(1, J=0, WPB=wpb, variant=5)
