@@ -1638,7 +1638,7 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
             cands.push_back(m);
         }
     if (bf16 && v5_ok(op))
-        for (int wpb : {2, 4, 8}) cands.push_back(mfma5_map_ctx(c, wpb));
+        for (int wpb : {2, 4, 6, 8}) cands.push_back(mfma5_map_ctx(c, wpb));
     {
         for (LaneMap m : cands) {
             {
@@ -1728,7 +1728,7 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
     }
     const HostOperator &op = c->op;
     if (variant == 5) {
-        if (precision != 1 || c->ang_prec == 0 || !v5_ok(op) || (WPB != 1 && WPB != 2 && WPB != 4 && WPB != 8))
+        if (precision != 1 || c->ang_prec == 0 || !v5_ok(op) || WPB < 1 || WPB > 8)
             return fail(TR_ERR_ARG, "tr_ctx_set_schedule: K-blocked matrix-core schedule unsupported here");
         c->map32 = mfma5_map_ctx(c, WPB);
         return TR_OK;

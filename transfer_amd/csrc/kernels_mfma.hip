@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 
 #include "kernels.h"
 
@@ -244,6 +245,51 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
             for (int c = 0; c < NCH; c++) acc[c][j] = 0;
 }
 
+// acc[c][j] += (w of lane S of the quad) * x[j], j = 0..3, for two channels:
+// the quad broadcast rides on the FMAs as their DPP source operand
+// (v_fmac_f32_dpp) instead of a separate v_mov_b32_dpp per channel.  The
+// leading s_nop 1 gives the two wait states a DPP read needs after a VALU
+// write of its source register, whatever the compiler placed before.
+#define TR_DPP_FMAC2(S0, S1)                                                                            \
+    asm volatile("s_nop 1\n"                                                                          \
+                 "v_fmac_f32_dpp %0, %8, %10 quad_perm:[" #S0 "," #S0 "," #S0 "," #S0 "] row_mask:0xf bank_mask:0xf\n" \
+                 "v_fmac_f32_dpp %1, %8, %11 quad_perm:[" #S0 "," #S0 "," #S0 "," #S0 "] row_mask:0xf bank_mask:0xf\n" \
+                 "v_fmac_f32_dpp %2, %8, %12 quad_perm:[" #S0 "," #S0 "," #S0 "," #S0 "] row_mask:0xf bank_mask:0xf\n" \
+                 "v_fmac_f32_dpp %3, %8, %13 quad_perm:[" #S0 "," #S0 "," #S0 "," #S0 "] row_mask:0xf bank_mask:0xf\n" \
+                 "v_fmac_f32_dpp %4, %9, %10 quad_perm:[" #S1 "," #S1 "," #S1 "," #S1 "] row_mask:0xf bank_mask:0xf\n" \
+                 "v_fmac_f32_dpp %5, %9, %11 quad_perm:[" #S1 "," #S1 "," #S1 "," #S1 "] row_mask:0xf bank_mask:0xf\n" \
+                 "v_fmac_f32_dpp %6, %9, %12 quad_perm:[" #S1 "," #S1 "," #S1 "," #S1 "] row_mask:0xf bank_mask:0xf\n" \
+                 "v_fmac_f32_dpp %7, %9, %13 quad_perm:[" #S1 "," #S1 "," #S1 "," #S1 "] row_mask:0xf bank_mask:0xf" \
+                 : "+v"(a0[0]), "+v"(a0[1]), "+v"(a0[2]), "+v"(a0[3]), "+v"(a1[0]), "+v"(a1[1]), "+v"(a1[2]), \
+                   "+v"(a1[3])                                                                          \
+                 : "v"(w0), "v"(w1), "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]))
+
+template <int S0, int S1>
+__device__ __forceinline__ void dpp_fmac2(float (&a0)[4], float (&a1)[4], float w0, float w1, const float *x) {
+    static_assert(S0 >= 0 && S0 < 4 && S1 >= 0 && S1 < 4, "quad lane");
+    if constexpr (S0 == 0 && S1 == 0) TR_DPP_FMAC2(0, 0);
+    else if constexpr (S0 == 0 && S1 == 1) TR_DPP_FMAC2(0, 1);
+    else if constexpr (S0 == 1 && S1 == 1) TR_DPP_FMAC2(1, 1);
+    else if constexpr (S0 == 1 && S1 == 2) TR_DPP_FMAC2(1, 2);
+    else if constexpr (S0 == 2 && S1 == 2) TR_DPP_FMAC2(2, 2);
+    else if constexpr (S0 == 2 && S1 == 3) TR_DPP_FMAC2(2, 3);
+    else if constexpr (S0 == 3 && S1 == 3) TR_DPP_FMAC2(3, 3);
+    else if constexpr (S0 == 0 && S1 == 2) TR_DPP_FMAC2(0, 2);
+    else if constexpr (S0 == 1 && S1 == 3) TR_DPP_FMAC2(1, 3);
+    else if constexpr (S0 == 0 && S1 == 3) TR_DPP_FMAC2(0, 3);
+    else static_assert(S0 <= S1, "channel pairs in order");
+}
+#undef TR_DPP_FMAC2
+
+// all NCH channels of one slot, two at a time (channel c's weight is
+// element c % PE of quad lane c / PE)
+template <int PE, int NCH, int... P>
+__device__ __forceinline__ void accum_dpp(float (&acc)[NCH][4], const float *c0, const float *x,
+                                          std::integer_sequence<int, P...>) {
+    (dpp_fmac2<(2 * P) / PE, (2 * P + 1) / PE>(acc[2 * P], acc[2 * P + 1], c0[(2 * P) % PE], c0[(2 * P + 1) % PE], x),
+     ...);
+}
+
 // The gather with the slots' coefficient records already in registers
 // (cf[s][e]: element q*PE + e of slot s's record, loaded once per tile by
 // the K-blocked schedule); only the neighbour rows stream, DEPTH slots in
@@ -262,16 +308,20 @@ __device__ __forceinline__ void gather_pre(T (&acc)[NCH][J], const uint32_t (&nb
             x[j] = bload1<T>(frs, base + (uint32_t)((64 * (j / E) + (j % E) * (int)sizeof(T))), 0u);
     };
     auto accum = [&](const T *x, const T *c0) __attribute__((always_inline)) {
+        if constexpr (std::is_same<T, float>::value && J == 4 && NCH % 2 == 0) {
+            accum_dpp<PE, NCH>(acc, c0, x, std::make_integer_sequence<int, NCH / 2>{});
+        } else {
 #pragma unroll
-        for (int c = 0; c < NCH; c++) {
-            const T w = quad_bcast(c0[c % PE], c / PE);
+            for (int c = 0; c < NCH; c++) {
+                const T w = quad_bcast(c0[c % PE], c / PE);
 #pragma unroll
-            for (int j = 0; j < J; j++) acc[c][j] = fma(w, x[j], acc[c][j]);
+                for (int j = 0; j < J; j++) acc[c][j] = fma(w, x[j], acc[c][j]);
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; c++)
+#pragma unroll
+                for (int j = 0; j < J; j++) asm volatile("" : "+v"(acc[c][j]));
         }
-#pragma unroll
-        for (int c = 0; c < NCH; c++)
-#pragma unroll
-            for (int j = 0; j < J; j++) asm volatile("" : "+v"(acc[c][j]));
     };
 #pragma unroll
     for (int d = 0; d < DEPTH && d < NS; d++) load(d, xs[d]);
@@ -623,7 +673,9 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
     const int vloc = lane & 15, q = lane >> 4;
     const int gv = lane >> 2, gq = lane & 3;
     // the tile's coefficient records (WT x PE per lane) stay in registers
-    // across the harmonic blocks
+    // across the harmonic blocks (two wavefronts per SIMD: capping the
+    // registers at 168 for a third spills and measured slower, 523 against
+    // 480 us on the 96^3 ball)
     constexpr int BUDGET = 256;
     constexpr int PE = (RS + 3) / 4;
     constexpr int OVH = WT + 4 * NT + 30 + WT * PE;

@@ -98,8 +98,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
     }
     const T alpha = (T)al_sh;
     double rr = 0, rz = 0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
+    auto one = [&](size_t i) __attribute__((always_inline)) {
         x[i] = fma(alpha, p[i], x[i]);
         const T ri = fma(-alpha, Ap[i], r[i]);
         r[i] = ri;
@@ -107,7 +106,16 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
         const T zi = dinv ? dinv[i] * ri : ri;
         rr = fma((double)ri, (double)ri, rr);
         rz = fma((double)ri, (double)zi, rz);
+    };
+    // four elements per thread and pass (each a full wavefront's coalesced
+    // stride apart), so four loads of every stream are in flight at once
+    const size_t S = (size_t)gridDim.x * blockDim.x;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * S < n; i += 4 * S) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) one(i + u * S);
     }
+    for (; i < n; i += S) one(i);
     block_sum2(rr, rz);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = rr;
@@ -161,11 +169,17 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
         __syncthreads();
     }
     const T beta = (T)be_sh;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
+    auto one = [&](size_t i) __attribute__((always_inline)) {
         const T zi = z ? z[i] : (dinv ? dinv[i] * r[i] : r[i]);
         p[i] = fma(beta, p[i], zi);
+    };
+    const size_t S = (size_t)gridDim.x * blockDim.x;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * S < n; i += 4 * S) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) one(i + u * S);
     }
+    for (; i < n; i += S) one(i);
 }
 
 // r = b - r (r holds A x on entry); z = M r; p = z; partials of r.r, r.z, b.b
