@@ -406,6 +406,83 @@ def bench_c3(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_c5(args):
+    """The learned part's protocol on the 32^3 blob cube (196,608 tets,
+    K = 3, kappa = 0.1 + 8 Gaussian blobs, albedo U[0, 0.99] per problem):
+    the network trained on the solutions of args.c5_train solver-generated
+    problems (seeds 0 ..), then, on args.c5_test held-out seeds, the mixed
+    solver (Jacobi-preconditioned FP32 CG inside FP64 refinement) to a 1e-8
+    FP64 residual from zero, from the learned guess, and from the guess with
+    the learned coarse correction in the preconditioner (use_prec 5); the
+    learned rows' time includes building the guess on the device."""
+    import numpy as np
+    import transfer_amd as ta
+    from transfer_amd import learned
+    tag = "bench-c5"
+    ntrain, ntest = args.c5_train, args.c5_test
+    t0 = time.perf_counter()
+
+    def prog(k):
+        log(tag, "training data: %d/%d problems solved, %.1f s" % (k, ntrain, time.perf_counter() - t0))
+
+    feat, targ, _ = learned.generate(list(range(ntrain)), n=32, maxk=3, chunk=128, progress=prog)
+    gen_s = time.perf_counter() - t0
+    g = learned.LearnedGuess(seed=3)
+    hist = g.fit(feat, targ, steps=3000, batch=16384, lr=2e-3)
+    train_s = time.perf_counter() - t0 - gen_s
+    log(tag, "trained on %d samples in %.1f s, loss %.2e" % (len(targ), train_s, float(hist[-10:].mean())))
+    del feat, targ
+    kw = dict(solver=ta.TR_SOLVER_MIXED, maxit=30, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000, inner_tol=1e-5)
+    rows = {k: dict(inner=[], seconds=[], converged=0, fallback=0) for k in ("zero_start", "learned_start",
+                                                                          "learned_coarse")}
+    pred_err, mean_err = [], []
+    seeds = list(range(100000, 100000 + ntest))
+    for c0 in range(0, ntest, 16):
+        for P in learned.cube_problems(seeds[c0:c0 + 16], n=32, maxk=3):
+            ctx = ta.HipContext(P.mesh, P.ang)
+            for mode in rows:
+                if mode == "zero_start":
+                    d = ctx.allocVector()
+                    st, _ = ctx.solve(d, use_prec=2, **kw)
+                    gs = 0.0
+                    U, _ = ctx.moments(d)
+                    t = U * (1 - P.albedo) / (4 * np.pi)
+                    y = g.predict(ta.mlp_features(P.mesh, P.albedo))
+                    pred_err.append(float(np.abs(y - t).mean()))
+                    mean_err.append(float(np.abs(t.mean() - t).mean()))
+                else:
+                    ctx.synchronize()
+                    t1 = time.perf_counter()
+                    d = g.write_guess(ctx, P.mesh, P.albedo)
+                    ctx.synchronize()
+                    gs = time.perf_counter() - t1
+                    st, _ = ctx.solve(d, use_prec=2 if mode == "learned_start" else 5, x0=True, **kw)
+                r = rows[mode]
+                r["inner"].append(st.preck)
+                r["seconds"].append(gs + st.seconds)
+                r["converged"] += st.status == 0
+                r["fallback"] += st.fallback != 0
+                ctx.deviceFree(d)
+            ctx.close()
+        log(tag, "held-out %d/%d: mean inner %s" % (min(c0 + 16, ntest), ntest,
+                                                    {k: round(float(np.mean(v["inner"])), 1) for k, v in rows.items()}))
+    base = rows["zero_start"]
+    out = dict(metric="learned part: held-out mean time and inner iterations to a 1e-8 FP64 residual", unit="s",
+               higher_is_better=False, n_gpus=1, data="synthetic", dtype="f32+f64",
+               config=dict(workload="cube32 blobs: 32^3 cubes (196,608 tets), K=3, kappa=0.1+8 Gaussian blobs, "
+                                    "albedo U[0,0.99]", train_problems=ntrain, held_out_problems=ntest),
+               training=dict(generate_seconds=gen_s, train_seconds=train_s, loss=float(hist[-10:].mean())),
+               prediction=dict(mean_abs_error=float(np.mean(pred_err)), mean_predictor_error=float(np.mean(mean_err))))
+    for k, v in rows.items():
+        out[k] = dict(mean_inner=float(np.mean(v["inner"])), mean_seconds=float(np.mean(v["seconds"])),
+                      converged=v["converged"], fallback=v["fallback"],
+                      inner_reduction=1 - float(np.mean(v["inner"])) / float(np.mean(base["inner"])),
+                      time_reduction=1 - float(np.mean(v["seconds"])) / float(np.mean(base["seconds"])),
+                      improved_problems=int(sum(a < b for a, b in zip(v["inner"], base["inner"]))))
+    out["value"] = out["learned_start"]["mean_seconds"]
+    print(json.dumps(out), flush=True)
+
+
 def bench_c2_brief(steps, warmup):
     """The 64^3 Planck ball (K = 3): both operators and the plain FP64 and
     best mixed solves."""
@@ -516,6 +593,9 @@ def main():
                     help="run the curved sphere workload with this many cubes per axis instead (48: the measured one)")
     ap.add_argument("--batch", type=int, default=0,
                     help="run the batching workload with this many problems instead (256 = the full one)")
+    ap.add_argument("--c5", action="store_true", help="run the learned part's protocol instead (cube32 blobs)")
+    ap.add_argument("--c5-train", type=int, default=2000, help="--c5: solver-generated training problems")
+    ap.add_argument("--c5-test", type=int, default=200, help="--c5: held-out problems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -527,6 +607,8 @@ def main():
         bench_sphere(args.sphere, min(args.steps, 50), min(args.warmup, 5))
     elif args.c2:
         bench_c2(args)
+    elif args.c5:
+        bench_c5(args)
     else:
         bench_c3(args)
 

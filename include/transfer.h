@@ -307,7 +307,11 @@ enum {
     TR_SOLVER_PCG_FP64 = 1,  /* CG with device-resident scalars and graph-
                                 captured iterations; use_prec 0 = none,
                                 2 = Jacobi (exact diagonal of A),
-                                3 = multilevel, 4 = Chebyshev polynomial   */
+                                3 = multilevel, 4 = Chebyshev polynomial,
+                                5 = Jacobi plus the learned coarse
+                                correction g (g.r)/(g.Ag), g the learned
+                                guess passed in f (x0 = 1; a rejected guess
+                                leaves Jacobi)                            */
     TR_SOLVER_MIXED = 2      /* FP64 iterative refinement (true residual
                                 b - A x in FP64) around the FP32 operator's
                                 device CG run to inner_tol (relative), or

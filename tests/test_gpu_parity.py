@@ -262,3 +262,49 @@ def test_reference_angular_data_through_the_view(K):
     ctx.computeLhs(d_f, d_r)
     assert rel(ctx.copyToHost(d_r), op.lhs(f)) <= TOL
     ctx.close()
+
+
+@pytest.mark.parametrize("solver", [ta.TR_SOLVER_REFERENCE, ta.TR_SOLVER_PCG_FP64])
+def test_pcg_iterations_match_oracle_ball32(solver):
+    """main.cpp:125-162's PCG (no invprec) on a K = 3 Planck ball of the
+    64^3 ball's construction at 32^3: the host-driven reference algorithm
+    and the device-resident PCG both take the oracle's iteration count to a
+    1e-8 relative residual (within 2: the sums run in another order) and
+    reach its solution to 1e-7."""
+    P = ta.problems.ball(32)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    kw = dict(maxit=5000, tol_abs=0.0, tol_rel=1e-8)
+    f_or, st_or, _ = op.pcg(**kw)
+    d_f = ctx.allocVector()
+    st, _ = ctx.solve(d_f, solver=solver, **kw)
+    f = ctx.copyToHost(d_f)
+    print("[gpu-parity] ball32 pcg solver=%d iters gpu %d oracle %d, solution rel diff %.2e" % (
+        solver, st.iters, st_or.iters, rel(f, f_or)), flush=True)
+    assert st.status == 0 and abs(st.iters - st_or.iters) <= 2
+    assert rel(f, f_or) <= 1e-7
+    ctx.close()
+
+
+def test_padding_lanes_of_inputs_are_ignored():
+    """Vectors cross the boundary in the reference layout [v][aslm]; the
+    library keeps [v][slm] inside, so padding lanes of an input are dropped
+    and every output's padding lanes are zero.  (The reference's vertex
+    kernel would carry nonzero input padding into the kappa V / 20 mass of
+    the output's padding lanes, kernels.cu:244-249, 286; its callers never
+    make such vectors: allocations are zero-filled and the BLAS-1 kernels
+    map zero padding to zero.)  The real lanes equal the oracle's operator
+    applied to the input with its padding cleared."""
+    P, ctx, op = setup("ball12-K2-hg")
+    f = rvec(op, 9).reshape(op.nP, op.aslm)
+    f[:, op.slm:] = 7.0
+    d_f, d_r = ctx.vector_from(f.ravel()), ctx.allocVector()
+    ctx.computeLhs(d_f, d_r)
+    r = ctx.copyToHost(d_r).reshape(op.nP, op.aslm)
+    g = f.copy()
+    g[:, op.slm:] = 0.0
+    want = op.lhs(g.ravel()).reshape(op.nP, op.aslm)
+    assert np.all(r[:, op.slm:] == 0.0)
+    assert rel(r[:, :op.slm], want[:, :op.slm]) <= TOL
+    ctx.deviceFree(d_f)
+    ctx.deviceFree(d_r)

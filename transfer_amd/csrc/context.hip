@@ -1100,7 +1100,10 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     if (!Ap) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
     int rc;
     ChebPrec<double> cheb;
-    if (prm->use_prec == 2 || prm->use_prec == 4) {
+    if (prm->use_prec == 5 && !prm->x0)
+        return fail(TR_ERR_ARG, "tr_solve: use_prec 5 (learned coarse correction) takes the learned guess in f "
+                                "with x0 = 1");
+    if (prm->use_prec == 2 || prm->use_prec == 4 || prm->use_prec == 5) {
         std::vector<double> d = operator_diagonal(c->op);
         for (double &x : d) x = 1.0 / x;
         dinv = w.get();
@@ -1123,11 +1126,34 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     } else if (prm->use_prec == 3) {
         if ((rc = build_multilevel(c, false, c->mg_coarsest, c->mg_agg))) return rc;
     } else if (prm->use_prec != 0) {
-        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none), 2 (Jacobi), 3 (multilevel) "
-                                "or 4 (Chebyshev)");
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_PCG_FP64 takes use_prec 0 (none), 2 (Jacobi), 3 (multilevel), "
+                                "4 (Chebyshev) or 5 (Jacobi + learned coarse correction)");
     }
+    tr_solve_stats st0{};
+    if (!st) st = &st0;
     if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
     CgSpace<double> S;
+    // the learned coarse correction: g = the scaled guess; without one (the
+    // guess rejected) the solve is Jacobi-preconditioned
+    double *gvec = nullptr, *ccp = nullptr;
+    if (prm->use_prec == 5 && !st->fallback) {
+        gvec = w.get();
+        S.z = w.get();
+        if (!gvec || !S.z) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
+        const int np = learned_coarse_partials();
+        HIPCHECK(hipMalloc((void **)&ccp, sizeof(double) * (np + 8)));
+        w.v.push_back(ccp);
+        HIPCHECK(hipMemcpyAsync(gvec, f, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = lhs(c, gvec, Ap))) return rc;
+        const double gAg = dot(c, gvec, Ap);
+        if (!(gAg > 0)) return fail(TR_ERR_HIP, "tr_solve: learned coarse vector with g.Ag <= 0");
+        HIPCHECK(hipMemcpy(ccp + np, &gAg, sizeof(double), hipMemcpyHostToDevice));
+        const hipStream_t cs = c->stream;
+        const double *gv = gvec, *dv = dinv;
+        S.prec = [gv, dv, ccp, np, n, cs](const double *rr, double *z, const double *skip) {
+            return launch_learned_coarse<double>(rr, z, dv, gv, n, ccp, ccp + np, skip, cs);
+        };
+    }
     if (prm->use_prec == 3) {
         S.z = w.get();
         if (!S.z) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
@@ -1270,7 +1296,10 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     float *dinv32 = nullptr;
     if (!Ap) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
     ChebPrec<float> cheb;
-    if (prm->use_prec == 2 || prm->use_prec == 4) {
+    if (prm->use_prec == 5 && !prm->x0)
+        return fail(TR_ERR_ARG, "tr_solve: use_prec 5 (learned coarse correction) takes the learned guess in f "
+                                "with x0 = 1");
+    if (prm->use_prec == 2 || prm->use_prec == 4 || prm->use_prec == 5) {
         std::vector<double> d = operator_diagonal(c->op);
         std::vector<float> di(d.size());
         for (size_t i = 0; i < d.size(); i++) di[i] = (float)(1.0 / d[i]);
@@ -1311,13 +1340,38 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     } else if (prm->use_prec == 3) {
         if ((rc = build_multilevel(c, true, c->mg_coarsest, c->mg_agg))) return rc;
     } else if (prm->use_prec != 0) {
-        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none), 2 (Jacobi), 3 (multilevel) or "
-                                "4 (Chebyshev)");
+        return fail(TR_ERR_ARG, "tr_solve: TR_SOLVER_MIXED takes use_prec 0 (none), 2 (Jacobi), 3 (multilevel), "
+                                "4 (Chebyshev) or 5 (Jacobi + learned coarse correction)");
     }
     auto t0 = std::chrono::steady_clock::now();
+    tr_solve_stats st0{};
+    if (!st) st = &st0;
     if ((rc = tr_compute_rhs(c, b)) || (rc = start_point(c, prm, f, b, r, st))) return rc;
     const double bn = std::sqrt(dot(c, b, b));
     CgSpace<float> S;
+    // the learned coarse correction of the inner solves: g = the scaled
+    // guess in the FP32 vectors' order, g.Ag with the FP64 operator
+    if (prm->use_prec == 5 && !st->fallback) {
+        const int np = learned_coarse_partials();
+        float *g32 = nullptr;
+        double *ccp = nullptr;
+        HIPCHECK(hipMalloc((void **)&g32, sizeof(float) * 2 * n + 16));
+        w.v.push_back((double *)g32);
+        HIPCHECK(hipMalloc((void **)&ccp, sizeof(double) * (np + 8)));
+        w.v.push_back(ccp);
+        S.z = g32 + n;
+        if ((rc = lhs(c, f, Ap))) return rc;
+        const double gAg = dot(c, f, Ap);
+        if (!(gAg > 0)) return fail(TR_ERR_HIP, "tr_solve: learned coarse vector with g.Ag <= 0");
+        HIPCHECK(launch_to_f32_scaled(f, g32, 1.0, n, c->stream, c->perm32(), c->op.slm));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        HIPCHECK(hipMemcpy(ccp + np, &gAg, sizeof(double), hipMemcpyHostToDevice));
+        const hipStream_t cs = c->stream;
+        const float *dv = dinv32;
+        S.prec = [g32, dv, ccp, np, n, cs](const float *rr, float *z, const double *skip) {
+            return launch_learned_coarse<float>(rr, z, dv, g32, n, ccp, ccp + np, skip, cs);
+        };
+    }
     if (prm->use_prec == 3) {
         S.z = v32 + 6 * n;
         S.prec = [c](const float *rr, float *z, const double *skip) { return multilevel_apply32(c, rr, z, skip); };

@@ -194,6 +194,13 @@ hipError_t launch_rhs(const double *rhs, uint32_t nP, uint32_t slm, double *b,
 hipError_t launch_axpby(double *x, double wx, const double *y, double wy, size_t n,
                         hipStream_t s);
 
+// The learned coarse correction z = dinv r + g (g.r)/(gAg) (coarse.hip);
+// part holds learned_coarse_partials() doubles, gAg one device double;
+// nothing is done when *skip != 0.
+template <typename T>
+hipError_t launch_learned_coarse(const T *r, T *z, const T *dinv, const T *g, size_t n, double *part,
+                                 const double *gAg, const double *skip, hipStream_t s);
+int learned_coarse_partials();
 // out[0] = sum_i x[i]*y[i] in FP64, deterministic two-pass reduction;
 // part needs red_partials() doubles.
 int red_partials();

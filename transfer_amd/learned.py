@@ -16,8 +16,17 @@ from .api import MLP, Angular, Batch, HipContext, mlp_features, TR_SOLVER_MIXED
 FOUR_PI = 4.0 * np.pi
 
 
+def _pool():
+    import concurrent.futures as cf
+    import os
+    return cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1))
+
+
 def cube_problems(seeds, n=32, maxk=3):
-    return [problems.cube(n, maxk=maxk, seed=int(s)) for s in seeds]
+    """The cube problems of `seeds`, built on the host cores (the mesh and
+    field generators run in the C library, outside the interpreter lock)."""
+    with _pool() as ex:
+        return list(ex.map(lambda s: problems.cube(n, maxk=maxk, seed=int(s)), seeds))
 
 
 def generate(seeds, n=32, maxk=3, chunk=64, inner_tol=1e-4, progress=None):
@@ -27,6 +36,8 @@ def generate(seeds, n=32, maxk=3, chunk=64, inner_tol=1e-4, progress=None):
     feats, targs, counts = [], [], []
     for c0 in range(0, len(seeds), chunk):
         P = cube_problems(seeds[c0:c0 + chunk], n, maxk)
+        with _pool() as ex:
+            F = list(ex.map(lambda p: mlp_features(p.mesh, p.albedo), P))
         B = Batch([p.mesh for p in P], Angular(maxk), [p.albedo for p in P])
         B.autotune(0)
         B.autotune(1)
@@ -36,7 +47,7 @@ def generate(seeds, n=32, maxk=3, chunk=64, inner_tol=1e-4, progress=None):
         B.deviceFree(d_x)
         for k, p in enumerate(P):
             lo, hi = int(B.voff[k]), int(B.voff[k + 1])
-            feats.append(mlp_features(p.mesh, p.albedo))
+            feats.append(F[k])
             targs.append(np.clip(U[lo:hi] * (1.0 - p.albedo) / FOUR_PI, 0.0, 1.0).astype(np.float32))
             counts.append(hi - lo)
         if progress:
