@@ -208,7 +208,10 @@ int tr_ctx_schedule(tr_ctx *ctx, int precision, int *out5);
 /* Forces a schedule (J harmonics per lane, WPB wavefronts per workgroup,
  * variant 1 = whole-wavefront LDS staging, 2 = batched staging, 3 = batched
  * staging with buffer loads and per-slot coefficient records, 4 = angular
- * contraction on the matrix cores, J ignored). */
+ * contraction on the matrix cores, J ignored; 5 = the same for 33..80
+ * harmonics in blocks of 16: FP32 with BF16 matrix-core products, or FP64
+ * (precision 0, WPB 8) in the reflection-parity classes where they apply:
+ * a single problem without reflecting planes or faces off the axes). */
 int tr_ctx_set_schedule(tr_ctx *ctx, int precision, int J, int WPB, int variant);
 /* (variant 5: the matrix-core schedule for 33..80 harmonics, FP32 operator
  * with a BF16 angular precision only, J ignored.) */
@@ -455,10 +458,12 @@ int tr_eig_estimate(tr_ctx *ctx, int steps, int jacobi, double tol, double *out4
 /* Seconds per operator application (median over reps), for benchmarking. */
 double tr_bench_lhs(tr_ctx *ctx, int reps);
 /* Times `steps` operator applications (precision 0 FP64, 1 the FP32
- * operator of the mixed solver) after `warmup` untimed ones, with HIP events
- * on the context's stream around each launch.  out[0] wall seconds of the
- * timed loop (device-synchronised on both sides), out[1] mean, out[2] min,
- * out[3] max kernel ms. */
+ * operator of the mixed solver, 2 the FP64 operator on vectors already in
+ * its schedule's harmonic order: under the FP64 K-blocked schedule the
+ * kernel alone, without the permutation of an internal-order input) after
+ * `warmup` untimed ones, with HIP events on the context's stream around
+ * each launch.  out[0] wall seconds of the timed loop (device-synchronised
+ * on both sides), out[1] mean, out[2] min, out[3] max ms per application. */
 int tr_bench_operator(tr_ctx *ctx, int precision, int warmup, int steps, double *out4);
 /* Name of the HIP kernel of the current operator schedule, as profilers
  * print it (without the namespace and argument list). */

@@ -53,6 +53,22 @@ def test_ball96_hg_operator_and_mixed_solve():
     e64 = rel(ctx.copyToHost(d_r), want)
     print("[gpu-configs] ball96-hg FP64 %s: rel err %.2e" % (ctx.kernel_name(0), e64), flush=True)
     assert e64 <= 1e-10
+    # the FP64 operator on the FP64 matrix cores (parity-class blocks), and
+    # the plain FP64 solve with it, which the oracle re-checks
+    default = ctx.schedule()
+    ctx.set_schedule(0, J=0, WPB=8, variant=5)
+    ctx.computeLhs(d_f, d_r)
+    e5 = rel(ctx.copyToHost(d_r), want)
+    print("[gpu-configs] ball96-hg FP64 %s: rel err %.2e" % (ctx.kernel_name(0), e5), flush=True)
+    assert e5 <= 1e-10
+    d_x = ctx.allocVector()
+    st, _ = ctx.solve(d_x, solver=ta.TR_SOLVER_PCG_FP64, use_prec=0, maxit=5000, tol_abs=0.0, tol_rel=1e-8)
+    res = oracle_residual(op, ctx.copyToHost(d_x))
+    print("[gpu-configs] ball96-hg FP64 PCG on %s: %d iterations, %.3f s, true rel %.2e (oracle %.2e)" % (
+        ctx.kernel_name(0), st.iters, st.seconds, st.true_rel, res), flush=True)
+    assert st.status == 0 and st.true_rel <= 1e-8 and res <= 1.5e-8
+    ctx.deviceFree(d_x)
+    ctx.set_schedule(0, J=default["J"], WPB=default["WPB"], variant=default["variant"])
     errs = {}
     for prec, bar in ((0, 1e-5), (3, 2e-4)):
         ctx.set_angular_precision(prec)

@@ -207,6 +207,57 @@ def test_bf16_blocked_projection(name):
         ctx.deviceFree(d_r)
 
 
+@pytest.mark.parametrize("name", ["ball16-K5", "ball10-K5-hg"])
+def test_fp64_blocked_operator_and_solves(name):
+    """The FP64 operator for 33..80 harmonics on the FP64 matrix cores in
+    parity-class blocks (variant 5 in FP64; seven channels in one group,
+    ten in two) against the oracle at the FP64 bar, the angular-diagonal
+    operator beside it, and FP64 PCG solves through its fused epilogues
+    (p.Ap with Jacobi, the Chebyshev steps) against the VALU schedule's: the
+    oracle re-checks every solution's residual and the iteration counts
+    agree to rounding (within 2%)."""
+    if name in cases():
+        P, ctx, op = setup(name)
+    else:
+        P = ta.problems.ball(10, maxk=5, kind="hg")
+        ctx = ta.HipContext(P.mesh, P.ang)
+        op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    default = ctx.schedule()
+    f = rvec(op, 11)
+    d_f, d_r = ctx.vector_from(f), ctx.allocVector()
+    b = op.rhs()
+    try:
+        assert try_schedule(ctx, 0, J=0, WPB=8, variant=5), "FP64 K-blocked schedule expected here"
+        ctx.computeLhs(d_f, d_r)
+        e = rel(ctx.copyToHost(d_r), op.lhs(f))
+        print("[gpu-parity] %s fp64-blocked %s lhs rel err %.2e" % (name, ctx.kernel_name(0), e), flush=True)
+        assert e <= TOL
+        ctx.computeLhsDiag(d_f, d_r)
+        assert rel(ctx.copyToHost(d_r), op.lhs_diag(f)) <= TOL
+        iters = {}
+        for use_prec in (0, 2, 4):
+            for variant in (5, default["variant"]):
+                if variant == 5:
+                    ctx.set_schedule(0, J=0, WPB=8, variant=5)
+                else:
+                    ctx.set_schedule(0, J=default["J"], WPB=default["WPB"], variant=default["variant"])
+                d_x = ctx.allocVector()
+                st, _ = ctx.solve(d_x, solver=ta.TR_SOLVER_PCG_FP64, use_prec=use_prec, maxit=5000, tol_abs=0.0,
+                                  tol_rel=1e-10)
+                res = op.norm(b - op.lhs(ctx.copyToHost(d_x))) / op.norm(b)
+                ctx.deviceFree(d_x)
+                print("[gpu-parity] %s fp64 PCG use_prec=%d variant=%d: %d iterations, true rel %.2e, oracle %.2e" % (
+                    name, use_prec, variant, st.iters, st.true_rel, res), flush=True)
+                assert st.status == 0 and res <= 1.5e-10
+                iters[(use_prec, variant)] = st.iters
+            a, c = iters[(use_prec, 5)], iters[(use_prec, default["variant"])]
+            assert abs(a - c) <= max(2, 0.02 * c)
+    finally:
+        ctx.set_schedule(0, J=default["J"], WPB=default["WPB"], variant=default["variant"])
+        ctx.deviceFree(d_f)
+        ctx.deviceFree(d_r)
+
+
 def test_copies_blas_reductions():
     P, ctx, op = setup("slab12-K1-iso")
     x, y = rvec(op, 2), rvec(op, 3)

@@ -660,7 +660,9 @@ class HipContext:
         return float(out[0])
 
     def bench_operator(self, warmup, steps, precision=0):
-        """(wall s, mean, min, max kernel ms) of the FP64 (0) or FP32 (1) operator"""
+        """(wall s, mean, min, max ms per application) of the FP64 (0) or FP32
+        (1) operator, or (2) the FP64 one on vectors in its own harmonic order
+        (the K-blocked FP64 kernel alone)"""
         out = np.zeros(4)
         check(lib.tr_bench_operator(self._h, int(precision), warmup, steps, ptr(out)))
         return tuple(out)

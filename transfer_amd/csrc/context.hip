@@ -236,6 +236,120 @@ static std::vector<unsigned char> mfma_tables5(const HostOperator &op, const std
     return out;
 }
 
+// The FP64 B blocks of variant 5 for the class order hp split into blocks
+// (h0, w): per input block kb, channel c and output block t whose couplings
+// are not all zero, 4 K steps x 64 lanes of doubles (lane l of step st: the
+// coupling of output hp[h0_t + l % 16] with input hp[h0_kb + mfma_harm(2,
+// l / 16, st)], the operand layout of v_mfma_f64_16x16x4f64 after the move
+// from the gather layout).  Volume blocks, then the boundary's.
+static std::vector<unsigned char> mfma_tables5d(const HostOperator &op, const std::vector<uint16_t> &hp,
+                                                const std::vector<uint32_t> &h0, const std::vector<uint32_t> &w,
+                                                Blk5 &b5) {
+    const uint32_t slm = op.slm, NT = (uint32_t)h0.size();
+    auto mat = [&](int sec, int c, uint32_t n, uint32_t m) -> double {
+        if (sec == 1) return op.onc[((size_t)c * slm + n) * slm + m];
+        if (c == 0) return n == m ? op.massfac[n] : 0.0;
+        return op.omc[((size_t)(c - 1) * slm + n) * slm + m];
+    };
+    b5 = Blk5();
+    b5.nb = NT;
+    b5.nsplitB = 1;
+    for (uint32_t k = 0; k < NT; k++) {
+        b5.h0[k] = h0[k];
+        b5.w[k] = w[k];
+    }
+    std::vector<unsigned char> out;
+    for (int sec = 0; sec < 2; sec++) {
+        const int nch = sec == 0 ? (int)op.NC : 3;
+        // the same threshold as the FP32 blocks: couplings below 1e-12 of
+        // the largest are rounding residues of entries that vanish by parity
+        double big = 0;
+        for (int c = 0; c < nch; c++)
+            for (uint32_t n = 0; n < slm; n++)
+                for (uint32_t m = 0; m < slm; m++) big = std::max(big, std::fabs(mat(sec, c, n, m)));
+        const double tiny = 1e-12 * big;
+        uint32_t count = 0;
+        for (uint32_t kb = 0; kb < NT; kb++) {
+            uint64_t mask = 0;
+            const uint32_t first = count;
+            for (int c = 0; c < nch; c++)
+                for (uint32_t tt = 0; tt < NT; tt++) {
+                    double blk[4 * 64];
+                    bool nz = false;
+                    for (int st = 0; st < 4; st++)
+                        for (int l = 0; l < 64; l++) {
+                            const uint32_t col = (uint32_t)(l % 16), kk = mfma_harm(2, l / 16, st);
+                            double x = 0.0;
+                            if (col < w[tt] && kk < w[kb]) x = mat(sec, c, hp[h0[tt] + col], hp[h0[kb] + kk]);
+                            if (std::fabs(x) <= tiny) x = 0.0;
+                            nz = nz || x != 0.0;
+                            blk[st * 64 + l] = x;
+                        }
+                    if (!nz) continue;
+                    mask |= 1ull << (c * NT + tt);
+                    const unsigned char *bp = reinterpret_cast<const unsigned char *>(blk);
+                    out.insert(out.end(), bp, bp + sizeof blk);
+                    count++;
+                }
+            if (sec == 0) {
+                b5.vmask[kb] = mask;
+                b5.voff[kb] = first;
+            } else {
+                b5.bmask[kb] = mask;
+                b5.boff[kb] = first;
+            }
+        }
+        (sec == 0 ? b5.nvol : b5.nbnd) = count;
+    }
+    return out;
+}
+
+// The parity-class blocks of an operator with 33..80 harmonics (at most 16
+// harmonics per block inside each class), or false when the classes do not
+// apply: batches, reflecting planes and faces off the axes index harmonics
+// in the internal order, and the blocks must number 3 to 5.
+static bool class_blocks(const tr_ctx *c, std::vector<uint16_t> &hp, std::vector<uint32_t> &h0,
+                         std::vector<uint32_t> &w) {
+    const HostOperator &op = c->op;
+    std::vector<uint32_t> sizes;
+    hp = parity_class_order(op.slm, sizes);
+    h0.clear();
+    w.clear();
+    for (int cls = 0, base = 0; cls < 4; base += (int)sizes[cls], cls++)
+        for (uint32_t k = 0; k < sizes[cls]; k += 16) {
+            h0.push_back((uint32_t)base + k);
+            w.push_back(std::min<uint32_t>(16, sizes[cls] - k));
+        }
+    return mfma5_supported(op.slm) && c->voff.empty() && op.gvert.empty() && op.mask.empty() && h0.size() >= 3 &&
+           h0.size() <= 5;
+}
+
+// FP64 variant 5 for this context where it applies: the class blocks, 15 or
+// 16 volume slots, FP64 arrays within 32-bit buffer offsets, and the stored
+// volume blocks within the LDS of one workgroup.  Leaves d_tab5d null
+// otherwise.
+static int build_tables5d(tr_ctx *c, int device) {
+    const HostOperator &op = c->op;
+    std::vector<uint16_t> hp;
+    std::vector<uint32_t> h0, w;
+    if (!class_blocks(c, hp, h0, w)) return TR_OK;
+    const double lim = 4294967295.0;
+    if ((double)op.nP * op.slm * 8 >= lim || (double)op.W * op.nP * 4 >= lim ||
+        (double)op.W * op.nP * rec_stride<double>((int)op.NC) * 8 >= lim || (double)op.WB * op.nB * 4 * 8 >= lim ||
+        (op.W != 15 && op.W != 16) || op.WB > TR_WBMAX || (op.NC != 7 && op.NC != 10))
+        return TR_OK;
+    Blk5 b5;
+    std::vector<unsigned char> tab = mfma_tables5d(op, hp, h0, w, b5);
+    int maxlds = 65536;
+    hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    if (apply5d_lds_bytes(b5) + 64 > (size_t)maxlds) return TR_OK;
+    int rc;
+    if ((rc = upload(c, (unsigned char **)&c->d_tab5d, tab)) || (rc = upload(c, &c->d_hp64, hp))) return rc;
+    c->b5d = b5;
+    c->hp64 = hp;
+    return TR_OK;
+}
+
 // Variant 5's FP32 order and blocks for this context: the parity classes
 // (blocks of at most 16 inside each class) for a single problem without
 // reflecting planes or faces off the axes (their kernels index harmonics in
@@ -245,14 +359,9 @@ static std::vector<unsigned char> mfma_tables5(const HostOperator &op, const std
 static int build_tables5(tr_ctx *c) {
     const HostOperator &op = c->op;
     const uint32_t slm = op.slm;
-    std::vector<uint32_t> sizes, h0, w;
-    std::vector<uint16_t> hp = parity_class_order(slm, sizes);
-    for (int cls = 0, base = 0; cls < 4; base += (int)sizes[cls], cls++)
-        for (uint32_t k = 0; k < sizes[cls]; k += 16) {
-            h0.push_back((uint32_t)base + k);
-            w.push_back(std::min<uint32_t>(16, sizes[cls] - k));
-        }
-    const bool classes = c->voff.empty() && op.gvert.empty() && op.mask.empty() && h0.size() >= 3 && h0.size() <= 5;
+    std::vector<uint32_t> h0, w;
+    std::vector<uint16_t> hp;
+    const bool classes = class_blocks(c, hp, h0, w);
     if (!classes) {
         hp.resize(slm);
         for (uint32_t k = 0; k < slm; k++) hp[k] = (uint16_t)k;
@@ -293,6 +402,28 @@ static LaneMap mfma5_map_ctx(const tr_ctx *c, int wpb) {
     m.grid_dot = apply5_grid(c->args32(m), NC, wpb, 1);
     m.grid_cheb = apply5_grid(c->args32(m), NC, wpb, 2);
     return m;
+}
+
+// The FP64 K-blocked schedule (variant 5) with its persistent grids; the
+// scratch vector of internal-order inputs is allocated on first use.
+static int mfma5d_map_ctx(tr_ctx *c, int wpb, LaneMap &m) {
+    if (!c->d_tab5d) return fail(TR_ERR_ARG, "FP64 K-blocked matrix-core schedule unsupported here");
+    if (!c->d_cls) {
+        HIPCHECK(hipMalloc((void **)&c->d_cls, sizeof(double) * std::max<size_t>(c->nint(), 1)));
+        c->owned.push_back(c->d_cls);
+    }
+    m = LaneMap();
+    m.variant = 5;
+    m.WPB = wpb;
+    m.J = 4;
+    m.L = 4;
+    m.VPW = 16;
+    m.mfma_mode = 0;
+    const int NC = (int)c->op.NC;
+    m.grid = apply5d_grid(c->args(m), NC, wpb, 0);
+    m.grid_dot = apply5d_grid(c->args(m), NC, wpb, 1);
+    m.grid_cheb = apply5d_grid(c->args(m), NC, wpb, 2);
+    return TR_OK;
 }
 
 // The matrix-core schedule of this context with its persistent grid sized
@@ -502,7 +633,8 @@ static int lhs(tr_ctx *c, const double *f, double *r) {
 }
 
 static int lhs_diag(tr_ctx *c, const double *f, double *r) {
-    HIPCHECK(launch_apply_diag<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream));
+    // the angular-diagonal operator has VALU schedules only
+    HIPCHECK(launch_apply_diag<double>(c->args(c->vmap), (int)c->op.NC, c->vmap, f, r, nullptr, nullptr, c->stream));
     return TR_OK;
 }
 
@@ -573,6 +705,7 @@ int tr::ctx_init(tr_ctx *c, int device) {
         tr_ctx_destroy(c);
         return fail(TR_ERR_ARG, "tr_ctx_create: angular order too large for LDS staging");
     }
+    c->vmap = c->map;
     const HostOperator &op = c->op;
     std::vector<double> stage;
     if ((rc = upload(c, &c->d_nbr, op.nbr)) || (rc = upload(c, &c->d_coef, op.coef)) ||
@@ -596,7 +729,8 @@ int tr::ctx_init(tr_ctx *c, int device) {
           (rc = upload(c, &c->d_gR, op.gR)) || (rc = upload(c, &c->d_gOd, op.gOd)) ||
           (rc = upload(c, &c->d_gOz, op.gOz)))) ||
         (rc = upload(c, (unsigned char **)&c->d_tab, table_records<double>(op))) ||
-        (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1))))) {
+        (mfma_supported(op.slm, 0) && (rc = upload(c, (unsigned char **)&c->d_tab4, mfma_tables(op, 0, 1)))) ||
+        (rc = build_tables5d(c, device))) {
         tr_ctx_destroy(c);
         return rc;
     }
@@ -1103,17 +1237,30 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     if (prm->use_prec == 5 && !prm->x0)
         return fail(TR_ERR_ARG, "tr_solve: use_prec 5 (learned coarse correction) takes the learned guess in f "
                                 "with x0 = 1");
+    // Under the FP64 K-blocked schedule the solve runs on vectors in its
+    // harmonic order (the parity classes), so that no application permutes:
+    // b and the start in, the solution out.  The multilevel and learned
+    // preconditioners index harmonics in the internal order: they keep it.
+    const uint16_t *hp = (prm->use_prec == 0 || prm->use_prec == 2 || prm->use_prec == 4) ? c->perm64() : nullptr;
     if (prm->use_prec == 2 || prm->use_prec == 4 || prm->use_prec == 5) {
-        std::vector<double> d = operator_diagonal(c->op);
+        std::vector<double> d = operator_diagonal(c->op), dp;
         for (double &x : d) x = 1.0 / x;
+        if (hp) {
+            const uint32_t slm = c->op.slm;
+            dp.resize(n);
+            for (size_t v = 0; v < c->op.nP; v++)
+                for (uint32_t k = 0; k < slm; k++) dp[v * slm + k] = d[v * slm + c->hp64[k]];
+        }
         dinv = w.get();
         if (!dinv) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
-        HIPCHECK(hipMemcpyAsync(dinv, d.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-        // d is a temporary: the copy must have read it before it goes
+        HIPCHECK(hipMemcpyAsync(dinv, hp ? dp.data() : d.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                                c->stream));
+        // d and dp are temporaries: the copy must have read them before they go
         HIPCHECK(hipStreamSynchronize(c->stream));
         if (prm->use_prec == 4) {
+            // lambda_max by the power iteration on the internal order: d, not dp
             if ((rc = cheb_lambda(c, d))) return rc;
-            cheb.a = c->args();
+            cheb.a = hp ? c->args_cls() : c->args();
             cheb.NC = (int)c->op.NC;
             cheb.map = c->map;
             cheb.dinv = dinv;
@@ -1171,7 +1318,7 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
         S.pre_scale = cheb.inv_theta;
         S.prec_nparts = cheb.nparts();
     }
-    S.a = c->args();
+    S.a = hp ? c->args_cls() : c->args();
     S.NC = (int)c->op.NC;
     S.map = c->map;
     S.n = n;
@@ -1184,6 +1331,13 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     S.part = part;
     S.part2 = part2;
     S.sc = sc;
+    if (hp) {
+        // r and Ap are free until the solve: the start and b in the class order
+        HIPCHECK(launch_perm_harm(f, r, hp, c->op.slm, n, false, nullptr, c->stream));
+        HIPCHECK(launch_perm_harm(b, Ap, hp, c->op.slm, n, false, nullptr, c->stream));
+        HIPCHECK(hipMemcpyAsync(f, r, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(hipMemcpyAsync(b, Ap, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
     CgParams cp;
     cp.maxit = prm->maxit;
     cp.tol_abs = prm->tol_abs;
@@ -1191,6 +1345,13 @@ static int solve_device(tr_ctx *c, const tr_solve_params *prm, double *f, tr_sol
     cp.chunk = 16;
     CgResult res;
     if ((rc = pcg_device<double>(S, cp, c->stream, res))) return rc;
+    if (hp) {
+        // the solution and b back in the internal order
+        HIPCHECK(launch_perm_harm(f, r, hp, c->op.slm, n, true, nullptr, c->stream));
+        HIPCHECK(launch_perm_harm(b, Ap, hp, c->op.slm, n, true, nullptr, c->stream));
+        HIPCHECK(hipMemcpyAsync(f, r, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(hipMemcpyAsync(b, Ap, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
     if ((rc = lhs(c, f, Ap)) || (rc = residual(c, r, b, Ap))) return rc;
     double trn = std::sqrt(dot(c, r, r)), bn = std::sqrt(res.bb);
     if (st) {
@@ -1531,7 +1692,7 @@ double tr_bench_lhs(tr_ctx *c, int reps) {
 }  // extern "C"
 
 extern "C" int tr_bench_operator(tr_ctx *c, int precision, int warmup, int steps, double *out) {
-    if (!c || !out || steps < 1 || warmup < 0 || precision < 0 || precision > 1)
+    if (!c || !out || steps < 1 || warmup < 0 || precision < 0 || precision > 2)
         return fail(TR_ERR_ARG, "tr_bench_operator: bad argument");
     int rc;
     if (precision == 1 && (rc = ensure_fp32(c))) return rc;
@@ -1546,7 +1707,10 @@ extern "C" int tr_bench_operator(tr_ctx *c, int precision, int warmup, int steps
     auto apply = [&]() -> hipError_t {
         if (precision == 1)
             return launch_apply<float>(c->args32(), (int)c->op.NC, c->map32, f32, r32, nullptr, nullptr, c->stream);
-        return launch_apply<double>(c->args(), (int)c->op.NC, c->map, f, r, nullptr, nullptr, c->stream);
+        // precision 2: the FP64 operator on vectors in its own harmonic
+        // order (the K-blocked schedule's kernel alone, as its solves run it)
+        return launch_apply<double>(precision == 2 ? c->args_cls() : c->args(), (int)c->op.NC, c->map, f, r,
+                                    nullptr, nullptr, c->stream);
     };
     for (int i = 0; i < warmup; i++) HIPCHECK(apply());
     std::vector<hipEvent_t> ev(2 * (size_t)steps);
@@ -1588,8 +1752,10 @@ extern "C" int tr_ctx_kernel_name(tr_ctx *c, int precision, char *buf, size_t ca
     const int NC = (int)c->op.NC, W = (int)c->op.W;
     std::string name;
     char tmp[160];
-    if (m.variant == 5)
-        snprintf(tmp, sizeof tmp, "k_apply5<%d, %d, %d, 0>", NC, W, (int)((c->op.slm + 15) / 16));
+    if (m.variant == 5 && precision == 0)
+        snprintf(tmp, sizeof tmp, "k_apply5d<%d, %d, %d, %d, 0>", NC, W, (int)c->b5d.nb, NC == 10 ? 5 : 7);
+    else if (m.variant == 5)
+        snprintf(tmp, sizeof tmp, "k_apply5<%d, %d, %d, 0>", NC, W, (int)c->b5.nb);
     else if (m.variant == 4)
         snprintf(tmp, sizeof tmp, "k_apply4<%s, %d, %d, %d, %d, 0>", T, precision ? m.mfma_mode : 0, m.J, NC, W);
     else if (m.variant == 3 && (W == 15 || W == 16))
@@ -1693,6 +1859,16 @@ static int autotune(tr_ctx *c, int precision, std::string &report) {
         }
     if (bf16 && v5_ok(op))
         for (int wpb : {2, 4, 6, 8}) cands.push_back(mfma5_map_ctx(c, wpb));
+    if (!f32 && c->d_tab5d) {
+        // one 512-lane workgroup per CU (the FP64 blocks' LDS): WPB 8 only
+        LaneMap m;
+        if ((rc = mfma5d_map_ctx(c, 8, m))) {
+            hipFree(x);
+            hipFree(y);
+            return rc;
+        }
+        cands.push_back(m);
+    }
     {
         for (LaneMap m : cands) {
             {
@@ -1782,7 +1958,16 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
     }
     const HostOperator &op = c->op;
     if (variant == 5) {
-        if (precision != 1 || c->ang_prec == 0 || !v5_ok(op) || WPB < 1 || WPB > 8)
+        if (precision == 0) {
+            if (!c->d_tab5d || WPB != 8)
+                return fail(TR_ERR_ARG, "tr_ctx_set_schedule: FP64 K-blocked matrix-core schedule unsupported here");
+            LaneMap m;
+            int rc = mfma5d_map_ctx(c, WPB, m);
+            if (rc) return rc;
+            c->map = m;
+            return TR_OK;
+        }
+        if (c->ang_prec == 0 || !v5_ok(op) || WPB < 1 || WPB > 8)
             return fail(TR_ERR_ARG, "tr_ctx_set_schedule: K-blocked matrix-core schedule unsupported here");
         c->map32 = mfma5_map_ctx(c, WPB);
         return TR_OK;

@@ -72,6 +72,17 @@ struct tr_ctx {
     tr::Blk5 b5;
     std::vector<uint16_t> hp32;
     uint16_t *d_hp32 = nullptr;
+    // FP64 variant 5 (the parity classes on the FP64 matrix cores), built
+    // at context creation where it applies (v5d_ok): its blocks, B tables,
+    // class order hp64 (device copy d_hp64) and d_cls, the scratch vector
+    // that holds an internal-order input in the class order; vmap is the
+    // VALU schedule the angular-diagonal operator keeps
+    tr::Blk5 b5d;
+    void *d_tab5d = nullptr;
+    std::vector<uint16_t> hp64;
+    uint16_t *d_hp64 = nullptr;
+    double *d_cls = nullptr;
+    tr::LaneMap vmap;
     int ang_prec = 3;  // FP32 operator's angular contraction: 0 FP32, 1 BF16, 3 BF16 hi/lo split
     std::set<void *> vectors, raw;
     // batches: internal vertex range of problem p = voff[p] .. voff[p+1]
@@ -95,6 +106,17 @@ struct tr_ctx {
     std::vector<void *> owned;
 
     tr::OpArgs<double> args() const { return args(map); }
+    // the FP64 operator on vectors already in its class order (variant 5;
+    // else the same as args())
+    tr::OpArgs<double> args_cls() const {
+        tr::OpArgs<double> a = args(map);
+        a.hpo = nullptr;
+        a.cls = nullptr;
+        return a;
+    }
+    // the FP64 vectors' harmonic order inside class-order solves (null:
+    // the internal order)
+    const uint16_t *perm64() const { return map.variant == 5 && !hp64.empty() ? d_hp64 : nullptr; }
     tr::OpArgs<double> args(const tr::LaneMap &map) const {
         tr::OpArgs<double> a;
         a.nP = op.nP;
@@ -119,8 +141,13 @@ struct tr_ctx {
         a.bval = d_bval;
         a.bdiag = d_bdiag;
         a.crec = d_crec;
-        a.tab = map.variant == 4 ? d_tab4 : d_tab;
+        a.tab = map.variant == 4 ? d_tab4 : (map.variant == 5 ? d_tab5d : d_tab);
         a.nsplit = 1;
+        if (map.variant == 5) {
+            a.b5 = b5d;
+            a.hpo = d_hp64;
+            a.cls = d_cls;
+        }
         a.bcrec = d_bcrec;
         a.mask = d_mask;
         a.nmask = (uint32_t)op.mask.size();

@@ -77,6 +77,12 @@ struct OpArgs {
     const uint32_t *gvert = nullptr, *gstart = nullptr, *gface = nullptr, *gpos = nullptr, *gfv = nullptr;
     const T *garea = nullptr, *gR = nullptr, *gOd = nullptr, *gOz = nullptr;
     Blk5 b5;               // variant 5 only
+    // FP64 variant 5 on vectors in the internal order: cls is a scratch
+    // vector for the input in the class order and hpo[k] the internal
+    // harmonic of class position k; both null when the vectors are in the
+    // class order already
+    const uint16_t *hpo = nullptr;
+    T *cls = nullptr;
 };
 
 // One step of the Chebyshev iteration on D^-1 A z = D^-1 rhs, written as
@@ -171,6 +177,17 @@ size_t apply5_lds_bytes(const Blk5 &b5);
 uint32_t apply5_grid(const OpArgs<float> &a, int NC, int wpb, int epi);
 hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid, const float *f, float *r,
                          double *part, const double *skip, hipStream_t s, const ChebStep<float> *cheb = nullptr);
+// variant 5 in FP64: the same blocks in the parity classes, contracted on
+// v_mfma_f64_16x16x4f64; tab = FP64 B blocks (2 KB each, volume then
+// boundary); a.cls / a.hpo as in OpArgs
+size_t apply5d_lds_bytes(const Blk5 &b5);
+uint32_t apply5d_grid(const OpArgs<double> &a, int NC, int wpb, int epi);
+hipError_t launch_apply5d(const OpArgs<double> &a, int NC, int wpb, uint32_t grid, const double *f, double *r,
+                          double *part, const double *skip, hipStream_t s, const ChebStep<double> *cheb = nullptr);
+// dst[v][k] = src[v][hp[k]] over the n = nP slm entries (inverse: dst[v][hp[k]]
+// = src[v][k]); nothing when *skip != 0
+hipError_t launch_perm_harm(const double *src, double *dst, const uint16_t *hp, uint32_t slm, size_t n,
+                            bool inverse, const double *skip, hipStream_t s);
 // workgroups of the persistent variant-4 grid for this operator and WPB,
 // for the launch with epilogue epi (0 plain, 1 fused dot product, 2
 // Chebyshev step)

@@ -602,7 +602,9 @@ hipError_t launch_apply(const OpArgs<T> &a, int NC, const LaneMap &m, const T *f
     if (part && a.ngv) return hipErrorInvalidValue;
     if (m.variant == 5) {
         if constexpr (sizeof(T) == 8) {
-            return hipErrorInvalidValue;
+            hipError_t e = launch_apply5d(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
+            if (e == hipSuccess) e = launch_bgen(a, false, f, r, skip, s);
+            return e != hipSuccess ? e : launch_zero_mask(a, r, skip, s);
         } else {
             hipError_t e = launch_apply5(a, NC, m.WPB, part ? m.grid_dot : m.grid, f, r, part, skip, s);
             if (e == hipSuccess) e = launch_bgen(a, false, f, r, skip, s);
@@ -822,7 +824,9 @@ hipError_t launch_cheb_step(const OpArgs<T> &a, int NC, const LaneMap &m, const 
     if (part && a.ngv) return hipErrorInvalidValue;
     if (m.variant == 5) {
         if constexpr (sizeof(T) == 8) {
-            return hipErrorInvalidValue;
+            e = launch_apply5d(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
+            if (e == hipSuccess) e = launch_bgen(a, false, z, out, skip, s, &cs);
+            return e != hipSuccess ? e : launch_zero_mask(a, out, skip, s);
         } else {
             e = launch_apply5(a, NC, m.WPB, m.grid_cheb, z, out, part, skip, s, &cs);
             if (e == hipSuccess) e = launch_bgen(a, false, z, out, skip, s, &cs);

@@ -800,6 +800,310 @@ __global__ __launch_bounds__(512) void k_apply5(OpArgs<float> a, const float *__
     }
 }
 
+// The FP64 contraction of one channel group (channels c0 .. c0 + CG - 1 of
+// the operator) against the stored FP64 B blocks: block idx holds, for K
+// step st and lane l, the coupling of output column h0_t + l % 16 with input
+// harmonic h0_kb + mfma_harm(2, l / 16, st), 2 KB per block.
+template <int CG, int NT>
+__device__ __forceinline__ void contract5d(const double (&acc)[CG][4], const unsigned char *B, int c0,
+                                           uint64_t mask, uint32_t off, int lane, d4v (&D)[NT]) {
+    constexpr size_t BLK = (size_t)TR_WAVE * 8 * 4;
+#pragma unroll
+    for (int cc = 0; cc < CG; cc++) {
+        const int c = c0 + cc;
+        if (!((mask >> (c * NT)) & ((1ull << NT) - 1))) continue;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const uint32_t bit = (uint32_t)(c * NT + t);
+            if (!((mask >> bit) & 1ull)) continue;
+            const uint32_t idx = off + (uint32_t)__builtin_popcountll(mask & ((1ull << bit) - 1ull));
+            const double *bp = reinterpret_cast<const double *>(B + (size_t)idx * BLK) + lane;
+#pragma unroll
+            for (int st = 0; st < 4; st++)
+                D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[cc][st], bp[st * TR_WAVE], D[t], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Variant 5 in FP64: the operator for 33 to 80 harmonics with its angular
+// contraction on the FP64 matrix cores (v_mfma_f64_16x16x4f64, four K steps
+// per 16-harmonic block), in the reflection-parity classes only (the stored
+// blocks, 2 KB each, must fit the LDS: 56 at K = 5 with ten channels).
+// The gather reads the vector fg, whose harmonics are in the class order;
+// the results go to position hpo[n] of the output row for class position
+// n (hpo null: the class order too).  The epilogue's reads (f, and the
+// Chebyshev step's vectors) are in the output order.  FP64 records and
+// accumulators need twice the registers of FP32, so the ten channels run
+// in groups of CG (two groups of five; seven channels in one), each group
+// with its quarter-records in registers across the harmonic blocks and the
+// neighbour rows gathered once per group.  EPI as k_apply4.
+template <int NC, int WT, int NT, int CG, int EPI>
+__global__ __launch_bounds__(512) void k_apply5d(OpArgs<double> a, const double *__restrict__ fg,
+                                                 const double *__restrict__ f, double *__restrict__ r,
+                                                 double *__restrict__ part, const double *__restrict__ skip,
+                                                 ChebStep<double> cs) {
+    constexpr bool DOT = EPI == 1 || EPI == 3;
+    static_assert(NC % CG == 0, "channel groups");
+    constexpr int NG = NC / CG;
+    if (skip && *skip != 0.0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int RS = rec_stride<double>(NC);
+    constexpr int J = 4;
+    constexpr int PE = (CG + 3) / 4;
+    const int lane = threadIdx.x & (TR_WAVE - 1), wave = threadIdx.x / TR_WAVE;
+    const int wpb = blockDim.x / TR_WAVE;
+    const uint32_t slm = a.slm, nP = a.nP;
+    const uint32_t vsec = a.b5.nvol * (uint32_t)(TR_WAVE * 8 * 4);
+    {
+        const i32x4m *src = reinterpret_cast<const i32x4m *>(a.tab);
+        i32x4m *dst = reinterpret_cast<i32x4m *>(smem_raw);
+        for (uint32_t i = threadIdx.x; i < vsec / 16u; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const unsigned char *gB = reinterpret_cast<const unsigned char *>(a.tab) + vsec;
+    const auto frs = rsrc4(fg, (size_t)nP * slm * sizeof(double));
+    const auto brs = rsrc4(a.bof, (size_t)nP * 4);
+    const auto nrs = rsrc4(a.nbr, (size_t)a.W * nP * 4);
+    const auto bnrs = rsrc4(a.bnbr, (size_t)TR_WBMAX * a.nB * 4);
+    const auto crs = rsrc4(a.crec, (size_t)a.W * nP * RS * sizeof(double));
+    const auto bcrs = rsrc4(a.bcrec, (size_t)TR_WBMAX * a.nB * 4 * sizeof(double));
+    const int vloc = lane & 15, q = lane >> 4;
+    const int gv = lane >> 2, gq = lane & 3;
+    // two wavefronts per SIMD (the LDS holds one 512-lane workgroup per CU):
+    // 256 registers.  Three slots in flight is the most that compiles
+    // without spills (the compiler's resource report: 4 spills 2 to 8
+    // registers of the ten-channel kernels)
+    constexpr int DEPTH = 3;
+    constexpr int DEPTHB = 4;
+
+    const uint32_t ntiles = (nP + 16u * wpb - 1) / (16u * wpb);
+    const uint32_t G = gridDim.x, X = 8;
+    uint32_t t0 = blockIdx.x, t1 = ntiles, tstep = G;
+    if (G % X == 0 && G <= ntiles) {
+        const uint32_t xcd = blockIdx.x % X, per = ntiles / X, rem = ntiles % X;
+        const uint32_t lo = xcd * per + (xcd < rem ? xcd : rem);
+        t0 = lo + blockIdx.x / X;
+        t1 = lo + per + (xcd < rem ? 1u : 0u);
+        tstep = G / X;
+    }
+    const uint32_t nwt = (nP + 15u) / 16u;
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane(wave);
+    double d = 0;
+    for (uint32_t tile = t0; tile < t1; tile += tstep) {
+        const uint32_t e = tile * (uint32_t)wpb + wv;
+        const uint32_t v0 = 16u * ((a.tiles && e < nwt) ? a.tiles[e] : e);
+        uint32_t v = v0 + gv;
+        asm volatile("" : "+v"(v));
+        const bool valid = v < nP;
+        uint32_t nbc[WT];
+        int b = -1;
+        if (valid) {
+            load_idx<WT>(nbc, nrs, v * 4u, nP * 4u, WT);
+            b = (int)__builtin_amdgcn_raw_buffer_load_b32(brs, v * 4u, 0u, 0);
+        } else {
+#pragma unroll
+            for (int s2 = 0; s2 < WT; s2++) nbc[s2] = 0;
+        }
+        d4v D[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) D[t] = (d4v){0, 0, 0, 0};
+        // the groups unrolled: with g a run-time value the channels' mask
+        // bits and block indices are too, and the compiler keeps them in
+        // SGPRs (about a hundred spilled)
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            // lane q's share of the group's channels in every slot's record
+            // (elements g CG + q PE .. + PE - 1; those past the group belong
+            // to no channel the group uses)
+            double cf[WT][PE];
+            {
+                uint32_t cv = v * (uint32_t)(RS * sizeof(double)) +
+                              (uint32_t)((g * CG + gq * PE) * (int)sizeof(double));
+                const uint32_t cstride = opaque_uniform(nP * (uint32_t)(RS * sizeof(double)));
+                asm volatile("" : "+v"(cv));
+#pragma unroll
+                for (int s2 = 0; s2 < WT; s2++)
+#pragma unroll
+                    for (int e2 = 0; e2 < PE; e2++)
+                        cf[s2][e2] = (valid && gq * PE + e2 < CG)
+                                         ? bload1<double>(crs, cv + (uint32_t)(e2 * (int)sizeof(double)),
+                                                          (uint32_t)s2 * cstride)
+                                         : 0.0;
+            }
+#pragma unroll 1
+            for (int kb = 0; kb < NT; kb++) {
+                double acc[CG][J];
+#pragma unroll
+                for (int c = 0; c < CG; c++)
+#pragma unroll
+                    for (int j = 0; j < J; j++) acc[c][j] = 0;
+                const uint32_t h0 = a.b5.h0[kb], hw = a.b5.w[kb];
+                if (valid) gather_pre<double, J, CG, PE, WT, DEPTH>(acc, nbc, frs, cf, slm, gq, 8u * h0, hw);
+                // per-vertex factor of the mass channel (channel 0) at
+                // harmonic 0 (register 0 of chunk 0 in block 0)
+                if (a.mf0 && valid && gq == 0 && kb == 0 && g == 0) acc[0][0] *= a.mf0[v];
+                to_mfma_layout<double, 0, J, CG>(acc, lane);
+                contract5d<CG, NT>(acc, smem_raw, g * CG, a.b5.vmask[kb], a.b5.voff[kb], lane, D);
+            }
+        }
+        if (__any(b >= 0)) {
+            uint32_t bn[TR_WBMAX];
+            if (b >= 0) load_idx<TR_WBMAX>(bn, bnrs, (uint32_t)b * 4u, a.nB * 4u, TR_WBMAX);
+#pragma unroll 1
+            for (int kb = 0; kb < NT; kb++) {
+                if (!a.b5.bmask[kb]) continue;
+                double bacc[3][J];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int j = 0; j < J; j++) bacc[c][j] = 0;
+                if (b >= 0)
+                    gather<double, J, 3, 4, TR_WBMAX, DEPTHB>(bacc, bn, frs, bcrs,
+                                                              a.nB * (uint32_t)(4 * sizeof(double)),
+                                                              (uint32_t)b * (uint32_t)(4 * sizeof(double)), TR_WBMAX,
+                                                              slm, gq, 8u * a.b5.h0[kb], 0u, a.b5.w[kb]);
+                to_mfma_layout<double, 0, J, 3>(bacc, lane);
+                contract5d<3, NT>(bacc, gB, 0, a.b5.bmask[kb], a.b5.boff[kb], lane, D);
+            }
+        }
+        // ---- write: D rows (l/16) + 4i, column l % 16 of output block t
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const uint32_t n = a.b5.h0[t] + (uint32_t)vloc;
+            const bool col_ok = (uint32_t)vloc < a.b5.w[t];
+            const uint32_t no = (col_ok && a.hpo) ? (uint32_t)a.hpo[n] : n;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t vr = v0 + (uint32_t)(q + 4 * i);
+                if (col_ok && vr < nP) {
+                    const double val = D[t][i];
+                    const size_t o = (size_t)vr * slm + no;
+                    if constexpr (EPI >= 2) {
+                        const double rh = cs.rhs[o];
+                        double nv = cs.zc * f[o] + cs.ga * cs.dinv[o] * (rh - val);
+                        if (cs.om != 0.0) nv -= cs.om * r[o];
+                        r[o] = nv;
+                        if (EPI == 3) d = fma(rh, nv, d);
+                    } else {
+                        r[o] = val;
+                        if (DOT) d = fma(f[o], val, d);
+                    }
+                }
+            }
+        }
+    }
+    if (DOT) {
+        double *w = reinterpret_cast<double *>(smem_raw);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, TR_WAVE);
+        __syncthreads();
+        if (lane == 0) w[wave] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (int i = 0; i < wpb; i++) t += w[i];
+            part[blockIdx.x] = t;
+        }
+    }
+}
+
+// dst[v][k] = src[v][hp[k]] (inverse: dst[v][hp[k]] = src[v][k]), the
+// harmonics of every row of n = nP slm entries
+__global__ __launch_bounds__(256) void k_perm_harm(const double *__restrict__ src, double *__restrict__ dst,
+                                                   const uint16_t *__restrict__ hp, uint32_t slm, size_t n,
+                                                   int inverse, const double *__restrict__ skip) {
+    if (skip && *skip != 0.0) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t v = i / slm, k = i - v * slm;
+        const size_t j = v * slm + hp[k];
+        if (inverse)
+            dst[j] = src[i];
+        else
+            dst[i] = src[j];
+    }
+}
+
+hipError_t launch_perm_harm(const double *src, double *dst, const uint16_t *hp, uint32_t slm, size_t n,
+                            bool inverse, const double *skip, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (!src || !dst || !hp || src == dst || slm == 0) return hipErrorInvalidValue;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_perm_harm, dim3(g), dim3(256), 0, s, src, dst, hp, slm, n, inverse ? 1 : 0, skip);
+    return hipGetLastError();
+}
+
+// LDS of the FP64 variant 5: the stored volume blocks, 2 KB each
+size_t apply5d_lds_bytes(const Blk5 &b5) { return std::max<size_t>((size_t)b5.nvol * 2048, 64); }
+
+template <int EPI>
+static const void *apply5d_kernel(int NC, uint32_t W, uint32_t NT) {
+#define TR_K5D(NCC, WW, NTT) reinterpret_cast<const void *>(&k_apply5d<NCC, WW, NTT, (NCC == 10 ? 5 : 7), EPI>)
+#define TR_K5DT(NCC, WW) (NT == 3 ? TR_K5D(NCC, WW, 3) : (NT == 4 ? TR_K5D(NCC, WW, 4) : TR_K5D(NCC, WW, 5)))
+#define TR_K5DW(NCC) (W == 15 ? TR_K5DT(NCC, 15) : TR_K5DT(NCC, 16))
+    if ((W != 15 && W != 16) || (NC != 7 && NC != 10) || NT < 3 || NT > 5) return nullptr;
+    return NC == 7 ? TR_K5DW(7) : TR_K5DW(10);
+#undef TR_K5DW
+#undef TR_K5DT
+#undef TR_K5D
+}
+
+static const void *apply5d_kernel_epi(int epi, int NC, uint32_t W, uint32_t NT) {
+    switch (epi) {
+        case 1: return apply5d_kernel<1>(NC, W, NT);
+        case 2: return apply5d_kernel<2>(NC, W, NT);
+        case 3: return apply5d_kernel<3>(NC, W, NT);
+        default: return apply5d_kernel<0>(NC, W, NT);
+    }
+}
+
+uint32_t apply5d_grid(const OpArgs<double> &a, int NC, int wpb, int epi) {
+    const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
+    const void *k = apply5d_kernel_epi(epi, NC, a.W, a.b5.nb);
+    if (!k || ntiles == 0) return ntiles;
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, TR_WAVE * wpb, apply5d_lds_bytes(a.b5)) != hipSuccess ||
+        nb < 1 || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        return ntiles;
+    }
+    uint32_t g = (uint32_t)nb * (uint32_t)cus;
+    if (g >= 8) g -= g % 8;
+    return std::min(g, ntiles);
+}
+
+// The FP64 variant 5 on vectors in the internal (lm) order when a.cls is
+// set: f is first permuted into the class order in a.cls (a scratch vector
+// of the context) and the results written back through a.hpo; with a.cls
+// null the vectors are in the class order already.
+hipError_t launch_apply5d(const OpArgs<double> &a, int NC, int wpb, uint32_t grid, const double *f, double *r,
+                          double *part, const double *skip, hipStream_t s, const ChebStep<double> *cheb) {
+    if (a.nP == 0) return hipSuccess;
+    const uint32_t per = 16u * (uint32_t)wpb, ntiles = (a.nP + per - 1) / per;
+    bool blocks_ok = a.b5.nb >= 3 && a.b5.nb <= 5;
+    for (uint32_t k = 0; blocks_ok && k < a.b5.nb; k++)
+        blocks_ok = a.b5.w[k] >= 1 && a.b5.w[k] <= 16 && a.b5.h0[k] + a.b5.w[k] <= a.slm;
+    if ((a.W != 15 && a.W != 16) || a.WB > TR_WBMAX || !a.crec || !a.tab || (a.nB && !a.bcrec) || wpb < 1 ||
+        wpb > 8 || !mfma5_supported(a.slm) || !blocks_ok || (a.cls && !a.hpo) || (!a.cls && a.hpo) ||
+        (a.cls && (a.cls == f || a.cls == r)))
+        return hipErrorInvalidValue;
+    if (cheb && (!cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
+    const void *k = apply5d_kernel_epi(cheb ? (part ? 3 : 2) : (part ? 1 : 0), NC, a.W, a.b5.nb);
+    if (!k) return hipErrorInvalidValue;
+    const double *fg = f;
+    if (a.cls) {
+        hipError_t e = launch_perm_harm(f, a.cls, a.hpo, a.slm, (size_t)a.nP * a.slm, false, skip, s);
+        if (e != hipSuccess) return e;
+        fg = a.cls;
+    }
+    dim3 g(grid ? std::min(grid, ntiles) : ntiles), b(TR_WAVE * wpb);
+    OpArgs<double> aa = a;
+    ChebStep<double> cs = cheb ? *cheb : ChebStep<double>{nullptr, nullptr, 0.0, 0.0, 0.0};
+    void *args[] = {&aa, &fg, &f, &r, &part, &skip, &cs};
+    return hipLaunchKernel(k, g, b, args, apply5d_lds_bytes(a.b5), s);
+}
+
 bool mfma5_supported(uint32_t slm) { return slm > 32 && slm <= 80; }
 
 // LDS of variant 5: the stored volume B blocks (hi, and lo with nsplitB 2)
