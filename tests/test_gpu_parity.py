@@ -181,7 +181,9 @@ def test_bf16_blocked_projection(name):
     projection in blocks of at most 16 harmonics (variant 5) against the
     FP64 oracle.  Plain BF16 (gathered values and couplings rounded to 8
     significant bits: bar 2e-2 relative to max|A f|, observed 3e-3) or the
-    hi/lo split of both (bar 2e-4, observed 6e-6); every workgroup size."""
+    hi/lo split of both (bar 2e-4, observed 6e-6); every workgroup size,
+    under the static class layout (straight-line contraction, k_apply5s: the
+    default where it applies) and the masked block layout (k_apply5)."""
     if name in cases():
         P, ctx, op = setup(name)
     else:
@@ -192,16 +194,21 @@ def test_bf16_blocked_projection(name):
     d_f, d_r = ctx.vector_from(f), ctx.allocVector()
     want = op.lhs(f)
     try:
-        for prec, bar in ((1, 2e-2), (3, 2e-4)):
-            ctx.set_angular_precision(prec)
-            for wpb in (2, 4, 8):
-                ctx.set_schedule(1, J=0, WPB=wpb, variant=5)
-                ctx.computeLhsPrec(1, d_f, d_r)
-                e = rel(ctx.copyToHost(d_r), want)
-                print("[gpu-parity] %s bf16-blocked prec=%d WPB=%d rel err %.2e (%s)" % (
-                    name, prec, wpb, e, ctx.kernel_name(1)), flush=True)
-                assert e <= bar
+        for layout in (0, 1):
+            for prec, bar in ((1, 2e-2), (3, 2e-4)):
+                ctx.set_angular_precision(prec)
+                for wpb in (2, 4, 8):
+                    ctx.set_schedule(1, J=layout, WPB=wpb, variant=5)
+                    kname = ctx.kernel_name(1)
+                    # the classes' static layout covers K = 4, 5 (seven or ten channels)
+                    assert kname.startswith("k_apply5s<" if layout == 0 else "k_apply5<"), kname
+                    ctx.computeLhsPrec(1, d_f, d_r)
+                    e = rel(ctx.copyToHost(d_r), want)
+                    print("[gpu-parity] %s bf16-blocked prec=%d WPB=%d rel err %.2e (%s)" % (
+                        name, prec, wpb, e, kname), flush=True)
+                    assert e <= bar
     finally:
+        ctx.set_schedule(1, J=0, WPB=8, variant=5)
         ctx.set_angular_precision(3)
         ctx.deviceFree(d_f)
         ctx.deviceFree(d_r)

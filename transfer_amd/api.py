@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtransfer_amd.so")
+# TR_LIBRARY: another build of the library (same ABI), e.g. a variant under test
+LIB_PATH = os.environ.get("TR_LIBRARY") or os.path.join(_HERE, "libtransfer_amd.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("transfer_amd: %s is missing; run `python transfer_amd/build.py`" % LIB_PATH)

@@ -236,6 +236,73 @@ static std::vector<unsigned char> mfma_tables5(const HostOperator &op, const std
     return out;
 }
 
+// The B blocks of the static class layout sl (kernels.h) for the class
+// order hp: every pair of the layout in cl_vol_index / cl_bnd_index order,
+// zero blocks included, hi parts then lo parts per section (volume, then
+// boundary).  False when a coupling outside the layout's pairs is not zero,
+// i.e. the layout does not describe this operator.
+static bool mfma_tables5s(const HostOperator &op, const std::vector<uint16_t> &hp, int sl, Blk5 &b5,
+                          std::vector<unsigned char> &out) {
+    const uint32_t slm = op.slm;
+    const int NT = cl_nblk(sl), NC = (int)op.NC;
+    if (slm != cl_slm(sl) || hp.size() != slm) return false;
+    auto mat = [&](int sec, int c, uint32_t n, uint32_t m) -> double {
+        if (sec == 1) return op.onc[((size_t)c * slm + n) * slm + m];
+        if (c == 0) return n == m ? op.massfac[n] : 0.0;
+        return op.omc[((size_t)(c - 1) * slm + n) * slm + m];
+    };
+    b5 = Blk5();
+    b5.nb = (uint32_t)NT;
+    b5.nsplitB = 2;
+    b5.stat = (uint32_t)sl;
+    for (int k = 0; k < NT; k++) {
+        b5.h0[k] = (uint32_t)cl_h0(sl, k);
+        b5.w[k] = (uint32_t)cl_w(sl, k);
+    }
+    out.clear();
+    for (int sec = 0; sec < 2; sec++) {
+        const int nch = sec == 0 ? NC : 3;
+        double big = 0;
+        for (int c = 0; c < nch; c++)
+            for (uint32_t n = 0; n < slm; n++)
+                for (uint32_t m = 0; m < slm; m++) big = std::max(big, std::fabs(mat(sec, c, n, m)));
+        const double tiny = 1e-12 * big;
+        std::vector<unsigned char> hi_part, lo_part;
+        for (int kb = 0; kb < NT; kb++)
+            for (int c = 0; c < nch; c++)
+                for (int t = 0; t < NT; t++) {
+                    unsigned char bh[64 * 8], bl[64 * 8];
+                    bool nz = false;
+                    for (int l = 0; l < 64; l++) {
+                        const uint32_t col = (uint32_t)(l % 16), q = (uint32_t)(l / 16);
+                        uint16_t h[4], g[4];
+                        for (int u = 0; u < 4; u++) {
+                            const uint32_t kk = 4u * q + (uint32_t)u;
+                            double x = 0.0;
+                            if (col < b5.w[t] && kk < b5.w[kb]) x = mat(sec, c, hp[b5.h0[t] + col], hp[b5.h0[kb] + kk]);
+                            if (std::fabs(x) <= tiny) x = 0.0;
+                            nz = nz || x != 0.0;
+                            h[u] = to_bf16((float)x);
+                            g[u] = to_bf16((float)x - from_bf16(h[u]));
+                        }
+                        std::memcpy(bh + l * 8, h, 8);
+                        std::memcpy(bl + l * 8, g, 8);
+                    }
+                    const bool pair = sec == 0 ? cl_vol_pair(sl, kb, c, t) : cl_bnd_pair(sl, kb, t);
+                    if (!pair) {
+                        if (nz) return false;
+                        continue;
+                    }
+                    hi_part.insert(hi_part.end(), bh, bh + sizeof bh);
+                    lo_part.insert(lo_part.end(), bl, bl + sizeof bl);
+                }
+        (sec == 0 ? b5.nvol : b5.nbnd) = (uint32_t)(hi_part.size() / sizeof(unsigned char[64 * 8]));
+        out.insert(out.end(), hi_part.begin(), hi_part.end());
+        out.insert(out.end(), lo_part.begin(), lo_part.end());
+    }
+    return b5.nvol == (uint32_t)cl_vol_count(sl, NC) && b5.nbnd == (uint32_t)cl_bnd_count(sl);
+}
+
 // The FP64 B blocks of variant 5 for the class order hp split into blocks
 // (h0, w): per input block kb, channel c and output block t whose couplings
 // are not all zero, 4 K steps x 64 lanes of doubles (lane l of step st: the
@@ -373,8 +440,25 @@ static int build_tables5(tr_ctx *c) {
         }
     }
     Blk5 b5;
-    std::vector<unsigned char> tab = mfma_tables5(op, hp, h0, w, true, b5);
-    if ((size_t)b5.nvol * 1024 > 80 * 1024) tab = mfma_tables5(op, hp, h0, w, false, b5);
+    std::vector<unsigned char> tab;
+    // the static class layout (straight-line contraction) where it applies
+    const int sl = op.slm == cl_slm(5) ? 5 : (op.slm == cl_slm(4) ? 4 : 0);
+    bool stat = classes && c->tab5_static && sl && (op.NC == 7 || op.NC == 10) && op.W == 15 &&
+                h0.size() == (size_t)cl_nblk(sl);
+    for (size_t k = 0; stat && k < h0.size(); k++)
+        stat = h0[k] == (uint32_t)cl_h0(sl, (int)k) && w[k] == (uint32_t)cl_w(sl, (int)k);
+    if (!(stat && mfma_tables5s(op, hp, sl, b5, tab))) {
+        tab = mfma_tables5(op, hp, h0, w, true, b5);
+        if ((size_t)b5.nvol * 1024 > 80 * 1024) tab = mfma_tables5(op, hp, h0, w, false, b5);
+    }
+    if (c->f_tab5) {
+        // a rebuild (the layout switched): the old table goes once no
+        // queued launch can read it
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->owned.erase(std::remove(c->owned.begin(), c->owned.end(), c->f_tab5), c->owned.end());
+        HIPCHECK(hipFree(c->f_tab5));
+        c->f_tab5 = nullptr;
+    }
     int rc = upload(c, (unsigned char **)&c->f_tab5, tab);
     if (rc) return rc;
     c->b5 = b5;
@@ -1754,6 +1838,8 @@ extern "C" int tr_ctx_kernel_name(tr_ctx *c, int precision, char *buf, size_t ca
     char tmp[160];
     if (m.variant == 5 && precision == 0)
         snprintf(tmp, sizeof tmp, "k_apply5d<%d, %d, %d, %d, 0>", NC, W, (int)c->b5d.nb, NC == 10 ? 5 : 7);
+    else if (m.variant == 5 && c->b5.stat)
+        snprintf(tmp, sizeof tmp, "k_apply5s<%d, %d, %d, 0>", NC, W, (int)c->b5.stat);
     else if (m.variant == 5)
         snprintf(tmp, sizeof tmp, "k_apply5<%d, %d, %d, 0>", NC, W, (int)c->b5.nb);
     else if (m.variant == 4)
@@ -1969,6 +2055,14 @@ extern "C" int tr_ctx_set_schedule(tr_ctx *c, int precision, int J, int WPB, int
         }
         if (c->ang_prec == 0 || !v5_ok(op) || WPB < 1 || WPB > 8)
             return fail(TR_ERR_ARG, "tr_ctx_set_schedule: K-blocked matrix-core schedule unsupported here");
+        // J = 1: the masked block layout even where the static class layout
+        // applies; otherwise the static one where it applies
+        const bool want_static = J != 1;
+        if (want_static != c->tab5_static) {
+            c->tab5_static = want_static;
+            int rc = build_tables5(c);
+            if (rc) return rc;
+        }
         c->map32 = mfma5_map_ctx(c, WPB);
         return TR_OK;
     }

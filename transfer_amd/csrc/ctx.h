@@ -64,6 +64,7 @@ struct tr_ctx {
     void *f_tab = nullptr;
     void *f_tab4 = nullptr, *f_tab4b1 = nullptr, *f_tab4b3 = nullptr;  // variant 4: FP32, BF16, BF16 hi/lo
     void *f_tab5 = nullptr;  // variant 5: BF16 B blocks (33..80 harmonics)
+    bool tab5_static = true; // variant 5: the static class layout where it applies (Blk5::stat)
     // variant 5: the harmonic blocks and stored B blocks (kernels.h, Blk5).
     // Under variant 5 the FP32 vectors hold harmonic hp32[k] of the
     // internal order at position k (reflection-parity classes, so that

@@ -41,7 +41,64 @@ struct Blk5 {
     uint32_t h0[8] = {}, w[8] = {};
     uint64_t vmask[8] = {}, bmask[8] = {};
     uint32_t voff[8] = {}, boff[8] = {};
+    // static class layout (cl_* below) whose stored order the blocks follow,
+    // 0 = none (vmask / voff describe them)
+    uint32_t stat = 0;
 };
+
+// Static parity-class layouts of the K-blocked schedule.  The harmonics of
+// the even P_K expansion in the four reflection-parity classes (cos with m
+// even, cos with m odd, sin with |m| even, sin with |m| odd), each class in
+// blocks of at most 16: layout 5 is K = 5 (66 harmonics: 21, 15, 15, 15 ->
+// blocks 16, 5 | 15 | 15 | 15), layout 4 is K = 4 (45 harmonics: 15, 10,
+// 10, 10, one block per class).
+__host__ __device__ constexpr int cl_nblk(int sl) { return sl; }
+__host__ __device__ constexpr uint32_t cl_slm(int sl) { return sl == 5 ? 66u : 45u; }
+__host__ __device__ constexpr int cl_h0(int sl, int b) {
+    return sl == 5 ? (b == 0 ? 0 : (b == 1 ? 16 : 21 + 15 * (b - 2))) : (b == 0 ? 0 : 15 + 10 * (b - 1));
+}
+__host__ __device__ constexpr int cl_w(int sl, int b) {
+    return sl == 5 ? (b == 0 ? 16 : (b == 1 ? 5 : 15)) : (b == 0 ? 15 : 10);
+}
+__host__ __device__ constexpr int cl_class(int sl, int b) { return sl == 5 ? (b < 2 ? 0 : b - 1) : b; }
+// The class that volume channel c maps class k onto: the mass channel and
+// K^xx, K^yy, K^zz keep it; the xy channels (symmetric part 4, antisymmetric
+// 7) swap 0 <-> 2 and 1 <-> 3, xz (5, 8) 0 <-> 1 and 2 <-> 3, yz (6, 9)
+// 0 <-> 3 and 1 <-> 2 (the mirror parities of Omega_a Omega_b).
+__host__ __device__ constexpr int cl_chan_map(int c, int k) {
+    return c < 4 ? k : ((c - 4) % 3 == 0 ? (k ^ 2) : ((c - 4) % 3 == 1 ? (k ^ 1) : 3 - k));
+}
+// Whether output block t can receive input block kb through volume channel
+// c (the mass channel is diagonal in the harmonics), or through any of the
+// three boundary channels (|Omega_n| keeps every class).
+__host__ __device__ constexpr bool cl_vol_pair(int sl, int kb, int c, int t) {
+    return c == 0 ? t == kb : cl_class(sl, t) == cl_chan_map(c, cl_class(sl, kb));
+}
+__host__ __device__ constexpr bool cl_bnd_pair(int sl, int kb, int t) { return cl_class(sl, t) == cl_class(sl, kb); }
+// Stored position of block (kb, c, t) among the pairs in (kb, c, t) order,
+// and the number of pairs; boundary blocks likewise over 3 channels.
+__host__ __device__ constexpr int cl_vol_index(int sl, int nc, int kb, int c, int t) {
+    int n = 0;
+    for (int k = 0; k < cl_nblk(sl); k++)
+        for (int cc = 0; cc < nc; cc++)
+            for (int tt = 0; tt < cl_nblk(sl); tt++) {
+                if (k == kb && cc == c && tt == t) return n;
+                if (cl_vol_pair(sl, k, cc, tt)) n++;
+            }
+    return n;
+}
+__host__ __device__ constexpr int cl_vol_count(int sl, int nc) { return cl_vol_index(sl, nc, cl_nblk(sl), 0, 0); }
+__host__ __device__ constexpr int cl_bnd_index(int sl, int kb, int c, int t) {
+    int n = 0;
+    for (int k = 0; k < cl_nblk(sl); k++)
+        for (int cc = 0; cc < 3; cc++)
+            for (int tt = 0; tt < cl_nblk(sl); tt++) {
+                if (k == kb && cc == c && tt == t) return n;
+                if (cl_bnd_pair(sl, k, tt)) n++;
+            }
+    return n;
+}
+__host__ __device__ constexpr int cl_bnd_count(int sl) { return cl_bnd_index(sl, cl_nblk(sl), 0, 0); }
 
 // Device operator data (see stencil.h), precision T.
 template <typename T>
@@ -173,6 +230,10 @@ hipError_t launch_apply4(const OpArgs<T> &a, int NC, int wpb, uint32_t grid, con
 // matrix-core projection in blocks of 16 harmonics; tab = B blocks
 // [block][channel][tile][lane] (volume, then boundary), nsplit 1 or 2
 bool mfma5_supported(uint32_t slm);
+// the static class-layout kernel of variant 5 (kernels_mfma5s.hip) for
+// layout SL (Blk5::stat), BF16 split ns (1..3) and epilogue epi (0..3):
+// 15 volume slots; null otherwise
+const void *apply5s_kernel_epi(int epi, int NC, uint32_t W, uint32_t SL, int ns);
 size_t apply5_lds_bytes(const Blk5 &b5);
 uint32_t apply5_grid(const OpArgs<float> &a, int NC, int wpb, int epi);
 hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid, const float *f, float *r,
