@@ -52,7 +52,7 @@ def source_digest():
     that runs."""
     import hashlib
     h = hashlib.sha256()
-    for name in ("kernels_mfma.hip", "kernels.hip", "kernels.h"):
+    for name in ("kernels_mfma.hip", "kernels_mfma5s.hip", "mfma_common.h", "kernels.hip", "kernels.h"):
         with open(os.path.join(ROOT, "transfer_amd", "csrc", name), "rb") as fh:
             h.update(fh.read())
     return h.hexdigest()[:16]

@@ -1839,7 +1839,7 @@ extern "C" int tr_ctx_kernel_name(tr_ctx *c, int precision, char *buf, size_t ca
     if (m.variant == 5 && precision == 0)
         snprintf(tmp, sizeof tmp, "k_apply5d<%d, %d, %d, %d, 0>", NC, W, (int)c->b5d.nb, NC == 10 ? 5 : 7);
     else if (m.variant == 5 && c->b5.stat)
-        snprintf(tmp, sizeof tmp, "k_apply5s<%d, %d, %d, 0>", NC, W, (int)c->b5.stat);
+        snprintf(tmp, sizeof tmp, "k_apply5s<%d, %d, %d, %d, 0>", NC, W, (int)c->b5.stat, c->args32(m).nsplit);
     else if (m.variant == 5)
         snprintf(tmp, sizeof tmp, "k_apply5<%d, %d, %d, 0>", NC, W, (int)c->b5.nb);
     else if (m.variant == 4)
