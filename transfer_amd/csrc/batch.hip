@@ -71,13 +71,23 @@ __device__ inline void bsum3(double &a, double &b, double &c) {
     }
 }
 
-// part[blk] = x . y over the block
+// Blocks of problems that have stopped (converged, capped or broken down)
+// skip the vector kernels: their alpha and beta are 0, their partials are
+// no longer read, and their p no longer matters (alpha = 0 keeps x and r).
+__device__ inline bool bstopped(const double *sc, const uint32_t *prob) {
+    return sc && sc[(size_t)prob[blockIdx.x] * BS_N + BS_CONV] != 0.0;
+}
+
+// part[blk] = x . y over the block (sc, prob: the stopped problems' blocks
+// skip, when given)
 template <typename T>
 __global__ __launch_bounds__(TR_BLK_THREADS) void k_bdot(const T *__restrict__ x, const T *__restrict__ y,
                                                          const uint32_t *__restrict__ lo,
                                                          const uint32_t *__restrict__ hi, double *__restrict__ part,
-                                                         const double *__restrict__ skip) {
-    if (skip && *skip != 0.0) return;
+                                                         const double *__restrict__ skip,
+                                                         const double *__restrict__ sc = nullptr,
+                                                         const uint32_t *__restrict__ prob = nullptr) {
+    if ((skip && *skip != 0.0) || bstopped(sc, prob)) return;
     double s = 0, u = 0, w = 0;
     for (uint32_t i = lo[blockIdx.x] + threadIdx.x; i < hi[blockIdx.x]; i += TR_BLK_THREADS)
         s = fma((double)x[i], (double)y[i], s);
@@ -116,7 +126,7 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bupdate(T *__restrict__ x, c
                                                             const double *__restrict__ sc, double *__restrict__ part,
                                                             uint32_t nblk, const double *__restrict__ skip,
                                                             const T *__restrict__ cdinv, T cscale, T *__restrict__ z1) {
-    if (skip && *skip != 0.0) return;
+    if ((skip && *skip != 0.0) || bstopped(sc, prob)) return;
     const double a = sc[(size_t)prob[blockIdx.x] * BS_N + BS_ALPHA];
     double rr = 0, rz = 0, w = 0;
     for (uint32_t i = lo[blockIdx.x] + threadIdx.x; i < hi[blockIdx.x]; i += TR_BLK_THREADS) {
@@ -179,7 +189,7 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bpupdate(T *__restrict__ p, 
                                                              const double *__restrict__ sc,
                                                              const double *__restrict__ skip,
                                                              const T *__restrict__ zb) {
-    if (skip && *skip != 0.0) return;
+    if ((skip && *skip != 0.0) || bstopped(sc, prob)) return;
     const double be = sc[(size_t)prob[blockIdx.x] * BS_N + BS_BETA];
     for (uint32_t i = lo[blockIdx.x] + threadIdx.x; i < hi[blockIdx.x]; i += TR_BLK_THREADS) {
         const double z = zb ? (double)zb[i] : (dinv ? (double)dinv[i] * (double)r[i] : (double)r[i]);
@@ -326,6 +336,10 @@ struct BatchCg {
 // polynomial over the union: one interval [lambda_max / ratio, lambda_max]
 // with lambda_max of the whole union's D^-1 A bounds every problem's
 // spectrum, so it is a valid SPD preconditioner for each of them.
+// The operator launches of the iterations (and the Chebyshev steps, whose
+// arguments the caller gives the same fields) skip the workgroups of
+// stopped problems: a.vprob and a.pstop = sc + BS_CONV.  sc is cleared
+// first, so the start's launches (before k_bfin_init) run everywhere.
 template <typename T>
 int pcg_batch(tr_ctx *c, const BlockTable &t, const OpArgs<T> &a, const LaneMap &map, T *x, const T *b, T *r, T *p,
               T *Ap, const T *dinv, double *part, double *sc, double *all, const BatchCg &prm, hipStream_t st,
@@ -333,6 +347,7 @@ int pcg_batch(tr_ctx *c, const BlockTable &t, const OpArgs<T> &a, const LaneMap 
     const int NC = (int)c->op.NC;
     const dim3 gb(t.nblk), tb(TR_BLK_THREADS);
     if (cheb && !z) return fail(TR_ERR_ARG, "batch: Chebyshev preconditioner without its buffer");
+    HIPCHECK(hipMemsetAsync(sc, 0, sizeof(double) * (size_t)t.nprob * BS_N, st));
     HIPCHECK(launch_apply<T>(a, NC, map, x, r, nullptr, nullptr, st));
     hipLaunchKernelGGL(k_binit<T>, gb, tb, 0, st, b, r, p, dinv, t.lo, t.hi, part, t.nblk);
     if (cheb) {
@@ -347,16 +362,22 @@ int pcg_batch(tr_ctx *c, const BlockTable &t, const OpArgs<T> &a, const LaneMap 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    OpArgs<T> ai = a;
+    ai.vprob = c->d_vprob;
+    ai.pstop = ai.vprob ? sc + BS_CONV : nullptr;
+    ai.pstride = BS_N;
     for (int it = 0; it < prm.chunk; it++) {
-        launch_apply<T>(a, NC, map, p, Ap, nullptr, all, st);
-        hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, p, Ap, t.lo, t.hi, part, all);
+        launch_apply<T>(ai, NC, map, p, Ap, nullptr, all, st);
+        hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, p, Ap, t.lo, t.hi, part, all, (const double *)sc,
+                           (const uint32_t *)t.prob);
         hipLaunchKernelGGL(k_bfin_alpha, dim3((t.nprob + 255) / 256), dim3(256), 0, st, part, t.b0, t.nprob, sc, all);
         hipLaunchKernelGGL(k_bupdate<T>, gb, tb, 0, st, x, p, r, Ap, cheb ? (const T *)nullptr : dinv, t.lo, t.hi,
                            t.prob, sc, part, t.nblk, all, cheb ? cheb->dinv : (const T *)nullptr,
                            cheb ? (T)cheb->inv_theta : (T)0, cheb ? cheb->first_buffer(z) : (T *)nullptr);
         if (cheb) {
             cheb->steps(r, z, all, st, nullptr);
-            hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, r, z, t.lo, t.hi, part + t.nblk, all);
+            hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, r, z, t.lo, t.hi, part + t.nblk, all, (const double *)sc,
+                               (const uint32_t *)t.prob);
         }
         hipLaunchKernelGGL(k_bfin_beta, dim3(1), tb, 0, st, part, t.b0, t.nprob, t.nblk, sc, prm.maxit, all);
         hipLaunchKernelGGL(k_bpupdate<T>, gb, tb, 0, st, p, r, dinv, t.lo, t.hi, t.prob, sc, all,
@@ -467,6 +488,14 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
     double *b = m.get<double>(n), *r = m.get<double>(n), *p = m.get<double>(n), *Ap = m.get<double>(n);
     double *part = m.get<double>(3 * (size_t)t.nblk + 16), *sc = m.get<double>((size_t)P * BS_N + 16);
     double *all = m.get<double>(2), *fac = m.get<double>(P);
+    if (!c->d_vprob) {
+        // problem of every vertex, for the operator's skipping of stopped
+        // problems (kept with the context)
+        std::vector<uint32_t> vp(c->op.nP);
+        for (uint32_t q = 0; q < P; q++)
+            for (uint32_t v = c->voff[q]; v < c->voff[q + 1]; v++) vp[v] = q;
+        if ((rc = upload(c, &c->d_vprob, vp))) return rc;
+    }
     double *dinv = nullptr;
     float *dinv32 = nullptr;
     if (!b || !r || !p || !Ap || !part || !sc || !all || !fac) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
@@ -501,6 +530,9 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
                 zc = m.get<double>(n);
                 if (!zc) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
                 cheb.setup(c->cheb_deg, c->cheb_lmax, c->cheb_ratio);
+                cheb.a.vprob = c->d_vprob;
+                cheb.a.pstop = sc + BS_CONV;
+                cheb.a.pstride = BS_N;
             } else {
                 cheb32.a = c->args32();
                 if (c->cheb_split > 0 && c->cheb_split < cheb32.a.nsplit &&
@@ -515,6 +547,9 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
                 zc32 = m.get<float>(n);
                 if (!zc32) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
                 cheb32.setup(c->cheb_deg, c->cheb_lmax, c->cheb_ratio);
+                cheb32.a.vprob = c->d_vprob;
+                cheb32.a.pstop = sc + BS_CONV;
+                cheb32.a.pstride = BS_N;
             }
         }
     }

@@ -89,6 +89,7 @@ struct tr_ctx {
     // batches: internal vertex range of problem p = voff[p] .. voff[p+1]
     // (empty for a single problem)
     std::vector<uint32_t> voff;
+    uint32_t *d_vprob = nullptr;  // batches: problem of every internal vertex (built on first solve)
     bool own_stream = true;        // false for the levels of a multilevel hierarchy
     tr::Multilevel *mg = nullptr;  // multilevel preconditioner (use_prec 3), built on first use
     uint32_t mg_coarsest = 3000, mg_agg = 16; // hierarchy shape: coarsest-level size, aggregate size

@@ -140,6 +140,13 @@ struct OpArgs {
     // class order already
     const uint16_t *hpo = nullptr;
     T *cls = nullptr;
+    // batches: vprob[v] = problem of vertex v and pstop[p * pstride] != 0
+    // when problem p has stopped; the buffer-load VALU schedule (variant 3)
+    // skips workgroups whose vertices all belong to stopped problems (their
+    // outputs keep their old values).  Null: every workgroup runs.
+    const uint32_t *vprob = nullptr;
+    const double *pstop = nullptr;
+    uint32_t pstride = 0;
 };
 
 // One step of the Chebyshev iteration on D^-1 A z = D^-1 rhs, written as

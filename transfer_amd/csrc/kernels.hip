@@ -360,6 +360,19 @@ __global__ __launch_bounds__(256) void k_apply3(OpArgs<T> a, const T *__restrict
         for (uint32_t i = threadIdx.x; i < ntab * (TB / 8); i += blockDim.x) dst[i] = src[i];
     }
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    if (a.pstop) {
+        // a workgroup whose vertices all belong to stopped problems has
+        // nothing to do (the decision is the same for all its threads)
+        const uint32_t vb0 = blk * (uint32_t)wpb * (uint32_t)a.VPW;
+        const uint32_t vb1 = min(vb0 + (uint32_t)wpb * (uint32_t)a.VPW, nP);
+        bool run = false;
+        if (vb0 < nP)
+            for (uint32_t p = a.vprob[vb0]; p <= a.vprob[vb1 - 1] && !run; p++) run = a.pstop[(size_t)p * a.pstride] == 0.0;
+        if (!run) {
+            if (DOT && threadIdx.x == 0) part[blockIdx.x] = 0;
+            return;
+        }
+    }
     const int vsub = lane / a.L, lv = lane - vsub * a.L;
     const uint32_t v0 = (blk * (uint32_t)wpb + wave) * (uint32_t)a.VPW;
     const uint32_t v = v0 + vsub;
