@@ -381,6 +381,7 @@ def bench_c3(args):
         # fused operator launch (the BF16 matrix-core operator's hi/lo split)
         ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
         mixed_kw = {"mixed_fp32_ir_jacobi": (use_exact, dict(use_prec=2)),
+                    "mixed_bf16mfma_ir_plain": (use_mfma, dict(use_prec=0)),
                     "mixed_bf16mfma_ir_jacobi": (use_mfma, dict(use_prec=2)),
                     "mixed_bf16mfma_ir_chebyshev": (use_mfma, dict(use_prec=4, inner_tol=1e-4))}
         runs = [("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),

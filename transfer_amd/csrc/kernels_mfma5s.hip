@@ -192,6 +192,24 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
         // a ring of DEPTH neighbour-row buffers runs through all the
         // harmonic blocks: the next block's first rows load while the
         // current one contracts
+        // the fused dot's f at this lane's outputs (rows 4q + i, column
+        // vloc of each output block), loaded now so the reads complete
+        // during the gather instead of at the write
+        float fo[EPI == 1 ? NT : 1][4];
+        if constexpr (EPI == 1) {
+            const auto fr2 = rsrc4(f, (size_t)nP * SLM * sizeof(float));
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t vr = v0 + (uint32_t)(4 * q + i);
+                    const bool ok = vloc < cl_w(SL, t) && vr < nP;
+                    // rows past nP read past the vector: zeros
+                    fo[t][i] = __builtin_bit_cast(
+                        float, __builtin_amdgcn_raw_buffer_load_b32(
+                                   fr2, ok ? (vr * SLM + (uint32_t)cl_h0(SL, t) + (uint32_t)vloc) * 4u : 0xfffffff0u, 0u, 0));
+                }
+        }
         float xs[DEPTH][4];
         auto ld = [&](int g, float *x) __attribute__((always_inline)) {
             const int kb = g / WT, s2 = g % WT;
@@ -277,7 +295,7 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
                         if (EPI == 3) d = fma((double)rh, (double)nv, d);
                     } else {
                         r[o] = val;
-                        if (DOT) d = fma((double)f[o], (double)val, d);
+                        if constexpr (EPI == 1) d = fma((double)fo[t][i], (double)val, d);
                     }
                 }
             }

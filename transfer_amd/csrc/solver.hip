@@ -107,15 +107,39 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
         rr = fma((double)ri, (double)ri, rr);
         rz = fma((double)ri, (double)zi, rz);
     };
-    // four elements per thread and pass (each a full wavefront's coalesced
-    // stride apart), so four loads of every stream are in flight at once
     const size_t S = (size_t)gridDim.x * blockDim.x;
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    for (; i + 3 * S < n; i += 4 * S) {
+    if constexpr (sizeof(T) == 4) {
+        // FP32: 16-byte accesses, four elements per thread and step
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const size_t n4 = n / 4;
+        f4 *x4 = reinterpret_cast<f4 *>(x), *r4 = reinterpret_cast<f4 *>(r);
+        const f4 *p4 = reinterpret_cast<const f4 *>(p), *a4 = reinterpret_cast<const f4 *>(Ap);
+        const f4 *d4 = reinterpret_cast<const f4 *>(dinv), *c4 = reinterpret_cast<const f4 *>(cdinv);
+        f4 *z4 = reinterpret_cast<f4 *>(z1);
+        for (size_t k = i; k < n4; k += S) {
+            x4[k] = alpha * p4[k] + x4[k];
+            const f4 rv = r4[k] - alpha * a4[k];
+            r4[k] = rv;
+            if (z1) z4[k] = cscale * c4[k] * rv;
+            const f4 zv = dinv ? d4[k] * rv : rv;
 #pragma unroll
-        for (int u = 0; u < 4; u++) one(i + u * S);
+            for (int u = 0; u < 4; u++) {
+                rr = fma((double)rv[u], (double)rv[u], rr);
+                rz = fma((double)rv[u], (double)zv[u], rz);
+            }
+        }
+        for (size_t k = 4 * n4 + i; k < n; k += S) one(k);
+    } else {
+        // four elements per thread and pass (each a full wavefront's
+        // coalesced stride apart), so four loads of every stream are in
+        // flight at once
+        for (; i + 3 * S < n; i += 4 * S) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) one(i + u * S);
+        }
+        for (; i < n; i += S) one(i);
     }
-    for (; i < n; i += S) one(i);
     block_sum2(rr, rz);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = rr;
@@ -175,11 +199,24 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
     };
     const size_t S = (size_t)gridDim.x * blockDim.x;
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    for (; i + 3 * S < n; i += 4 * S) {
+    if constexpr (sizeof(T) == 4) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const size_t n4 = n / 4;
+        f4 *p4 = reinterpret_cast<f4 *>(p);
+        const f4 *r4 = reinterpret_cast<const f4 *>(r), *d4 = reinterpret_cast<const f4 *>(dinv);
+        const f4 *zz4 = reinterpret_cast<const f4 *>(z);
+        for (size_t k = i; k < n4; k += S) {
+            const f4 zv = z ? zz4[k] : (dinv ? d4[k] * r4[k] : r4[k]);
+            p4[k] = beta * p4[k] + zv;
+        }
+        for (size_t k = 4 * n4 + i; k < n; k += S) one(k);
+    } else {
+        for (; i + 3 * S < n; i += 4 * S) {
 #pragma unroll
-        for (int u = 0; u < 4; u++) one(i + u * S);
+            for (int u = 0; u < 4; u++) one(i + u * S);
+        }
+        for (; i < n; i += S) one(i);
     }
-    for (; i < n; i += S) one(i);
 }
 
 // r = b - r (r holds A x on entry); z = M r; p = z; partials of r.r, r.z, b.b
