@@ -109,7 +109,29 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
     };
     const size_t S = (size_t)gridDim.x * blockDim.x;
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if constexpr (sizeof(T) == 4) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "FP32 or FP64 vectors");
+    if constexpr (sizeof(T) == 8) {
+        // FP64: 16-byte accesses, two elements per thread and step
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const size_t n2 = n / 2;
+        d2 *x2 = reinterpret_cast<d2 *>(x), *r2 = reinterpret_cast<d2 *>(r);
+        const d2 *p2 = reinterpret_cast<const d2 *>(p), *a2 = reinterpret_cast<const d2 *>(Ap);
+        const d2 *dd2 = reinterpret_cast<const d2 *>(dinv), *c2 = reinterpret_cast<const d2 *>(cdinv);
+        d2 *z2 = reinterpret_cast<d2 *>(z1);
+        for (size_t k = i; k < n2; k += S) {
+            x2[k] = alpha * p2[k] + x2[k];
+            const d2 rv = r2[k] - alpha * a2[k];
+            r2[k] = rv;
+            if (z1) z2[k] = cscale * c2[k] * rv;
+            const d2 zv = dinv ? dd2[k] * rv : rv;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                rr = fma(rv[u], rv[u], rr);
+                rz = fma(rv[u], zv[u], rz);
+            }
+        }
+        for (size_t k = 2 * n2 + i; k < n; k += S) one(k);
+    } else if constexpr (sizeof(T) == 4) {
         // FP32: 16-byte accesses, four elements per thread and step
         typedef float f4 __attribute__((ext_vector_type(4)));
         const size_t n4 = n / 4;
@@ -130,15 +152,6 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
             }
         }
         for (size_t k = 4 * n4 + i; k < n; k += S) one(k);
-    } else {
-        // four elements per thread and pass (each a full wavefront's
-        // coalesced stride apart), so four loads of every stream are in
-        // flight at once
-        for (; i + 3 * S < n; i += 4 * S) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) one(i + u * S);
-        }
-        for (; i < n; i += S) one(i);
     }
     block_sum2(rr, rz);
     if (threadIdx.x == 0) {
@@ -199,7 +212,18 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
     };
     const size_t S = (size_t)gridDim.x * blockDim.x;
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 8) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const size_t n2 = n / 2;
+        d2 *p2 = reinterpret_cast<d2 *>(p);
+        const d2 *r2 = reinterpret_cast<const d2 *>(r), *dd2 = reinterpret_cast<const d2 *>(dinv);
+        const d2 *zz2 = reinterpret_cast<const d2 *>(z);
+        for (size_t k = i; k < n2; k += S) {
+            const d2 zv = z ? zz2[k] : (dinv ? dd2[k] * r2[k] : r2[k]);
+            p2[k] = beta * p2[k] + zv;
+        }
+        for (size_t k = 2 * n2 + i; k < n; k += S) one(k);
+    } else if constexpr (sizeof(T) == 4) {
         typedef float f4 __attribute__((ext_vector_type(4)));
         const size_t n4 = n / 4;
         f4 *p4 = reinterpret_cast<f4 *>(p);
@@ -210,12 +234,6 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
             p4[k] = beta * p4[k] + zv;
         }
         for (size_t k = 4 * n4 + i; k < n; k += S) one(k);
-    } else {
-        for (; i + 3 * S < n; i += 4 * S) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) one(i + u * S);
-        }
-        for (; i < n; i += S) one(i);
     }
 }
 
