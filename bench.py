@@ -235,6 +235,7 @@ def run_solves(ctx, runs, tag, only=""):
         setup = kw.pop("setup", None)
         if setup:
             setup()
+        ctx.set_residual_replacement(kw.pop("replacement", True))
         d = ctx.allocVector()
         p = dict(maxit=20000 if solver != ta.TR_SOLVER_MIXED else 30, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000,
                  inner_tol=1e-5)
@@ -280,6 +281,7 @@ def learned_mixed(ctx, P, g, solver_kw, tag):
     kw = dict(maxit=30, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000, inner_tol=1e-5)
     kw.update(solver_kw)
     prec = kw.pop("use_prec")
+    ctx.set_residual_replacement(kw.pop("replacement", True))
     st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_MIXED, use_prec=prec, x0=True, **kw)
     ctx.deviceFree(d_f)
     out = dict(seconds=guess_s + st.seconds, guess_seconds=guess_s, solve_seconds=st.seconds, iters=st.iters,
@@ -380,9 +382,14 @@ def bench_c3(args):
         # Chebyshev degree 6 over [lambda_max / 300, lambda_max], each step one
         # fused operator launch (the BF16 matrix-core operator's hi/lo split)
         ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
-        mixed_kw = {"mixed_fp32_ir_jacobi": (use_exact, dict(use_prec=2)),
-                    "mixed_bf16mfma_ir_plain": (use_mfma, dict(use_prec=0)),
-                    "mixed_bf16mfma_ir_jacobi": (use_mfma, dict(use_prec=2)),
+        # residual replacement (the default) with a replacement every 1e-2
+        # of inner reduction; the restarted refinement (each step's inner CG
+        # from zero to 1e-5) beside it
+        mixed_kw = {"mixed_fp32_ir_jacobi": (use_exact, dict(use_prec=2, inner_tol=1e-2)),
+                    "mixed_bf16mfma_ir_plain": (use_mfma, dict(use_prec=0, inner_tol=1e-2)),
+                    "mixed_bf16mfma_ir_plain_restart": (use_mfma, dict(use_prec=0, inner_tol=1e-5,
+                                                                       replacement=False)),
+                    "mixed_bf16mfma_ir_jacobi": (use_mfma, dict(use_prec=2, inner_tol=1e-2)),
                     "mixed_bf16mfma_ir_chebyshev": (use_mfma, dict(use_prec=4, inner_tol=1e-4))}
         runs = [("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
                 ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
@@ -444,16 +451,24 @@ def bench_c5(args):
     train_s = time.perf_counter() - t0 - gen_s
     log(tag, "trained on %d samples in %.1f s, loss %.2e" % (len(targ), train_s, float(hist[-10:].mean())))
     del feat, targ
+    # the protocol's solver restarts its inner CG at every refinement step;
+    # the last row is the default mixed solver (residual replacement, inner
+    # tolerance 1e-2) from zero, for comparison
     kw = dict(solver=ta.TR_SOLVER_MIXED, maxit=30, tol_abs=0.0, tol_rel=1e-8, inner_maxit=20000, inner_tol=1e-5)
     rows = {k: dict(inner=[], seconds=[], converged=0, fallback=0) for k in ("zero_start", "learned_start",
-                                                                          "learned_coarse")}
+                                                                          "learned_coarse", "zero_start_replacement")}
     pred_err, mean_err = [], []
     seeds = list(range(100000, 100000 + ntest))
     for c0 in range(0, ntest, 16):
         for P in learned.cube_problems(seeds[c0:c0 + 16], n=32, maxk=3):
             ctx = ta.HipContext(P.mesh, P.ang)
             for mode in rows:
-                if mode == "zero_start":
+                ctx.set_residual_replacement(mode == "zero_start_replacement")
+                if mode == "zero_start_replacement":
+                    d = ctx.allocVector()
+                    st, _ = ctx.solve(d, use_prec=2, **dict(kw, inner_tol=1e-2))
+                    gs = 0.0
+                elif mode == "zero_start":
                     d = ctx.allocVector()
                     st, _ = ctx.solve(d, use_prec=2, **kw)
                     gs = 0.0

@@ -58,6 +58,39 @@ def test_solver_reaches_fp64_residual(name, solver, prec):
     ctx.close()
 
 
+@pytest.mark.parametrize("name", ["ball16", "ball12-hg", "ball10-K5-hg"])
+@pytest.mark.parametrize("prec", [0, 2, 4])
+@pytest.mark.parametrize("inner_tol", [1e-2, 1e-5])
+def test_residual_replacement(name, prec, inner_tol):
+    """The mixed solver's refinement steps with residual replacement (the
+    inner CG continued across steps, its residual replaced by the true FP64
+    one; the default) and restarted: both reach the FP64 residual, checked
+    again by the oracle, and the oracle's solution, with no fallback to the
+    FP64 solver.  At 1e-5 the inner CG's recursive residual can drift from
+    the true one before a replacement; such steps restart."""
+    P = build(name)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    b = op.rhs()
+    ref, _, _ = op.pcg(tol_abs=0.0, tol_rel=1e-12, maxit=40000)
+    for replace in (True, False):
+        ctx.set_residual_replacement(replace)
+        d_f = ctx.allocVector()
+        st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_MIXED, use_prec=prec, maxit=30, tol_abs=0.0, tol_rel=1e-8,
+                          inner_maxit=20000, inner_tol=inner_tol)
+        f = ctx.copyToHost(d_f)
+        ctx.deviceFree(d_f)
+        res = op.norm(b - op.lhs(f)) / op.norm(b)
+        err = np.abs(f - ref).max() / np.abs(ref).max()
+        print("[gpu-solvers] %s prec=%d inner_tol=%g replacement=%d: %d steps, %d inner, true_rel %.2e "
+              "(oracle %.2e), err_vs_fp64 %.2e, fallback %d" % (name, prec, inner_tol, replace, st.iters, st.preck,
+                                                                st.true_rel, res, err, st.fallback), flush=True)
+        assert st.status == 0 and st.fallback == 0
+        assert st.true_rel <= 1e-8 * 1.01 and res <= 1.5e-8
+        assert err <= 1e-6
+    ctx.close()
+
+
 def test_multilevel_hierarchy_shapes():
     """Every hierarchy shape gives a converged solve with the same solution
     (the preconditioner changes the path, not the answer)."""

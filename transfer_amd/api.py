@@ -107,6 +107,7 @@ _sig("tr_ctx_schedule", C.c_int, vp, C.c_int, C.POINTER(C.c_int))
 _sig("tr_ctx_set_schedule", C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int)
 _sig("tr_ctx_set_angular_precision", C.c_int, vp, C.c_int)
 _sig("tr_ctx_set_refinement", C.c_int, vp, C.c_double, C.c_int)
+_sig("tr_ctx_set_residual_replacement", C.c_int, vp, C.c_int)
 _sig("tr_dot_noalloc", C.c_int, vp, vp, vp, vp)
 _sig("tr_abs_normal_matrix", C.c_int, C.c_int, f64p, f64p)
 _sig("tr_abs_normal_column", C.c_int, C.c_int, f64p, f64p)
@@ -483,6 +484,11 @@ class HipContext:
         """Mixed solver: FP64 fallback when a refinement step cuts the true
         residual by less than `stagnation`; inner CG stagnation window."""
         check(lib.tr_ctx_set_refinement(self._h, float(stagnation), int(inner_stall_window)))
+
+    def set_residual_replacement(self, on=True):
+        """Mixed solver: continue the inner CG across refinement steps
+        (residual replacement, the default) or restart it."""
+        check(lib.tr_ctx_set_residual_replacement(self._h, int(bool(on))))
 
     def permutation(self):
         p = np.empty(self.nP, np.uint32)

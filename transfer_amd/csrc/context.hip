@@ -1495,6 +1495,10 @@ int ensure_fp32(tr_ctx *c) {
 
 // FP64 iterative refinement around an FP32 device CG:
 //   r = b - A64 x;  d = (A32)^-1 (r / |r|) to inner_tol;  x += |r| d
+// By default every step after the first continues the previous step's CG
+// instead of restarting it (residual replacement, cg_replace_residual): its
+// residual becomes the new true FP64 residual and its last direction is
+// kept, so the Krylov space built so far is not thrown away.
 // The plain preconditioned FP64 solver (TR_SOLVER_PCG_FP64, same use_prec)
 // takes over from the last good iterate when the FP64 residual fails to
 // drop: a refinement step that does not cut it by kStagnation, an update
@@ -1504,6 +1508,7 @@ int ensure_fp32(tr_ctx *c) {
 // (Defaults; tr_ctx_set_refinement changes them per context.)
 static constexpr double kStagnation = 0.5;
 static constexpr int kInnerStallWindow = 100;
+static constexpr double kReplaceDrift = 2.0;
 
 static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
     int rc = ensure_fp32(c);
@@ -1656,6 +1661,10 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     int outer = 0, inner = 0, flag = TR_ERR_NOCONV;
     bool stalled = false, have_backup = false;
     double rn = bn, rn_prev = INFINITY;
+    // residual replacement: the last inner solve stopped converged with a
+    // direction, its scale and that direction's r.z
+    bool resumable = false;
+    double scale_prev = 0, rz_dir = 0, rr_rec = 0;
     for (; outer <= prm->maxit; outer++) {
         if ((rc = lhs(c, f, Ap)) || (rc = residual(c, r, b, Ap))) return rc;
         rn = std::sqrt(dot(c, r, r));
@@ -1687,10 +1696,25 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
             const double goal = std::max(prm->tol_rel * bn, prm->tol_abs);
             cp.tol_rel = std::max(prm->inner_tol, std::min(0.1, 0.6 * goal / rn));
         }
-        HIPCHECK(launch_to_f32_scaled(r, b32, 1.0 / rn, n, c->stream, c->perm32(), c->op.slm));
-        HIPCHECK(hipMemsetAsync(d32, 0, sizeof(float) * n, c->stream));
         CgResult res;
-        if ((rc = pcg_device<float>(S, cp, c->stream, res))) return rc;
+        bool resumed = false;
+        HIPCHECK(hipMemsetAsync(d32, 0, sizeof(float) * n, c->stream));
+        // continue the inner CG: its recursive residual replaced by the
+        // true one in the new scale, its last direction kept; unless the
+        // true residual (in the stopped solve's scale, rn / scale_prev) has
+        // drifted from the recursive one by more than kReplaceDrift: then
+        // the direction's r.z no longer matches, and the step restarts
+        if (c->ir_reliable && resumable && rn / scale_prev <= kReplaceDrift * std::sqrt(rr_rec)) {
+            HIPCHECK(launch_to_f32_scaled(r, r32, 1.0 / rn, n, c->stream, c->perm32(), c->op.slm));
+            CgParams cq = cp;
+            cq.resume = 1;
+            if ((rc = cg_replace_residual<float>(S, cq, scale_prev / rn, rz_dir, c->stream, resumed))) return rc;
+            if (resumed && (rc = pcg_device<float>(S, cq, c->stream, res))) return rc;
+        }
+        if (!resumed) {
+            HIPCHECK(launch_to_f32_scaled(r, b32, 1.0 / rn, n, c->stream, c->perm32(), c->op.slm));
+            if ((rc = pcg_device<float>(S, cp, c->stream, res))) return rc;
+        }
         inner += res.iters;
         // a breakdown (NaN, or p.Ap <= 0 from an operator that is not SPD in
         // FP32) leaves no usable correction: the update is not applied
@@ -1698,6 +1722,10 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
             stalled = true;
             break;
         }
+        resumable = res.flag == 1 && res.iters > 0;
+        scale_prev = rn;
+        rz_dir = res.rz_dir;
+        rr_rec = res.rr;
         HIPCHECK(hipMemcpyAsync(xb, f, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
         have_backup = true;
         HIPCHECK(launch_add_f32_scaled(f, d32, rn, n, c->stream, c->perm32(), c->op.slm));
@@ -2264,9 +2292,6 @@ extern "C" int tr_ctx_set_multilevel(tr_ctx *c, uint32_t coarsest, uint32_t agg,
     return TR_OK;
 }
 
-// Angular precision of the FP32 operator: 0 = FP32 arithmetic throughout,
-// 1 = BF16 matrix-core projection with FP32 accumulation, 3 = the same with
-// the hi/lo split (three BF16 products, ~16 significant bits).
 extern "C" int tr_ctx_set_refinement(tr_ctx *c, double stagnation, int inner_stall_window) {
     if (!c || !(stagnation > 0) || inner_stall_window < 0)
         return fail(TR_ERR_ARG, "tr_ctx_set_refinement: stagnation must be > 0 and the window >= 0");
@@ -2274,6 +2299,16 @@ extern "C" int tr_ctx_set_refinement(tr_ctx *c, double stagnation, int inner_sta
     c->ir_stall_window = inner_stall_window;
     return TR_OK;
 }
+
+extern "C" int tr_ctx_set_residual_replacement(tr_ctx *c, int on) {
+    if (!c || (on != 0 && on != 1)) return fail(TR_ERR_ARG, "tr_ctx_set_residual_replacement: 0 or 1");
+    c->ir_reliable = on;
+    return TR_OK;
+}
+
+// Angular precision of the FP32 operator: 0 = FP32 arithmetic throughout,
+// 1 = BF16 matrix-core projection with FP32 accumulation, 3 = the same with
+// the hi/lo split (three BF16 products, ~16 significant bits).
 
 extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
     if (!c || (prec != 0 && prec != 1 && prec != 3)) return fail(TR_ERR_ARG, "tr_ctx_set_angular_precision: 0, 1 or 3");

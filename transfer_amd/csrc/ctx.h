@@ -105,6 +105,9 @@ struct tr_ctx {
     // the inner CG (0 = none)
     double ir_stagnation = 0.5;
     int ir_stall_window = 100;
+    // mixed solver: 1 = each refinement step continues the inner CG from
+    // its last direction (residual replacement); 0 = restarts it
+    int ir_reliable = 1;
     std::vector<void *> owned;
 
     tr::OpArgs<double> args() const { return args(map); }

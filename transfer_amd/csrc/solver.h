@@ -24,6 +24,8 @@ enum {
     SC_RZ2 = 9,   // r.z of odd iterations (SC_RZ holds it for even ones)
     SC_RRMIN = 10, // smallest r.r so far
     SC_RRWIN = 11, // SC_RRMIN at the last stagnation check
+    SC_RUN = 12,   // 1 when the iteration's update pass ran (its p update
+                   // pass follows it), 0 when the solve had stopped
     SC_N = 16
 };
 
@@ -66,16 +68,34 @@ struct CgParams {
     // r.r so far must have dropped below 0.81 of its value at the previous
     // check (||r|| by 10%), else the solve stops with flag 4
     int stall_window = 0;
+    // 1: continue a stopped solve: skip r = b - A x and p = z; the caller
+    // has set r, p and the scalars (cg_replace_residual)
+    int resume = 0;
 };
 
 struct CgResult {
     int iters = 0, flag = 0;
     double rr = 0, bb = 0, seconds = 0;
+    double rz_dir = 0;  // r.z that built the last direction (0 if none)
 };
 
 // Solves from the x held on entry.
 template <typename T>
 int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResult &res);
+
+// Residual replacement (a "reliable update", Sleijpen and van der Vorst
+// 1996): a solve stopped converged (flag 1) with its last direction p_k in
+// S.p and rz_dir = r_k.z_k; the caller has overwritten S.r with a new
+// residual whose units are 1/s of the stopped solve's (a vector q of that
+// solve is s q in the new units) and reset S.x.  The direction becomes
+// p = z + beta s p_k' with beta = r.z / (s^2 rz_dir) and p_k' = p_k less
+// its component along the new r (the recursive residual is orthogonal to
+// p_k), and the scalars are set for a solve with prm.resume = 1 to
+// tolerance prm.tol_rel relative to |r|.  ok = false (nothing changed but
+// z) when r.r, r.z, rz_dir or s is not positive and finite.
+template <typename T>
+int cg_replace_residual(const CgSpace<T> &S, const CgParams &prm, double s, double rz_dir, hipStream_t st,
+                        bool &ok);
 
 // dst = (float)(src * s);  x += s * (double)d; hp (with slm): the FP32
 // vectors' harmonic order, position k of a vertex's FP32 row holding

@@ -5,8 +5,8 @@
 // (CudaContext.cpp:175-198).  Here alpha, beta and the residual norm stay
 // in a small device array; one iteration is three launches
 //   apply A p (+ per-workgroup p.Ap)
-//   alpha = rz / p.Ap; x += alpha p, r -= alpha Ap, z = M r (+ r.r, r.z partials)
-//   beta = rz_new / rz, convergence test; p = z + beta p
+//   alpha = rz / p.Ap; r -= alpha Ap, z = M r (+ r.r, r.z partials)
+//   beta = rz_new / rz, convergence test; x += alpha p, p = z + beta p
 // (with a general preconditioner z = M r runs between the last two, with
 // its own r.z pass).  Every workgroup of the two vector launches sums the
 // previous launch's partials itself, in one fixed order, so all agree on
@@ -70,9 +70,10 @@ __device__ inline void sum_parts(const double *part, int n, int n2, double &a, d
     block_sum2(a, b);
 }
 
-// x += alpha p, r -= alpha Ap, z = M r and the r.r, r.z partials, alpha =
-// rz / p.Ap from the nb p.Ap partials of the operator launch; workgroup 0
-// records p.Ap and alpha
+// r -= alpha Ap, z = M r and the r.r, r.z partials, alpha = rz / p.Ap from
+// the nb p.Ap partials of the operator launch; workgroup 0 records p.Ap,
+// alpha and whether the pass ran (SC_RUN).  With x != null also x +=
+// alpha p (else the p update pass does it, reading p there anyway).
 // With z1 != null also z1 = cscale cdinv r (the first Chebyshev iterate of
 // a polynomial preconditioner) from the new r.
 template <typename T>
@@ -82,7 +83,10 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
                                                       double *__restrict__ sc, const double *__restrict__ pap_part,
                                                       int nb, int rzs, double *__restrict__ part,
                                                       const T *__restrict__ cdinv, T cscale, T *__restrict__ z1) {
-    if (sc[SC_CONV] != 0.0) return;
+    if (sc[SC_CONV] != 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_RUN] = 0.0;
+        return;
+    }
     __shared__ double al_sh;
     {
         double pap, dummy;
@@ -92,6 +96,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
             if (blockIdx.x == 0) {
                 sc[SC_PAP] = pap;
                 sc[SC_ALPHA] = al_sh;
+                sc[SC_RUN] = 1.0;
             }
         }
         __syncthreads();
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
     const T alpha = (T)al_sh;
     double rr = 0, rz = 0;
     auto one = [&](size_t i) __attribute__((always_inline)) {
-        x[i] = fma(alpha, p[i], x[i]);
+        if (x) x[i] = fma(alpha, p[i], x[i]);
         const T ri = fma(-alpha, Ap[i], r[i]);
         r[i] = ri;
         if (z1) z1[i] = cscale * cdinv[i] * ri;
@@ -119,7 +124,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
         const d2 *dd2 = reinterpret_cast<const d2 *>(dinv), *c2 = reinterpret_cast<const d2 *>(cdinv);
         d2 *z2 = reinterpret_cast<d2 *>(z1);
         for (size_t k = i; k < n2; k += S) {
-            x2[k] = alpha * p2[k] + x2[k];
+            if (x) x2[k] = alpha * p2[k] + x2[k];
             const d2 rv = r2[k] - alpha * a2[k];
             r2[k] = rv;
             if (z1) z2[k] = cscale * c2[k] * rv;
@@ -140,7 +145,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
         const f4 *d4 = reinterpret_cast<const f4 *>(dinv), *c4 = reinterpret_cast<const f4 *>(cdinv);
         f4 *z4 = reinterpret_cast<f4 *>(z1);
         for (size_t k = i; k < n4; k += S) {
-            x4[k] = alpha * p4[k] + x4[k];
+            if (x) x4[k] = alpha * p4[k] + x4[k];
             const f4 rv = r4[k] - alpha * a4[k];
             r4[k] = rv;
             if (z1) z4[k] = cscale * c4[k] * rv;
@@ -162,17 +167,21 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
 
 // beta = rz_new / rz from the G r.r / r.z partials (rz in slot rz_old, the
 // new value into slot rz_new by workgroup 0, with the iteration count and
-// the convergence flag); p = z + beta p, z = M r (dinv) or z given
-// (rr partials part[0 .. G), r.z partials part[TR_UPD_BLOCKS ..
-// TR_UPD_BLOCKS + Grz))
+// the convergence flag); with x != null x += alpha p; p = z + beta p, z =
+// M r (dinv) or z given (rr partials part[0 .. G), r.z partials
+// part[TR_UPD_BLOCKS .. TR_UPD_BLOCKS + Grz)).  It runs when the
+// iteration's update pass ran (SC_RUN, which no workgroup of this launch
+// writes), so every workgroup updates x, also in the iteration that stops
+// the solve.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const T *__restrict__ r,
                                                        const T *__restrict__ dinv, const T *__restrict__ z,
-                                                       size_t n, double *__restrict__ sc,
+                                                       T *__restrict__ x, size_t n, double *__restrict__ sc,
                                                        const double *__restrict__ part, int G, int Grz, int rz_old,
                                                        int rz_new, int maxit, int stall) {
-    if (sc[SC_CONV] != 0.0) return;
-    __shared__ double be_sh;
+    if (sc[SC_RUN] == 0.0) return;
+    __shared__ double be_sh, al_sh;
+    __shared__ int stop_sh;
     {
         double rr, rz, dummy;
         if (Grz == G) {
@@ -184,6 +193,13 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
         }
         if (threadIdx.x == 0) {
             be_sh = rz / sc[rz_old];
+            al_sh = sc[SC_ALPHA];
+            // converged or broken down: every workgroup reaches this from
+            // the same sums and from scalars no workgroup of this launch
+            // writes, so none of them updates p, and the stopped solve
+            // keeps its last direction whole (a resumed solve continues
+            // from it)
+            stop_sh = !(rr == rr) || !(rz == rz) || !(sc[SC_PAP] > 0.0) || rr <= sc[SC_THR];
             if (blockIdx.x == 0) {
                 sc[SC_RR] = rr;
                 sc[SC_BETA] = be_sh;
@@ -205,33 +221,47 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
         }
         __syncthreads();
     }
-    const T beta = (T)be_sh;
+    const T beta = (T)be_sh, alpha = (T)al_sh;
+    const bool pupd = !stop_sh;
+    if (!pupd && !x) return;
     auto one = [&](size_t i) __attribute__((always_inline)) {
-        const T zi = z ? z[i] : (dinv ? dinv[i] * r[i] : r[i]);
-        p[i] = fma(beta, p[i], zi);
+        const T pi = p[i];
+        if (x) x[i] = fma(alpha, pi, x[i]);
+        if (pupd) {
+            const T zi = z ? z[i] : (dinv ? dinv[i] * r[i] : r[i]);
+            p[i] = fma(beta, pi, zi);
+        }
     };
     const size_t S = (size_t)gridDim.x * blockDim.x;
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if constexpr (sizeof(T) == 8) {
         typedef double d2 __attribute__((ext_vector_type(2)));
         const size_t n2 = n / 2;
-        d2 *p2 = reinterpret_cast<d2 *>(p);
+        d2 *p2 = reinterpret_cast<d2 *>(p), *x2 = reinterpret_cast<d2 *>(x);
         const d2 *r2 = reinterpret_cast<const d2 *>(r), *dd2 = reinterpret_cast<const d2 *>(dinv);
         const d2 *zz2 = reinterpret_cast<const d2 *>(z);
         for (size_t k = i; k < n2; k += S) {
-            const d2 zv = z ? zz2[k] : (dinv ? dd2[k] * r2[k] : r2[k]);
-            p2[k] = beta * p2[k] + zv;
+            const d2 pv = p2[k];
+            if (x) x2[k] = alpha * pv + x2[k];
+            if (pupd) {
+                const d2 zv = z ? zz2[k] : (dinv ? dd2[k] * r2[k] : r2[k]);
+                p2[k] = beta * pv + zv;
+            }
         }
         for (size_t k = 2 * n2 + i; k < n; k += S) one(k);
     } else if constexpr (sizeof(T) == 4) {
         typedef float f4 __attribute__((ext_vector_type(4)));
         const size_t n4 = n / 4;
-        f4 *p4 = reinterpret_cast<f4 *>(p);
+        f4 *p4 = reinterpret_cast<f4 *>(p), *x4 = reinterpret_cast<f4 *>(x);
         const f4 *r4 = reinterpret_cast<const f4 *>(r), *d4 = reinterpret_cast<const f4 *>(dinv);
         const f4 *zz4 = reinterpret_cast<const f4 *>(z);
         for (size_t k = i; k < n4; k += S) {
-            const f4 zv = z ? zz4[k] : (dinv ? d4[k] * r4[k] : r4[k]);
-            p4[k] = beta * p4[k] + zv;
+            const f4 pv = p4[k];
+            if (x) x4[k] = alpha * pv + x4[k];
+            if (pupd) {
+                const f4 zv = z ? zz4[k] : (dinv ? d4[k] * r4[k] : r4[k]);
+                p4[k] = beta * pv + zv;
+            }
         }
         for (size_t k = 4 * n4 + i; k < n; k += S) one(k);
     }
@@ -279,6 +309,42 @@ __global__ __launch_bounds__(256) void k_rz(const T *__restrict__ r, const T *__
     if (threadIdx.x == 0) part[blockIdx.x] = rz;
 }
 
+// Residual replacement: partials of r.r, r.z and r.p for a new residual r
+// and the stopped solve's last direction p; z = M r given, or dinv r, or r
+template <typename T>
+__global__ __launch_bounds__(256) void k_replace_dots(const T *__restrict__ r, const T *__restrict__ z,
+                                                      const T *__restrict__ dinv, const T *__restrict__ p,
+                                                      size_t n, double *__restrict__ part) {
+    double rr = 0, rz = 0, rp = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double ri = (double)r[i];
+        const double zi = z ? (double)z[i] : (dinv ? (double)(dinv[i] * r[i]) : ri);
+        rr = fma(ri, ri, rr);
+        rz = fma(ri, zi, rz);
+        rp = fma(ri, (double)p[i], rp);
+    }
+    block_sum2(rr, rz);
+    double dummy = 0;
+    __syncthreads();
+    block_sum2(rp, dummy);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = rr;
+        part[TR_UPD_BLOCKS + blockIdx.x] = rz;
+        part[2 * TR_UPD_BLOCKS + blockIdx.x] = rp;
+    }
+}
+
+// p = b1 p + z - cr r  (z = M r given, or dinv r, or r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_replace_p(T *__restrict__ p, const T *__restrict__ r,
+                                                   const T *__restrict__ z, const T *__restrict__ dinv, size_t n,
+                                                   T b1, T cr) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const T zi = z ? z[i] : (dinv ? dinv[i] * r[i] : r[i]);
+        p[i] = fma(b1, p[i], fma(-cr, r[i], zi));
+    }
+}
+
 // sc from the init partials: rr, rz, bb, and the stopping threshold
 __global__ __launch_bounds__(256) void k_fin_init(const double *__restrict__ part, int n,
                                                   double *__restrict__ sc, double tol_abs,
@@ -321,19 +387,22 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
         e = (x);                                                                            \
         if (e != hipSuccess) return fail(TR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e)); \
     } while (0)
-    // r = b - A x ; z = M r ; p = z
-    if (prm.diag_op)
-        HC(launch_apply_diag<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
-    else
-        HC(launch_apply<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
     const bool gen = (bool)S.prec;
-    hipLaunchKernelGGL(k_init<T>, dim3(G), dim3(256), 0, st, S.b, S.r, S.p, gen ? nullptr : S.dinv, n, S.part);
-    if (gen) {
-        HC(S.prec(S.r, S.z, nullptr));
-        hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, S.p, n, S.part + TR_UPD_BLOCKS, nullptr);
+    if (!prm.resume) {
+        // r = b - A x ; z = M r ; p = z
+        if (prm.diag_op)
+            HC(launch_apply_diag<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
+        else
+            HC(launch_apply<T>(S.a, S.NC, S.map, S.x, S.r, nullptr, nullptr, st));
+        hipLaunchKernelGGL(k_init<T>, dim3(G), dim3(256), 0, st, S.b, S.r, S.p, gen ? nullptr : S.dinv, n, S.part);
+        if (gen) {
+            HC(S.prec(S.r, S.z, nullptr));
+            hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, S.p, n, S.part + TR_UPD_BLOCKS,
+                               nullptr);
+        }
+        hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, st, S.part, (int)G, S.sc, prm.tol_abs, prm.tol_rel);
+        HC(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, st, S.part, (int)G, S.sc, prm.tol_abs, prm.tol_rel);
-    HC(hipGetLastError());
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -359,7 +428,7 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
         // a polynomial preconditioner's first iterate comes out of the
         // update pass, and its last step may write the r.z partials itself
         const bool first = gen && S.prec_loop && S.pre_out;
-        hipLaunchKernelGGL(k_update_fused<T>, dim3(G), dim3(256), 0, st, S.x, S.p, S.r, S.Ap,
+        hipLaunchKernelGGL(k_update_fused<T>, dim3(G), dim3(256), 0, st, (T *)nullptr, S.p, S.r, S.Ap,
                            gen ? nullptr : S.dinv, n, S.sc, S.part, (int)nb, rz_old, S.part2,
                            first ? S.pre_dinv : (const T *)nullptr, S.pre_scale, first ? S.pre_out : (T *)nullptr);
         keep(hipGetLastError());
@@ -376,7 +445,7 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
                                    S.part2 + TR_UPD_BLOCKS, S.sc);
         }
         hipLaunchKernelGGL(k_pupdate_fused<T>, dim3(G), dim3(256), 0, st, S.p, S.r, gen ? nullptr : S.dinv,
-                           gen ? (const T *)S.z : nullptr, n, S.sc, S.part2, (int)G, Grz, rz_old, rz_new,
+                           gen ? (const T *)S.z : nullptr, S.x, n, S.sc, S.part2, (int)G, Grz, rz_old, rz_new,
                            prm.maxit, prm.stall_window);
         keep(hipGetLastError());
     }
@@ -408,13 +477,68 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
     res.rr = sc[SC_RR];
     res.bb = sc[SC_BB];
     res.flag = (int)sc[SC_CONV];
+    // iteration k (from 0) read the r.z of its direction from slot k % 2
+    res.rz_dir = res.iters > 0 ? sc[((res.iters - 1) & 1) ? SC_RZ2 : SC_RZ] : 0.0;
     res.seconds = std::chrono::duration<double>(t1 - t0).count();
+    return TR_OK;
+#undef HC
+}
+
+template <typename T>
+int cg_replace_residual(const CgSpace<T> &S, const CgParams &prm, double s, double rz_dir, hipStream_t st,
+                        bool &ok) {
+    ok = false;
+    const size_t n = S.n;
+    const unsigned G = upd_grid(n);
+    hipError_t e;
+#define HC(x)                                                                               \
+    do {                                                                                    \
+        e = (x);                                                                            \
+        if (e != hipSuccess) return fail(TR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e)); \
+    } while (0)
+    const bool gen = (bool)S.prec;
+    if (gen) HC(S.prec(S.r, S.z, nullptr));
+    hipLaunchKernelGGL(k_replace_dots<T>, dim3(G), dim3(256), 0, st, (const T *)S.r, gen ? (const T *)S.z : nullptr,
+                       gen ? nullptr : S.dinv, (const T *)S.p, n, S.part);
+    HC(hipGetLastError());
+    std::vector<double> part(3 * TR_UPD_BLOCKS);
+    HC(hipMemcpyAsync(part.data(), S.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    double rr = 0, rz = 0, rp = 0;
+    for (unsigned i = 0; i < G; i++) {
+        rr += part[i];
+        rz += part[TR_UPD_BLOCKS + i];
+        rp += part[2 * TR_UPD_BLOCKS + i];
+    }
+    if (!(rr > 0) || !(rz > 0) || !(rz_dir > 0) || !(s > 0) || !std::isfinite(rr) || !std::isfinite(rz) ||
+        !std::isfinite(rp) || !std::isfinite(s * rz_dir))
+        return TR_OK;
+    // the direction in the new scale is s p; take out its component along
+    // the new r (zero for the recursive residual), then p = z + beta (s p)
+    // with beta = r.z / (s^2 rz_dir)
+    const double b1 = rz / (s * rz_dir), c = rp / rr;
+    hipLaunchKernelGGL(k_replace_p<T>, dim3(G), dim3(256), 0, st, S.p, (const T *)S.r, gen ? (const T *)S.z : nullptr,
+                       gen ? nullptr : S.dinv, n, (T)b1, (T)(b1 * c));
+    HC(hipGetLastError());
+    double sc[SC_N] = {};
+    sc[SC_RZ] = rz;
+    sc[SC_RR] = rr;
+    sc[SC_BB] = rr;
+    double thr = prm.tol_abs * prm.tol_abs;
+    if (prm.tol_rel > 0) thr = std::max(thr, prm.tol_rel * prm.tol_rel * rr);
+    sc[SC_THR] = thr;
+    sc[SC_RRMIN] = rr;
+    sc[SC_RRWIN] = rr;
+    HC(hipMemcpyAsync(S.sc, sc, sizeof(sc), hipMemcpyHostToDevice, st));
+    HC(hipStreamSynchronize(st));
+    ok = true;
     return TR_OK;
 #undef HC
 }
 
 template int pcg_device<double>(const CgSpace<double> &, const CgParams &, hipStream_t, CgResult &);
 template int pcg_device<float>(const CgSpace<float> &, const CgParams &, hipStream_t, CgResult &);
+template int cg_replace_residual<float>(const CgSpace<float> &, const CgParams &, double, double, hipStream_t, bool &);
 
 }  // namespace tr
 
