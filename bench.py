@@ -334,6 +334,10 @@ def bench_c3(args):
         ctx.set_angular_precision(3)
         ctx.set_schedule(1, J=mfma32["J"], WPB=mfma32["WPB"], variant=mfma32["variant"])
 
+    def use_f32mfma():
+        ctx.set_angular_precision(4)
+        ctx.set_schedule(1, J=mfma32["J"], WPB=mfma32["WPB"], variant=mfma32["variant"])
+
     wall, op32 = time_operator(ctx, P, 1, steps, warmup, 4)
     _, op64 = time_operator(ctx, P, 0, steps, warmup, 8)
     if op64["schedule"]["variant"] == 5:
@@ -346,8 +350,10 @@ def bench_c3(args):
             k64["kernel_ms"] * 1e3, k64["achieved_gbs"]))
     use_exact()
     _, op32x = time_operator(ctx, P, 1, steps, warmup, 4)
+    use_f32mfma()
+    _, op32m = time_operator(ctx, P, 1, steps, warmup, 4)
     use_mfma()
-    for k, v in (("FP32/BF16", op32), ("FP64", op64), ("exact FP32", op32x)):
+    for k, v in (("FP32/BF16", op32), ("FP64", op64), ("exact FP32", op32x), ("FP32 matrix-core", op32m)):
         log(tag, "%s operator %s %.1f us (%.3g DOF*ordinate/s)" % (k, v["kernel"], v["kernel_ms"] * 1e3,
                                                                    v["dof_ordinate_per_s"]))
     # each FP32 operator's error against the FP64 one on a random vector
@@ -355,7 +361,7 @@ def bench_c3(args):
     ctx.copyToDev(f, np.random.default_rng(1).standard_normal(ctx.N()))
     ctx.computeLhs(f, a64)
     A64 = ctx.copyToHost(a64)
-    for d, setup in ((op32x, use_exact), (op32, use_mfma)):
+    for d, setup in ((op32x, use_exact), (op32m, use_f32mfma), (op32, use_mfma)):
         setup()
         ctx.computeLhsPrec(1, f, a32)
         d["rel_error_vs_fp64"] = float(np.linalg.norm(ctx.copyToHost(a32) - A64) / np.linalg.norm(A64))
@@ -377,6 +383,7 @@ def bench_c3(args):
     out["operator_fp32_bf16"] = op32
     out["operator_fp64"] = op64
     out["operator_fp32_exact"] = op32x
+    out["operator_fp32_matrix_core"] = op32m
     out["kernel_source_digest"] = source_digest()
     if not args.no_solve:
         # Chebyshev degree 6 over [lambda_max / 300, lambda_max], each step one
@@ -390,7 +397,9 @@ def bench_c3(args):
                     "mixed_bf16mfma_ir_plain_restart": (use_mfma, dict(use_prec=0, inner_tol=1e-5,
                                                                        replacement=False)),
                     "mixed_bf16mfma_ir_jacobi": (use_mfma, dict(use_prec=2, inner_tol=1e-2)),
-                    "mixed_bf16mfma_ir_chebyshev": (use_mfma, dict(use_prec=4, inner_tol=1e-4))}
+                    "mixed_bf16mfma_ir_chebyshev": (use_mfma, dict(use_prec=4, inner_tol=1e-4)),
+                    "mixed_f32mfma_ir_plain": (use_f32mfma, dict(use_prec=0, inner_tol=1e-2)),
+                    "mixed_f32mfma_ir_chebyshev": (use_f32mfma, dict(use_prec=4, inner_tol=1e-2))}
         runs = [("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
                 ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
                 ("pcg_fp64_device_chebyshev", ta.TR_SOLVER_PCG_FP64, 4, {})]

@@ -233,9 +233,11 @@ int tr_ctx_set_chebyshev(tr_ctx *ctx, int degree, double ratio, int step_split);
 /* Angular precision of the FP32 operator used by TR_SOLVER_MIXED:
  * 0 = FP32 throughout, 1 = BF16 matrix-core projection with FP32
  * accumulation, 3 (default) = BF16 with the hi/lo split (three BF16
- * products per term, ~16 significant bits; for 33..80 harmonics, whose
- * schedule keeps the couplings in BF16 only, the gathered values alone are
- * split: two products per term). */
+ * products per term, ~16 significant bits; two, the gathered values alone
+ * split, where the couplings' lo parts do not fit the LDS), 4 = for 33..80
+ * harmonics under the parity-class layout (one problem, K = 4 or 5): FP32
+ * couplings and products on the FP32 matrix cores (error ~2e-7 of
+ * max|A f| against ~6e-6 for 3); elsewhere as 3. */
 int tr_ctx_set_angular_precision(tr_ctx *ctx, int prec);
 /* Robustness of TR_SOLVER_MIXED (extension; the reference has only its
  * iteration caps, main.cpp:142, 232): when a refinement step cuts the true

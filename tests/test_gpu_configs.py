@@ -70,7 +70,7 @@ def test_ball96_hg_operator_and_mixed_solve():
     ctx.deviceFree(d_x)
     ctx.set_schedule(0, J=default["J"], WPB=default["WPB"], variant=default["variant"])
     errs = {}
-    for prec, bar in ((0, 1e-5), (3, 2e-4)):
+    for prec, bar in ((0, 1e-5), (3, 2e-4), (4, 1e-6)):
         ctx.set_angular_precision(prec)
         ctx.computeLhsPrec(1, d_f, d_r)
         errs[prec] = rel(ctx.copyToHost(d_r), want)

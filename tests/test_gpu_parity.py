@@ -183,7 +183,10 @@ def test_bf16_blocked_projection(name):
     significant bits: bar 2e-2 relative to max|A f|, observed 3e-3) or the
     hi/lo split of both (bar 2e-4, observed 6e-6); every workgroup size,
     under the static class layout (straight-line contraction, k_apply5s: the
-    default where it applies) and the masked block layout (k_apply5)."""
+    default where it applies) and the masked block layout (k_apply5).
+    Angular precision 4: FP32 blocks and products on the FP32 matrix cores
+    under the static layout (bar 1e-6, observed 2e-7 to 3e-7), the hi/lo
+    split under the masked one."""
     if name in cases():
         P, ctx, op = setup(name)
     else:
@@ -195,13 +198,15 @@ def test_bf16_blocked_projection(name):
     want = op.lhs(f)
     try:
         for layout in (0, 1):
-            for prec, bar in ((1, 2e-2), (3, 2e-4)):
+            for prec, bar in ((1, 2e-2), (3, 2e-4), (4, 1e-6 if layout == 0 else 2e-4)):
                 ctx.set_angular_precision(prec)
                 for wpb in (2, 4, 8):
                     ctx.set_schedule(1, J=layout, WPB=wpb, variant=5)
                     kname = ctx.kernel_name(1)
                     # the classes' static layout covers K = 4, 5 (seven or ten channels)
                     assert kname.startswith("k_apply5s<" if layout == 0 else "k_apply5<"), kname
+                    if layout == 0:
+                        assert kname.endswith(", %d, 0>" % (4 if prec == 4 else 3 if prec == 3 else 1)), kname
                     ctx.computeLhsPrec(1, d_f, d_r)
                     e = rel(ctx.copyToHost(d_r), want)
                     print("[gpu-parity] %s bf16-blocked prec=%d WPB=%d rel err %.2e (%s)" % (

@@ -477,7 +477,8 @@ class HipContext:
         check(lib.tr_ctx_set_schedule(self._h, int(precision), int(J), int(WPB), int(variant)))
 
     def set_angular_precision(self, prec):
-        """0 FP32, 1 BF16 matrix-core projection, 3 BF16 hi/lo split"""
+        """0 FP32, 1 BF16 matrix-core projection, 3 BF16 hi/lo split, 4 FP32
+        matrix-core products (33..80 harmonics under the class layout)"""
         check(lib.tr_ctx_set_angular_precision(self._h, int(prec)))
 
     def set_refinement(self, stagnation=0.5, inner_stall_window=100):

@@ -239,10 +239,11 @@ static std::vector<unsigned char> mfma_tables5(const HostOperator &op, const std
 // The B blocks of the static class layout sl (kernels.h) for the class
 // order hp: every pair of the layout in cl_vol_index / cl_bnd_index order,
 // zero blocks included, hi parts then lo parts per section (volume, then
-// boundary).  False when a coupling outside the layout's pairs is not zero,
-// i.e. the layout does not describe this operator.
+// boundary), or (f32) the FP32 blocks per section.  False when a coupling
+// outside the layout's pairs is not zero, i.e. the layout does not
+// describe this operator.
 static bool mfma_tables5s(const HostOperator &op, const std::vector<uint16_t> &hp, int sl, Blk5 &b5,
-                          std::vector<unsigned char> &out) {
+                          std::vector<unsigned char> &out, bool f32) {
     const uint32_t slm = op.slm;
     const int NT = cl_nblk(sl), NC = (int)op.NC;
     if (slm != cl_slm(sl) || hp.size() != slm) return false;
@@ -255,6 +256,7 @@ static bool mfma_tables5s(const HostOperator &op, const std::vector<uint16_t> &h
     b5.nb = (uint32_t)NT;
     b5.nsplitB = 2;
     b5.stat = (uint32_t)sl;
+    b5.f32 = f32 ? 1u : 0u;
     for (int k = 0; k < NT; k++) {
         b5.h0[k] = (uint32_t)cl_h0(sl, k);
         b5.w[k] = (uint32_t)cl_w(sl, k);
@@ -268,14 +270,16 @@ static bool mfma_tables5s(const HostOperator &op, const std::vector<uint16_t> &h
                 for (uint32_t m = 0; m < slm; m++) big = std::max(big, std::fabs(mat(sec, c, n, m)));
         const double tiny = 1e-12 * big;
         std::vector<unsigned char> hi_part, lo_part;
+        uint32_t npairs = 0;
         for (int kb = 0; kb < NT; kb++)
             for (int c = 0; c < nch; c++)
                 for (int t = 0; t < NT; t++) {
-                    unsigned char bh[64 * 8], bl[64 * 8];
+                    unsigned char bh[64 * 8], bl[64 * 8], bf[64 * 16];
                     bool nz = false;
                     for (int l = 0; l < 64; l++) {
                         const uint32_t col = (uint32_t)(l % 16), q = (uint32_t)(l / 16);
                         uint16_t h[4], g[4];
+                        float xf[4];
                         for (int u = 0; u < 4; u++) {
                             const uint32_t kk = 4u * q + (uint32_t)u;
                             double x = 0.0;
@@ -284,19 +288,28 @@ static bool mfma_tables5s(const HostOperator &op, const std::vector<uint16_t> &h
                             nz = nz || x != 0.0;
                             h[u] = to_bf16((float)x);
                             g[u] = to_bf16((float)x - from_bf16(h[u]));
+                            xf[u] = (float)x;
                         }
                         std::memcpy(bh + l * 8, h, 8);
                         std::memcpy(bl + l * 8, g, 8);
+                        // lane l, element u: B[4 (l / 16) + u][l % 16], slice u
+                        // of the four FP32 products
+                        std::memcpy(bf + l * 16, xf, 16);
                     }
                     const bool pair = sec == 0 ? cl_vol_pair(sl, kb, c, t) : cl_bnd_pair(sl, kb, t);
                     if (!pair) {
                         if (nz) return false;
                         continue;
                     }
-                    hi_part.insert(hi_part.end(), bh, bh + sizeof bh);
-                    lo_part.insert(lo_part.end(), bl, bl + sizeof bl);
+                    npairs++;
+                    if (f32) {
+                        hi_part.insert(hi_part.end(), bf, bf + sizeof bf);
+                    } else {
+                        hi_part.insert(hi_part.end(), bh, bh + sizeof bh);
+                        lo_part.insert(lo_part.end(), bl, bl + sizeof bl);
+                    }
                 }
-        (sec == 0 ? b5.nvol : b5.nbnd) = (uint32_t)(hi_part.size() / sizeof(unsigned char[64 * 8]));
+        (sec == 0 ? b5.nvol : b5.nbnd) = npairs;
         out.insert(out.end(), hi_part.begin(), hi_part.end());
         out.insert(out.end(), lo_part.begin(), lo_part.end());
     }
@@ -447,7 +460,7 @@ static int build_tables5(tr_ctx *c) {
                 h0.size() == (size_t)cl_nblk(sl);
     for (size_t k = 0; stat && k < h0.size(); k++)
         stat = h0[k] == (uint32_t)cl_h0(sl, (int)k) && w[k] == (uint32_t)cl_w(sl, (int)k);
-    if (!(stat && mfma_tables5s(op, hp, sl, b5, tab))) {
+    if (!(stat && mfma_tables5s(op, hp, sl, b5, tab, c->ang_prec == 4))) {
         tab = mfma_tables5(op, hp, h0, w, true, b5);
         if ((size_t)b5.nvol * 1024 > 80 * 1024) tab = mfma_tables5(op, hp, h0, w, false, b5);
     }
@@ -2308,13 +2321,20 @@ extern "C" int tr_ctx_set_residual_replacement(tr_ctx *c, int on) {
 
 // Angular precision of the FP32 operator: 0 = FP32 arithmetic throughout,
 // 1 = BF16 matrix-core projection with FP32 accumulation, 3 = the same with
-// the hi/lo split (three BF16 products, ~16 significant bits).
-
+// the hi/lo split (three BF16 products, ~16 significant bits), 4 = FP32
+// products on the FP32 matrix cores where the static class layout applies
+// (33..80 harmonics), else as 3.
 extern "C" int tr_ctx_set_angular_precision(tr_ctx *c, int prec) {
-    if (!c || (prec != 0 && prec != 1 && prec != 3)) return fail(TR_ERR_ARG, "tr_ctx_set_angular_precision: 0, 1 or 3");
+    if (!c || (prec != 0 && prec != 1 && prec != 3 && prec != 4))
+        return fail(TR_ERR_ARG, "tr_ctx_set_angular_precision: 0, 1, 3 or 4");
     c->ang_prec = prec;
     if (c->have32) {
         const HostOperator &op = c->op;
+        // the K-blocked schedule's tables hold FP32 or BF16 blocks
+        if (v5_ok(op) && c->b5.stat && (c->b5.f32 != 0) != (prec == 4)) {
+            const int rc = build_tables5(c);
+            if (rc) return rc;
+        }
         if (prec != 0 && v4_ok(op))
             c->map32 = mfma_map_ctx(c, 4, 2);
         else if (prec != 0 && v5_ok(op))

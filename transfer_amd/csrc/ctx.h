@@ -200,8 +200,10 @@ struct tr_ctx {
                 : map32.variant != 4 ? f_tab
                                      : (map32.mfma_mode == 1 ? f_tab4 : (ang_prec == 1 ? f_tab4b1 : f_tab4b3));
         // variant 5 has B lo parts only when its stored blocks fit the LDS
-        // twice: else the A operand split at most
-        a.nsplit = ang_prec == 1 ? 1 : (map32.variant == 5 ? (b5.nsplitB == 2 ? 3 : 2) : 3);
+        // twice: else the A operand split at most; FP32 blocks: 4
+        a.nsplit = (map32.variant == 5 && b5.f32) ? 4
+                   : ang_prec == 1                ? 1
+                                                  : (map32.variant == 5 ? (b5.nsplitB == 2 ? 3 : 2) : 3);
         a.b5 = b5;
         a.bcrec = f_bcrec;
         a.mask = d.mask;

@@ -44,6 +44,9 @@ struct Blk5 {
     // static class layout (cl_* below) whose stored order the blocks follow,
     // 0 = none (vmask / voff describe them)
     uint32_t stat = 0;
+    // 1: FP32 blocks in the operand layout of v_mfma_f32_16x16x4f32 (static
+    // layouts only; 1 KB each, the bytes of both BF16 parts: nsplitB 2)
+    uint32_t f32 = 0;
 };
 
 // Static parity-class layouts of the K-blocked schedule.  The harmonics of

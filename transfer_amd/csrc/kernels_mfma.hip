@@ -814,8 +814,9 @@ hipError_t launch_apply5(const OpArgs<float> &a, int NC, int wpb, uint32_t grid,
     for (uint32_t k = 0; blocks_ok && k < a.b5.nb; k++)
         blocks_ok = a.b5.w[k] >= 1 && a.b5.w[k] <= 16 && a.b5.h0[k] + a.b5.w[k] <= a.slm;
     if ((a.W != 15 && a.W != 16) || a.WB > TR_WBMAX || !a.crec || !a.tab || (a.nB && !a.bcrec) || wpb < 1 ||
-        wpb > 8 || !mfma5_supported(a.slm) || a.nsplit < 1 || a.nsplit > 3 || !blocks_ok ||
-        (a.nsplit == 3 && a.b5.nsplitB != 2))
+        wpb > 8 || !mfma5_supported(a.slm) || a.nsplit < 1 || a.nsplit > 4 || !blocks_ok ||
+        (a.nsplit >= 3 && a.b5.nsplitB != 2) || ((a.nsplit == 4) != (a.b5.f32 != 0)) ||
+        (a.b5.f32 && !a.b5.stat))
         return hipErrorInvalidValue;
     if (cheb && (!cheb->rhs || !cheb->dinv)) return hipErrorInvalidValue;
     // the static layout: its blocks, slm and both B parts as the kernel assumes

@@ -70,41 +70,79 @@ __device__ __forceinline__ void split_bf16x4(const float (&x)[4], s4v &hi, s4v &
 // A_c B_(KB,c,t) for every pair of the layout, NCH channels; rd(o) reads
 // this lane's 8 bytes of the block at byte o of its section (hi parts,
 // lo parts lo bytes further).  BND: the three boundary channels against
-// the boundary blocks.
+// the boundary blocks.  NS 1..3: BF16 products (1: A_hi B_hi, 2: + A_lo
+// B_hi, 3: + A_hi B_lo); NS 4: FP32 products on the FP32 matrix cores,
+// the blocks in FP32 (1 KB each, block index times 1024 bytes; rd(o) then
+// reads this lane's 16 bytes): slice u of v_mfma_f32_16x16x4f32 takes
+// element u of this lane's A values, K index 4 (lane / 16) + u, against
+// B rows 4 (lane / 16) + u, so four products cover the 16 K indices with
+// the A values as they are.
 template <int SL, int NCH, int KB, bool BND, int NS, typename RD>
 __device__ __forceinline__ void contract5s(const float (&acc)[NCH][4], RD &&rd, uint32_t lo, f4v (&D)[cl_nblk(SL)]) {
-    static_for<NCH>([&](auto cc) {
-        constexpr int c = decltype(cc)::value;
-        s4v ah, al;
-        split_bf16x4(acc[c], ah, al);
-        static_for<cl_nblk(SL)>([&](auto tt) {
-            constexpr int t = decltype(tt)::value;
-            constexpr bool pair = BND ? cl_bnd_pair(SL, KB, t) : cl_vol_pair(SL, KB, c, t);
-            if constexpr (pair) {
-                constexpr uint32_t off = 512u * (uint32_t)(BND ? cl_bnd_index(SL, KB, c, t)
-                                                               : cl_vol_index(SL, NCH, KB, c, t));
-                const s4v bh = rd(off);
-                D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, D[t], 0, 0, 0);
-                if constexpr (NS > 1) {
-                    D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, D[t], 0, 0, 0);
-                    if constexpr (NS > 2) {
-                        const s4v bl = rd(off + lo);
-                        D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
+    if constexpr (NS == 4) {
+        constexpr int NT = cl_nblk(SL);
+        // the next channel's blocks load while this channel's products run
+        // (a product right behind its LDS read would wait out the read)
+        f4v bq[2][NT];
+        auto load = [&](auto cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(cc)::value;
+            static_for<NT>([&](auto tt) {
+                constexpr int t = decltype(tt)::value;
+                if constexpr (BND ? cl_bnd_pair(SL, KB, t) : cl_vol_pair(SL, KB, c, t))
+                    bq[c & 1][t] = rd(1024u * (uint32_t)(BND ? cl_bnd_index(SL, KB, c, t)
+                                                            : cl_vol_index(SL, NCH, KB, c, t)));
+            });
+        };
+        load(std::integral_constant<int, 0>{});
+        static_for<NCH>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            if constexpr (c + 1 < NCH) load(std::integral_constant<int, c + 1>{});
+            // slices outermost: consecutive products go to different output
+            // blocks where the channel couples to more than one
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                static_for<NT>([&](auto tt) {
+                    constexpr int t = decltype(tt)::value;
+                    if constexpr (BND ? cl_bnd_pair(SL, KB, t) : cl_vol_pair(SL, KB, c, t))
+                        D[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(acc[c][u], bq[c & 1][t][u], D[t], 0, 0, 0);
+                });
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    } else {
+        static_for<NCH>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            s4v ah, al;
+            split_bf16x4(acc[c], ah, al);
+            static_for<cl_nblk(SL)>([&](auto tt) {
+                constexpr int t = decltype(tt)::value;
+                constexpr bool pair = BND ? cl_bnd_pair(SL, KB, t) : cl_vol_pair(SL, KB, c, t);
+                if constexpr (pair) {
+                    constexpr uint32_t off = 512u * (uint32_t)(BND ? cl_bnd_index(SL, KB, c, t)
+                                                                   : cl_vol_index(SL, NCH, KB, c, t));
+                    const s4v bh = rd(off);
+                    D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, D[t], 0, 0, 0);
+                    if constexpr (NS > 1) {
+                        D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, D[t], 0, 0, 0);
+                        if constexpr (NS > 2) {
+                            const s4v bl = rd(off + lo);
+                            D[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, D[t], 0, 0, 0);
+                        }
                     }
                 }
-            }
+            });
+            // one channel's B reads at a time: hoisted above all the MFMAs they
+            // would hold every block in registers at once
+            __builtin_amdgcn_sched_barrier(0);
         });
-        // one channel's B reads at a time: hoisted above all the MFMAs they
-        // would hold every block in registers at once
-        __builtin_amdgcn_sched_barrier(0);
-    });
+    }
 }
 
 // Variant 5 under the static class layout SL (5 or 4; the vectors'
 // harmonics in that class order, slm = cl_slm(SL)).  a.tab: the volume
 // blocks' hi parts in cl_vol_index order, their lo parts, then the
-// boundary blocks likewise (cl_bnd_index); the volume section is staged in
-// LDS.  Lanes, tiles, grid and epilogues as k_apply5.
+// boundary blocks likewise (cl_bnd_index); with NS 4 the FP32 volume
+// blocks, then the FP32 boundary blocks (the same bytes); the volume
+// section is staged in LDS.  Lanes, tiles, grid and epilogues as k_apply5.
 template <int NC, int WT, int SL, int NS, int EPI>
 __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *__restrict__ f,
                                                  float *__restrict__ r, double *__restrict__ part,
@@ -246,10 +284,19 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
             if constexpr (kb == 0)
                 if (a.mf0 && valid && gq == 0) acc[0][0] *= a.mf0[v];
             to_mfma_layout<float, 2, 4, NC>(acc, lane);
-            const unsigned char *lb = smem_raw + (uint32_t)lane * 8u;
-            contract5s<SL, NC, kb, false, NS>(
-                acc, [&](uint32_t o) __attribute__((always_inline)) { return *reinterpret_cast<const s4v *>(lb + o); },
-                VLO, D);
+            if constexpr (NS == 4) {
+                const unsigned char *lb = smem_raw + (uint32_t)lane * 16u;
+                contract5s<SL, NC, kb, false, NS>(
+                    acc,
+                    [&](uint32_t o) __attribute__((always_inline)) { return *reinterpret_cast<const f4v *>(lb + o); },
+                    0u, D);
+            } else {
+                const unsigned char *lb = smem_raw + (uint32_t)lane * 8u;
+                contract5s<SL, NC, kb, false, NS>(
+                    acc,
+                    [&](uint32_t o) __attribute__((always_inline)) { return *reinterpret_cast<const s4v *>(lb + o); },
+                    VLO, D);
+            }
         });
         if (__any(b >= 0)) {
             uint32_t bn[TR_WBMAX];
@@ -267,13 +314,23 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
                                                             SLM, gq, 4u * (uint32_t)cl_h0(SL, kb), 0u,
                                                             (uint32_t)cl_w(SL, kb));
                 to_mfma_layout<float, 2, 4, 3>(bacc, lane);
-                const uint32_t lo8 = (uint32_t)lane * 8u;
-                contract5s<SL, 3, kb, true, NS>(
-                    bacc,
-                    [&](uint32_t o) __attribute__((always_inline)) {
-                        return __builtin_bit_cast(s4v, __builtin_amdgcn_raw_buffer_load_b64(gbrs, lo8, o, 0));
-                    },
-                    BLO, D);
+                if constexpr (NS == 4) {
+                    const uint32_t lo16 = (uint32_t)lane * 16u;
+                    contract5s<SL, 3, kb, true, NS>(
+                        bacc,
+                        [&](uint32_t o) __attribute__((always_inline)) {
+                            return __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(gbrs, lo16, o, 0));
+                        },
+                        0u, D);
+                } else {
+                    const uint32_t lo8 = (uint32_t)lane * 8u;
+                    contract5s<SL, 3, kb, true, NS>(
+                        bacc,
+                        [&](uint32_t o) __attribute__((always_inline)) {
+                            return __builtin_bit_cast(s4v, __builtin_amdgcn_raw_buffer_load_b64(gbrs, lo8, o, 0));
+                        },
+                        BLO, D);
+                }
             });
         }
         // ---- write: D rows 4(l/16) + i, column l % 16 of output block t
@@ -317,13 +374,16 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
 }
 
 // the static class-layout kernel for layout SL (Blk5::stat), BF16 split
-// ns (1..3) and epilogue EPI: ten or seven channels, 15 slots
+// ns (1..3) or FP32 products (4) and epilogue EPI: ten or seven channels,
+// 15 slots
 template <int EPI>
 static const void *apply5s_kernel(int NC, uint32_t W, uint32_t SL, int ns) {
 #define TR_K5S(NCC, SLL, NSS) reinterpret_cast<const void *>(&k_apply5s<NCC, 15, SLL, NSS, EPI>)
-#define TR_K5SN(NCC, SLL) (ns == 1 ? TR_K5S(NCC, SLL, 1) : (ns == 2 ? TR_K5S(NCC, SLL, 2) : TR_K5S(NCC, SLL, 3)))
+#define TR_K5SN(NCC, SLL)                                                                                    \
+    (ns == 1 ? TR_K5S(NCC, SLL, 1)                                                                          \
+             : (ns == 2 ? TR_K5S(NCC, SLL, 2) : (ns == 3 ? TR_K5S(NCC, SLL, 3) : TR_K5S(NCC, SLL, 4))))
 #define TR_K5SL(NCC) (SL == 5 ? TR_K5SN(NCC, 5) : TR_K5SN(NCC, 4))
-    if (W != 15 || (NC != 7 && NC != 10) || (SL != 4 && SL != 5) || ns < 1 || ns > 3) return nullptr;
+    if (W != 15 || (NC != 7 && NC != 10) || (SL != 4 && SL != 5) || ns < 1 || ns > 4) return nullptr;
     return NC == 7 ? TR_K5SL(7) : TR_K5SL(10);
 #undef TR_K5SL
 #undef TR_K5SN
