@@ -58,28 +58,32 @@ def source_digest():
     return h.hexdigest()[:16]
 
 
-def measured_traffic(kernel, workload):
+def measured_traffic(kernel, workload, wpb=None):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the
     factor the same counter reads on a plain copy of known size).  Rows
-    carry the source digest they were measured on; (None, reason) when no
-    row covers this kernel on these sources."""
+    carry the source digest they were measured on and the workgroup size
+    ("WPB n": the traffic depends on it); (None, reason) when no row covers
+    this kernel at this workgroup size on these sources."""
     fn = os.path.join(ROOT, "profiles", "traffic_%s.txt" % workload)
     if not os.path.exists(fn):
         return None, "no counter summary for %s" % workload
     dig = source_digest()
-    stale = False
+    why = "kernel not in the summary"
     for line in open(fn):
         parts = [x.strip() for x in line.split("|")]
         if len(parts) >= 4 and parts[0] == kernel:
             if parts[1] != dig:
-                stale = True
+                why = "summary measured on other kernel sources"
+                continue
+            if wpb is not None and (len(parts) < 8 or parts[2] != "WPB %d" % wpb):
+                why = "summary measured at another workgroup size"
                 continue
             try:
                 return float(parts[-1]), "profiles/traffic_%s.txt" % workload
             except ValueError:
                 return None, "unreadable row"
-    return None, ("summary measured on other kernel sources" if stale else "kernel not in the summary")
+    return None, why
 
 
 def _oracle_problem(P, nthreads=0):
@@ -368,7 +372,7 @@ def bench_c3(args):
     for v in (f, a64, a32):
         ctx.deviceFree(v)
     use_mfma()
-    traffic, tsrc = measured_traffic(op32["kernel"], P.name)
+    traffic, tsrc = measured_traffic(op32["kernel"], P.name, op32["schedule"]["WPB"])
     out = dict(metric=METRIC, value=op32["dof_ordinate_per_s"], unit="DOF*ordinate/s", n_gpus=1, steps=steps,
                warmup=warmup, ms_per_step=wall / steps * 1e3, higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="f32+bf16", data="synthetic",
@@ -398,8 +402,7 @@ def bench_c3(args):
                                                                        replacement=False)),
                     "mixed_bf16mfma_ir_jacobi": (use_mfma, dict(use_prec=2, inner_tol=1e-2)),
                     "mixed_bf16mfma_ir_chebyshev": (use_mfma, dict(use_prec=4, inner_tol=1e-4)),
-                    "mixed_f32mfma_ir_plain": (use_f32mfma, dict(use_prec=0, inner_tol=1e-2)),
-                    "mixed_f32mfma_ir_chebyshev": (use_f32mfma, dict(use_prec=4, inner_tol=1e-2))}
+                    "mixed_f32mfma_ir_plain": (use_f32mfma, dict(use_prec=0, inner_tol=1e-2))}
         runs = [("pcg_fp64_device", ta.TR_SOLVER_PCG_FP64, 0, {}),
                 ("pcg_fp64_device_jacobi", ta.TR_SOLVER_PCG_FP64, 2, {}),
                 ("pcg_fp64_device_chebyshev", ta.TR_SOLVER_PCG_FP64, 4, {})]
