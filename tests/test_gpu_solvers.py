@@ -27,6 +27,8 @@ def build(name):
         return ta.problems.ball(12, maxk=2, kind="hg")
     if name == "ball10-K5-hg":
         return ta.problems.ball(10, maxk=5, kind="hg")
+    if name == "ball16-K5":
+        return ta.problems.ball(16, maxk=5)
     raise KeyError(name)
 
 
@@ -88,6 +90,36 @@ def test_residual_replacement(name, prec, inner_tol):
         assert st.status == 0 and st.fallback == 0
         assert st.true_rel <= 1e-8 * 1.01 and res <= 1.5e-8
         assert err <= 1e-6
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["ball10-K5-hg", "ball16-K5"])
+@pytest.mark.parametrize("prec", [0, 2, 4])
+def test_fp32_matrix_core_operator_solves(name, prec):
+    """The mixed solver with the FP32 matrix-core operator (angular precision
+    4: k_apply5s with FP32 products), its fused p.Ap dot (no
+    preconditioner, Jacobi) and its fused Chebyshev steps, against the
+    oracle: the FP64 residual reached and re-checked by the oracle, the
+    oracle's solution to 1e-6, no fallback."""
+    P = build(name)
+    ctx = ta.HipContext(P.mesh, P.ang)
+    op = oracle.OracleProblem(P.mesh.arrays(), ang_dict(P.ang))
+    ctx.set_angular_precision(4)
+    d_f = ctx.allocVector()
+    st, _ = ctx.solve(d_f, solver=ta.TR_SOLVER_MIXED, use_prec=prec, maxit=30, tol_abs=0.0, tol_rel=1e-8,
+                      inner_maxit=20000, inner_tol=1e-2 if prec != 4 else 1e-4)
+    kname = ctx.kernel_name(1)
+    f = ctx.copyToHost(d_f)
+    b = op.rhs()
+    res = op.norm(b - op.lhs(f)) / op.norm(b)
+    ref, _, _ = op.pcg(tol_abs=0.0, tol_rel=1e-12, maxit=40000)
+    err = np.abs(f - ref).max() / np.abs(ref).max()
+    print("[gpu-solvers] %s FP32 matrix-core %s prec=%d: %d steps, %d inner, true_rel %.2e (oracle %.2e), "
+          "err_vs_fp64 %.2e" % (name, kname, prec, st.iters, st.preck, st.true_rel, res, err), flush=True)
+    assert kname.endswith(", 4, 0>"), kname
+    assert st.status == 0 and st.fallback == 0
+    assert st.true_rel <= 1e-8 * 1.01 and res <= 1.5e-8
+    assert err <= 1e-6
     ctx.close()
 
 
