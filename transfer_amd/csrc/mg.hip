@@ -131,28 +131,30 @@ static hipError_t vcycle(Multilevel &mg, size_t k, const T *r, T *z, const doubl
     return hipGetLastError();
 }
 
-// Largest eigenvalue of D^-1 A by power iteration (FP64, 20 steps).
+// Largest eigenvalue of D^-1 A by power iteration (FP64, 20 steps); the
+// norm and the normalisation on the device (one scalar read per step)
 int lambda_max(tr_ctx *c, const double *dinv, double *x, double *y, double &lam) {
     const size_t n = c->nint();
-    std::vector<double> h(n);
-    for (size_t i = 0; i < n; i++) h[i] = 0.5 + rng_uniform(11, 7, i);
-    HIPCHECK(hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    {
+        std::vector<double> h(n);
+        for (size_t i = 0; i < n; i++) h[i] = 0.5 + rng_uniform(11, 7, i);
+        // synchronous: h goes out of scope here
+        HIPCHECK(hipMemcpy(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
     lam = 0;
     for (int it = 0; it < 20; it++) {
         HIPCHECK(launch_apply<double>(c->args(), (int)c->op.NC, c->map, x, y, nullptr, nullptr, c->stream));
         hipLaunchKernelGGL(k_jac0<double>, dim3(grid_of(n)), dim3(256), 0, c->stream, y, dinv, 1.0, y, n, nullptr);
-        HIPCHECK(hipMemcpyAsync(h.data(), y, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipStreamSynchronize(c->stream));
+        HIPCHECK(launch_dot(y, y, n, c->d_part, c->d_red, c->stream));
         double s = 0;
-        for (double v : h) s += v * v;
+        HIPCHECK(hipMemcpyAsync(&s, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
         s = std::sqrt(s);
-        if (!(s > 0)) return fail(TR_ERR_HIP, "multilevel: power iteration broke down");
+        if (!(s > 0) || !std::isfinite(s)) return fail(TR_ERR_HIP, "power iteration broke down");
         // x has unit norm after the first step, so |D^-1 A x| estimates lambda
         if (it > 0) lam = s;
-        for (double &v : h) v /= s;
-        HIPCHECK(hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(launch_axpby(x, 0.0, y, 1.0 / s, n, c->stream));
     }
-    // h is local: the last copy must have read it
     HIPCHECK(hipStreamSynchronize(c->stream));
     return TR_OK;
 }
