@@ -1669,6 +1669,14 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
     cp.tol_rel = prm->inner_tol;
     cp.chunk = 16;
     cp.stall_window = c->ir_stall_window;
+    // one iteration graph for every refinement step of this solve
+    struct GraphKeep {
+        hipGraphExec_t g = nullptr;
+        ~GraphKeep() {
+            if (g) hipGraphExecDestroy(g);
+        }
+    } keep_graph;
+    cp.graph = &keep_graph.g;
     double *xb = w.get();  // the iterate before the last update
     if (!xb) return fail(TR_ERR_HIP, "tr_solve: out of device memory");
     int outer = 0, inner = 0, flag = TR_ERR_NOCONV;

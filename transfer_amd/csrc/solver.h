@@ -71,6 +71,10 @@ struct CgParams {
     // 1: continue a stopped solve: skip r = b - A x and p = z; the caller
     // has set r, p and the scalars (cg_replace_residual)
     int resume = 0;
+    // optional: the iteration graph kept across calls on the same space and
+    // parameters (captured on the first call; the caller destroys it with
+    // hipGraphExecDestroy); null: captured and destroyed per call
+    hipGraphExec_t *graph = nullptr;
 };
 
 struct CgResult {

@@ -405,60 +405,61 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
     }
 
     hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = prm.graph ? *prm.graph : nullptr;
     // an even number of iterations per graph: iteration k reads r.z from
     // slot k % 2 and writes slot (k + 1) % 2, and every launch starts at 0
     const int chunk = std::max(2, prm.chunk + (prm.chunk & 1));
-    HC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    // the first error of a launch made during the capture (the graph would
-    // silently lack that node otherwise)
-    hipError_t ce = hipSuccess;
-    auto keep = [&ce](hipError_t x) {
-        if (ce == hipSuccess && x != hipSuccess) ce = x;
-    };
-    for (int it = 0; it < chunk; it++) {
-        const int rz_old = (it & 1) ? SC_RZ2 : SC_RZ, rz_new = (it & 1) ? SC_RZ : SC_RZ2;
-        if (prm.diag_op)
-            keep(launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st));
-        else
-            keep(launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st));
-        if (!fdot)
-            hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, (const T *)S.p, (const T *)S.Ap, (T *)nullptr, n,
-                               S.part, S.sc);
-        // a polynomial preconditioner's first iterate comes out of the
-        // update pass, and its last step may write the r.z partials itself
-        const bool first = gen && S.prec_loop && S.pre_out;
-        hipLaunchKernelGGL(k_update_fused<T>, dim3(G), dim3(256), 0, st, (T *)nullptr, S.p, S.r, S.Ap,
-                           gen ? nullptr : S.dinv, n, S.sc, S.part, (int)nb, rz_old, S.part2,
-                           first ? S.pre_dinv : (const T *)nullptr, S.pre_scale, first ? S.pre_out : (T *)nullptr);
-        keep(hipGetLastError());
-        int Grz = (int)G;
-        if (gen) {
-            if (first) {
-                keep(S.prec_loop(S.r, S.z, S.sc + SC_CONV, S.part2 + TR_UPD_BLOCKS));
-                if (S.prec_nparts) Grz = S.prec_nparts;
-            } else {
-                keep(S.prec(S.r, S.z, S.sc + SC_CONV));
+    if (!exec) {  // capture the iteration graph (once, when the caller keeps it)
+        HC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        // the first error of a launch made during the capture (the graph would
+        // silently lack that node otherwise)
+        hipError_t ce = hipSuccess;
+        auto keep = [&ce](hipError_t x) {
+            if (ce == hipSuccess && x != hipSuccess) ce = x;
+        };
+        for (int it = 0; it < chunk; it++) {
+            const int rz_old = (it & 1) ? SC_RZ2 : SC_RZ, rz_new = (it & 1) ? SC_RZ : SC_RZ2;
+            if (prm.diag_op)
+                keep(launch_apply_diag<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st));
+            else
+                keep(launch_apply<T>(S.a, S.NC, S.map, S.p, S.Ap, fdot ? S.part : nullptr, S.sc + SC_CONV, st));
+            if (!fdot)
+                hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, (const T *)S.p, (const T *)S.Ap, (T *)nullptr, n,
+                                   S.part, S.sc);
+            // a polynomial preconditioner's first iterate comes out of the
+            // update pass, and its last step may write the r.z partials itself
+            const bool first = gen && S.prec_loop && S.pre_out;
+            hipLaunchKernelGGL(k_update_fused<T>, dim3(G), dim3(256), 0, st, (T *)nullptr, S.p, S.r, S.Ap,
+                               gen ? nullptr : S.dinv, n, S.sc, S.part, (int)nb, rz_old, S.part2,
+                               first ? S.pre_dinv : (const T *)nullptr, S.pre_scale, first ? S.pre_out : (T *)nullptr);
+            keep(hipGetLastError());
+            int Grz = (int)G;
+            if (gen) {
+                if (first) {
+                    keep(S.prec_loop(S.r, S.z, S.sc + SC_CONV, S.part2 + TR_UPD_BLOCKS));
+                    if (S.prec_nparts) Grz = S.prec_nparts;
+                } else {
+                    keep(S.prec(S.r, S.z, S.sc + SC_CONV));
+                }
+                if (!(first && S.prec_nparts))
+                    hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, (T *)nullptr, n,
+                                       S.part2 + TR_UPD_BLOCKS, S.sc);
             }
-            if (!(first && S.prec_nparts))
-                hipLaunchKernelGGL(k_rz<T>, dim3(G), dim3(256), 0, st, S.r, S.z, (T *)nullptr, n,
-                                   S.part2 + TR_UPD_BLOCKS, S.sc);
+            hipLaunchKernelGGL(k_pupdate_fused<T>, dim3(G), dim3(256), 0, st, S.p, S.r, gen ? nullptr : S.dinv,
+                               gen ? (const T *)S.z : nullptr, S.x, n, S.sc, S.part2, (int)G, Grz, rz_old, rz_new,
+                               prm.maxit, prm.stall_window);
+            keep(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_pupdate_fused<T>, dim3(G), dim3(256), 0, st, S.p, S.r, gen ? nullptr : S.dinv,
-                           gen ? (const T *)S.z : nullptr, S.x, n, S.sc, S.part2, (int)G, Grz, rz_old, rz_new,
-                           prm.maxit, prm.stall_window);
-        keep(hipGetLastError());
-    }
-    e = hipStreamEndCapture(st, &graph);
-    if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    if (ce != hipSuccess) {
+        e = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        if (ce != hipSuccess) {
+            hipGraphDestroy(graph);
+            return fail(TR_ERR_HIP, std::string("graph capture: a captured launch failed: ") + hipGetErrorString(ce));
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
-        return fail(TR_ERR_HIP, std::string("graph capture: a captured launch failed: ") + hipGetErrorString(ce));
-    }
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        hipGraphDestroy(graph);
-        return fail(TR_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        if (e != hipSuccess) return fail(TR_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        if (prm.graph) *prm.graph = exec;
     }
     double sc[SC_N];
     int launches = 0;
@@ -470,8 +471,7 @@ int pcg_device(const CgSpace<T> &S, const CgParams &prm, hipStream_t st, CgResul
         HC(hipGraphLaunch(exec, st));
         launches++;
     }
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
+    if (!prm.graph) hipGraphExecDestroy(exec);
     auto t1 = std::chrono::steady_clock::now();
     res.iters = (int)sc[SC_ITER];
     res.rr = sc[SC_RR];
