@@ -155,6 +155,10 @@ def bench_batch(count, steps, warmup):
     print("[bench-batch] %d problems, %d vertices, %d tets; meshes %.1f s, assembly+upload %.1f s"
           % (count, nP, nT, t1 - t0, t2 - t1), file=sys.stderr, flush=True)
     ctx.autotune(0)
+    # FP32 products on the matrix cores: as fast as the BF16 split on the
+    # union (2.49 against 2.50 ms) and exact to FP32, which the Chebyshev
+    # steps need (mixed Chebyshev 4.84 s against 6.95 s)
+    ctx.set_angular_precision(0)
     ctx.autotune(1)
     out = dict(metric="batched time-to-1e-8 residual & DOF*ordinate/s", unit="DOF*ordinate/s", n_gpus=1,
                steps=steps, warmup=warmup, higher_is_better=True, data="synthetic", dtype="f32+f64",

@@ -89,12 +89,12 @@ LaneMap choose_map(uint32_t slm, int maxJ) {
 // Whether the matrix-core schedule covers this operator: ceil(slm/4) in
 // {2, 4, 7}, exactly 15 or 16 volume slots, at most TR_WBMAX boundary slots.
 static bool v4_ok(const HostOperator &op) {
-    // 32-bit buffer offsets: the vector, the neighbour table and the
-    // coefficient records (FP64 sizes) each stay below 4 GiB
+    // 32-bit buffer offsets: the vector, the neighbour table and one slot's
+    // coefficient records (FP64 sizes; a descriptor per slot) each stay
+    // below 4 GiB
     const double lim = 4294967295.0;
     if ((double)op.nP * op.slm * 8 >= lim || (double)op.W * op.nP * 4 >= lim ||
-        (double)op.W * op.nP * rec_stride<double>((int)op.NC) * 8 >= lim ||
-        (double)op.WB * op.nB * 4 * 8 >= lim)
+        (double)op.nP * rec_stride<double>((int)op.NC) * 8 >= lim || (double)op.WB * op.nB * 4 * 8 >= lim)
         return false;
     return mfma_supported(op.slm, 0) && (op.W == 15 || op.W == 16) && op.WB <= TR_WBMAX &&
            (op.NC == 7 || op.NC == 10);

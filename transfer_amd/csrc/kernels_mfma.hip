@@ -136,9 +136,12 @@ __global__ __launch_bounds__(512) void k_apply4(OpArgs<T> a, const T *__restrict
         for (int c = 0; c < NC; c++)
 #pragma unroll
             for (int j = 0; j < J; j++) acc[c][j] = 0;
+        // one descriptor per slot: a slot's records stay below 4 GiB where
+        // all W of them need not (unions of many problems)
         if (valid)
-            gather<T, J, NC, RS, WT, DEPTH>(acc, nbc, frs, crs, nP * (uint32_t)(RS * sizeof(T)),
-                                             v * (uint32_t)(RS * sizeof(T)), WT, slm, gq);
+            gather<T, J, NC, RS, WT, DEPTH, true>(acc, nbc, frs, crs, nP * (uint32_t)(RS * sizeof(T)),
+                                                   v * (uint32_t)(RS * sizeof(T)), WT, slm, gq, 0u, 0u, 0xffffffffu,
+                                                   a.crec);
         // per-vertex factor of the mass channel at lm = 0 (isotropic albedo
         // per problem in a batch: 1 - albedo, with massfac[0] = 1); harmonic
         // 0 is register 0 of chunk 0

@@ -173,11 +173,16 @@ __device__ inline double quad_bcast(double x, int src) {
 // pin each slot's FMAs before the next load offsets are formed.
 // hoff / h0: byte and harmonic offset of the harmonic block gathered (the
 // K-blocked schedule of kernels with more than 32 harmonics).
-template <typename T, int J, int NCH, int RS, int NS, int DEPTH>
+// SLOTRS: the records of slot s are read through a descriptor of their own
+// (base cbase + s * cstride bytes, cstride bytes long) instead of crs at
+// offset s * cstride, so that one slot's records, not all of them, must fit
+// the 32-bit buffer offsets (unions of many problems).  The descriptors are
+// formed per call from the opaque stride, one slot at a time.
+template <typename T, int J, int NCH, int RS, int NS, int DEPTH, bool SLOTRS = false>
 __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS], __amdgpu_buffer_rsrc_t frs,
                                        __amdgpu_buffer_rsrc_t crs, uint32_t cstride, uint32_t coff, int nslot,
                                        uint32_t slm, int q, uint32_t hoff = 0, uint32_t h0 = 0,
-                                       uint32_t hw = 0xffffffffu) {
+                                       uint32_t hw = 0xffffffffu, const T *cbase = nullptr) {
     constexpr int E = 16 / (int)sizeof(T), PE = (RS + 3) / 4;
     // opaque here, so that the per-slot products and tests are formed where
     // used instead of hoisted out of the tile loop (kept live, they spill)
@@ -186,13 +191,19 @@ __device__ __forceinline__ void gather(T (&acc)[NCH][J], const uint32_t (&nb)[NS
     T xs[DEPTH][J], cs[DEPTH][PE];
     auto load = [&](int s, T *x, T *cf) __attribute__((always_inline)) {
         uint32_t base = nb[s] * slm * (uint32_t)sizeof(T) + 16u * (uint32_t)q + hoff;
-        uint32_t cv = coff + (uint32_t)(q * PE * (int)sizeof(T)) + (uint32_t)s * cstride;
+        uint32_t cv = coff + (uint32_t)(q * PE * (int)sizeof(T)) + (SLOTRS ? 0u : (uint32_t)s * cstride);
         asm volatile("" : "+v"(base), "+v"(cv));
 #pragma unroll
         for (int j = 0; j < J; j++)
             x[j] = bload1<T>(frs, base + (uint32_t)((64 * (j / E) + (j % E) * (int)sizeof(T))), 0u);
+        if constexpr (SLOTRS) {
+            const auto cs1 = rsrc4(reinterpret_cast<const unsigned char *>(cbase) + (size_t)s * cstride, cstride);
 #pragma unroll
-        for (int e = 0; e < PE; e++) cf[e] = bload1<T>(crs, cv + (uint32_t)(e * (int)sizeof(T)), 0u);
+            for (int e = 0; e < PE; e++) cf[e] = bload1<T>(cs1, cv + (uint32_t)(e * (int)sizeof(T)), 0u);
+        } else {
+#pragma unroll
+            for (int e = 0; e < PE; e++) cf[e] = bload1<T>(crs, cv + (uint32_t)(e * (int)sizeof(T)), 0u);
+        }
     };
     auto accum = [&](const T *x, const T *cf) __attribute__((always_inline)) {
 #pragma unroll
