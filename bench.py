@@ -164,10 +164,15 @@ def bench_batch(count, steps, warmup):
                steps=steps, warmup=warmup, higher_is_better=True, data="synthetic", dtype="f32+f64",
                config=dict(workload="batch%d: slabs n x n x 4n, n~U{8..32}, ln kappa~N(0,1), albedo~U[0,0.95], K=2"
                                     % count, nP=nP, nT=nT, slm=slm, problems=count))
-    for prec, key in ((0, "operator_fp64"), (1, "operator_fp32")):
+    nF = sum(m.nF for m in B.meshes)
+    for prec, key, p in ((0, "operator_fp64", 8), (1, "operator_fp32", 4)):
         w, k, _, _ = ctx.bench_operator(warmup, steps, precision=prec)
-        out[key] = dict(kernel=ctx.kernel_name(prec), kernel_ms=k, dof_ordinate_per_s=nP * slm * steps / w)
-        print("[bench-batch] %s %s %.3f ms" % (key, ctx.kernel_name(prec), k), file=sys.stderr, flush=True)
+        b = operator_bytes(nT, nP, nF, slm, p)
+        out[key] = dict(kernel=ctx.kernel_name(prec), kernel_ms=k, dof_ordinate_per_s=nP * slm * steps / w,
+                        algorithmic_bytes=b, achieved_gbs=b / (k * 1e-3) / 1e9,
+                        hbm_frac=b / (k * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        print("[bench-batch] %s %s %.3f ms (%.0f GB/s algorithmic, %.3f of HBM)" % (
+            key, ctx.kernel_name(prec), k, out[key]["achieved_gbs"], out[key]["hbm_frac"]), file=sys.stderr, flush=True)
     out["value"] = out["operator_fp32"]["dof_ordinate_per_s"]
     out["ms_per_step"] = out["operator_fp32"]["kernel_ms"]
     solves = {}
