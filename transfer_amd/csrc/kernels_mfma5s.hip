@@ -49,6 +49,48 @@ __device__ __forceinline__ void slot_accum(float (&acc)[NC][4], const float *cw,
         for (int j = 0; j < 4; j++) asm volatile("" : "+v"(acc[c][j]));
 }
 
+// The gather of a block five harmonics wide (harmonics h0 .. h0 + 4, the
+// second block of class 0 at K = 5): lane gq of a vertex's quad takes
+// harmonic h0 + gq for every channel (acc1, the coefficients broadcast as
+// in slot_accum) and harmonic h0 + 4 for its own record's channels
+// gq * PE + e (acc4, no broadcast); x[0], x[1] hold those two values of the
+// neighbour row.  NC + PE FMAs and NC moves per slot instead of 4 NC FMAs
+// and NC moves.
+template <int PE, int NC, bool FIRST>
+__device__ __forceinline__ void slot_accum5(float (&acc1)[NC], float (&acc4)[PE], const float *cw, const float *x) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        float w = quad_bcast(cw[c % PE], c / PE);
+        asm volatile("" : "+v"(w));
+        acc1[c] = FIRST ? w * x[0] : fmaf(w, x[0], acc1[c]);
+    }
+#pragma unroll
+    for (int e = 0; e < PE; e++) acc4[e] = FIRST ? cw[e] * x[1] : fmaf(cw[e], x[1], acc4[e]);
+#pragma unroll
+    for (int c = 0; c < NC; c++) asm volatile("" : "+v"(acc1[c]));
+#pragma unroll
+    for (int e = 0; e < PE; e++) asm volatile("" : "+v"(acc4[e]));
+}
+
+// The five-wide block's sums in the MFMA operand layout (lane v + 16 q,
+// element u: K index 4 q + u of vertex v): K 0..3 from acc1 of the vertex's
+// lanes 0..3, K 4 (q = 1, u = 0) from acc4 of lane c / PE, the rest zero.
+template <int PE, int NC>
+__device__ __forceinline__ void narrow_to_mfma(const float (&acc1)[NC], const float (&acc4)[PE], float (&acc)[NC][4],
+                                               int lane) {
+    const int v = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        float t[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] = bperm(4 * (4 * v + u), acc1[c]);
+        const float t4 = bperm(4 * (4 * v + c / PE), acc4[c % PE]);
+        acc[c][0] = q == 0 ? t[0] : (q == 1 ? t4 : 0.0f);
+#pragma unroll
+        for (int u = 1; u < 4; u++) acc[c][u] = q == 0 ? t[u] : 0.0f;
+    }
+}
+
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef __bf16 b2v __attribute__((ext_vector_type(2)));
 
@@ -249,8 +291,19 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
                 }
         }
         float xs[DEPTH][4];
+        const uint32_t nq = 12u * (uint32_t)gq, nq4 = 16u * (uint32_t)gq;
+        // ro[] holds this lane's 16-byte piece (+16 gq bytes); the five-wide
+        // block reads harmonics h0 + gq and h0 + 4 instead
         auto ld = [&](int g, float *x) __attribute__((always_inline)) {
             const int kb = g / WT, s2 = g % WT;
+            if (cl_w(SL, kb) == 5) {
+                x[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                     frs, ro[s2] + 4u * (uint32_t)cl_h0(SL, kb) - nq, 0u, 0));
+                x[1] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                     frs, ro[s2] + 4u * (uint32_t)cl_h0(SL, kb) + 16u - nq4, 0u, 0));
+                x[2] = 0; x[3] = 0;
+                return;
+            }
             const f32x4m t4 = __builtin_bit_cast(
                 f32x4m, __builtin_amdgcn_raw_buffer_load_b128(frs, ro[s2] + 4u * (uint32_t)cl_h0(SL, kb), 0u, 0));
             x[0] = t4.x;
@@ -263,18 +316,44 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
         static_for<NT>([&](auto kbc) {
             constexpr int kb = decltype(kbc)::value;
             float acc[NC][4];
+            if constexpr (cl_w(SL, kb) == 5) {
+                float acc1[NC], acc4[PE];
 #pragma unroll
-            for (int s2 = 0; s2 < WT; s2++) {
-                const int g = kb * WT + s2;
-                const float *cw = cf[s2];
-                if (s2 == 0)
-                    slot_accum<PE, NC, true>(acc, cw, xs[g % DEPTH]);
-                else
-                    slot_accum<PE, NC, false>(acc, cw, xs[g % DEPTH]);
-                if (g + DEPTH < NG) ld(g + DEPTH, xs[g % DEPTH]);
+                for (int s2 = 0; s2 < WT; s2++) {
+                    const int g = kb * WT + s2;
+                    const float *cw = cf[s2];
+                    if (s2 == 0)
+                        slot_accum5<PE, NC, true>(acc1, acc4, cw, xs[g % DEPTH]);
+                    else
+                        slot_accum5<PE, NC, false>(acc1, acc4, cw, xs[g % DEPTH]);
+                    if (g + DEPTH < NG) ld(g + DEPTH, xs[g % DEPTH]);
+                }
+                narrow_to_mfma<PE, NC>(acc1, acc4, acc, lane);
+            } else {
+#pragma unroll
+                for (int s2 = 0; s2 < WT; s2++) {
+                    const int g = kb * WT + s2;
+                    const float *cw = cf[s2];
+                    if (s2 == 0)
+                        slot_accum<PE, NC, true>(acc, cw, xs[g % DEPTH]);
+                    else
+                        slot_accum<PE, NC, false>(acc, cw, xs[g % DEPTH]);
+                    if (g + DEPTH < NG) ld(g + DEPTH, xs[g % DEPTH]);
+                }
             }
             // harmonics past the block's width read the next block's values
-            if constexpr (cl_w(SL, kb) < 16) {
+            if constexpr (cl_w(SL, kb) == 5) {
+                // the five-wide block is in the MFMA layout already (K index
+                // 4 (lane / 16) + j), its entries past K 4 zero: this leaves
+                // them so.  Without these selects the compiler's schedule of
+                // the kernel needs 256 VGPRs and spills (measured); with
+                // them 199
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (4 * (lane >> 4) + j >= 5)
+#pragma unroll
+                        for (int c = 0; c < NC; c++) acc[c][j] = 0;
+            } else if constexpr (cl_w(SL, kb) < 16) {
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     if (4 * gq + j >= cl_w(SL, kb))
@@ -283,7 +362,7 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
             }
             if constexpr (kb == 0)
                 if (a.mf0 && valid && gq == 0) acc[0][0] *= a.mf0[v];
-            to_mfma_layout<float, 2, 4, NC>(acc, lane);
+            if constexpr (cl_w(SL, kb) != 5) to_mfma_layout<float, 2, 4, NC>(acc, lane);
             if constexpr (NS == 4) {
                 const unsigned char *lb = smem_raw + (uint32_t)lane * 16u;
                 contract5s<SL, NC, kb, false, NS>(
