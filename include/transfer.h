@@ -249,12 +249,15 @@ int tr_ctx_set_angular_precision(tr_ctx *ctx, int prec);
  * norm has not dropped by 10% over `inner_stall_window` iterations (0 =
  * never).  Defaults 0.5 and 100. */
 int tr_ctx_set_refinement(tr_ctx *ctx, double stagnation, int inner_stall_window);
-/* TR_SOLVER_MIXED's refinement steps (extension; the reference has no
- * mixed solver): 1 (default) = each step after the first continues the
- * inner CG of the previous one, its residual replaced by the new true FP64
- * residual and its last search direction kept (a "reliable update",
- * Sleijpen and van der Vorst 1996); 0 = each step restarts the inner CG
- * from a zero correction. */
+/* TR_SOLVER_MIXED's refinement steps, in tr_solve and (per problem) in
+ * tr_batch_solve (extension; the reference has no mixed solver):
+ * 1 (default) = each step after the first continues the inner CG of the
+ * previous one, its residual replaced by the new true FP64 residual and its
+ * last search direction kept (a "reliable update", Sleijpen and van der
+ * Vorst 1996), while the true residual stays within 2% of the inner CG's
+ * recursive one (else that step restarts); 0 = each step restarts the
+ * inner CG from a zero correction.  inner_tol is then the inner reduction
+ * between two replacements: 1e-2 measured best. */
 int tr_ctx_set_residual_replacement(tr_ctx *ctx, int on);
 /* Returns the internal numbering: perm[new] = caller vertex id. */
 int tr_ctx_permutation(const tr_ctx *ctx, uint32_t *perm);

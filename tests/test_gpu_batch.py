@@ -58,20 +58,27 @@ def test_batch_operator_matches_each_problem():
     ctx.deviceFree(d_r)
 
 
-@pytest.mark.parametrize("solver", [ta.TR_SOLVER_PCG_FP64, ta.TR_SOLVER_MIXED])
+@pytest.mark.parametrize("solver,inner_tol,replace", [(ta.TR_SOLVER_PCG_FP64, 1e-4, True),
+                                                      (ta.TR_SOLVER_MIXED, 1e-4, True),
+                                                      (ta.TR_SOLVER_MIXED, 1e-2, True),
+                                                      (ta.TR_SOLVER_MIXED, 1e-2, False)])
 @pytest.mark.parametrize("prec", [2, 4])
-def test_batch_solve_matches_individual_solves(solver, prec):
+def test_batch_solve_matches_individual_solves(solver, inner_tol, replace, prec):
     """Every problem of a batched solve, with Jacobi (2) or the Chebyshev
     polynomial over the union (4), reaches the FP64 residual and the
-    solution of the same problem solved alone."""
+    solution of the same problem solved alone; the mixed solver with its
+    refinement steps continuing each problem's inner CG (residual
+    replacement, per problem) and restarting it."""
     B = small_batch()
-    print("[gpu-batch] solve solver=%d prec=%d" % (solver, prec), flush=True)
+    print("[gpu-batch] solve solver=%d prec=%d inner_tol=%g replacement=%d" % (solver, prec, inner_tol, replace),
+          flush=True)
     ctx = ta.Batch(B.meshes, B.ang, B.albedo)
     ctx.autotune(0)
     if solver == ta.TR_SOLVER_MIXED:
         ctx.autotune(1)
+    ctx.set_residual_replacement(replace)
     d_x = ctx.allocVector()
-    st, per = ctx.solve_batch(d_x, solver=solver, use_prec=prec, tol_rel=1e-8)
+    st, per = ctx.solve_batch(d_x, solver=solver, use_prec=prec, tol_rel=1e-8, inner_tol=inner_tol)
     assert st.status == 0
     x = ctx.copyToHost(d_x).reshape(ctx.nP, ctx.aslm)
     for p, m in enumerate(B.meshes):

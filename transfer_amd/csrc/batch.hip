@@ -33,7 +33,9 @@ namespace tr {
 #define TR_BLK_THREADS 256
 
 // per-problem scalar slots
-enum { BS_RZ, BS_RR, BS_BB, BS_ALPHA, BS_BETA, BS_THR, BS_CONV, BS_ITER, BS_N };
+// BS_RZD: r.z of the direction the last iteration used; BS_B1, BS_C: the
+// residual replacement's coefficients (k_brep_fin)
+enum { BS_RZ, BS_RR, BS_BB, BS_ALPHA, BS_BETA, BS_THR, BS_CONV, BS_ITER, BS_RZD, BS_B1, BS_C, BS_N };
 
 // Blocks of entries aligned to problems.
 struct BlockTable {
@@ -171,6 +173,7 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bfin_beta(const double *__re
         else if (s[BS_ITER] >= maxit)
             s[BS_CONV] = 2;
         s[BS_BETA] = s[BS_CONV] != 0.0 ? 0.0 : rz / s[BS_RZ];
+        s[BS_RZD] = s[BS_RZ];
         s[BS_RZ] = rz;
         mine |= s[BS_CONV] == 0.0;
     }
@@ -250,6 +253,44 @@ __global__ __launch_bounds__(TR_BLK_THREADS) void k_bfin_init(const double *__re
     if (mine) atomicOr(&running, 1);
     __syncthreads();
     if (threadIdx.x == 0) all[0] = running ? 0.0 : 1.0;
+}
+
+// Residual replacement per problem (cg_replace_residual of solver.hip, one
+// problem at a time): rep[2p] = s, the stopped solve's scale over the new
+// one (0: the problem restarts), rep[2p + 1] = the r.z of its last
+// direction; part: the r.psave partials of each block.  Sets BS_B1 = r.z /
+// (s rz_dir) and BS_C = r.psave / r.r for the problems that continue, 0
+// for the others.
+__global__ void k_brep_fin(const double *__restrict__ part, const uint32_t *__restrict__ b0, uint32_t nprob,
+                           double *__restrict__ sc, const double *__restrict__ rep) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
+        double *s = sc + (size_t)p * BS_N;
+        s[BS_B1] = 0;
+        s[BS_C] = 0;
+        const double sp = rep[2 * (size_t)p], rzd = rep[2 * (size_t)p + 1];
+        if (!(sp > 0) || s[BS_CONV] != 0.0) continue;
+        double rp = 0;
+        for (uint32_t b = b0[p]; b < b0[p + 1]; b++) rp += part[b];
+        const double rr = s[BS_RR], rz = s[BS_RZ];
+        if (!(rr > 0) || !(rz > 0) || !(rzd > 0) || !isfinite(rp) || !isfinite(sp * rzd)) continue;
+        s[BS_B1] = rz / (sp * rzd);
+        s[BS_C] = rp / rr;
+    }
+}
+
+// p += b1 (psave - c r) for the problems that continue (p holds z = M r
+// after the start, psave the stopped solve's last direction)
+template <typename T>
+__global__ __launch_bounds__(TR_BLK_THREADS) void k_brep_p(T *__restrict__ p, const T *__restrict__ psave,
+                                                           const T *__restrict__ r, const uint32_t *__restrict__ lo,
+                                                           const uint32_t *__restrict__ hi,
+                                                           const uint32_t *__restrict__ prob,
+                                                           const double *__restrict__ sc) {
+    const double *s = sc + (size_t)prob[blockIdx.x] * BS_N;
+    const double b1 = s[BS_B1], c = s[BS_C];
+    if (b1 == 0.0) return;
+    for (uint32_t i = lo[blockIdx.x] + threadIdx.x; i < hi[blockIdx.x]; i += TR_BLK_THREADS)
+        p[i] = (T)((double)p[i] + b1 * ((double)psave[i] - c * (double)r[i]));
 }
 
 // dst = (float)(src * s_p);  x += s_p * (double)d   (per-problem factors)
@@ -340,10 +381,14 @@ struct BatchCg {
 // arguments the caller gives the same fields) skip the workgroups of
 // stopped problems: a.vprob and a.pstop = sc + BS_CONV.  sc is cleared
 // first, so the start's launches (before k_bfin_init) run everywhere.
+// With psave and rep (residual replacement; b holds each problem's new
+// residual and x is zero), the problems rep names continue the direction
+// saved in psave (k_brep_fin, k_brep_p) instead of starting from p = z.
 template <typename T>
 int pcg_batch(tr_ctx *c, const BlockTable &t, const OpArgs<T> &a, const LaneMap &map, T *x, const T *b, T *r, T *p,
               T *Ap, const T *dinv, double *part, double *sc, double *all, const BatchCg &prm, hipStream_t st,
-              int &launches, const ChebPrec<T> *cheb = nullptr, T *z = nullptr) {
+              int &launches, const ChebPrec<T> *cheb = nullptr, T *z = nullptr, const T *psave = nullptr,
+              const double *rep = nullptr) {
     const int NC = (int)c->op.NC;
     const dim3 gb(t.nblk), tb(TR_BLK_THREADS);
     if (cheb && !z) return fail(TR_ERR_ARG, "batch: Chebyshev preconditioner without its buffer");
@@ -358,6 +403,13 @@ int pcg_batch(tr_ctx *c, const BlockTable &t, const OpArgs<T> &a, const LaneMap 
                        all);
     if (cheb)  // p = z (beta = 0 after k_bfin_init)
         hipLaunchKernelGGL(k_bpupdate<T>, gb, tb, 0, st, p, r, dinv, t.lo, t.hi, t.prob, sc, all, (const T *)z);
+    if (psave && rep) {
+        hipLaunchKernelGGL(k_bdot<T>, gb, tb, 0, st, (const T *)r, psave, t.lo, t.hi, part, (const double *)nullptr,
+                           (const double *)sc, (const uint32_t *)t.prob);
+        hipLaunchKernelGGL(k_brep_fin, dim3((t.nprob + 255) / 256), dim3(256), 0, st, part, t.b0, t.nprob, sc, rep);
+        hipLaunchKernelGGL(k_brep_p<T>, gb, tb, 0, st, p, psave, (const T *)r, t.lo, t.hi, t.prob,
+                           (const double *)sc);
+    }
     HIPCHECK(hipGetLastError());
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -578,6 +630,14 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
         float *x32 = m.get<float>(n), *b32 = m.get<float>(n), *r32 = m.get<float>(n), *p32 = m.get<float>(n),
               *Ap32 = m.get<float>(n);
         if (!Ap32) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
+        // residual replacement (as tr_solve's): each problem whose inner CG
+        // stopped converged, and whose true residual has not drifted from
+        // the recursive one, continues its last direction (saved in psave)
+        float *psave = c->ir_reliable ? m.get<float>(n) : nullptr;
+        double *rep = c->ir_reliable ? m.get<double>(2 * (size_t)P) : nullptr;
+        if (c->ir_reliable && (!psave || !rep)) return fail(TR_ERR_HIP, "tr_batch_solve: out of device memory");
+        std::vector<double> reph(2 * (size_t)P, 0.0), scale_prev(P, 0.0), rr_rec(P, 0.0), rz_dir(P, 0.0);
+        std::vector<char> resumable(P, 0);
         BatchCg icg;
         icg.maxit = prm->inner_maxit;
         icg.tol_rel = prm->inner_tol;
@@ -618,16 +678,39 @@ int tr_batch_solve(tr_ctx *c, const tr_solve_params *prm, double *x, tr_solve_st
             HIPCHECK(hipStreamSynchronize(st));
             hipLaunchKernelGGL(k_bscale_f32, dim3(t.nblk), dim3(TR_BLK_THREADS), 0, st, r, b32, fac, t.lo, t.hi, t.prob);
             HIPCHECK(hipMemsetAsync(x32, 0, sizeof(float) * n, st));
+            bool any_rep = false;
+            if (psave) {
+                for (uint32_t q = 0; q < P; q++) {
+                    const double s_q = rnorm[q] > 0 ? scale_prev[q] / rnorm[q] : 0.0;
+                    const bool go = !done[q] && resumable[q] && s_q > 0 &&
+                                    1.0 / s_q <= kReplaceDrift * std::sqrt(rr_rec[q]);
+                    reph[2 * (size_t)q] = go ? s_q : 0.0;
+                    reph[2 * (size_t)q + 1] = go ? rz_dir[q] : 0.0;
+                    any_rep = any_rep || go;
+                }
+                if (any_rep) {
+                    // reph is rewritten next step: a synchronous copy
+                    HIPCHECK(hipMemcpy(rep, reph.data(), sizeof(double) * reph.size(), hipMemcpyHostToDevice));
+                    HIPCHECK(hipMemcpyAsync(psave, p32, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+                }
+            }
             int l = 0;
             if ((rc = pcg_batch<float>(c, t, a32, c->map32, x32, b32, r32, p32, Ap32, dinv32, part, sc, all, icg, st,
-                                       l, zc32 ? &cheb32 : nullptr, zc32)))
+                                       l, zc32 ? &cheb32 : nullptr, zc32, any_rep ? (const float *)psave : nullptr,
+                                       any_rep ? (const double *)rep : nullptr)))
                 return rc;
             launches += l;
             HIPCHECK(hipMemcpyAsync(scv.data(), sc, sizeof(double) * scv.size(), hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
             for (uint32_t q = 0; q < P; q++) {
-                inner[q] += (int)scv[(size_t)q * BS_N + BS_ITER];
+                const double *sq = scv.data() + (size_t)q * BS_N;
+                inner[q] += (int)sq[BS_ITER];
                 f[q] = done[q] ? 0.0 : rnorm[q];
+                // converged with a direction: the next step may continue it
+                resumable[q] = !done[q] && sq[BS_CONV] == 1.0 && sq[BS_ITER] > 0 && sq[BS_RZD] > 0;
+                scale_prev[q] = rnorm[q];
+                rr_rec[q] = sq[BS_RR];
+                rz_dir[q] = sq[BS_RZD];
             }
             HIPCHECK(hipMemcpyAsync(fac, f.data(), sizeof(double) * P, hipMemcpyHostToDevice, st));
             HIPCHECK(hipStreamSynchronize(st));

@@ -1521,7 +1521,6 @@ int ensure_fp32(tr_ctx *c) {
 // (Defaults; tr_ctx_set_refinement changes them per context.)
 static constexpr double kStagnation = 0.5;
 static constexpr int kInnerStallWindow = 100;
-static constexpr double kReplaceDrift = 1.02;
 
 static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solve_stats *st) {
     int rc = ensure_fp32(c);
@@ -1723,10 +1722,9 @@ static int solve_mixed(tr_ctx *c, const tr_solve_params *prm, double *f, tr_solv
         // continue the inner CG: its recursive residual replaced by the
         // true one in the new scale, its last direction kept; unless the
         // true residual (in the stopped solve's scale, rn / scale_prev) has
-        // drifted from the recursive one by more than 2%: then the kept
-        // direction no longer fits the residual, and the step restarts
-        // (measured: replacements at ratios up to 1.004 saved 5-12% of the
-        // inner iterations, at 1.05 to 1.9 they cost up to 60%)
+        // drifted from the recursive one by more than kReplaceDrift: then
+        // the kept direction no longer fits the residual, and the step
+        // restarts
         if (c->ir_reliable && resumable && rn / scale_prev <= kReplaceDrift * std::sqrt(rr_rec)) {
             HIPCHECK(launch_to_f32_scaled(r, r32, 1.0 / rn, n, c->stream, c->perm32(), c->op.slm));
             CgParams cq = cp;
