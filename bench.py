@@ -136,7 +136,7 @@ def cpu_baseline(P, small, c1, seconds=8.0):
     return out
 
 
-def bench_batch(count, steps, warmup):
+def bench_batch(count, steps, warmup, solve=True, copy_calibration=False):
     """The batching workload: `count` independent slabs (n ~ U{8..32}),
     log-normal opacity, per-problem albedo, K = 2, solved together to a 1e-8
     relative FP64 residual per problem."""
@@ -175,6 +175,12 @@ def bench_batch(count, steps, warmup):
             key, ctx.kernel_name(prec), k, out[key]["achieved_gbs"], out[key]["hbm_frac"]), file=sys.stderr, flush=True)
     out["value"] = out["operator_fp32"]["dof_ordinate_per_s"]
     out["ms_per_step"] = out["operator_fp32"]["kernel_ms"]
+    if copy_calibration:
+        cms = ctx.bench_copy(256 << 20, 10)
+        print("[bench-batch] copy 256 MiB: %.3f ms" % cms, file=sys.stderr, flush=True)
+    if not solve:
+        print(json.dumps(out), flush=True)
+        return
     solves = {}
     ctx.set_chebyshev(CHEB_DEGREE, CHEB_RATIO, CHEB_SPLIT)
     # the mixed solves' inner tolerances (residual replacement every 1e-2),
@@ -654,7 +660,7 @@ def main():
               "(the multi-GPU sharding is a design note, not an implementation)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.batch:
-        bench_batch(args.batch, args.steps, args.warmup)
+        bench_batch(args.batch, args.steps, args.warmup, not args.no_solve, args.copy_calibration)
     elif args.sphere:
         bench_sphere(args.sphere, min(args.steps, 50), min(args.warmup, 5))
     elif args.c2:
