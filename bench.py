@@ -371,6 +371,13 @@ def bench_c3(args):
         op64["kernel_only"] = {k: k64[k] for k in ("kernel_ms", "kernel_ms_min", "kernel_ms_max", "achieved_gbs")}
         log(tag, "FP64 operator kernel alone %.1f us (%.0f GB/s algorithmic)" % (
             k64["kernel_ms"] * 1e3, k64["achieved_gbs"]))
+    # the same FP32 operator as a solver iteration launches it: with the
+    # fused p.Ap dot, and with a 1 GiB copy before each application (the
+    # cache state that the iteration's vector passes leave behind)
+    for mode, key in ((3, "fused_dot_ms"), (4, "fused_dot_after_copy_ms")):
+        op32[key] = ctx.bench_operator(warmup, steps, precision=mode)[1]
+    log(tag, "FP32/BF16 operator with the fused dot %.1f us, after a 1 GiB copy %.1f us" % (
+        op32["fused_dot_ms"] * 1e3, op32["fused_dot_after_copy_ms"] * 1e3))
     use_exact()
     _, op32x = time_operator(ctx, P, 1, steps, warmup, 4)
     use_f32mfma()

@@ -474,8 +474,12 @@ double tr_bench_lhs(tr_ctx *ctx, int reps);
  * its schedule's harmonic order: under the FP64 K-blocked schedule the
  * kernel alone, without the permutation of an internal-order input) after
  * `warmup` untimed ones, with HIP events on the context's stream around
- * each launch.  out[0] wall seconds of the timed loop (device-synchronised
- * on both sides), out[1] mean, out[2] min, out[3] max ms per application. */
+ * each launch.  Precision 3 times the FP32 operator with its fused f.Af
+ * dot (as a solver iteration launches it), 4 the same with a 1 GiB copy
+ * before each application (the cache state after a solve's vector passes;
+ * the copy is outside the timed events).  out[0] wall seconds of the timed
+ * loop (device-synchronised on both sides), out[1] mean, out[2] min, out[3]
+ * max ms per application. */
 int tr_bench_operator(tr_ctx *ctx, int precision, int warmup, int steps, double *out4);
 /* Name of the HIP kernel of the current operator schedule, as profilers
  * print it (without the namespace and argument list). */

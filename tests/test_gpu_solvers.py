@@ -270,3 +270,19 @@ def test_zero_opacity_is_rejected():
     with pytest.raises(ta.TransferError) as e:
         ta.HipContext(m, ta.Angular(1), mesh_view=ta.mesh_view_of(ar))
     assert e.value.code == ta.TR_ERR_ARG
+
+
+def test_operator_timing_modes():
+    """tr_bench_operator's modes: the FP64 and FP32 operators (0, 1), the
+    FP64 kernel alone (2), the FP32 operator with its fused dot (3) and that
+    after a 1 GiB copy (4) all run and report positive per-application
+    times, and a bad mode is an argument error."""
+    P = build("ball16-K5")
+    ctx = ta.HipContext(P.mesh, P.ang)
+    ctx.set_angular_precision(3)
+    for mode in range(5):
+        w, k, kmin, kmax = ctx.bench_operator(1, 3, precision=mode)
+        print("[gpu-solvers] bench_operator mode %d: %.1f us" % (mode, k * 1e3), flush=True)
+        assert w > 0 and 0 < kmin <= k <= kmax
+    with pytest.raises(ta.TransferError):
+        ctx.bench_operator(1, 3, precision=5)

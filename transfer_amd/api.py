@@ -670,7 +670,8 @@ class HipContext:
     def bench_operator(self, warmup, steps, precision=0):
         """(wall s, mean, min, max ms per application) of the FP64 (0) or FP32
         (1) operator, or (2) the FP64 one on vectors in its own harmonic order
-        (the K-blocked FP64 kernel alone)"""
+        (the K-blocked FP64 kernel alone); (3) the FP32 operator with its
+        fused f.Af dot, (4) that after a 1 GiB copy before each application"""
         out = np.zeros(4)
         check(lib.tr_bench_operator(self._h, int(precision), warmup, steps, ptr(out)))
         return tuple(out)

@@ -1824,14 +1824,44 @@ double tr_bench_lhs(tr_ctx *c, int reps) {
 
 }  // extern "C"
 
+namespace tr {
+__global__ void k_copy_calib(const double4 *__restrict__ src, double4 *__restrict__ dst, size_t n);
+}
+
 extern "C" int tr_bench_operator(tr_ctx *c, int precision, int warmup, int steps, double *out) {
-    if (!c || !out || steps < 1 || warmup < 0 || precision < 0 || precision > 2)
+    if (!c || !out || steps < 1 || warmup < 0 || precision < 0 || precision > 4)
         return fail(TR_ERR_ARG, "tr_bench_operator: bad argument");
     int rc;
+    // 3: the FP32 operator with its fused f.Af dot (the launch of a solver
+    // iteration); 4: the same after a 1 GiB copy before each application,
+    // so the caches hold what a solve's vector passes leave in them
+    const bool dotp = precision >= 3, flush = precision == 4;
+    if (dotp) precision = 1;
     if (precision == 1 && (rc = ensure_fp32(c))) return rc;
     Work w(c);
-    double *f = w.get(), *r = w.get();
-    if (!r) return fail(TR_ERR_HIP, "tr_bench_operator: out of device memory");
+    double *f = w.get(), *r = w.get(), *part = dotp ? w.get() : nullptr;
+    if (!r || (dotp && !part)) return fail(TR_ERR_HIP, "tr_bench_operator: out of device memory");
+    const size_t fl_n = (size_t)1 << 25;  // 32-byte elements: 1 GiB read and written
+    double4 *fl_a = nullptr, *fl_b = nullptr;
+    if (flush) {
+        if (hipMalloc((void **)&fl_a, fl_n * 32) != hipSuccess) return fail(TR_ERR_HIP, "tr_bench_operator: out of device memory");
+        if (hipMalloc((void **)&fl_b, fl_n * 32) != hipSuccess) {
+            hipFree(fl_a);
+            return fail(TR_ERR_HIP, "tr_bench_operator: out of device memory");
+        }
+        hipMemsetAsync(fl_a, 0, fl_n * 32, c->stream);
+    }
+    struct FlushFree {
+        double4 *a, *b;
+        hipStream_t s;
+        ~FlushFree() {
+            if (a) {
+                hipStreamSynchronize(s);
+                hipFree(a);
+                hipFree(b);
+            }
+        }
+    } fl_free{fl_a, fl_b, c->stream};
     // a non-trivial input: f = rhs (converted to FP32 for precision 1, in
     // the first half of the FP64 buffer f; the result goes to r)
     HIPCHECK(launch_rhs(c->d_rhs, c->op.nP, c->op.slm, f, c->stream));
@@ -1839,7 +1869,7 @@ extern "C" int tr_bench_operator(tr_ctx *c, int precision, int warmup, int steps
     if (precision == 1) HIPCHECK(launch_to_f32_scaled(f, f32, 1.0, c->nint(), c->stream, c->perm32(), c->op.slm));
     auto apply = [&]() -> hipError_t {
         if (precision == 1)
-            return launch_apply<float>(c->args32(), (int)c->op.NC, c->map32, f32, r32, nullptr, nullptr, c->stream);
+            return launch_apply<float>(c->args32(), (int)c->op.NC, c->map32, f32, r32, part, nullptr, c->stream);
         // precision 2: the FP64 operator on vectors in its own harmonic
         // order (the K-blocked schedule's kernel alone, as its solves run it)
         return launch_apply<double>(precision == 2 ? c->args_cls() : c->args(), (int)c->op.NC, c->map, f, r,
@@ -1851,6 +1881,7 @@ extern "C" int tr_bench_operator(tr_ctx *c, int precision, int warmup, int steps
     HIPCHECK(hipStreamSynchronize(c->stream));
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < steps; i++) {
+        if (flush) hipLaunchKernelGGL(k_copy_calib, dim3(256 * 8 * 4), dim3(256), 0, c->stream, fl_a, fl_b, fl_n);
         HIPCHECK(hipEventRecord(ev[2 * i], c->stream));
         HIPCHECK(apply());
         HIPCHECK(hipEventRecord(ev[2 * i + 1], c->stream));
