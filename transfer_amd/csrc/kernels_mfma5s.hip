@@ -258,8 +258,15 @@ __global__ __launch_bounds__(512) void k_apply5s(OpArgs<float> a, const float *_
         auto ldc = [&](int s2, float *c) __attribute__((always_inline)) {
 #pragma unroll
             for (int e2 = 0; e2 < PE; e2++)
-                c[e2] = bload1<float>(crs, cv + (uint32_t)(e2 * (int)sizeof(float)), (uint32_t)s2 * cstride);
+                c[e2] = bload1<float, kNonTemporal>(crs, cv + (uint32_t)(e2 * (int)sizeof(float)),
+                                                    (uint32_t)s2 * cstride);
         };
+        // the records stream through non-temporal loads: read once per
+        // application, they would otherwise push out of the caches the
+        // vectors that the solver's next passes read (the mixed solve on
+        // the 96^3 ball: 0.353 against 0.373 s).  The FP64 operator's
+        // records load normally: there the non-temporal form was slower
+        // (0.888 against 0.851 s for the plain FP64 solve).
         // records held in registers across the blocks (reloading them with
         // each block's rows frees registers for a third wavefront per SIMD,
         // but measured slower: 490 against 370 us on the 96^3 ball)

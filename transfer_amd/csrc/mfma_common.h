@@ -38,12 +38,16 @@ __device__ inline __amdgpu_buffer_rsrc_t rsrc4(const T *p, size_t bytes) {
                                              0x00020000);
 }
 
-template <typename T>
+// AUX: the cache policy bits; kNonTemporal for streams that are read once
+// per operator application (the coefficient records), so they do not
+// displace the vectors that the next kernel reads again
+constexpr int kNonTemporal = 2;
+template <typename T, int AUX = 0>
 __device__ inline T bload1(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
     if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, AUX));
     else
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, AUX));
 }
 
 // coefficient record of one slot (RS elements, a 16-byte multiple)

@@ -75,7 +75,9 @@ __device__ inline void sum_parts(const double *part, int n, int n2, double &a, d
 // alpha and whether the pass ran (SC_RUN).  With x != null also x +=
 // alpha p (else the p update pass does it, reading p there anyway).
 // With z1 != null also z1 = cscale cdinv r (the first Chebyshev iterate of
-// a polynomial preconditioner) from the new r.
+// a polynomial preconditioner) from the new r.  The old r and Ap are read
+// non-temporally (neither is read again this iteration), so the caches
+// keep the new r for the p update pass.
 template <typename T>
 __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T *__restrict__ p,
                                                       T *__restrict__ r, const T *__restrict__ Ap,
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
         d2 *z2 = reinterpret_cast<d2 *>(z1);
         for (size_t k = i; k < n2; k += S) {
             if (x) x2[k] = alpha * p2[k] + x2[k];
-            const d2 rv = r2[k] - alpha * a2[k];
+            const d2 rv = __builtin_nontemporal_load(&r2[k]) - alpha * __builtin_nontemporal_load(&a2[k]);
             r2[k] = rv;
             if (z1) z2[k] = cscale * c2[k] * rv;
             const d2 zv = dinv ? dd2[k] * rv : rv;
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
         f4 *z4 = reinterpret_cast<f4 *>(z1);
         for (size_t k = i; k < n4; k += S) {
             if (x) x4[k] = alpha * p4[k] + x4[k];
-            const f4 rv = r4[k] - alpha * a4[k];
+            const f4 rv = __builtin_nontemporal_load(&r4[k]) - alpha * __builtin_nontemporal_load(&a4[k]);
             r4[k] = rv;
             if (z1) z4[k] = cscale * c4[k] * rv;
             const f4 zv = dinv ? d4[k] * rv : rv;
@@ -172,7 +174,8 @@ __global__ __launch_bounds__(256) void k_update_fused(T *__restrict__ x, const T
 // part[TR_UPD_BLOCKS .. TR_UPD_BLOCKS + Grz)).  It runs when the
 // iteration's update pass ran (SC_RUN, which no workgroup of this launch
 // writes), so every workgroup updates x, also in the iteration that stops
-// the solve.
+// the solve.  r, the old p and x go through non-temporal accesses, so the
+// caches keep the new p for the operator that reads it next.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const T *__restrict__ r,
                                                        const T *__restrict__ dinv, const T *__restrict__ z,
@@ -241,10 +244,11 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
         const d2 *r2 = reinterpret_cast<const d2 *>(r), *dd2 = reinterpret_cast<const d2 *>(dinv);
         const d2 *zz2 = reinterpret_cast<const d2 *>(z);
         for (size_t k = i; k < n2; k += S) {
-            const d2 pv = p2[k];
-            if (x) x2[k] = alpha * pv + x2[k];
+            const d2 pv = __builtin_nontemporal_load(&p2[k]);
+            if (x) __builtin_nontemporal_store(alpha * pv + __builtin_nontemporal_load(&x2[k]), &x2[k]);
             if (pupd) {
-                const d2 zv = z ? zz2[k] : (dinv ? dd2[k] * r2[k] : r2[k]);
+                const d2 rv = __builtin_nontemporal_load(&r2[k]);
+                const d2 zv = z ? zz2[k] : (dinv ? dd2[k] * rv : rv);
                 p2[k] = beta * pv + zv;
             }
         }
@@ -256,10 +260,11 @@ __global__ __launch_bounds__(256) void k_pupdate_fused(T *__restrict__ p, const 
         const f4 *r4 = reinterpret_cast<const f4 *>(r), *d4 = reinterpret_cast<const f4 *>(dinv);
         const f4 *zz4 = reinterpret_cast<const f4 *>(z);
         for (size_t k = i; k < n4; k += S) {
-            const f4 pv = p4[k];
-            if (x) x4[k] = alpha * pv + x4[k];
+            const f4 pv = __builtin_nontemporal_load(&p4[k]);
+            if (x) __builtin_nontemporal_store(alpha * pv + __builtin_nontemporal_load(&x4[k]), &x4[k]);
             if (pupd) {
-                const f4 zv = z ? zz4[k] : (dinv ? d4[k] * r4[k] : r4[k]);
+                const f4 rv = __builtin_nontemporal_load(&r4[k]);
+                const f4 zv = z ? zz4[k] : (dinv ? d4[k] * rv : rv);
                 p4[k] = beta * pv + zv;
             }
         }
